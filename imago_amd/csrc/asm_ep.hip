@@ -1,0 +1,6 @@
+// asm_ep.hip - k_assemble instances for the EP integral kind.
+#include "assemble_launch.hpp"
+
+namespace imago {
+hipError_t launch_assemble_ep(const AssembleLaunch &L) { return launch_kind<K_EP>(L); }
+}  // namespace imago

@@ -1,0 +1,59 @@
+// assemble_launch.hpp - launchers of k_assemble, one translation unit per
+// integral kind (asm_ol.hip, asm_ke.hip, asm_np.hip, asm_ep.hip) so the
+// template instances compile in parallel.
+#pragma once
+#include "assemble.hpp"
+
+namespace imago {
+
+struct AssembleLaunch {
+    DevSys S;
+    const PairRec *pairs;
+    const ImgRec *imgs;
+    const double2 *phase;
+    const CandRec *cands;
+    const int *cand_off;
+    int term0;
+    double *out;
+    long term_stride;
+    int npairs, nterms;
+    size_t lds;
+    hipStream_t stream;
+    int lmax, ne;
+    bool gamma;
+};
+
+hipError_t launch_assemble_ol(const AssembleLaunch &L);
+hipError_t launch_assemble_ke(const AssembleLaunch &L);
+hipError_t launch_assemble_np(const AssembleLaunch &L);
+hipError_t launch_assemble_ep(const AssembleLaunch &L);
+
+template <int KIND, int LMAX, int NE, bool G>
+hipError_t launch_one(const AssembleLaunch &L)
+{
+    hipLaunchKernelGGL((k_assemble<KIND, LMAX, NE, G>), dim3(L.npairs, L.nterms), dim3(64), L.lds,
+                       L.stream, L.S, L.pairs, L.imgs, L.phase, L.cands, L.cand_off, L.term0,
+                       L.out, L.term_stride);
+    return hipGetLastError();
+}
+
+template <int KIND>
+hipError_t launch_kind(const AssembleLaunch &L)
+{
+#define IMAGO_CASE(LM)                                                          \
+    case LM:                                                                    \
+        if (L.ne == 4) return L.gamma ? launch_one<KIND, LM, 4, true>(L)        \
+                                      : launch_one<KIND, LM, 4, false>(L);      \
+        return L.gamma ? launch_one<KIND, LM, 16, true>(L)                      \
+                       : launch_one<KIND, LM, 16, false>(L);
+    switch (L.lmax) {
+        IMAGO_CASE(0)
+        IMAGO_CASE(1)
+        IMAGO_CASE(2)
+        IMAGO_CASE(3)
+    default: return hipErrorInvalidValue;
+    }
+#undef IMAGO_CASE
+}
+
+}  // namespace imago

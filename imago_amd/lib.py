@@ -1,0 +1,221 @@
+"""ctypes host interface to libimago_hip.so (include/imago_hip.h).
+
+Mirrors the reference's call surface for the integral and solve stages:
+``Context.assemble`` = gaussOverlapOL/KE/NP + elecPotGaussOverlap
+(src/imago/integrals.F90), ``Context.secular`` = secularEqnSCF's H build and
+zhegv/dsygv (secularEqn.F90:37-328, interfaces.F90:73-200),
+``Context.density_traces`` = the traces of makeValenceRho
+(valeCharge.F90:230-393).  Errors raise ``ImagoError`` carrying the
+library's message, the counterpart of the reference's ``stop``.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is
+visible, construction fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from ._desc import make_desc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libimago_hip.so")
+
+OL, KE, NP, EP = 1, 2, 3, 4
+KIND_CODES = {"ol": OL, "ke": KE, "np": NP, "ep": EP}
+
+ERRORS = {
+    -1: "IMAGO_E_ARG", -2: "IMAGO_E_HIP", -3: "IMAGO_E_LATTICE", -4: "IMAGO_E_NOTPD",
+    -5: "IMAGO_E_SOLVE", -6: "IMAGO_E_STATE", -7: "IMAGO_E_SCREEN", -8: "IMAGO_E_NOMEM",
+}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+EXPORTS = [
+    "imago_ctx_create", "imago_ctx_destroy", "imago_last_error", "imago_set_system",
+    "imago_primitive_batch", "imago_boys_batch", "imago_assemble", "imago_get_packed",
+    "imago_secular", "imago_get_eigenvectors", "imago_density_traces", "imago_zhegv",
+    "imago_dsygv", "imago_last_timing",
+]
+
+
+class ImagoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+        self.name = ERRORS.get(code, "?")
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libimago_hip.so is not built: run __graft_entry__.build() "
+                           "or python -m imago_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    L.imago_last_error.restype = ctypes.c_char_p
+    L.imago_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.imago_ctx_destroy.argtypes = [ctypes.c_void_p]
+    L.imago_set_system.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _ip, _ip]
+    L.imago_primitive_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                        _ip, _dp]
+    L.imago_boys_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    L.imago_assemble.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.imago_get_packed.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _dp]
+    L.imago_secular.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _ip]
+    L.imago_get_eigenvectors.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.imago_density_traces.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    L.imago_zhegv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _ip]
+    L.imago_dsygv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _ip]
+    L.imago_last_timing.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc):
+    if rc != 0:
+        raise ImagoError(rc, load().imago_last_error().decode())
+
+
+class Context:
+    """One library context on one GPU (not thread-safe)."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = ctypes.c_void_p()
+        _check(L.imago_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.system = None
+        self.vale_dim = 0
+        self.pot_dim = 0
+
+    def close(self):
+        if self._h:
+            load().imago_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set-up -------------------------------------------------------------
+    def set_system(self, system):
+        desc, keep = make_desc(system)
+        n = ctypes.c_int()
+        pd = ctypes.c_int()
+        _check(load().imago_set_system(self._h, ctypes.byref(desc), ctypes.byref(n),
+                                       ctypes.byref(pd)))
+        del keep
+        self.system = system
+        self.vale_dim = n.value
+        self.pot_dim = pd.value
+        return self
+
+    # -- primitives ---------------------------------------------------------
+    def primitive_batch(self, kind, a, xyz, sw):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 9)
+        sw = np.ascontiguousarray(sw, dtype=np.int32).reshape(-1)
+        n = len(sw)
+        out = np.zeros((n, 256))
+        _check(load().imago_primitive_batch(self._h, KIND_CODES.get(kind, kind), n, _p(a),
+                                            _p(xyz), sw.ctypes.data_as(_ip), _p(out)))
+        return out.reshape(n, 16, 16).transpose(0, 2, 1)   # sh[t][i][j] = sh(i,j)
+
+    def boys_batch(self, T):
+        T = np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
+        F = np.zeros((len(T), 7))
+        _check(load().imago_boys_batch(self._h, len(T), _p(T), _p(F)))
+        return F
+
+    # -- assembly -----------------------------------------------------------
+    def assemble(self, kind):
+        _check(load().imago_assemble(self._h, KIND_CODES.get(kind, kind)))
+        return self
+
+    def get_packed(self, kind, term=0, kpoint=0):
+        n = self.vale_dim
+        npack = n * (n + 1) // 2
+        if self.system.gamma:
+            out = np.zeros(npack)
+            _check(load().imago_get_packed(self._h, KIND_CODES.get(kind, kind), term, kpoint,
+                                           _p(out)))
+            return out
+        out = np.zeros(2 * npack)
+        _check(load().imago_get_packed(self._h, KIND_CODES.get(kind, kind), term, kpoint,
+                                       _p(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    # -- secular equation ---------------------------------------------------
+    def secular(self, pot_coeffs, num_states):
+        nk = self.system.num_kpoints
+        c = np.ascontiguousarray(pot_coeffs, dtype=np.float64)
+        ev = np.zeros(nk * num_states)
+        info = np.zeros(nk, dtype=np.int32)
+        rc = load().imago_secular(self._h, _p(c), int(num_states), _p(ev),
+                                  info.ctypes.data_as(_ip))
+        self.last_info = info
+        _check(rc)
+        return ev.reshape(nk, num_states)
+
+    def eigenvectors(self, kpoint, num_states):
+        n = self.vale_dim
+        if self.system.gamma:
+            out = np.zeros(n * num_states)
+            _check(load().imago_get_eigenvectors(self._h, kpoint, _p(out)))
+            return out.reshape(num_states, n).T
+        out = np.zeros(2 * n * num_states)
+        _check(load().imago_get_eigenvectors(self._h, kpoint, _p(out)))
+        z = out[0::2] + 1j * out[1::2]
+        return z.reshape(num_states, n).T
+
+    def density_traces(self, occupations):
+        occ = np.ascontiguousarray(occupations, dtype=np.float64)
+        tr = np.zeros(3)
+        pr = np.zeros(max(self.pot_dim, 1))
+        _check(load().imago_density_traces(self._h, _p(occ), _p(tr), _p(pr)))
+        return tr, pr[:self.pot_dim]
+
+    def timing(self, stage):
+        v = ctypes.c_double()
+        _check(load().imago_last_timing(self._h, stage, ctypes.byref(v)))
+        return v.value
+
+    # -- single problems ----------------------------------------------------
+    def hegv(self, H, S):
+        """zhegv/dsygv itype=1 'V' 'U' on the GPU: returns (w, vectors)."""
+        n = H.shape[0]
+        info = ctypes.c_int()
+        w = np.zeros(n)
+        if np.iscomplexobj(H):
+            A = np.asfortranarray(H, dtype=np.complex128).copy(order="F")
+            B = np.asfortranarray(S, dtype=np.complex128).copy(order="F")
+            _check(load().imago_zhegv(self._h, n, A.ctypes.data_as(_dp), B.ctypes.data_as(_dp),
+                                      _p(w), ctypes.byref(info)))
+        else:
+            A = np.asfortranarray(H, dtype=np.float64).copy(order="F")
+            B = np.asfortranarray(S, dtype=np.float64).copy(order="F")
+            _check(load().imago_dsygv(self._h, n, A.ctypes.data_as(_dp), B.ctypes.data_as(_dp),
+                                      _p(w), ctypes.byref(info)))
+        return w, A

@@ -1,0 +1,195 @@
+/*
+ * imago_hip.h - C ABI of the MI355X integral + k-point eigensolve library.
+ *
+ * Replaces the body of the Imago/OLCAO engine's setup-time integral stage
+ * and its per-iteration secular-equation stage (UMKC-CPG/imago):
+ *   src/imago/gaussIntegrals.f90  primitive Obara-Saika integrals + Boys
+ *   src/imago/integrals.F90       lattice-summed OL/KE/NP/EP assembly
+ *   src/imago/intgSaving.F90      contraction, Bloch phases, block save
+ *   src/imago/secularEqn.F90      H(k) = NP + KE + sum_j c_j V_j, solve
+ *   src/imago/interfaces.F90      zhegv / dsygv boundary
+ *   src/imago/valeCharge.F90      density matrix and Tr(rho M) traces
+ *
+ * Conventions (all entry points):
+ *   - every function returns IMAGO_OK (0) or a negative IMAGO_E* code;
+ *     imago_last_error() gives the message of the most recent failure
+ *     (the reference writes a message to unit 20 and stops instead);
+ *   - arrays are column-major like the Fortran host's, caller-owned, host
+ *     memory unless the name says _dev;
+ *   - "packed" = upper triangle column by column, element (r,c), r<=c,
+ *     1-based, at index c(c-1)/2 + r; complex packed matrices interleave
+ *     (re,im) pairs exactly like the reference's real(2, n(n+1)/2) arrays
+ *     (src/imago/matrixSubs.F90:506-535, hdf5SCFIntg.F90:134);
+ *   - a context is not thread-safe; use one per host thread.
+ *
+ * No symbol here takes a torch or HIP type: the library owns its device
+ * memory and stream.
+ */
+#ifndef IMAGO_HIP_H
+#define IMAGO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMAGO_OK 0
+#define IMAGO_E_ARG (-1)        /* invalid argument / inconsistent sizes   */
+#define IMAGO_E_HIP (-2)        /* HIP runtime or library failure          */
+#define IMAGO_E_LATTICE (-3)    /* cell list too small (integrals.F90:378) */
+#define IMAGO_E_NOTPD (-4)      /* overlap not positive definite (zhegv
+                                   info > n, interfaces.F90:106-109)       */
+#define IMAGO_E_SOLVE (-5)      /* eigensolver did not converge (0<info<=n)*/
+#define IMAGO_E_STATE (-6)      /* call order: e.g. assemble before set    */
+#define IMAGO_E_SCREEN (-7)     /* screening table not monotone: the
+                                   reference's alpha walk would differ     */
+#define IMAGO_E_NOMEM (-8)      /* device allocation failed                */
+
+/* integral kinds (opcodes of integrals.F90 / ortho()) */
+#define IMAGO_OL 1   /* overlap            gaussOverlapOL  integrals.F90:151  */
+#define IMAGO_KE 2   /* kinetic            gaussOverlapKE  integrals.F90:656  */
+#define IMAGO_NP 3   /* nuclear potential  gaussOverlapNP  integrals.F90:1531 */
+#define IMAGO_EP 4   /* electronic pot.    elecPotGaussOverlap :2357          */
+
+/* Boys-function modes for the nuclear integral */
+#define IMAGO_BOYS_REFERENCE 0  /* gaussIntegrals.f90:16080-16221 verbatim */
+
+/*
+ * System description: what the reference keeps as module state after
+ * setupSCF's input/lattice/k-point/basis steps (imago.F90:103-256).
+ * All indices are 0-based.  Elements are small integers shared by types
+ * of the same element: the reference keys its screening tables by element
+ * and takes the exponents of the FIRST type of each element
+ * (gaussRelations.f90:64-278); this library does the same.
+ */
+typedef struct imago_system_desc {
+    /* atom types (atomicTypes.f90; basis.f90:55-243) */
+    int num_atom_types;
+    int max_alphas;               /* leading dimension of type_alphas      */
+    int max_states;               /* leading dimension of type_basis       */
+    const int *type_element;      /* [num_atom_types]                      */
+    const int *type_num_orb_alphas; /* [num_atom_types][4] numOrbAlphas(l+1),
+                                     non-increasing in l                   */
+    const double *type_alphas;    /* [num_atom_types][max_alphas] ascending */
+    const int *type_num_states;   /* [num_atom_types] valence states        */
+    const double *type_basis;     /* [num_atom_types][max_states][max_alphas]
+                                     radial coefficient * angNorm(lm)      */
+    const int *type_lm_index;     /* [num_atom_types][max_states] 1..16     */
+
+    /* atom sites (atomicSites.f90) */
+    int num_atoms;
+    const int *atom_type;         /* [num_atoms]                            */
+    const double *atom_pos;       /* [num_atoms][3] Cartesian, bohr         */
+
+    /* potential types and sites (potTypes.f90, potSites.f90) */
+    int num_pot_types;
+    int max_pot_alphas;
+    const int *pot_type_element;  /* [num_pot_types]                        */
+    const int *pot_type_num_alphas; /* [num_pot_types]                      */
+    const double *pot_type_alphas;  /* [num_pot_types][max_pot_alphas] asc. */
+    const double *pot_type_nuc_charge; /* [num_pot_types] Z                 */
+    const double *pot_type_nuc_alpha;  /* [num_pot_types] nucAlpha          */
+    const int *pot_type_first_site;    /* [num_pot_types] first site index  */
+    const int *pot_type_multiplicity;  /* [num_pot_types] contiguous sites  */
+    int num_pot_sites;
+    const int *pot_site_type;     /* [num_pot_sites]                        */
+    const double *pot_site_pos;   /* [num_pot_sites][3]                     */
+
+    /* real-space lattice (lattice.f90:310-696): cells sorted by |R|^2 */
+    double real_vectors[9];       /* a,b,c as columns: real_vectors[3*i+x]  */
+    int num_cells;
+    const double *cell_dims;      /* [num_cells][3]                         */
+    const double *cell_sizes;     /* [num_cells] |R|^2 ascending            */
+
+    /* k-points, Cartesian with the 2*pi included (kpoints.f90:543-576) */
+    int num_kpoints;
+    const double *kpoints;        /* [num_kpoints][3]                       */
+    const double *kpoint_weights; /* [num_kpoints]                          */
+    int gamma;                    /* 1: real Gamma-only build (imagoG)      */
+
+    double log_basis_fn_thresh;   /* -ln(basisFnConvg) (lattice.f90:239)    */
+} imago_system_desc;
+
+typedef struct imago_ctx imago_ctx;
+
+/* ---- context ---------------------------------------------------------- */
+int imago_ctx_create(int device, imago_ctx **ctx);
+int imago_ctx_destroy(imago_ctx *ctx);
+const char *imago_last_error(void);
+
+/* Upload a system.  Replaces the module state that initializeAtomSite /
+ * initializePotSite read (basis.f90:257-383), the screening tables of
+ * makeAlphaDist/makeAlphaNucDist/makeAlphaPotDist (gaussRelations.f90:
+ * 64-278) and the phase factors of computePhaseFactors (kpoints.f90:578).
+ * Returns valeDim and potDim through the out pointers. */
+int imago_set_system(imago_ctx *ctx, const imago_system_desc *sys,
+                     int *vale_dim, int *pot_dim);
+
+/* ---- primitive seam: gaussIntegrals.f90, evaluated on the GPU ---------- *
+ * Batched versions of overlap2CIntg (:26), kinetic2CIntg (:4422),
+ * electron3CIntg (:11624), nuclear3CIntg (:16223) and boys (:16080).
+ * For n tuples: a[3n] exponents (a3 unused for 2-centre kinds), centres
+ * xyz[9n] as A,B,C, l1l2switch[n] as in integrals.F90:533-536.  sh[256n]
+ * receives each sh(16,16) column-major; entries outside the leading
+ * lmAlphaIndex(a1) x lmAlphaIndex(a2) block are written as 0. */
+int imago_primitive_batch(imago_ctx *ctx, int kind, int n, const double *a,
+                          const double *xyz, const int *l1l2switch,
+                          double *sh);
+int imago_boys_batch(imago_ctx *ctx, int n, const double *T, double *F7);
+
+/* ---- assembly: gaussOverlapOL/KE/NP, elecPotGaussOverlap -------------- *
+ * Computes the lattice-summed, Bloch-phased matrices of one kind into
+ * device-resident packed panels (per k for the complex build; one real
+ * panel for gamma).  For IMAGO_EP all potDim terms are built, term j in the
+ * reference's order (potential type, then alpha; integrals.F90:2411-2476).
+ * coreDim must be 0, so orthoOL/ortho reduce to packing (intgOrtho.F90). */
+int imago_assemble(imago_ctx *ctx, int kind);
+
+/* Copy one assembled packed matrix to the host: kind, term (EP only, else
+ * 0), k-point.  out holds 2*n(n+1)/2 doubles (complex) or n(n+1)/2 (gamma).
+ * This is the array the reference hands to h5dwrite_f in ortho(). */
+int imago_get_packed(imago_ctx *ctx, int kind, int term, int kpoint,
+                     double *out);
+
+/* ---- secular equation: secularEqnSCF (secularEqn.F90:37-328) ---------- *
+ * H(k) = NP(k) + KE(k) + sum_j c_j V_j(k) for every k-point, with the
+ * reference's rule that a coefficient of exactly 0.0 adds V_j unscaled
+ * (readPackedMatrixAccum, matrixSubs.F90:369-373); then the generalized
+ * problem H c = e S c of every k-point is solved on the GPU (zhegv/dsygv
+ * itype=1, 'V', 'U'; interfaces.F90:73-200).  eigenvalues[nk*num_states]
+ * (k-major) receive the lowest num_states values; eigenvectors stay on the
+ * device for imago_density_traces.  info_out[nk] gets LAPACK-style info. */
+int imago_secular(imago_ctx *ctx, const double *pot_coeffs, int num_states,
+                  double *eigenvalues, int *info_out);
+
+/* Copy the eigenvectors of one k-point (n x num_states, column-major,
+ * complex interleaved or real for gamma) to the host. */
+int imago_get_eigenvectors(imago_ctx *ctx, int kpoint, double *out);
+
+/* ---- valence density and traces: makeValenceRho (valeCharge.F90) ----- *
+ * rho(k) = sum_n f(n,k) c_n c_n^H (zher, upper), then Re Tr(rho M) via
+ * matrixElementMult (matrixSubs.F90:66-112) summed over k for
+ * M = S, NP, KE (traces[0..2]) and every V_j (pot_rho[j]).
+ * occupations[nk*num_states] are the structured electron populations. */
+int imago_density_traces(imago_ctx *ctx, const double *occupations,
+                         double *traces3, double *pot_rho);
+
+/* ---- single generalized eigenproblem on the GPU ----------------------- *
+ * solveZHEGV (interfaces.F90:73-120): A (n x n complex, interleaved,
+ * column-major, upper triangle read) is overwritten by the eigenvectors,
+ * B by its Cholesky factor, w[n] gets all eigenvalues, *info as LAPACK. */
+int imago_zhegv(imago_ctx *ctx, int n, double *A, double *B, double *w,
+                int *info);
+/* solveDSYGV (interfaces.F90:152-200), real symmetric. */
+int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
+                int *info);
+
+/* ---- timing ----------------------------------------------------------- *
+ * Milliseconds of the last launch of the named stage measured with HIP
+ * events on the library's stream: 0 = last imago_assemble, 1 = H build of
+ * the last imago_secular, 2 = its eigensolve, 3 = last density+traces. */
+int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMAGO_HIP_H */
