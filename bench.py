@@ -44,7 +44,7 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-terms", type=int, default=16,
+    p.add_argument("--cpu-terms", type=int, default=48,
                    help="EP terms the CPU baseline evaluates (sample, extrapolated)")
     p.add_argument("--small", action="store_true", help="tiny workload for smoke runs")
     return p.parse_args()
@@ -136,34 +136,42 @@ def cpu_baseline(system, coeffs, num_states, cpu_terms):
     t["ol_ke_np"] = time.perf_counter() - t0
     nt = min(cpu_terms, pot_dim)
     t0 = time.perf_counter()
-    V = oracle.assemble(system, "ep", 0, nt)
+    V = oracle.assemble(system, "ep", 0, nt, raw=True)
     t["ep_sample"] = time.perf_counter() - t0
     t["ep_full_est"] = t["ep_sample"] * pot_dim / nt
+    Sr = oracle.assemble(system, "ol", raw=True)
+    Kr = oracle.assemble(system, "ke", raw=True)
+    Nr = oracle.assemble(system, "np", raw=True)
     # H(k) build: the reference's readPackedMatrixAccum loop over potDim
-    # terms per k; time nt terms and scale
+    # terms per k (C loop); time nt terms at one k-point and scale
     t0 = time.perf_counter()
-    h = oracle.build_hamiltonian(N[0], K[0], V[:, 0], coeffs[:nt])
+    h = oracle.build_hamiltonian_raw(Nr[0], Kr[0], [V[j, 0] for j in range(nt)],
+                                     coeffs[:nt])
     t["hbuild_sample"] = time.perf_counter() - t0
     t["hbuild_full_est"] = t["hbuild_sample"] * (pot_dim / nt) * nk
     # generalized eigensolve (LAPACK zhegv, one k-point) and scale by nk
-    H = oracle.unpack(h, n)
+    H = oracle.unpack(h[0::2] + 1j * h[1::2], n)
     Sf = oracle.unpack(S[0], n)
     t0 = time.perf_counter()
     w, vec, info = oracle.hegv(H, Sf)
     t["zhegv_one_k"] = time.perf_counter() - t0
     t["zhegv_full_est"] = t["zhegv_one_k"] * nk
-    # density matrix + potDim traces (matrixElementMult over the panel)
+    # density matrix (zher over occupied states) + potDim traces
+    # (matrixElementMult, C loop over the packed panel)
     t0 = time.perf_counter()
     occ = np.zeros(n)
     occ[: electrons(system) // 2] = 2.0 / nk
     rho = (vec * occ) @ vec.conj().T
+    t["rho_one_k"] = time.perf_counter() - t0
     cols = np.concatenate([np.full(c + 1, c) for c in range(n)])
     rows = np.concatenate([np.arange(c + 1) for c in range(n)])
     rp = rho[rows, cols]
+    rpr = np.ascontiguousarray(np.stack([rp.real, rp.imag], -1).ravel())
+    t0 = time.perf_counter()
     for j in range(nt):
-        oracle.matrix_element_mult(V[j, 0], rp, n)
+        oracle.matrix_element_mult_raw(V[j, 0], rpr, n)
     t["traces_sample"] = time.perf_counter() - t0
-    t["traces_full_est"] = t["traces_sample"] * (pot_dim / nt) * nk
+    t["traces_full_est"] = (t["traces_sample"] * (pot_dim + 3) / nt + t["rho_one_k"]) * nk
     total = (t["ol_ke_np"] + t["ep_full_est"] + t["hbuild_full_est"] + t["zhegv_full_est"]
              + t["traces_full_est"])
     if limiter is not None:

@@ -43,6 +43,27 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FBIN = os.path.join(FORTRAN, "_bin")
+FLANG = "/opt/rocm/bin/amdflang"
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the iso_c_binding module and the small Fortran host that
+    links against libimago_hip.so (imago_amd/fortran/_bin/hip_driver)."""
+    os.makedirs(FBIN, exist_ok=True)
+    mod = [FLANG, "-c", "-J" + FBIN, os.path.join(FORTRAN, "imago_hip.f90"), "-o",
+           os.path.join(FBIN, "imago_hip.o")]
+    drv = [FLANG, "-J" + FBIN, os.path.join(FORTRAN, "hip_driver.f90"),
+           os.path.join(FBIN, "imago_hip.o"), "-L" + HERE, "-limago_hip",
+           "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(FBIN, "hip_driver")]
+    for cmd in (mod, drv):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return os.path.join(FBIN, "hip_driver")
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     jobs = min(len(SOURCES), max(1, (os.cpu_count() or 2)))
@@ -55,6 +76,8 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    if os.path.exists(FLANG):
+        build_fortran(verbose)
     return LIB
 
 

@@ -1,0 +1,227 @@
+! imago_hip.f90 - iso_c_binding interface to libimago_hip.so for the
+! reference's Fortran host (UMKC-CPG/imago, src/imago).
+!
+! The bind(C) interfaces mirror include/imago_hip.h one to one.  The module
+! procedures below them keep the argument lists of the reference routines
+! they replace, and keep the reference's error convention (a message on
+! unit 20, then stop):
+!   solveZHEGV_hip  <- O_LAPACKZHEGV::solveZHEGV  (interfaces.F90:73-120)
+!   solveDSYGV_hip  <- O_LAPACKDSYGV::solveDSYGV  (interfaces.F90:152-200)
+!   hip_check       <- the "stop" after each failing call
+module O_ImagoHIP
+   use iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: IMAGO_OK = 0
+   integer(c_int), parameter :: IMAGO_OL = 1, IMAGO_KE = 2, IMAGO_NP = 3, &
+         & IMAGO_EP = 4
+
+   ! imago_system_desc (include/imago_hip.h), field for field
+   type, bind(C) :: imago_system_desc
+      integer(c_int) :: num_atom_types, max_alphas, max_states
+      type(c_ptr) :: type_element, type_num_orb_alphas, type_alphas
+      type(c_ptr) :: type_num_states, type_basis, type_lm_index
+      integer(c_int) :: num_atoms
+      type(c_ptr) :: atom_type, atom_pos
+      integer(c_int) :: num_pot_types, max_pot_alphas
+      type(c_ptr) :: pot_type_element, pot_type_num_alphas, pot_type_alphas
+      type(c_ptr) :: pot_type_nuc_charge, pot_type_nuc_alpha
+      type(c_ptr) :: pot_type_first_site, pot_type_multiplicity
+      integer(c_int) :: num_pot_sites
+      type(c_ptr) :: pot_site_type, pot_site_pos
+      real(c_double) :: real_vectors(9)
+      integer(c_int) :: num_cells
+      type(c_ptr) :: cell_dims, cell_sizes
+      integer(c_int) :: num_kpoints
+      type(c_ptr) :: kpoints, kpoint_weights
+      integer(c_int) :: gamma
+      real(c_double) :: log_basis_fn_thresh
+   end type imago_system_desc
+
+   type(c_ptr), save :: imagoCtx = c_null_ptr
+
+   interface
+      integer(c_int) function imago_ctx_create(device, ctx) &
+            & bind(C, name="imago_ctx_create")
+         import :: c_int, c_ptr
+         integer(c_int), value :: device
+         type(c_ptr) :: ctx
+      end function imago_ctx_create
+
+      integer(c_int) function imago_ctx_destroy(ctx) &
+            & bind(C, name="imago_ctx_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function imago_ctx_destroy
+
+      type(c_ptr) function imago_last_error() bind(C, name="imago_last_error")
+         import :: c_ptr
+      end function imago_last_error
+
+      integer(c_int) function imago_set_system(ctx, sys, valeDim, potDim) &
+            & bind(C, name="imago_set_system")
+         import :: c_int, c_ptr, imago_system_desc
+         type(c_ptr), value :: ctx
+         type(imago_system_desc), intent(in) :: sys
+         integer(c_int), intent(out) :: valeDim, potDim
+      end function imago_set_system
+
+      integer(c_int) function imago_primitive_batch(ctx, kind, n, a, xyz, &
+            & l1l2switch, sh) bind(C, name="imago_primitive_batch")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: kind, n
+         real(c_double), intent(in) :: a(*), xyz(*)
+         integer(c_int), intent(in) :: l1l2switch(*)
+         real(c_double), intent(out) :: sh(*)
+      end function imago_primitive_batch
+
+      integer(c_int) function imago_boys_batch(ctx, n, T, F7) &
+            & bind(C, name="imago_boys_batch")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n
+         real(c_double), intent(in) :: T(*)
+         real(c_double), intent(out) :: F7(*)
+      end function imago_boys_batch
+
+      integer(c_int) function imago_assemble(ctx, kind) &
+            & bind(C, name="imago_assemble")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: kind
+      end function imago_assemble
+
+      integer(c_int) function imago_get_packed(ctx, kind, term, kpoint, out) &
+            & bind(C, name="imago_get_packed")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: kind, term, kpoint
+         real(c_double), intent(out) :: out(*)
+      end function imago_get_packed
+
+      integer(c_int) function imago_secular(ctx, potCoeffs, numStates, &
+            & eigenValues, info) bind(C, name="imago_secular")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: potCoeffs(*)
+         integer(c_int), value :: numStates
+         real(c_double), intent(out) :: eigenValues(*)
+         integer(c_int), intent(out) :: info(*)
+      end function imago_secular
+
+      integer(c_int) function imago_get_eigenvectors(ctx, kpoint, out) &
+            & bind(C, name="imago_get_eigenvectors")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: kpoint
+         real(c_double), intent(out) :: out(*)
+      end function imago_get_eigenvectors
+
+      integer(c_int) function imago_density_traces(ctx, occupations, traces3, &
+            & potRho) bind(C, name="imago_density_traces")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: occupations(*)
+         real(c_double), intent(out) :: traces3(3), potRho(*)
+      end function imago_density_traces
+
+      integer(c_int) function imago_zhegv(ctx, n, A, B, w, info) &
+            & bind(C, name="imago_zhegv")
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n
+         complex(c_double_complex), intent(inout) :: A(n,n), B(n,n)
+         real(c_double), intent(out) :: w(n)
+         integer(c_int), intent(out) :: info
+      end function imago_zhegv
+
+      integer(c_int) function imago_dsygv(ctx, n, A, B, w, info) &
+            & bind(C, name="imago_dsygv")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n
+         real(c_double), intent(inout) :: A(n,n), B(n,n)
+         real(c_double), intent(out) :: w(n)
+         integer(c_int), intent(out) :: info
+      end function imago_dsygv
+
+      integer(c_int) function imago_last_timing(ctx, stage, ms) &
+            & bind(C, name="imago_last_timing")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: stage
+         real(c_double), intent(out) :: ms
+      end function imago_last_timing
+   end interface
+
+contains
+
+   ! Stop with the library's message on unit 20, the reference's convention.
+   subroutine hip_check(status, what)
+      integer(c_int), intent(in) :: status
+      character(len=*), intent(in) :: what
+      character(kind=c_char), pointer :: msg(:)
+      integer :: i
+      if (status == IMAGO_OK) return
+      call c_f_pointer(imago_last_error(), msg, [512])
+      write (20,*) what, ' failed, status=', status
+      do i = 1, 512
+         if (msg(i) == c_null_char) exit
+         write (20,fmt="(a1)",advance="no") msg(i)
+      enddo
+      write (20,*)
+      stop
+   end subroutine hip_check
+
+   subroutine initImagoHIP(device)
+      integer, intent(in) :: device
+      if (c_associated(imagoCtx)) return
+      call hip_check(imago_ctx_create(int(device,c_int), imagoCtx), &
+            & 'imago_ctx_create')
+   end subroutine initImagoHIP
+
+   ! Same arguments as solveZHEGV (interfaces.F90:73-120).
+   subroutine solveZHEGV_hip(N, resultDim, A, B, eigenValues)
+      integer, intent(in) :: N, resultDim
+      complex(c_double_complex), dimension(N,N), intent(inout) :: A
+      complex(c_double_complex), dimension(N,N), intent(inout) :: B
+      real(c_double), dimension(resultDim), intent(out) :: eigenValues
+      real(c_double), allocatable :: w(:)
+      integer(c_int) :: info, status
+      call initImagoHIP(0)
+      allocate (w(N))
+      status = imago_zhegv(imagoCtx, int(N,c_int), A, B, w, info)
+      if (info /= 0) then
+         write (20,*) 'ZHEGV error code=', info
+         stop
+      endif
+      call hip_check(status, 'imago_zhegv')
+      eigenValues(:resultDim) = w(:resultDim)
+      deallocate (w)
+   end subroutine solveZHEGV_hip
+
+   ! Same arguments as solveDSYGV (interfaces.F90:152-200).
+   subroutine solveDSYGV_hip(valeDim, numStates, valeValeGamma, &
+         & valeValeOLGamma, eigenValues)
+      integer, intent(in) :: valeDim, numStates
+      real(c_double), dimension(valeDim,valeDim), intent(inout) :: valeValeGamma
+      real(c_double), dimension(valeDim,valeDim), intent(inout) :: &
+            & valeValeOLGamma
+      real(c_double), dimension(valeDim), intent(out) :: eigenValues
+      real(c_double), allocatable :: w(:)
+      integer(c_int) :: info, status
+      call initImagoHIP(0)
+      allocate (w(valeDim))
+      status = imago_dsygv(imagoCtx, int(valeDim,c_int), valeValeGamma, &
+            & valeValeOLGamma, w, info)
+      if (info /= 0) then
+         write (20,*) 'DSYGV error code=', info
+         stop
+      endif
+      call hip_check(status, 'imago_dsygv')
+      eigenValues(:numStates) = w(:numStates)
+      deallocate (w)
+   end subroutine solveDSYGV_hip
+
+end module O_ImagoHIP
