@@ -47,7 +47,22 @@ def parse():
     p.add_argument("--cpu-terms", type=int, default=48,
                    help="EP terms the CPU baseline evaluates (sample, extrapolated)")
     p.add_argument("--small", action="store_true", help="tiny workload for smoke runs")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run only a plain 4 GiB device-to-device copy kernel (scale factor "
+                        "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return p.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 29   # 2^29 doubles = 4 GiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "launches": 3}), flush=True)
 
 
 def electrons(system):
@@ -182,6 +197,9 @@ def cpu_baseline(system, coeffs, num_states, cpu_terms):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(it would shard the integrals by atom pair / potential term and the "
@@ -196,14 +214,19 @@ def main():
     coeffs = configs.pot_coefficients(system, seed=7)
     num_states = system.vale_dim
     ctx = Context(0)
-    for _ in range(args.warmup):
-        gpu_step(ctx, system, coeffs, num_states, populate)
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    log("system %s n=%d nk=%d potDim=%d" % (system.name, system.vale_dim, system.num_kpoints,
+                                          system.pot_dim))
+    for i in range(args.warmup):
+        dt, st, _ = gpu_step(ctx, system, coeffs, num_states, populate)
+        log("warmup %d: %.3f s %s" % (i, dt, {k: round(v, 2) for k, v in st.items()}))
     times, stages = [], []
     t_all = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         dt, st, tr = gpu_step(ctx, system, coeffs, num_states, populate)
         times.append(dt)
         stages.append(st)
+        log("step %d: %.3f s" % (i, dt))
     wall = time.perf_counter() - t_all
     step_s = wall / args.steps
     st = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
@@ -228,6 +251,7 @@ def main():
                 "traffic": None}
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline leg (%d EP terms)" % args.cpu_terms)
         total, parts, nt = cpu_baseline(system, coeffs, num_states, args.cpu_terms)
         cpu = {"value": round(total, 3), "unit": "s per step (extrapolated)", "cores": 1,
                "kind": "port",

@@ -66,6 +66,8 @@ struct DevSys {
     const int *pot_el, *pot_first, *pot_mult;
     const int *term_type, *term_alpha;                   // per EP term
     const double *cell_dims;      // [cell][3]
+    const int *pot_nalpha;        // per pot type
+    __device__ __forceinline__ int pot_nalpha_of(int t) const { return pot_nalpha[t]; }
 };
 
 __device__ __forceinline__ int lma_to_l(int lma)

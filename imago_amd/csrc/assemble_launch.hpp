@@ -3,6 +3,7 @@
 // template instances compile in parallel.
 #pragma once
 #include "assemble.hpp"
+#include "ep_kernel.hpp"
 
 namespace imago {
 
@@ -21,6 +22,8 @@ struct AssembleLaunch {
     hipStream_t stream;
     int lmax, ne;
     bool gamma;
+    const int *type_term0;  // EP: first term index of each potential type
+    int num_pot_types;
 };
 
 hipError_t launch_assemble_ol(const AssembleLaunch &L);
@@ -35,6 +38,32 @@ hipError_t launch_one(const AssembleLaunch &L)
                        L.stream, L.S, L.pairs, L.imgs, L.phase, L.cands, L.cand_off, L.term0,
                        L.out, L.term_stride);
     return hipGetLastError();
+}
+
+template <int LMAX, int NE, bool G>
+hipError_t launch_ep_one(const AssembleLaunch &L)
+{
+    hipLaunchKernelGGL((k_assemble_ep<LMAX, NE, G>), dim3(L.npairs, L.num_pot_types), dim3(64),
+                       L.lds, L.stream, L.S, L.pairs, L.imgs, L.phase, L.cands, L.cand_off,
+                       L.type_term0, L.out, L.term_stride);
+    return hipGetLastError();
+}
+
+inline hipError_t launch_ep(const AssembleLaunch &L)
+{
+#define IMAGO_EP_CASE(LM)                                                        \
+    case LM:                                                                     \
+        if (L.ne == 4) return L.gamma ? launch_ep_one<LM, 4, true>(L)            \
+                                      : launch_ep_one<LM, 4, false>(L);          \
+        return L.gamma ? launch_ep_one<LM, 16, true>(L) : launch_ep_one<LM, 16, false>(L);
+    switch (L.lmax) {
+        IMAGO_EP_CASE(0)
+        IMAGO_EP_CASE(1)
+        IMAGO_EP_CASE(2)
+        IMAGO_EP_CASE(3)
+    default: return hipErrorInvalidValue;
+    }
+#undef IMAGO_EP_CASE
 }
 
 template <int KIND>

@@ -62,7 +62,7 @@ struct HostSys {
     std::vector<int> el_nalpha, el_npot;
     std::vector<double> el_alpha, el_pot_alpha, el_nuc_alpha;
     std::vector<double> adist, acenter, andist, apdist;
-    std::vector<int> term_type, term_alpha;
+    std::vector<int> term_type, term_alpha, type_term0;
     int n = 0, pot_dim = 0;
 };
 
@@ -93,6 +93,7 @@ struct imago_ctx {
     int *cand_off = nullptr;
     int npairs = 0;
     long nimgs = 0, ncands = 0;
+    const int *type_term0 = nullptr;
     int lmax = 0, ne = 0;
     // panels: 0 OL, 1 KE, 2 NP, 3 EP
     double *panel[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -123,6 +124,7 @@ static void ctx_free_system(imago_ctx *c)
         c->assembled[k] = false;
     }
     c->pairs = nullptr;
+    c->type_term0 = nullptr;
     c->imgs = nullptr;
     c->phase = nullptr;
     c->cands = nullptr;
@@ -280,6 +282,15 @@ static int build_host(HostSys &h, const imago_system_desc *s)
             if (h.site_type[s2] != t)
                 return fail(IMAGO_E_ARG, "potential type %d: sites not contiguous", t);
     }
+    // potential coefficients are read in type order (initPotCoeffs,
+    // potential.f90:456-470) while the terms are built in the order of each
+    // type's first site (integrals.F90:2411-2476): the two agree only when
+    // types are numbered by first site, as the reference's input writer does
+    for (int t = 1; t < h.npt; ++t)
+        if (h.pot_first[t] <= h.pot_first[t - 1])
+            return fail(IMAGO_E_ARG, "potential types must be numbered in the order of their "
+                                     "first site (type %d starts at site %d)", t + 1,
+                        h.pot_first[t] + 1);
     for (int k = 1; k < h.ncells; ++k)
         if (h.cell_sizes[k] < h.cell_sizes[k - 1])
             return fail(IMAGO_E_ARG, "cell list not sorted by |R|^2");
@@ -391,11 +402,14 @@ static int build_host(HostSys &h, const imago_system_desc *s)
     for (int t = 0; t < h.npt; ++t) order[t] = t;
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return h.pot_first[a] < h.pot_first[b]; });
-    for (int t : order)
+    h.type_term0.assign(h.npt, 0);
+    for (int t : order) {
+        h.type_term0[t] = (int)h.term_type.size();
         for (int a = 0; a < h.pot_nalpha[t]; ++a) {
             h.term_type.push_back(t);
             h.term_alpha.push_back(a);
         }
+    }
     h.pot_dim = (int)h.term_type.size();
     return IMAGO_OK;
 }
@@ -645,6 +659,12 @@ extern "C" int imago_set_system(imago_ctx *c, const imago_system_desc *s, int *v
     RCHK(dev_upload(c, &d.term_type, h.term_type));
     RCHK(dev_upload(c, &d.term_alpha, h.term_alpha));
     RCHK(dev_upload(c, &d.cell_dims, h.cell_dims));
+    RCHK(dev_upload(c, &d.pot_nalpha, h.pot_nalpha));
+    {
+        const int *tt0 = nullptr;
+        RCHK(dev_upload(c, &tt0, h.type_term0));
+        c->type_term0 = tt0;
+    }
     RCHK(build_geometry(c));
     int lmax = 0;
     for (int t = 0; t < h.nt; ++t)
@@ -664,13 +684,14 @@ static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride
                            int nterms)
 {
     const HostSys &h = c->h;
-    const size_t lds = assemble_lds_bytes(c->lmax, h.maxa, h.maxs, h.nk);
+    const size_t lds = kind == IMAGO_EP ? ep_lds_bytes(c->lmax, h.maxa, h.maxs, h.nk)
+                                        : assemble_lds_bytes(c->lmax, h.maxa, h.maxs, h.nk);
     if (lds > 64 * 1024)
         return fail(IMAGO_E_ARG, "LDS requirement %zu bytes exceeds 64 KiB (max_states %d, nk %d)",
                     lds, h.maxs, h.nk);
     AssembleLaunch L{c->d, c->pairs, c->imgs, c->phase, c->cands, c->cand_off, term0, out,
                      term_stride, c->npairs, nterms, lds, c->stream, c->lmax,
-                     c->ne <= 4 ? 4 : 16, h.gamma != 0};
+                     c->ne <= 4 ? 4 : 16, h.gamma != 0, c->type_term0, h.npt};
     hipError_t e;
     switch (kind) {
     case IMAGO_OL: e = launch_assemble_ol(L); break;
@@ -701,10 +722,10 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
     const int nterms = kind == IMAGO_EP ? c->h.pot_dim : 1;
     if (!c->panel[slot]) RCHK(dev_alloc(c, &c->panel[slot], (size_t)per * nterms));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    // EP: launch in slices of terms so a single grid stays within limits
-    const int slice = 4096;
-    for (int t0 = 0; t0 < nterms; t0 += slice)
-        RCHK(launch_assemble(c, kind, c->panel[slot], per, t0, std::min(slice, nterms - t0)));
+    if (kind == IMAGO_EP)
+        RCHK(launch_assemble(c, kind, c->panel[slot], per, 0, nterms));
+    else
+        RCHK(launch_assemble(c, kind, c->panel[slot], per, 0, 1));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms = 0;

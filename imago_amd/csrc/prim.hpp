@@ -16,6 +16,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Primitive evaluators compile for the host as well, so a CPU build can be
+// checked against the oracle without a GPU.
+#define IMAGO_HD __host__ __device__ __forceinline__
+
 namespace imago {
 
 constexpr double kPi = 3.141592653589793;
@@ -62,7 +66,7 @@ __host__ __device__ constexpr double sh_w(int lm, int t)
 // below T <= 10E-2 (a single-precision literal) the 7-term series with
 // single-precision Gamma(n+1/2) constants, above it the closed forms in
 // erf(sqrt T) and exp(-T).
-__device__ __forceinline__ double boys_gamma_half(int k)
+IMAGO_HD double boys_gamma_half(int k)
 {
     // Gamma(k + 1/2), k = 0..13, rounded to single precision as the
     // reference's default-real literals are.
@@ -85,7 +89,7 @@ __device__ __forceinline__ double boys_gamma_half(int k)
 }
 
 template <int M>
-__device__ __forceinline__ void boys_ref(double T, double (&F)[M + 1])
+IMAGO_HD void boys_ref(double T, double (&F)[M + 1])
 {
 #pragma clang fp contract(off)
     const double exp_T = exp(-T);
@@ -140,7 +144,7 @@ __device__ __forceinline__ void boys_ref(double T, double (&F)[M + 1])
 //   s(i+1,j) = PA s(i,j) + (i s(i-1,j) + j s(i,j-1)) / (2p)
 //   s(i,j+1) = PB s(i,j) + (i s(i-1,j) + j s(i,j-1)) / (2p)
 template <int LA, int LB>
-__device__ __forceinline__ void os1d(double i2p, double PA, double PB,
+IMAGO_HD void os1d(double i2p, double PA, double PB,
                                      double (&s)[LA + 1][LB + 1])
 {
     s[0][0] = 1.0;
@@ -162,7 +166,7 @@ __device__ __forceinline__ void os1d(double i2p, double PA, double PB,
 // Accumulates sh(lm1,lm2) += scale * pref * sum combos over Cartesian values
 // given by a functor cart(ca, cb).
 template <int LA, int LB, class Cart, class Acc>
-__device__ __forceinline__ void combine_sh(double f, const Cart &cart, Acc &acc)
+IMAGO_HD void combine_sh(double f, const Cart &cart, Acc &acc)
 {
     constexpr int NA = (LA + 1) * (LA + 1), NB = (LB + 1) * (LB + 1);
 #pragma unroll
@@ -190,7 +194,7 @@ struct Vec3 {
 
 // overlap2CIntg: <a|b>, prefactor (pi/zeta)^1.5 exp(-xi |A-B|^2)
 template <int LA, int LB, class Acc>
-__device__ __forceinline__ void prim_overlap(double a1, double a2, Vec3 A, Vec3 B,
+IMAGO_HD void prim_overlap(double a1, double a2, Vec3 A, Vec3 B,
                                              double scale, Acc &acc)
 {
     const double zeta = a1 + a2, xi = a1 * a2 / zeta, i2p = 0.5 / zeta;
@@ -210,11 +214,46 @@ __device__ __forceinline__ void prim_overlap(double a1, double a2, Vec3 A, Vec3 
     combine_sh<LA, LB>(scale * pref, cart, acc);
 }
 
-// kinetic2CIntg: <a| -1/2 nabla^2 |b>; per direction
-//   t(i,j) = -1/2 [ j(j-1) s(i,j-2) - 2 a2 (2j+1) s(i,j) + 4 a2^2 s(i,j+2) ]
+// kinetic2CIntg: <a| -1/2 nabla^2 |b> as T = tx sy sz + sx ty sz + sx sy tz
+// with the 1-D Obara-Saika kinetic recursion per direction (relative to the
+// s-s overlap, X = A - B):
+//   t(0,0)   = xi - 2 xi^2 X^2
+//   t(i+1,j) = PA t(i,j) + (i t(i-1,j) + j t(i,j-1))/(2p)
+//              + 2 xi (s(i+1,j) - i/(2 a1) s(i-1,j))
+//   t(i,j+1) = PB t(i,j) + (i t(i-1,j) + j t(i,j-1))/(2p)
+//              + 2 xi (s(i,j+1) - j/(2 a2) s(i,j-1))
+// (stable for very different exponents, unlike derivative formulas)
+template <int LA, int LB>
+IMAGO_HD void os1d_kinetic(double i2p, double xi, double a1, double a2, double X, double PA,
+                           double PB, const double (&s)[LA + 1][LB + 1],
+                           double (&t)[LA + 1][LB + 1])
+{
+    t[0][0] = xi - 2.0 * xi * xi * X * X;
+#pragma unroll
+    for (int i = 1; i <= LA; ++i) {
+        double v = PA * t[i - 1][0] + 2.0 * xi * s[i][0];
+        if (i > 1) {
+            v += (i - 1) * i2p * t[i > 1 ? i - 2 : 0][0];
+            v -= 2.0 * xi * ((i - 1) / (2.0 * a1)) * s[i > 1 ? i - 2 : 0][0];
+        }
+        t[i][0] = v;
+    }
+#pragma unroll
+    for (int j = 1; j <= LB; ++j)
+#pragma unroll
+        for (int i = 0; i <= LA; ++i) {
+            double v = PB * t[i][j - 1] + 2.0 * xi * s[i][j];
+            if (i > 0) v += i * i2p * t[i > 0 ? i - 1 : 0][j - 1];
+            if (j > 1) {
+                v += (j - 1) * i2p * t[i][j > 1 ? j - 2 : 0];
+                v -= 2.0 * xi * ((j - 1) / (2.0 * a2)) * s[i][j > 1 ? j - 2 : 0];
+            }
+            t[i][j] = v;
+        }
+}
+
 template <int LA, int LB, class Acc>
-__device__ __forceinline__ void prim_kinetic(double a1, double a2, Vec3 A, Vec3 B,
-                                             double scale, Acc &acc)
+IMAGO_HD void prim_kinetic(double a1, double a2, Vec3 A, Vec3 B, double scale, Acc &acc)
 {
     const double zeta = a1 + a2, xi = a1 * a2 / zeta, i2p = 0.5 / zeta;
     const double Px = (a1 * A.x + a2 * B.x) / zeta, Py = (a1 * A.y + a2 * B.y) / zeta,
@@ -222,28 +261,14 @@ __device__ __forceinline__ void prim_kinetic(double a1, double a2, Vec3 A, Vec3 
     const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
     const double d2 = dx * dx + dy * dy + dz * dz;
     const double pref = (kPi / zeta) * sqrt(kPi / zeta) * exp(-xi * d2);
-    double sx[LA + 1][LB + 3], sy[LA + 1][LB + 3], sz[LA + 1][LB + 3];
-    os1d<LA, LB + 2>(i2p, Px - A.x, Px - B.x, sx);
-    os1d<LA, LB + 2>(i2p, Py - A.y, Py - B.y, sy);
-    os1d<LA, LB + 2>(i2p, Pz - A.z, Pz - B.z, sz);
+    double sx[LA + 1][LB + 1], sy[LA + 1][LB + 1], sz[LA + 1][LB + 1];
+    os1d<LA, LB>(i2p, Px - A.x, Px - B.x, sx);
+    os1d<LA, LB>(i2p, Py - A.y, Py - B.y, sy);
+    os1d<LA, LB>(i2p, Pz - A.z, Pz - B.z, sz);
     double tx[LA + 1][LB + 1], ty[LA + 1][LB + 1], tz[LA + 1][LB + 1];
-#pragma unroll
-    for (int i = 0; i <= LA; ++i)
-#pragma unroll
-        for (int j = 0; j <= LB; ++j) {
-            const double c0 = 2.0 * a2 * (2 * j + 1), c2 = 4.0 * a2 * a2;
-            double vx = c0 * sx[i][j] - c2 * sx[i][j + 2];
-            double vy = c0 * sy[i][j] - c2 * sy[i][j + 2];
-            double vz = c0 * sz[i][j] - c2 * sz[i][j + 2];
-            if (j > 1) {
-                vx -= j * (j - 1) * sx[i][j > 1 ? j - 2 : 0];
-                vy -= j * (j - 1) * sy[i][j > 1 ? j - 2 : 0];
-                vz -= j * (j - 1) * sz[i][j > 1 ? j - 2 : 0];
-            }
-            tx[i][j] = 0.5 * vx;
-            ty[i][j] = 0.5 * vy;
-            tz[i][j] = 0.5 * vz;
-        }
+    os1d_kinetic<LA, LB>(i2p, xi, a1, a2, dx, Px - A.x, Px - B.x, sx, tx);
+    os1d_kinetic<LA, LB>(i2p, xi, a1, a2, dy, Py - A.y, Py - B.y, sy, ty);
+    os1d_kinetic<LA, LB>(i2p, xi, a1, a2, dz, Pz - A.z, Pz - B.z, sz, tz);
     auto cart = [&](int ca, int cb) {
         const int ax = cart_e(ca, 0), ay = cart_e(ca, 1), az = cart_e(ca, 2);
         const int bx = cart_e(cb, 0), by = cart_e(cb, 1), bz = cart_e(cb, 2);
@@ -257,7 +282,7 @@ __device__ __forceinline__ void prim_kinetic(double a1, double a2, Vec3 A, Vec3 
 // G = (a1 A + a2 B + a3 C)/zeta3; prefactor
 // (pi/zeta3)^1.5 exp(-xi |A-B|^2 - (zeta a3/zeta3) |P-C|^2).
 template <int LA, int LB, class Acc>
-__device__ __forceinline__ void prim_elec3c(double a1, double a2, double a3, Vec3 A,
+IMAGO_HD void prim_elec3c(double a1, double a2, double a3, Vec3 A,
                                             Vec3 B, Vec3 C, double scale, Acc &acc)
 {
     const double zeta = a1 + a2, z3 = zeta + a3, xi = a1 * a2 / zeta, i2p = 0.5 / z3;
@@ -287,7 +312,7 @@ __device__ __forceinline__ void prim_elec3c(double a1, double a2, double a3, Vec
 //   E^{i+1,j}_t = E^{ij}_{t-1}/(2p) + GA E^{ij}_t + (t+1) E^{ij}_{t+1}
 //   E^{i,j+1}_t = E^{ij}_{t-1}/(2p) + GB E^{ij}_t + (t+1) E^{ij}_{t+1}
 template <int LA, int LB>
-__device__ __forceinline__ void hermite_e(double i2p, double GA, double GB,
+IMAGO_HD void hermite_e(double i2p, double GA, double GB,
                                           double (&E)[LA + 1][LB + 1][LA + LB + 1])
 {
 #pragma unroll
@@ -319,80 +344,78 @@ __device__ __forceinline__ void hermite_e(double i2p, double GA, double GB,
             }
 }
 
-// Hermite Coulomb integrals R^0_{tuv}, t+u+v <= L:
+// Hermite Coulomb integrals R^0_{tuv}, t+u+v <= L, stored compactly by
+// total order s = t+u+v:
+//   idx(t,u,v) = s(s+1)(s+2)/6 + (s-t)(s-t+1)/2 + v
+// Recursion (McMurchie-Davidson):
 //   R^n_{000} = (-2p)^n F_n(T);  R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X R^{n+1}_{tuv}
+__host__ __device__ constexpr int tuv_index(int t, int u, int v)
+{
+    return (t + u + v) * (t + u + v + 1) * (t + u + v + 2) / 6 +
+           (u + v) * (u + v + 1) / 2 + v;
+}
+__host__ __device__ constexpr int tuv_count(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+
 template <int L>
 struct HermiteR {
-    double r[L + 1][L + 1][L + 1];
+    double r[tuv_count(L)];
+    IMAGO_HD double operator()(int t, int u, int v) const { return r[tuv_index(t, u, v)]; }
 };
 
 template <int L>
-__device__ __forceinline__ void hermite_r(double p, double X, double Y, double Z,
-                                          const double (&F)[L + 1], HermiteR<L> &out)
+IMAGO_HD void hermite_r(double p, double X, double Y, double Z, const double (&F)[L + 1],
+                        HermiteR<L> &out)
 {
-    // level-by-level from n = L down to 0; cur holds R^{n}
-    double cur[L + 1][L + 1][L + 1];
-    double nxt[L + 1][L + 1][L + 1];
+    constexpr int NC = tuv_count(L);
+    double nxt[NC], cur[NC];
 #pragma unroll
-    for (int t = 0; t <= L; ++t)
-#pragma unroll
-        for (int u = 0; u <= L; ++u)
-#pragma unroll
-            for (int v = 0; v <= L; ++v) nxt[t][u][v] = 0.0;
-    double m2p = -2.0 * p;
+    for (int q = 0; q < NC; ++q) nxt[q] = cur[q] = 0.0;
+    const double m2p = -2.0 * p;
     double pw[L + 1];
     pw[0] = 1.0;
 #pragma unroll
     for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * m2p;
-    nxt[0][0][0] = pw[L] * F[L];
+    nxt[0] = pw[L] * F[L];
 #pragma unroll
     for (int n = L - 1; n >= 0; --n) {
-        const int lev = L - n;  // total order reachable at this n
+        const int lev = L - n;
 #pragma unroll
-        for (int t = 0; t <= L; ++t)
+        for (int s = 0; s <= L; ++s) {
+            if (s > lev) continue;
 #pragma unroll
-            for (int u = 0; u <= L; ++u)
+            for (int t = s; t >= 0; --t) {
 #pragma unroll
-                for (int v = 0; v <= L; ++v) {
-                    if (t + u + v > lev) {
-                        cur[t][u][v] = 0.0;
-                        continue;
-                    }
+                for (int v = 0; v <= s - t; ++v) {
+                    const int u = s - t - v;
                     double val;
-                    if (t + u + v == 0) {
+                    if (s == 0) {
                         val = pw[n] * F[n];
                     } else if (t > 0) {
-                        val = X * nxt[t - 1][u][v];
-                        if (t > 1) val += (t - 1) * nxt[t > 1 ? t - 2 : 0][u][v];
+                        val = X * nxt[tuv_index(t - 1, u, v)];
+                        if (t > 1) val += (t - 1) * nxt[tuv_index(t > 1 ? t - 2 : 0, u, v)];
                     } else if (u > 0) {
-                        val = Y * nxt[t][u - 1][v];
-                        if (u > 1) val += (u - 1) * nxt[t][u > 1 ? u - 2 : 0][v];
+                        val = Y * nxt[tuv_index(0, u - 1, v)];
+                        if (u > 1) val += (u - 1) * nxt[tuv_index(0, u > 1 ? u - 2 : 0, v)];
                     } else {
-                        val = Z * nxt[t][u][v - 1];
-                        if (v > 1) val += (v - 1) * nxt[t][u][v > 1 ? v - 2 : 0];
+                        val = Z * nxt[tuv_index(0, 0, v - 1)];
+                        if (v > 1) val += (v - 1) * nxt[tuv_index(0, 0, v > 1 ? v - 2 : 0)];
                     }
-                    cur[t][u][v] = val;
+                    cur[tuv_index(t, u, v)] = val;
                 }
+            }
+        }
 #pragma unroll
-        for (int t = 0; t <= L; ++t)
-#pragma unroll
-            for (int u = 0; u <= L; ++u)
-#pragma unroll
-                for (int v = 0; v <= L; ++v) nxt[t][u][v] = cur[t][u][v];
+        for (int q = 0; q < NC; ++q) nxt[q] = cur[q];
     }
 #pragma unroll
-    for (int t = 0; t <= L; ++t)
-#pragma unroll
-        for (int u = 0; u <= L; ++u)
-#pragma unroll
-            for (int v = 0; v <= L; ++v) out.r[t][u][v] = nxt[t][u][v];
+    for (int q = 0; q < NC; ++q) out.r[q] = nxt[q];
 }
 
 // nuclear3CIntg: int G_A G_B exp(-a3|r-C|^2) / |r-C|, i.e.
 // 2 pi/zeta3 exp(-(a1a2|A-B|^2 + a1a3|A-C|^2 + a2a3|B-C|^2)/zeta3)
 //   * sum_tuv E_t E_u E_v R_tuv(zeta3, G-C)
 template <int LA, int LB, class Acc>
-__device__ __forceinline__ void prim_nuc3c(double a1, double a2, double a3, Vec3 A,
+IMAGO_HD void prim_nuc3c(double a1, double a2, double a3, Vec3 A,
                                            Vec3 B, Vec3 C, double scale, Acc &acc)
 {
     constexpr int L = LA + LB;
@@ -431,7 +454,7 @@ __device__ __forceinline__ void prim_nuc3c(double a1, double a2, double a3, Vec3
 #pragma unroll
                 for (int v = 0; v <= LA + LB; ++v) {
                     if (v > az + bz || t + u + v > L) continue;
-                    s += exy * ez[az][bz][v] * R.r[t][u][v];
+                    s += exy * ez[az][bz][v] * R(t, u, v);
                 }
             }
         }
@@ -444,7 +467,7 @@ enum PrimKind { K_OL = 1, K_KE = 2, K_NP = 3, K_EP = 4 };
 
 // Dispatch on the runtime (l_a, l_b) class to the template instance.
 template <int KIND, int LMAX, class Acc>
-__device__ __forceinline__ void prim_dispatch(int la, int lb, double a1, double a2,
+IMAGO_HD void prim_dispatch(int la, int lb, double a1, double a2,
                                               double a3, Vec3 A, Vec3 B, Vec3 C,
                                               double scale, Acc &acc)
 {

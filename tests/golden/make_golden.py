@@ -40,6 +40,19 @@ def main():
             A = rng.normal(scale=1.5, size=3)
             B = A + rng.normal(scale=1.0, size=3) / np.sqrt(a[:2].min())
             C = 0.5 * (A + B) + rng.normal(scale=1.0, size=3)
+            # stay inside the region the assembly evaluates (alphaDist and
+            # threeAlphaDist screens at the default -ln(1e-16))
+            L = -np.log(1e-16)
+            xi = a[0] * a[1] / (a[0] + a[1])
+            d2 = np.sum((B - A) ** 2)
+            if xi * d2 > L:
+                B = A + (B - A) * np.sqrt(L / (xi * d2))
+            z = a[0] + a[1]
+            P = (a[0] * A + a[1] * B) / z
+            w = z * a[2] / (z + a[2])
+            c2 = np.sum((C - P) ** 2)
+            if w * c2 > L:
+                C = P + (C - P) * np.sqrt(L / (w * c2))
             for kind in oracle.KINDS:
                 sh = oracle.primitive(kind, a, A, B, C, sw, source="ref")
                 rec[kind]["a"].append(a)
