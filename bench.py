@@ -208,7 +208,7 @@ def main():
     from imago_amd import configs
     from imago_amd.lib import Context
     if args.small:
-        system = configs.si64_supercell(reps=(1, 1, 1), cutoff_bohr=12.0)
+        system = configs.si64_supercell(reps=(1, 1, 1), cutoff_bohr=16.0)
     else:
         system = configs.si64_supercell()
     coeffs = configs.pot_coefficients(system, seed=7)

@@ -2,5 +2,5 @@
 #include "assemble_launch.hpp"
 
 namespace imago {
-hipError_t launch_assemble_ep(const AssembleLaunch &L) { return launch_ep(L); }
+hipError_t launch_assemble_ep(const AssembleLaunch &L) { return launch_ep<>(L); }
 }  // namespace imago

@@ -24,6 +24,7 @@ struct AssembleLaunch {
     bool gamma;
     const int *type_term0;  // EP: first term index of each potential type
     int num_pot_types;
+    const int *order;       // EP: pairs ordered by column atom, then row atom
 };
 
 hipError_t launch_assemble_ol(const AssembleLaunch &L);
@@ -43,13 +44,16 @@ hipError_t launch_one(const AssembleLaunch &L)
 template <int LMAX, int NE, bool G>
 hipError_t launch_ep_one(const AssembleLaunch &L)
 {
-    hipLaunchKernelGGL((k_assemble_ep<LMAX, NE, G>), dim3(L.npairs, L.num_pot_types), dim3(64),
+    const long total = (long)L.npairs * L.num_pot_types;
+    const long per = (total + kNumXcd - 1) / kNumXcd;
+    hipLaunchKernelGGL((k_assemble_ep<LMAX, NE, G>), dim3((unsigned)(per * kNumXcd)), dim3(64),
                        L.lds, L.stream, L.S, L.pairs, L.imgs, L.phase, L.cands, L.cand_off,
-                       L.type_term0, L.out, L.term_stride);
+                       L.type_term0, L.order, L.npairs, L.out, L.term_stride);
     return hipGetLastError();
 }
 
-inline hipError_t launch_ep(const AssembleLaunch &L)
+template <int UNUSED = 0>
+hipError_t launch_ep(const AssembleLaunch &L)
 {
 #define IMAGO_EP_CASE(LM)                                                        \
     case LM:                                                                     \
@@ -69,6 +73,8 @@ inline hipError_t launch_ep(const AssembleLaunch &L)
 template <int KIND>
 hipError_t launch_kind(const AssembleLaunch &L)
 {
+    // nuclear integrals with f shells are refused by imago_assemble, so the
+    // LMAX = 3 instance is not built for them
 #define IMAGO_CASE(LM)                                                          \
     case LM:                                                                    \
         if (L.ne == 4) return L.gamma ? launch_one<KIND, LM, 4, true>(L)        \
@@ -79,7 +85,13 @@ hipError_t launch_kind(const AssembleLaunch &L)
         IMAGO_CASE(0)
         IMAGO_CASE(1)
         IMAGO_CASE(2)
-        IMAGO_CASE(3)
+    case 3:
+        if constexpr (KIND != K_NP) {
+            if (L.ne == 4) return L.gamma ? launch_one<KIND, 3, 4, true>(L)
+                                          : launch_one<KIND, 3, 4, false>(L);
+            return L.gamma ? launch_one<KIND, 3, 16, true>(L) : launch_one<KIND, 3, 16, false>(L);
+        }
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 #undef IMAGO_CASE

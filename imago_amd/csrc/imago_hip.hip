@@ -11,6 +11,8 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -94,10 +96,12 @@ struct imago_ctx {
     int npairs = 0;
     long nimgs = 0, ncands = 0;
     const int *type_term0 = nullptr;
+    const int *pair_order = nullptr;
     int lmax = 0, ne = 0;
     // panels: 0 OL, 1 KE, 2 NP, 3 EP
     double *panel[4] = {nullptr, nullptr, nullptr, nullptr};
     bool assembled[4] = {false, false, false, false};
+    bool chol_valid = false;   // Sfull/Sreal hold the Cholesky factor of S(k)
     // secular buffers
     double2 *Hfull = nullptr, *Sfull = nullptr;
     double *Hreal = nullptr, *Sreal = nullptr;
@@ -123,8 +127,10 @@ static void ctx_free_system(imago_ctx *c)
         c->panel[k] = nullptr;
         c->assembled[k] = false;
     }
+    c->chol_valid = false;
     c->pairs = nullptr;
     c->type_term0 = nullptr;
+    c->pair_order = nullptr;
     c->imgs = nullptr;
     c->phase = nullptr;
     c->cands = nullptr;
@@ -414,18 +420,29 @@ static int build_host(HostSys &h, const imago_system_desc *s)
     return IMAGO_OK;
 }
 
-// Pair/image table (integrals.F90:323-430) and 3-centre candidate images.
-static int build_geometry(imago_ctx *c)
-{
-    HostSys &h = c->h;
-    const int nk = h.nk;
+// Pair/image table (integrals.F90:323-430) and 3-centre candidate images,
+// built for the pairs (i, j >= i) of one atom i; atoms run in parallel and
+// their parts are concatenated in atom order.
+struct GeomPart {
     std::vector<PairRec> pairs;
     std::vector<ImgRec> imgs;
     std::vector<double2> phase;
     std::vector<CandRec> cands;
-    std::vector<int> cand_off;
+    std::vector<int> cand_off;   // relative to this part's cands
+    int rc = IMAGO_OK;
+    std::string err;
+};
+
+static void build_geometry_atom(const HostSys &h, int i, GeomPart &g)
+{
+    const int nk = h.nk;
+    auto &pairs = g.pairs;
+    auto &imgs = g.imgs;
+    auto &phase = g.phase;
+    auto &cands = g.cands;
+    auto &cand_off = g.cand_off;
     const double lastsz = h.cell_sizes[h.ncells - 1];
-    for (int i = 0; i < h.natoms; ++i) {
+    {
         const int t1 = h.atom_type[i], e1 = h.type_el[t1];
         const double *A = &h.atom_pos[3 * i];
         for (int j = i; j < h.natoms; ++j) {
@@ -442,11 +459,15 @@ static int build_geometry(imago_ctx *c)
                 continue;
             }
             const double maxLR = sepsq + negl + 2.0 * std::sqrt(sepsq * negl);
-            if (maxLR > lastsz)
-                return fail(IMAGO_E_LATTICE,
-                            "More lattice points needed for atom pair (%d,%d): requested %g, "
-                            "available %g",
-                            i + 1, j + 1, maxLR, lastsz);
+            if (maxLR > lastsz) {
+                char buf[256];
+                snprintf(buf, sizeof buf,
+                         "More lattice points needed for atom pair (%d,%d): requested %g, "
+                         "available %g", i + 1, j + 1, maxLR, lastsz);
+                g.rc = IMAGO_E_LATTICE;
+                g.err = buf;
+                return;
+            }
             // origin shift e^{ik.L}, skipped when |k.L| < 1e-8
             std::vector<double2> shift(nk, make_double2(1.0, 0.0));
             if (!h.gamma)
@@ -503,11 +524,15 @@ static int build_geometry(imago_ctx *c)
                     find_lattice_vector(h, mv, L2);
                     double cm[3] = {mv[0] - L2[0], mv[1] - L2[1], mv[2] - L2[2]};
                     double rreq = std::sqrt(cm[0] * cm[0] + cm[1] * cm[1] + cm[2] * cm[2]) + half + rb;
-                    if (rreq * rreq > lastsz)
-                        return fail(IMAGO_E_LATTICE,
-                                    "More lattice points needed for potential site %d near atom "
-                                    "pair (%d,%d)",
-                                    s2 + 1, i + 1, j + 1);
+                    if (rreq * rreq > lastsz) {
+                        char buf[256];
+                        snprintf(buf, sizeof buf,
+                                 "More lattice points needed for potential site %d near atom "
+                                 "pair (%d,%d)", s2 + 1, i + 1, j + 1);
+                        g.rc = IMAGO_E_LATTICE;
+                        g.err = buf;
+                        return;
+                    }
                     for (int n = 0; n < h.ncells; ++n) {
                         if (h.cell_sizes[n] > rreq * rreq) break;
                         const double *Rn = &h.cell_dims[3 * n];
@@ -542,7 +567,51 @@ static int build_geometry(imago_ctx *c)
             pairs.push_back(pr);
         }
     }
+}
+
+static int build_geometry(imago_ctx *c)
+{
+    HostSys &h = c->h;
+    std::vector<GeomPart> parts(h.natoms);
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        for (int i = next++; i < h.natoms; i = next++) build_geometry_atom(h, i, parts[i]);
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nth; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+    std::vector<PairRec> pairs;
+    std::vector<ImgRec> imgs;
+    std::vector<double2> phase;
+    std::vector<CandRec> cands;
+    std::vector<int> cand_off;
+    for (auto &g : parts) {
+        if (g.rc != IMAGO_OK) return fail(g.rc, "%s", g.err.c_str());
+        const int ib = (int)imgs.size(), cb = (int)cands.size();
+        for (PairRec pr : g.pairs) {
+            pr.img_off += ib;
+            pairs.push_back(pr);
+        }
+        imgs.insert(imgs.end(), g.imgs.begin(), g.imgs.end());
+        phase.insert(phase.end(), g.phase.begin(), g.phase.end());
+        cands.insert(cands.end(), g.cands.begin(), g.cands.end());
+        for (int o : g.cand_off) cand_off.push_back(o + cb);
+    }
     c->npairs = (int)pairs.size();
+    // pairs ordered by column atom j, then row atom i (the packed layout's
+    // order) for the EP launch
+    {
+        std::vector<int> order(pairs.size());
+        for (size_t q = 0; q < pairs.size(); ++q) order[q] = (int)q;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return pairs[a].j != pairs[b].j ? pairs[a].j < pairs[b].j : pairs[a].i < pairs[b].i;
+        });
+        const int *po = nullptr;
+        RCHK(dev_upload(c, &po, order));
+        c->pair_order = po;
+    }
     c->nimgs = (long)imgs.size();
     c->ncands = (long)cands.size();
     RCHK(dev_alloc(c, &c->pairs, pairs.size()));
@@ -691,7 +760,7 @@ static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride
                     lds, h.maxs, h.nk);
     AssembleLaunch L{c->d, c->pairs, c->imgs, c->phase, c->cands, c->cand_off, term0, out,
                      term_stride, c->npairs, nterms, lds, c->stream, c->lmax,
-                     c->ne <= 4 ? 4 : 16, h.gamma != 0, c->type_term0, h.npt};
+                     c->ne <= 4 ? 4 : 16, h.gamma != 0, c->type_term0, h.npt, c->pair_order};
     hipError_t e;
     switch (kind) {
     case IMAGO_OL: e = launch_assemble_ol(L); break;
@@ -716,6 +785,9 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
 {
     if (!c || !c->has_system) return fail(IMAGO_E_STATE, "imago_assemble before imago_set_system");
     if (kind < IMAGO_OL || kind > IMAGO_EP) return fail(IMAGO_E_ARG, "unknown integral kind %d", kind);
+    if (kind == IMAGO_NP && c->lmax > 2)
+        return fail(IMAGO_E_ARG, "nuclear-potential integrals with f shells (lmAlphaIndex 16) are "
+                                 "not supported by this build");
     HIPCHK(hipSetDevice(c->device));
     const int slot = kind - 1;
     const long per = panel_doubles(c);
@@ -732,6 +804,7 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[0] = ms;
     c->assembled[slot] = true;
+    if (kind == IMAGO_OL) c->chol_valid = false;
     return IMAGO_OK;
 }
 
@@ -835,6 +908,88 @@ static int ensure_secular_buffers(imago_ctx *c)
     return IMAGO_OK;
 }
 
+// Generalized Hermitian/symmetric eigensolve H c = e S c for a batch of
+// problems, the algorithm of LAPACK zhegv/dsygv itype=1 (interfaces.F90:
+// 73-200) with every stage on BLAS3 kernels:
+//   S = U^H U (potrf), C = U^-H H U^-1 (two trsm), C = Y E Y^H (heevd,
+//   divide and conquer), X = U^-1 Y (trsm).
+// factor_gv_batched runs the Cholesky step on B (S on entry, U on exit);
+// solve_gv_factored runs the rest on A (H on entry, X on exit) given U.
+// Matrices are n x n column-major with stride n*n; info[k] follows zhegv:
+// n + i when the leading minor i of S(k) is not positive definite, i when
+// heevd fails.
+static int factor_gv_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int *dinfo, int *hinfo,
+                             bool *bad)
+{
+    const rocblas_stride nn = (rocblas_stride)n * n;
+    if (cx) {
+        BLASCHK(rocsolver_zpotrf_strided_batched(c->blas, rocblas_fill_upper, n,
+                                                 (rocblas_double_complex *)B, n, nn, dinfo, nb));
+    } else {
+        BLASCHK(rocsolver_dpotrf_strided_batched(c->blas, rocblas_fill_upper, n, (double *)B, n, nn,
+                                                 dinfo, nb));
+    }
+    HIPCHK(hipMemcpyAsync(hinfo, dinfo, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *bad = false;
+    for (int k = 0; k < nb; ++k)
+        if (hinfo[k] != 0) {
+            hinfo[k] += n;
+            *bad = true;
+        }
+    return IMAGO_OK;
+}
+
+static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, const void *B,
+                             double *evals, double *ework, int *dinfo, int *hinfo)
+{
+    const rocblas_stride nn = (rocblas_stride)n * n;
+    if (cx) {
+        const rocblas_double_complex one(1.0, 0.0);
+        auto *a = (rocblas_double_complex *)A;
+        auto *b = (const rocblas_double_complex *)B;
+        BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
+                                              rocblas_operation_conjugate_transpose,
+                                              rocblas_diagonal_non_unit, n, n, &one, b, n, nn, a, n,
+                                              nn, nb));
+        BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
+                                              rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
+                                              &one, b, n, nn, a, n, nn, nb));
+        BLASCHK(rocsolver_zheevd_strided_batched(c->blas, rocblas_evect_original, rocblas_fill_upper,
+                                                 n, a, n, nn, evals, n, ework, n, dinfo, nb));
+        BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
+                                              rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
+                                              &one, b, n, nn, a, n, nn, nb));
+    } else {
+        const double one = 1.0;
+        auto *a = (double *)A;
+        auto *b = (const double *)B;
+        BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
+                                              rocblas_operation_transpose, rocblas_diagonal_non_unit,
+                                              n, n, &one, b, n, nn, a, n, nn, nb));
+        BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
+                                              rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
+                                              &one, b, n, nn, a, n, nn, nb));
+        BLASCHK(rocsolver_dsyevd_strided_batched(c->blas, rocblas_evect_original, rocblas_fill_upper,
+                                                 n, a, n, nn, evals, n, ework, n, dinfo, nb));
+        BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
+                                              rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
+                                              &one, b, n, nn, a, n, nn, nb));
+    }
+    HIPCHK(hipMemcpyAsync(hinfo, dinfo, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IMAGO_OK;
+}
+
+static int solve_gv_batched(imago_ctx *c, bool cx, int n, int nb, void *A, void *B,
+                            double *evals, double *ework, int *dinfo, int *hinfo)
+{
+    bool bad = false;
+    RCHK(factor_gv_batched(c, cx, n, nb, B, dinfo, hinfo, &bad));
+    if (bad) return IMAGO_OK;
+    return solve_gv_factored(c, cx, n, nb, A, B, evals, ework, dinfo, hinfo);
+}
+
 extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_states,
                              double *eigenvalues, int *info_out)
 {
@@ -859,32 +1014,38 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     const long nn = (long)h.n * h.n;
     const unsigned ug = (unsigned)((nn * h.nk + 255) / 256);
-    if (h.gamma) {
-        hipLaunchKernelGGL(k_unpack<false>, dim3(ug), dim3(256), 0, c->stream, c->packedH, npack, h.n,
-                           1, c->Hreal);
-        hipLaunchKernelGGL(k_unpack<false>, dim3(ug), dim3(256), 0, c->stream, c->panel[0], npack,
-                           h.n, 1, c->Sreal);
-    } else {
+    const bool cx = !h.gamma;
+    void *Hm = cx ? (void *)c->Hfull : (void *)c->Hreal;
+    void *Sm = cx ? (void *)c->Sfull : (void *)c->Sreal;
+    const int nb = cx ? h.nk : 1;
+    if (cx)
         hipLaunchKernelGGL(k_unpack<true>, dim3(ug), dim3(256), 0, c->stream, c->packedH, npack, h.n,
-                           h.nk, (double *)c->Hfull);
-        hipLaunchKernelGGL(k_unpack<true>, dim3(ug), dim3(256), 0, c->stream, c->panel[0], npack,
-                           h.n, h.nk, (double *)c->Sfull);
-    }
+                           h.nk, (double *)Hm);
+    else
+        hipLaunchKernelGGL(k_unpack<false>, dim3(ug), dim3(256), 0, c->stream, c->packedH, npack, h.n,
+                           1, (double *)Hm);
     HIPCHK(hipGetLastError());
     hipEvent_t e2, e3;
     HIPCHK(hipEventCreate(&e2));
     HIPCHK(hipEventCreate(&e3));
     HIPCHK(hipEventRecord(e2, c->stream));
-    if (h.gamma) {
-        BLASCHK(rocsolver_dsygvd_strided_batched(c->blas, rocblas_eform_ax, rocblas_evect_original,
-                                                 rocblas_fill_upper, h.n, c->Hreal, h.n, nn, c->Sreal,
-                                                 h.n, nn, c->evals, h.n, c->ework, h.n, c->info, 1));
-    } else {
-        BLASCHK(rocsolver_zhegvd_strided_batched(
-            c->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, h.n,
-            (rocblas_double_complex *)c->Hfull, h.n, nn, (rocblas_double_complex *)c->Sfull, h.n, nn,
-            c->evals, h.n, c->ework, h.n, c->info, h.nk));
+    std::vector<int> inf(h.nk, 0);
+    // S(k) does not change between calls on one assembled overlap, so its
+    // Cholesky factor is computed once and reused until OL is reassembled.
+    bool bad = false;
+    if (!c->chol_valid) {
+        if (cx)
+            hipLaunchKernelGGL(k_unpack<true>, dim3(ug), dim3(256), 0, c->stream, c->panel[0], npack,
+                               h.n, h.nk, (double *)Sm);
+        else
+            hipLaunchKernelGGL(k_unpack<false>, dim3(ug), dim3(256), 0, c->stream, c->panel[0],
+                               npack, h.n, 1, (double *)Sm);
+        HIPCHK(hipGetLastError());
+        RCHK(factor_gv_batched(c, cx, h.n, nb, Sm, c->info, inf.data(), &bad));
+        c->chol_valid = !bad;
     }
+    if (!bad)
+        RCHK(solve_gv_factored(c, cx, h.n, nb, Hm, Sm, c->evals, c->ework, c->info, inf.data()));
     HIPCHK(hipEventRecord(e3, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms1 = 0, ms2 = 0;
@@ -894,8 +1055,6 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     (void)hipEventDestroy(e3);
     c->timing[1] = ms1;
     c->timing[2] = ms2;
-    std::vector<int> inf(h.nk);
-    HIPCHK(hipMemcpy(inf.data(), c->info, sizeof(int) * h.nk, hipMemcpyDeviceToHost));
     std::vector<double> ev((size_t)h.nk * h.n);
     HIPCHK(hipMemcpy(ev.data(), c->evals, sizeof(double) * ev.size(), hipMemcpyDeviceToHost));
     if (info_out)
@@ -945,10 +1104,10 @@ extern "C" int imago_get_eigenvectors(imago_ctx *c, int kpoint, double *out)
 // rho = W W^H because every population is >= 0 (populate.F90:190-485).
 template <bool COMPLEX>
 __global__ void k_scale_vectors(const double *__restrict__ C, const double *__restrict__ occ, int n,
-                                int ns, int nk, long ldstride, double *__restrict__ W)
+                                int nocc, int ns, int nk, long ldstride, double *__restrict__ W)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long per = (long)n * ns;
+    const long per = (long)n * nocc;
     if (t >= per * nk) return;
     const int k = (int)(t / per);
     const long q = t % per;
@@ -989,25 +1148,41 @@ __global__ void k_pack_weighted(const double *__restrict__ rho, int n, int nk, l
     }
 }
 
-// partial[term][blk] = sum over a chunk of the term's panel of M . rw
+// partial[term][blk] = sum over a chunk of the term's panel of M . rw for a
+// group of TG terms at once, so each chunk of the weighted density rw is
+// read once per group instead of once per term (the panel is the only
+// stream that has to come from HBM).
+template <int TG>
 __global__ __launch_bounds__(256) void k_trace_partial(const double *__restrict__ M, long len,
-                                                       const double *__restrict__ rw,
+                                                       int nterms, const double *__restrict__ rw,
                                                        int nblk, double *__restrict__ partial)
 {
-    const int term = blockIdx.y, blk = blockIdx.x;
-    const double *m = M + (size_t)term * len;
-    const long chunk = (len + nblk - 1) / nblk;
+    const int blk = blockIdx.x;
+    const int t0 = blockIdx.y * TG;
+    M += (size_t)t0 * len;
+    const long chunk = ((len + nblk - 1) / nblk + 1) & ~1L;
     const long lo = blk * chunk, hi = min(len, lo + chunk);
-    double s = 0.0;
-    for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += m[i] * rw[i];
-    __shared__ double red[256];
-    red[threadIdx.x] = s;
+    double s[TG];
+#pragma unroll
+    for (int t = 0; t < TG; ++t) s[t] = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const double w = rw[i];
+#pragma unroll
+        for (int t = 0; t < TG; ++t)
+            if (t0 + t < nterms) s[t] += M[(size_t)t * len + i] * w;
+    }
+    __shared__ double red[TG][257];
+#pragma unroll
+    for (int t = 0; t < TG; ++t) red[t][threadIdx.x] = s[t];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int t = 0; t < TG; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[(size_t)term * nblk + blk] = red[0];
+    if ((int)threadIdx.x < TG && t0 + (int)threadIdx.x < nterms)
+        partial[(size_t)(t0 + threadIdx.x) * nblk + blk] = red[threadIdx.x][0];
 }
 
 __global__ void k_trace_final(const double *__restrict__ partial, int nblk, int nterms,
@@ -1028,8 +1203,13 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     if (!occupations) return fail(IMAGO_E_ARG, "null occupations");
     const HostSys &h = c->h;
     const int ns = c->num_states;
-    for (long q = 0; q < (long)h.nk * ns; ++q)
+    // states past the last one with a positive occupation at any k-point add
+    // nothing to rho, so the rank-k update stops there
+    int nocc = 0;
+    for (long q = 0; q < (long)h.nk * ns; ++q) {
         if (occupations[q] < 0.0) return fail(IMAGO_E_ARG, "negative occupation at %ld", q);
+        if (occupations[q] > 0.0) nocc = std::max(nocc, (int)(q % ns) + 1);
+    }
     HIPCHK(hipSetDevice(c->device));
     const long nn = (long)h.n * h.n;
     const long npack = (long)h.n * (h.n + 1) / 2;
@@ -1040,28 +1220,34 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     if (!c->rhow) RCHK(dev_alloc(c, &c->rhow, (size_t)per));
     if (!c->occ) RCHK(dev_alloc(c, &c->occ, (size_t)h.nk * h.n));
     double *W = nullptr;
-    RCHK(dev_alloc(c, &W, (size_t)h.n * ns * h.nk * dsz));
+    RCHK(dev_alloc(c, &W, (size_t)h.n * std::max(nocc, 1) * h.nk * dsz));
     HIPCHK(hipMemcpyAsync(c->occ, occupations, sizeof(double) * h.nk * ns, hipMemcpyHostToDevice,
                           c->stream));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    const long tot = (long)h.n * ns * h.nk;
-    if (cx)
-        hipLaunchKernelGGL(k_scale_vectors<true>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                           c->stream, (const double *)c->Hfull, c->occ, h.n, ns, h.nk, nn, W);
-    else
-        hipLaunchKernelGGL(k_scale_vectors<false>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                           c->stream, c->Hreal, c->occ, h.n, ns, h.nk, nn, W);
-    HIPCHK(hipGetLastError());
-    if (cx) {
+    const long tot = (long)h.n * nocc * h.nk;
+    if (tot > 0) {
+        if (cx)
+            hipLaunchKernelGGL(k_scale_vectors<true>, dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                               0, c->stream, (const double *)c->Hfull, c->occ, h.n, nocc, ns, h.nk,
+                               nn, W);
+        else
+            hipLaunchKernelGGL(k_scale_vectors<false>, dim3((unsigned)((tot + 255) / 256)),
+                               dim3(256), 0, c->stream, c->Hreal, c->occ, h.n, nocc, ns, h.nk, nn,
+                               W);
+        HIPCHK(hipGetLastError());
+    }
+    if (nocc == 0) {
+        HIPCHK(hipMemsetAsync(c->rho, 0, sizeof(double) * nn * h.nk * dsz, c->stream));
+    } else if (cx) {
         const double one = 1.0, zero = 0.0;
         BLASCHK(rocblas_zherk_strided_batched(
-            c->blas, rocblas_fill_upper, rocblas_operation_none, h.n, ns, &one,
-            (const rocblas_double_complex *)W, h.n, (rocblas_stride)h.n * ns, &zero,
+            c->blas, rocblas_fill_upper, rocblas_operation_none, h.n, nocc, &one,
+            (const rocblas_double_complex *)W, h.n, (rocblas_stride)h.n * std::max(nocc, 1), &zero,
             (rocblas_double_complex *)c->rho, h.n, (rocblas_stride)nn, h.nk));
     } else {
         const double one = 1.0, zero = 0.0;
-        BLASCHK(rocblas_dsyrk(c->blas, rocblas_fill_upper, rocblas_operation_none, h.n, ns, &one, W,
-                              h.n, &zero, c->rho, h.n));
+        BLASCHK(rocblas_dsyrk(c->blas, rocblas_fill_upper, rocblas_operation_none, h.n, nocc, &one,
+                              W, h.n, &zero, c->rho, h.n));
     }
     const long pt = npack * h.nk;
     if (cx)
@@ -1076,14 +1262,15 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     if (!c->partial) RCHK(dev_alloc(c, &c->partial, (size_t)nterms * nblk));
     double *outd = nullptr;
     RCHK(dev_alloc(c, &outd, (size_t)nterms));
-    // S, NP, KE
+    // S, NP, KE (one term each), then the potential panel in groups of 16
     const int order3[3] = {0, 2, 1};
     for (int q = 0; q < 3; ++q)
-        hipLaunchKernelGGL(k_trace_partial, dim3(nblk, 1), dim3(256), 0, c->stream,
-                           c->panel[order3[q]], per, c->rhow, nblk, c->partial + (size_t)q * nblk);
+        hipLaunchKernelGGL(k_trace_partial<1>, dim3(nblk, 1), dim3(256), 0, c->stream,
+                           c->panel[order3[q]], per, 1, c->rhow, nblk, c->partial + (size_t)q * nblk);
     if (h.pot_dim > 0)
-        hipLaunchKernelGGL(k_trace_partial, dim3(nblk, h.pot_dim), dim3(256), 0, c->stream,
-                           c->panel[3], per, c->rhow, nblk, c->partial + (size_t)3 * nblk);
+        hipLaunchKernelGGL(k_trace_partial<1>, dim3(nblk, h.pot_dim), dim3(256), 0, c->stream,
+                           c->panel[3], per, h.pot_dim, c->rhow, nblk,
+                           c->partial + (size_t)3 * nblk);
     hipLaunchKernelGGL(k_trace_final, dim3((nterms + 255) / 256), dim3(256), 0, c->stream, c->partial,
                        nblk, nterms, outd);
     HIPCHK(hipGetLastError());
@@ -1125,26 +1312,20 @@ static int single_gv(imago_ctx *c, bool cx, int n, double *A, double *B, double 
     HIPCHK(hipMalloc(&di, sizeof(int)));
     HIPCHK(hipMemcpy(dA, A, nn * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dB, B, nn * sizeof(double), hipMemcpyHostToDevice));
-    rocblas_status st;
-    if (cx)
-        st = rocsolver_zhegvd(c->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
-                              (rocblas_double_complex *)dA, n, (rocblas_double_complex *)dB, n, dw, de,
-                              di);
-    else
-        st = rocsolver_dsygvd(c->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
-                              dA, n, dB, n, dw, de, di);
-    if (st != rocblas_status_success) return fail(IMAGO_E_HIP, "rocsolver status %d", (int)st);
-    HIPCHK(hipStreamSynchronize(c->stream));
     int inf = 0;
-    HIPCHK(hipMemcpy(&inf, di, sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(A, dA, nn * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(B, dB, nn * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(w, dw, n * sizeof(double), hipMemcpyDeviceToHost));
+    int rc = solve_gv_batched(c, cx, n, 1, dA, dB, dw, de, di, &inf);
+    if (rc == IMAGO_OK && hipMemcpy(A, dA, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+        hipMemcpy(B, dB, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+        hipMemcpy(w, dw, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
+    } else if (rc == IMAGO_OK) {
+        rc = fail(IMAGO_E_HIP, "copy of eigenproblem results failed");
+    }
     (void)hipFree(dA);
     (void)hipFree(dB);
     (void)hipFree(dw);
     (void)hipFree(de);
     (void)hipFree(di);
+    if (rc != IMAGO_OK) return rc;
     if (info) *info = inf;
     if (inf > n) return fail(IMAGO_E_NOTPD, "%s error code=%d: B not positive definite",
                              cx ? "ZHEGV" : "DSYGV", inf);

@@ -105,3 +105,40 @@ def test_secular_before_assembly_is_a_state_error(gpu_ctx):
     with pytest.raises(ImagoError) as e:
         gpu_ctx.secular(np.zeros(s.pot_dim), 4)
     assert e.value.name == "IMAGO_E_STATE"
+
+
+def test_repeated_secular_reuses_overlap_factor(gpu_ctx, oracle_mod):
+    """Successive secular calls on one assembled overlap (the SCF loop of
+    mainSCF.F90) reuse the Cholesky factor of S(k); each must still match a
+    fresh LAPACK solve of its own H(k), also after OL is reassembled."""
+    s = configs.small_test_system(seed=11)
+    print("[gpu] repeated secular", flush=True)
+    _setup(gpu_ctx, s)
+    n = s.vale_dim
+    S = oracle_mod.assemble(s, "ol")
+    K = oracle_mod.assemble(s, "ke")
+    N = oracle_mod.assemble(s, "np")
+    V = oracle_mod.assemble(s, "ep")
+    for it, seed in enumerate((1, 2, 3)):
+        if it == 2:
+            gpu_ctx.assemble("ol")
+        coeffs = configs.pot_coefficients(s, seed=seed)
+        ev = gpu_ctx.secular(coeffs, n)
+        for k in range(s.num_kpoints):
+            H = oracle_mod.unpack(oracle_mod.build_hamiltonian(N[k], K[k], V[:, k], coeffs), n)
+            w, _, info = oracle_mod.hegv(H, oracle_mod.unpack(S[k], n))
+            assert info == 0
+            assert np.max(np.abs(ev[k] - w)) < 1e-8, (it, k)
+        # only the lowest states occupied: the density update stops there
+        occ = np.zeros((s.num_kpoints, n))
+        occ[:, :3] = 2.0 / s.num_kpoints
+        tr, _ = gpu_ctx.density_traces(occ)
+        ref = 0.0
+        for k in range(s.num_kpoints):
+            X = gpu_ctx.eigenvectors(k, n)
+            rho = (X * occ[k]) @ X.conj().T
+            Sf = oracle_mod.unpack(S[k], n)
+            ref += np.real(np.sum(rho * Sf.T))
+        assert tr[0] == pytest.approx(ref, rel=1e-10)
+        # Tr(rho S) = sum of the occupations: 3 states x 2/nk at nk k-points
+        assert tr[0] == pytest.approx(6.0, rel=1e-9)
