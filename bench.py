@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--cpu-terms", type=int, default=48,
                    help="EP terms the CPU baseline evaluates (sample, extrapolated)")
     p.add_argument("--small", action="store_true", help="tiny workload for smoke runs")
+    p.add_argument("--ep-mode", choices=("direct", "panel"), default="direct",
+                   help="direct: EP terms from the stored primitive blocks each iteration "
+                        "(IMAGO_EP_DIRECT); panel: all potDim EP panels stored (IMAGO_EP)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 4 GiB device-to-device copy kernel (scale factor "
                         "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -70,14 +73,17 @@ def electrons(system):
     return 14 * system.num_atoms
 
 
-def gpu_step(ctx, system, coeffs, num_states, oracle_populate):
+def gpu_step(ctx, system, coeffs, num_states, oracle_populate, ep_kind):
     t0 = time.perf_counter()
     ctx.set_system(system)
     t1 = time.perf_counter()
     stage = {}
-    for kind in ("ol", "ke", "np", "ep"):
+    for kind in ("ol", "ke", "np", ep_kind):
         ctx.assemble(kind)
         stage["assemble_" + kind] = ctx.timing(0)
+        if kind.startswith("ep"):
+            stage["ep_screen"] = ctx.timing(5)
+            stage["ep_store"] = ctx.timing(4)
     t2 = time.perf_counter()
     ev = ctx.secular(coeffs, num_states)
     stage["hbuild"] = ctx.timing(1)
@@ -213,17 +219,18 @@ def main():
         system = configs.si64_supercell()
     coeffs = configs.pot_coefficients(system, seed=7)
     num_states = system.vale_dim
+    ep_kind = "ep_direct" if args.ep_mode == "direct" else "ep"
     ctx = Context(0)
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     log("system %s n=%d nk=%d potDim=%d" % (system.name, system.vale_dim, system.num_kpoints,
                                           system.pot_dim))
     for i in range(args.warmup):
-        dt, st, _ = gpu_step(ctx, system, coeffs, num_states, populate)
+        dt, st, _ = gpu_step(ctx, system, coeffs, num_states, populate, ep_kind)
         log("warmup %d: %.3f s %s" % (i, dt, {k: round(v, 2) for k, v in st.items()}))
     times, stages = [], []
     t_all = time.perf_counter()
     for i in range(args.steps):
-        dt, st, tr = gpu_step(ctx, system, coeffs, num_states, populate)
+        dt, st, tr = gpu_step(ctx, system, coeffs, num_states, populate, ep_kind)
         times.append(dt)
         stages.append(st)
         log("step %d: %.3f s" % (i, dt))
@@ -233,7 +240,7 @@ def main():
     n = system.vale_dim
     npack = n * (n + 1) // 2
     ep_bytes = system.pot_dim * system.num_kpoints * npack * 16.0
-    ep_ms = st["assemble_ep"]
+    ep_ms = st["assemble_" + ep_kind]
     hb_bytes = (system.pot_dim + 3) * system.num_kpoints * npack * 16.0
     eig_flop = 32.0 * n ** 3 * system.num_kpoints
     kernels = {

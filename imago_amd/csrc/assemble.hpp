@@ -32,6 +32,7 @@ namespace imago {
 struct PairRec {
     int i, j;          // atoms, i <= j
     int img_off, img_cnt;
+    int blk;           // first entry of the pair's block in the block layout
 };
 
 struct ImgRec {
@@ -47,8 +48,8 @@ struct CandRec {
 
 struct DevSys {
     int num_types, max_alphas, max_states, num_elements, max_pot_alphas;
-    int num_atoms, num_pot_sites, num_pot_types, num_cells, nk, vale_dim;
-    long npack;
+    int num_atoms, num_pot_sites, num_pot_types, num_cells, nk, vale_dim, pot_dim;
+    long nbe;         // block-layout entries per k-point
     const double *type_alphas;    // [t][a]
     const double *type_basis;     // [t][s][a]
     const int *type_lm;           // [t][s] 1..16
@@ -292,23 +293,24 @@ __global__ __launch_bounds__(64) void k_assemble(DevSys S, const PairRec *__rest
         __syncthreads();
     }
 
-    // store: packed upper triangle, column by column (matrixSubs.F90:506)
-    const int r0 = S.cumul[pr.i], c0g = S.cumul[pr.j];
+    // store: the pair's block, column-major over (row state, column state),
+    // at its offset in the block layout; the strict lower triangle of a
+    // diagonal block is zero (valeValeSaving, intgSaving.F90:433-446)
     double *o = out + (size_t)term * term_stride;
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
         const int ee = lane + 64 * e;
         if (ee >= nent) continue;
-        const long row = r0 + em1[e], col = c0g + em2[e];
-        if (row > col) continue;  // diagonal block: strict lower zeroed
-        const long idx = col * (col + 1) / 2 + row;
+        const bool zero = pr.i == pr.j && em1[e] > em2[e];
+        const long idx = (long)pr.blk + ee;
         if (GAMMA) {
-            o[idx] = Q[2 * ee];
+            o[idx] = zero ? 0.0 : Q[2 * ee];
         } else {
             for (int k = 0; k < nk; ++k) {
-                double2 v = make_double2(Q[2 * ((size_t)k * nent + ee)],
-                                         Q[2 * ((size_t)k * nent + ee) + 1]);
-                reinterpret_cast<double2 *>(o)[(size_t)k * S.npack + idx] = v;
+                double2 v = zero ? make_double2(0.0, 0.0)
+                                 : make_double2(Q[2 * ((size_t)k * nent + ee)],
+                                                Q[2 * ((size_t)k * nent + ee) + 1]);
+                reinterpret_cast<double2 *>(o)[(size_t)k * S.nbe + idx] = v;
             }
         }
     }

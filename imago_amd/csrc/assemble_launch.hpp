@@ -25,12 +25,54 @@ struct AssembleLaunch {
     const int *type_term0;  // EP: first term index of each potential type
     int num_pot_types;
     const int *order;       // EP: pairs ordered by column atom, then row atom
+    EpTuple *ep_tuples;     // EP: screened tuples of every (type, pair)
+    const long *ep_tseg;    // EP: [types * pairs + 1] offsets into ep_tuples
+    int *ep_item_tuple;     // EP: item -> tuple
+    double *ep_blocks;      // EP: primitive block of every item
+    long ep_nitems;
+    int nk;
 };
 
 hipError_t launch_assemble_ol(const AssembleLaunch &L);
 hipError_t launch_assemble_ke(const AssembleLaunch &L);
 hipError_t launch_assemble_np(const AssembleLaunch &L);
 hipError_t launch_assemble_ep(const AssembleLaunch &L);
+hipError_t launch_ep_screen(const AssembleLaunch &L, bool fill, long *tseg, long *iseg,
+                            size_t lds);
+hipError_t launch_ep_prims(const AssembleLaunch &L);
+
+// direct (panel-free) EP contributions, asm_epd.hip; H is summed over groups
+// of this many potential types before the final addition
+constexpr int kEpdTypesPerGroup = 8;
+
+struct EpdLaunch {
+    DevSys S;
+    const PairRec *pairs;
+    const double2 *phase;
+    const EpTuple *tuples;
+    const long *tseg;
+    const double *blocks;
+    const double *coef_eff;   // hsum: per term, 0.0 replaced by 1.0
+    const double *rw;         // trace: weighted density, block layout
+    const int *type_term0;
+    const int *order;
+    int npairs, ngroups;
+    double *partial;          // hsum: [ngroups][group_stride]
+    long group_stride;
+    double *tpart;            // trace: [npairs][pot_dim]
+    int lmax, ne;
+    bool gamma;
+    hipStream_t stream;
+};
+hipError_t launch_epd_hsum(const EpdLaunch &L);
+hipError_t launch_epd_trace(const EpdLaunch &L);
+hipError_t launch_epd_hfinal(const double *np, const double *ke, const double *partial, int ngroups,
+                             long len, double *H, hipStream_t stream);
+hipError_t launch_epd_trace_final(const double *tpart, int npairs, int pot_dim, double *out,
+                                  hipStream_t stream);
+// exclusive prefix sum of n longs (tmp == nullptr: only sets *tmp_bytes)
+hipError_t ep_exclusive_scan(const long *in, long *out, long n, void *tmp, size_t *tmp_bytes,
+                             hipStream_t stream);
 
 template <int KIND, int LMAX, int NE, bool G>
 hipError_t launch_one(const AssembleLaunch &L)
@@ -41,15 +83,24 @@ hipError_t launch_one(const AssembleLaunch &L)
     return hipGetLastError();
 }
 
-template <int LMAX, int NE, bool G>
+template <int LMAX, int NE, int NKR, bool G>
 hipError_t launch_ep_one(const AssembleLaunch &L)
 {
-    const long total = (long)L.npairs * L.num_pot_types;
+    const long total = (long)L.npairs * L.nterms;
     const long per = (total + kNumXcd - 1) / kNumXcd;
-    hipLaunchKernelGGL((k_assemble_ep<LMAX, NE, G>), dim3((unsigned)(per * kNumXcd)), dim3(64),
-                       L.lds, L.stream, L.S, L.pairs, L.imgs, L.phase, L.cands, L.cand_off,
-                       L.type_term0, L.order, L.npairs, L.out, L.term_stride);
+    if (per * kNumXcd >= (1L << 32)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_ep_store<LMAX, NE, NKR, G>), dim3((unsigned)(per * kNumXcd)), dim3(64),
+                       L.lds, L.stream, L.S, L.pairs, L.phase, L.ep_tuples, L.ep_tseg,
+                       L.ep_blocks, L.order, L.npairs, L.out, L.term_stride);
     return hipGetLastError();
+}
+
+template <int LMAX, int NE>
+hipError_t launch_ep_ne(const AssembleLaunch &L)
+{
+    if (L.gamma) return launch_ep_one<LMAX, NE, 1, true>(L);
+    if (L.nk <= 4) return launch_ep_one<LMAX, NE, 4, false>(L);
+    return launch_ep_one<LMAX, NE, 0, false>(L);
 }
 
 template <int UNUSED = 0>
@@ -57,9 +108,7 @@ hipError_t launch_ep(const AssembleLaunch &L)
 {
 #define IMAGO_EP_CASE(LM)                                                        \
     case LM:                                                                     \
-        if (L.ne == 4) return L.gamma ? launch_ep_one<LM, 4, true>(L)            \
-                                      : launch_ep_one<LM, 4, false>(L);          \
-        return L.gamma ? launch_ep_one<LM, 16, true>(L) : launch_ep_one<LM, 16, false>(L);
+        return L.ne == 4 ? launch_ep_ne<LM, 4>(L) : launch_ep_ne<LM, 16>(L);
     switch (L.lmax) {
         IMAGO_EP_CASE(0)
         IMAGO_EP_CASE(1)
@@ -68,6 +117,47 @@ hipError_t launch_ep(const AssembleLaunch &L)
     default: return hipErrorInvalidValue;
     }
 #undef IMAGO_EP_CASE
+}
+
+// Screening pass of the EP terms: count (fill == false) writes the tuple
+// and item counts of every (type, pair) task into tseg / iseg; fill writes
+// the tuples and the item -> tuple map from the offsets in tseg / iseg.
+template <int UNUSED = 0>
+hipError_t launch_ep_screen_pass(const AssembleLaunch &L, bool fill, long *tseg, long *iseg,
+                                 size_t lds)
+{
+    const unsigned tasks = (unsigned)((long)L.npairs * L.num_pot_types);
+    if (fill)
+        hipLaunchKernelGGL(k_ep_screen<true>, dim3(tasks), dim3(64), lds, L.stream, L.S, L.pairs,
+                           L.imgs, L.cands, L.cand_off, L.npairs, tseg, iseg, L.ep_tuples,
+                           L.ep_item_tuple);
+    else
+        hipLaunchKernelGGL(k_ep_screen<false>, dim3(tasks), dim3(64), lds, L.stream, L.S, L.pairs,
+                           L.imgs, L.cands, L.cand_off, L.npairs, tseg, iseg, L.ep_tuples,
+                           L.ep_item_tuple);
+    return hipGetLastError();
+}
+
+// Primitive blocks of every item (k_ep_prims), one lane per item.
+template <int UNUSED = 0>
+hipError_t launch_ep_prims_pass(const AssembleLaunch &L)
+{
+    if (L.ep_nitems == 0) return hipSuccess;
+    const unsigned g = (unsigned)((L.ep_nitems + 63) / 64);
+#define IMAGO_EPP_CASE(LM)                                                                    \
+    case LM:                                                                                  \
+        hipLaunchKernelGGL(k_ep_prims<LM>, dim3(g), dim3(64), 0, L.stream, L.S, L.pairs,       \
+                           L.imgs, L.cands, L.ep_tuples, L.ep_item_tuple, L.ep_nitems,         \
+                           L.npairs, L.ep_blocks);                                            \
+        return hipGetLastError();
+    switch (L.lmax) {
+        IMAGO_EPP_CASE(0)
+        IMAGO_EPP_CASE(1)
+        IMAGO_EPP_CASE(2)
+        IMAGO_EPP_CASE(3)
+    default: return hipErrorInvalidValue;
+    }
+#undef IMAGO_EPP_CASE
 }
 
 template <int KIND>

@@ -97,6 +97,12 @@ struct imago_ctx {
     long nimgs = 0, ncands = 0;
     const int *type_term0 = nullptr;
     const int *pair_order = nullptr;
+    // block layout of the panels: each pair's block (row states x column
+    // states, column-major) back to back; nbe entries per k-point
+    long nbe = 0;
+    std::vector<PairRec> hpairs;
+    const int *atom_of_state = nullptr;   // [n]
+    const int *pair_index = nullptr;      // [natoms][natoms]
     int lmax = 0, ne = 0;
     // panels: 0 OL, 1 KE, 2 NP, 3 EP
     double *panel[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -108,7 +114,7 @@ struct imago_ctx {
     double *evals = nullptr, *ework = nullptr;
     int *info = nullptr;
     double *coef = nullptr;
-    double *packedH = nullptr;
+    double *blockH = nullptr;   // H(k) in the block layout
     int num_states = 0;
     bool have_vectors = false;
     // density / traces
@@ -116,8 +122,22 @@ struct imago_ctx {
     double *rhow = nullptr;     // weighted packed
     double *occ = nullptr;
     double *partial = nullptr;
-    double timing[4] = {0, 0, 0, 0};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // EP screening: tuples of every (potential type, pair) and the primitive
+    // block of every (tuple, exponent) item, built once per system
+    EpTuple *ep_tuples = nullptr;
+    long *ep_tseg = nullptr;     // [types * pairs + 1] offsets into ep_tuples
+    int *ep_item_tuple = nullptr;
+    double *ep_blocks = nullptr;
+    long ep_ntuples = 0, ep_nitems = 0;
+    bool ep_screened = false;
+    // direct EP path (IMAGO_EP_DIRECT): H(k) and the traces from the
+    // primitive blocks each iteration, no potDim panels
+    bool ep_direct = false;
+    double *coef_eff = nullptr;   // [pot_dim]
+    double *hpartial = nullptr;   // [groups][panel]
+    double *tpart = nullptr;      // [pairs][pot_dim]
+    double timing[6] = {0, 0, 0, 0, 0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -128,9 +148,21 @@ static void ctx_free_system(imago_ctx *c)
         c->assembled[k] = false;
     }
     c->chol_valid = false;
+    c->ep_tuples = nullptr;
+    c->ep_tseg = nullptr;
+    c->ep_item_tuple = nullptr;
+    c->ep_blocks = nullptr;
+    c->ep_ntuples = c->ep_nitems = 0;
+    c->ep_screened = false;
+    c->ep_direct = false;
+    c->coef_eff = c->hpartial = c->tpart = nullptr;
     c->pairs = nullptr;
     c->type_term0 = nullptr;
     c->pair_order = nullptr;
+    c->nbe = 0;
+    c->hpairs.clear();
+    c->atom_of_state = nullptr;
+    c->pair_index = nullptr;
     c->imgs = nullptr;
     c->phase = nullptr;
     c->cands = nullptr;
@@ -140,7 +172,7 @@ static void ctx_free_system(imago_ctx *c)
     c->evals = c->ework = nullptr;
     c->info = nullptr;
     c->coef = nullptr;
-    c->packedH = nullptr;
+    c->blockH = nullptr;
     c->rho = c->rhow = c->occ = c->partial = nullptr;
     c->have_vectors = false;
     c->has_system = false;
@@ -282,6 +314,9 @@ static int build_host(HostSys &h, const imago_system_desc *s)
         if (h.site_type[i] < 0 || h.site_type[i] >= h.npt)
             return fail(IMAGO_E_ARG, "potential site %d: bad type", i);
     for (int t = 0; t < h.npt; ++t) {
+        if (h.pot_nalpha[t] <= 0 || h.pot_nalpha[t] > kEpMaxPotAlphas || h.pot_nalpha[t] > h.maxpa)
+            return fail(IMAGO_E_ARG, "potential type %d: %d exponents (1..%d supported)", t,
+                        h.pot_nalpha[t], kEpMaxPotAlphas);
         if (h.pot_first[t] < 0 || h.pot_mult[t] <= 0 || h.pot_first[t] + h.pot_mult[t] > h.nsites)
             return fail(IMAGO_E_ARG, "potential type %d: bad site range", t);
         for (int s2 = h.pot_first[t]; s2 < h.pot_first[t] + h.pot_mult[t]; ++s2)
@@ -563,6 +598,11 @@ static void build_geometry_atom(const HostSys &h, int i, GeomPart &g)
                 }
                 cand_off.push_back((int)cands.size());
                 pr.img_cnt++;
+                if (pr.img_cnt >= 65536) {
+                    g.rc = IMAGO_E_LATTICE;
+                    g.err = "more than 65535 lattice images for one atom pair";
+                    return;
+                }
             }
             pairs.push_back(pr);
         }
@@ -611,7 +651,34 @@ static int build_geometry(imago_ctx *c)
         const int *po = nullptr;
         RCHK(dev_upload(c, &po, order));
         c->pair_order = po;
+        // block layout: the pairs' blocks back to back in the same order, so
+        // consecutive work-groups of the EP launch write consecutive memory
+        long off = 0;
+        for (int q : order) {
+            PairRec &pr = pairs[q];
+            const long sz = (long)h.nstates[h.atom_type[pr.i]] * h.nstates[h.atom_type[pr.j]];
+            if (off + sz >= (1L << 31)) return fail(IMAGO_E_ARG, "block layout exceeds 2^31 entries");
+            pr.blk = (int)off;
+            off += sz;
+        }
+        c->nbe = off;
+        c->d.nbe = off;
     }
+    // unpack tables: the atom of each basis function and the pair of each
+    // (row atom, column atom), -1 below the diagonal
+    {
+        std::vector<int> atom_of(h.n), pidx((size_t)h.natoms * h.natoms, -1);
+        for (int a = 0; a < h.natoms; ++a)
+            for (int m = 0; m < h.nstates[h.atom_type[a]]; ++m) atom_of[h.cumul[a] + m] = a;
+        for (size_t q = 0; q < pairs.size(); ++q)
+            pidx[(size_t)pairs[q].i * h.natoms + pairs[q].j] = (int)q;
+        const int *ao = nullptr, *pi = nullptr;
+        RCHK(dev_upload(c, &ao, atom_of));
+        RCHK(dev_upload(c, &pi, pidx));
+        c->atom_of_state = ao;
+        c->pair_index = pi;
+    }
+    c->hpairs = pairs;
     c->nimgs = (long)imgs.size();
     c->ncands = (long)cands.size();
     RCHK(dev_alloc(c, &c->pairs, pairs.size()));
@@ -650,6 +717,7 @@ extern "C" int imago_ctx_create(int device, imago_ctx **out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev2);
     if (e != hipSuccess) {
         delete c;
         return fail(IMAGO_E_HIP, "context set-up failed: %s", hipGetErrorString(e));
@@ -673,6 +741,7 @@ extern "C" int imago_ctx_destroy(imago_ctx *c)
     if (c->blas) rocblas_destroy_handle(c->blas);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return IMAGO_OK;
@@ -700,7 +769,7 @@ extern "C" int imago_set_system(imago_ctx *c, const imago_system_desc *s, int *v
     d.num_cells = h.ncells;
     d.nk = h.nk;
     d.vale_dim = h.n;
-    d.npack = (long)h.n * (h.n + 1) / 2;
+    d.pot_dim = h.pot_dim;
     std::vector<int> nal(h.nt);
     for (int t = 0; t < h.nt; ++t) nal[t] = h.noa[4 * t];
     RCHK(dev_upload(c, &d.type_alphas, h.alphas));
@@ -749,18 +818,87 @@ extern "C" int imago_set_system(imago_ctx *c, const imago_system_desc *s, int *v
 // ---------------------------------------------------------------------------
 // assembly
 
+static AssembleLaunch make_launch(imago_ctx *c, double *out, long term_stride, int term0,
+                                  int nterms, size_t lds)
+{
+    const HostSys &h = c->h;
+    return AssembleLaunch{c->d, c->pairs, c->imgs, c->phase, c->cands, c->cand_off, term0, out,
+                          term_stride, c->npairs, nterms, lds, c->stream, c->lmax,
+                          c->ne <= 4 ? 4 : 16, h.gamma != 0, c->type_term0, h.npt, c->pair_order,
+                          c->ep_tuples, c->ep_tseg, c->ep_item_tuple, c->ep_blocks,
+                          c->ep_nitems, h.nk};
+}
+
+static void dev_release(imago_ctx *c, void *p)
+{
+    auto &v = c->sysbuf.ptrs;
+    v.erase(std::remove(v.begin(), v.end(), p), v.end());
+    (void)hipFree(p);
+}
+
+// Screening of the EP terms (k_ep_screen) and their primitive blocks
+// (k_ep_prims): count the tuples and items of every (potential type, pair),
+// scan the counts into offsets, fill the tuples, then evaluate every item.
+// Runs once per system: none of it depends on the potential coefficients.
+static int ep_prepare(imago_ctx *c)
+{
+    if (c->ep_screened) return IMAGO_OK;
+    const HostSys &h = c->h;
+    const size_t lds = ep_screen_lds_bytes(h.maxa);
+    if (lds > 64 * 1024)
+        return fail(IMAGO_E_ARG, "EP screening needs %zu bytes of LDS (max_alphas %d)", lds, h.maxa);
+    const long ntask = (long)c->npairs * h.npt;
+    if (ntask >= (1L << 31)) return fail(IMAGO_E_ARG, "%ld (pair, potential type) tasks", ntask);
+    long *tcnt = nullptr, *icnt = nullptr, *toff = nullptr, *ioff = nullptr;
+    RCHK(dev_alloc(c, &tcnt, (size_t)ntask + 1));
+    RCHK(dev_alloc(c, &icnt, (size_t)ntask + 1));
+    RCHK(dev_alloc(c, &toff, (size_t)ntask + 1));
+    RCHK(dev_alloc(c, &ioff, (size_t)ntask + 1));
+    HIPCHK(hipMemsetAsync(tcnt + ntask, 0, sizeof(long), c->stream));
+    HIPCHK(hipMemsetAsync(icnt + ntask, 0, sizeof(long), c->stream));
+    AssembleLaunch L = make_launch(c, nullptr, 0, 0, 0, 0);
+    HIPCHK(launch_ep_screen(L, false, tcnt, icnt, lds));
+    size_t tmp_bytes = 0;
+    HIPCHK(ep_exclusive_scan(tcnt, toff, ntask + 1, nullptr, &tmp_bytes, c->stream));
+    char *tmp = nullptr;
+    RCHK(dev_alloc(c, &tmp, tmp_bytes));
+    HIPCHK(ep_exclusive_scan(tcnt, toff, ntask + 1, tmp, &tmp_bytes, c->stream));
+    HIPCHK(ep_exclusive_scan(icnt, ioff, ntask + 1, tmp, &tmp_bytes, c->stream));
+    long tot[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&tot[0], toff + ntask, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&tot[1], ioff + ntask, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (void *p : {(void *)tcnt, (void *)icnt, (void *)tmp}) dev_release(c, p);
+    if (tot[1] >= (1L << 31))
+        return fail(IMAGO_E_ARG, "%ld EP primitive items exceed the 2^31 item index", tot[1]);
+    const int sh = (c->lmax + 1) * (c->lmax + 1);
+    RCHK(dev_alloc(c, &c->ep_tuples, (size_t)std::max(tot[0], 1L)));
+    RCHK(dev_alloc(c, &c->ep_item_tuple, (size_t)std::max(tot[1], 1L)));
+    RCHK(dev_alloc(c, &c->ep_blocks, (size_t)std::max(tot[1], 1L) * sh * sh));
+    c->ep_tseg = toff;
+    c->ep_ntuples = tot[0];
+    c->ep_nitems = tot[1];
+    L = make_launch(c, nullptr, 0, 0, 0, 0);
+    HIPCHK(launch_ep_screen(L, true, toff, ioff, lds));
+    HIPCHK(launch_ep_prims(L));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, ioff);
+    c->ep_screened = true;
+    return IMAGO_OK;
+}
+
 static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride, int term0,
                            int nterms)
 {
     const HostSys &h = c->h;
-    const size_t lds = kind == IMAGO_EP ? ep_lds_bytes(c->lmax, h.maxa, h.maxs, h.nk)
-                                        : assemble_lds_bytes(c->lmax, h.maxa, h.maxs, h.nk);
+    const size_t lds =
+        kind == IMAGO_EP ? ep_store_lds_bytes(h.maxs, h.nk, h.gamma || h.nk <= 4)
+                         : assemble_lds_bytes(c->lmax, h.maxa, h.maxs, h.nk);
     if (lds > 64 * 1024)
         return fail(IMAGO_E_ARG, "LDS requirement %zu bytes exceeds 64 KiB (max_states %d, nk %d)",
                     lds, h.maxs, h.nk);
-    AssembleLaunch L{c->d, c->pairs, c->imgs, c->phase, c->cands, c->cand_off, term0, out,
-                     term_stride, c->npairs, nterms, lds, c->stream, c->lmax,
-                     c->ne <= 4 ? 4 : 16, h.gamma != 0, c->type_term0, h.npt, c->pair_order};
+    AssembleLaunch L = make_launch(c, out, term_stride, term0, nterms, lds);
     hipError_t e;
     switch (kind) {
     case IMAGO_OL: e = launch_assemble_ol(L); break;
@@ -777,14 +915,30 @@ static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride
 
 static long panel_doubles(const imago_ctx *c)
 {
-    const long npack = (long)c->h.n * (c->h.n + 1) / 2;
-    return c->h.gamma ? npack : 2 * npack * c->h.nk;
+    return c->h.gamma ? c->nbe : 2 * c->nbe * c->h.nk;
 }
 
 extern "C" int imago_assemble(imago_ctx *c, int kind)
 {
     if (!c || !c->has_system) return fail(IMAGO_E_STATE, "imago_assemble before imago_set_system");
-    if (kind < IMAGO_OL || kind > IMAGO_EP) return fail(IMAGO_E_ARG, "unknown integral kind %d", kind);
+    if (kind < IMAGO_OL || kind > IMAGO_EP_DIRECT)
+        return fail(IMAGO_E_ARG, "unknown integral kind %d", kind);
+    if (kind == IMAGO_EP_DIRECT) {
+        if (!c->h.gamma && c->h.nk > 4)
+            return fail(IMAGO_E_ARG, "IMAGO_EP_DIRECT supports Gamma or at most 4 k-points (%d); "
+                                     "assemble IMAGO_EP instead", c->h.nk);
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        RCHK(ep_prepare(c));
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->timing[0] = c->timing[5] = ms;
+        c->timing[4] = 0.0;
+        c->ep_direct = true;
+        return IMAGO_OK;
+    }
     if (kind == IMAGO_NP && c->lmax > 2)
         return fail(IMAGO_E_ARG, "nuclear-potential integrals with f shells (lmAlphaIndex 16) are "
                                  "not supported by this build");
@@ -794,12 +948,25 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
     const int nterms = kind == IMAGO_EP ? c->h.pot_dim : 1;
     if (!c->panel[slot]) RCHK(dev_alloc(c, &c->panel[slot], (size_t)per * nterms));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    if (kind == IMAGO_EP)
+    if (kind == IMAGO_EP) {
+        // stage 5: screening and primitive blocks (first EP assembly of a
+        // system); stage 4: the k_ep_store launch alone; stage 0: both
+        const bool screen = !c->ep_screened;
+        RCHK(ep_prepare(c));
+        HIPCHK(hipEventRecord(c->ev2, c->stream));
         RCHK(launch_assemble(c, kind, c->panel[slot], per, 0, nterms));
-    else
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms_s = 0, ms_e = 0;
+        HIPCHK(hipEventElapsedTime(&ms_s, c->ev0, c->ev2));
+        HIPCHK(hipEventElapsedTime(&ms_e, c->ev2, c->ev1));
+        c->timing[5] = screen ? ms_s : 0.0;
+        c->timing[4] = ms_e;
+    } else {
         RCHK(launch_assemble(c, kind, c->panel[slot], per, 0, 1));
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[0] = ms;
@@ -817,11 +984,26 @@ extern "C" int imago_get_packed(imago_ctx *c, int kind, int term, int kpoint, do
     if (term < 0 || term >= nterms || kpoint < 0 || kpoint >= c->h.nk || !out)
         return fail(IMAGO_E_ARG, "term %d / k-point %d out of range", term, kpoint);
     HIPCHK(hipSetDevice(c->device));
-    const long npack = (long)c->h.n * (c->h.n + 1) / 2;
+    const HostSys &h = c->h;
+    const int dsz = h.gamma ? 1 : 2;
     const long per = panel_doubles(c);
-    const long len = c->h.gamma ? npack : 2 * npack;
+    const long len = dsz * c->nbe;
     const double *src = c->panel[kind - 1] + (size_t)term * per + (size_t)kpoint * len;
-    HIPCHK(hipMemcpy(out, src, len * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> blk(len);
+    HIPCHK(hipMemcpy(blk.data(), src, len * sizeof(double), hipMemcpyDeviceToHost));
+    // block layout -> packed upper triangle, column by column (matrixSubs.F90:506)
+    for (const PairRec &pr : c->hpairs) {
+        const int ns1 = h.nstates[h.atom_type[pr.i]], ns2 = h.nstates[h.atom_type[pr.j]];
+        const long r0 = h.cumul[pr.i], c0 = h.cumul[pr.j];
+        for (int s2 = 0; s2 < ns2; ++s2)
+            for (int s1 = 0; s1 < ns1; ++s1) {
+                const long row = r0 + s1, col = c0 + s2;
+                if (row > col) continue;
+                const long pk = col * (col + 1) / 2 + row;
+                const long be = (long)pr.blk + s1 + (long)ns1 * s2;
+                for (int x = 0; x < dsz; ++x) out[dsz * pk + x] = blk[dsz * be + x];
+            }
+    }
     return IMAGO_OK;
 }
 
@@ -867,11 +1049,20 @@ __global__ __launch_bounds__(256) void k_hbuild(const double *__restrict__ np,
     H[e] = a;
 }
 
-// unpackMatrix fullFlag=1 (matrixSubs.F90:405-460): packed upper -> full
-// Hermitian column-major, one thread per (row, col) of each k.
+__device__ __forceinline__ int cumul_states(int a, const int *cumul, int natoms, int n)
+{
+    return (a + 1 < natoms ? cumul[a + 1] : n) - cumul[a];
+}
+
+// Block layout -> full n x n Hermitian (symmetric for Gamma) column-major
+// matrix per k, one thread per (row, col): the entry comes from the block
+// of the pair (atom of min(row, col), atom of max(row, col)), conjugated
+// below the diagonal (unpackMatrix, matrixSubs.F90:405-460).
 template <bool COMPLEX>
-__global__ void k_unpack(const double *__restrict__ packed, long npack, int n, int nk,
-                         double *__restrict__ full)
+__global__ void k_unpack(const double *__restrict__ blk, long nbe, int n, int nk,
+                         const PairRec *__restrict__ pairs, const int *__restrict__ atom_of,
+                         const int *__restrict__ pair_index, const int *__restrict__ cumul,
+                         int natoms, double *__restrict__ full)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nn = (long)n * n;
@@ -880,12 +1071,15 @@ __global__ void k_unpack(const double *__restrict__ packed, long npack, int n, i
     const long rc = t % nn;
     const int r = (int)(rc % n), col = (int)(rc / n);
     const int lo = min(r, col), hi = max(r, col);
-    const long idx = (long)hi * (hi + 1) / 2 + lo;
+    const int a = atom_of[lo], b = atom_of[hi];
+    const PairRec pr = pairs[pair_index[(size_t)a * natoms + b]];
+    const int ns1 = cumul_states(a, cumul, natoms, n);
+    const long e = (long)pr.blk + (lo - cumul[a]) + (long)ns1 * (hi - cumul[b]);
     if (COMPLEX) {
-        const double2 v = reinterpret_cast<const double2 *>(packed)[(size_t)k * npack + idx];
+        const double2 v = reinterpret_cast<const double2 *>(blk)[(size_t)k * nbe + e];
         reinterpret_cast<double2 *>(full)[t] = make_double2(v.x, r <= col ? v.y : -v.y);
     } else {
-        full[t] = packed[(size_t)k * npack + idx];
+        full[t] = blk[(size_t)k * nbe + e];
     }
 }
 
@@ -894,7 +1088,7 @@ static int ensure_secular_buffers(imago_ctx *c)
     const HostSys &h = c->h;
     const long nn = (long)h.n * h.n;
     if (!c->coef) RCHK(dev_alloc(c, &c->coef, std::max(h.pot_dim, 1)));
-    if (!c->packedH) RCHK(dev_alloc(c, &c->packedH, (size_t)panel_doubles(c)));
+    if (!c->blockH) RCHK(dev_alloc(c, &c->blockH, (size_t)panel_doubles(c)));
     if (!c->evals) RCHK(dev_alloc(c, &c->evals, (size_t)h.nk * h.n));
     if (!c->ework) RCHK(dev_alloc(c, &c->ework, (size_t)h.nk * h.n));
     if (!c->info) RCHK(dev_alloc(c, &c->info, (size_t)h.nk));
@@ -995,21 +1189,43 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
 {
     if (!c || !c->has_system) return fail(IMAGO_E_STATE, "imago_secular before imago_set_system");
     const HostSys &h = c->h;
-    if (!c->assembled[0] || !c->assembled[1] || !c->assembled[2] || !c->assembled[3])
-        return fail(IMAGO_E_STATE, "imago_secular needs OL, KE, NP and EP assembled");
+    if (!c->assembled[0] || !c->assembled[1] || !c->assembled[2] ||
+        !(c->assembled[3] || c->ep_direct))
+        return fail(IMAGO_E_STATE,
+                    "imago_secular needs OL, KE, NP and EP (or EP_DIRECT) assembled");
+    // the stored EP panels when they exist, else the direct path
+    const bool direct = !c->assembled[3];
     if (num_states <= 0 || num_states > h.n)
         return fail(IMAGO_E_ARG, "num_states %d out of range 1..%d", num_states, h.n);
     if (!pot_coeffs && h.pot_dim > 0) return fail(IMAGO_E_ARG, "null potential coefficients");
     HIPCHK(hipSetDevice(c->device));
     RCHK(ensure_secular_buffers(c));
-    const long npack = (long)h.n * (h.n + 1) / 2;
     const long per = panel_doubles(c);
     HIPCHK(hipMemcpyAsync(c->coef, pot_coeffs, sizeof(double) * h.pot_dim, hipMemcpyHostToDevice,
                           c->stream));
+    if (direct) {
+        // readPackedMatrixAccum: a factor of exactly 0.0 adds the term unscaled
+        std::vector<double> ce(h.pot_dim);
+        for (int j = 0; j < h.pot_dim; ++j) ce[j] = pot_coeffs[j] == 0.0 ? 1.0 : pot_coeffs[j];
+        if (!c->coef_eff) RCHK(dev_alloc(c, &c->coef_eff, std::max(h.pot_dim, 1)));
+        HIPCHK(hipMemcpyAsync(c->coef_eff, ce.data(), sizeof(double) * h.pot_dim,
+                              hipMemcpyHostToDevice, c->stream));
+    }
+    const int ngroups = (h.npt + kEpdTypesPerGroup - 1) / kEpdTypesPerGroup;
+    if (direct && !c->hpartial) RCHK(dev_alloc(c, &c->hpartial, (size_t)per * ngroups));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_hbuild, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, c->stream,
-                       c->panel[2], c->panel[1], c->panel[3], c->coef, h.pot_dim, per,
-                       c->packedH);
+    if (direct) {
+        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg, c->ep_blocks, c->coef_eff,
+                    nullptr, c->type_term0, c->pair_order, c->npairs, ngroups, c->hpartial, per,
+                    nullptr, c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
+        HIPCHK(launch_epd_hsum(L));
+        HIPCHK(launch_epd_hfinal(c->panel[2], c->panel[1], c->hpartial, ngroups, per, c->blockH,
+                                 c->stream));
+    } else {
+        hipLaunchKernelGGL(k_hbuild, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, c->stream,
+                           c->panel[2], c->panel[1], c->panel[3], c->coef, h.pot_dim, per,
+                           c->blockH);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     const long nn = (long)h.n * h.n;
@@ -1019,11 +1235,13 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     void *Sm = cx ? (void *)c->Sfull : (void *)c->Sreal;
     const int nb = cx ? h.nk : 1;
     if (cx)
-        hipLaunchKernelGGL(k_unpack<true>, dim3(ug), dim3(256), 0, c->stream, c->packedH, npack, h.n,
-                           h.nk, (double *)Hm);
+        hipLaunchKernelGGL(k_unpack<true>, dim3(ug), dim3(256), 0, c->stream, c->blockH, c->nbe,
+                           h.n, h.nk, c->pairs, c->atom_of_state, c->pair_index, c->d.cumul,
+                           h.natoms, (double *)Hm);
     else
-        hipLaunchKernelGGL(k_unpack<false>, dim3(ug), dim3(256), 0, c->stream, c->packedH, npack, h.n,
-                           1, (double *)Hm);
+        hipLaunchKernelGGL(k_unpack<false>, dim3(ug), dim3(256), 0, c->stream, c->blockH, c->nbe,
+                           h.n, 1, c->pairs, c->atom_of_state, c->pair_index, c->d.cumul,
+                           h.natoms, (double *)Hm);
     HIPCHK(hipGetLastError());
     hipEvent_t e2, e3;
     HIPCHK(hipEventCreate(&e2));
@@ -1035,11 +1253,13 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     bool bad = false;
     if (!c->chol_valid) {
         if (cx)
-            hipLaunchKernelGGL(k_unpack<true>, dim3(ug), dim3(256), 0, c->stream, c->panel[0], npack,
-                               h.n, h.nk, (double *)Sm);
+            hipLaunchKernelGGL(k_unpack<true>, dim3(ug), dim3(256), 0, c->stream, c->panel[0],
+                               c->nbe, h.n, h.nk, c->pairs, c->atom_of_state, c->pair_index,
+                               c->d.cumul, h.natoms, (double *)Sm);
         else
             hipLaunchKernelGGL(k_unpack<false>, dim3(ug), dim3(256), 0, c->stream, c->panel[0],
-                               npack, h.n, 1, (double *)Sm);
+                               c->nbe, h.n, 1, c->pairs, c->atom_of_state, c->pair_index,
+                               c->d.cumul, h.natoms, (double *)Sm);
         HIPCHK(hipGetLastError());
         RCHK(factor_gv_batched(c, cx, h.n, nb, Sm, c->info, inf.data(), &bad));
         c->chol_valid = !bad;
@@ -1122,36 +1342,44 @@ __global__ void k_scale_vectors(const double *__restrict__ C, const double *__re
     }
 }
 
-// packMatrix of rho with the matrixElementMult weights folded in:
-// off-diagonal entries x2, diagonal entries real part only.
+// rho in the block layout with the matrixElementMult weights folded in
+// (matrixSubs.F90:66-112): off-diagonal entries x2, diagonal entries real
+// part only, the strict lower triangle of a diagonal block 0.  rho holds
+// the upper triangle (zherk); one work-group per (pair, k).
 template <bool COMPLEX>
-__global__ void k_pack_weighted(const double *__restrict__ rho, int n, int nk, long npack,
-                                double *__restrict__ rw)
+__global__ void k_pack_weighted(const double *__restrict__ rho, int n, long nbe,
+                                const PairRec *__restrict__ pairs, const int *__restrict__ cumul,
+                                int natoms, double *__restrict__ rw)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npack * nk) return;
-    const int k = (int)(t / npack);
-    const long idx = t % npack;
-    // invert idx = col(col+1)/2 + row
-    long col = (long)((sqrt(8.0 * (double)idx + 1.0) - 1.0) / 2.0);
-    while (col * (col + 1) / 2 > idx) --col;
-    while ((col + 1) * (col + 2) / 2 <= idx) ++col;
-    const long row = idx - col * (col + 1) / 2;
+    const PairRec pr = pairs[blockIdx.x];
+    const int k = blockIdx.y;
+    const int ns1 = cumul_states(pr.i, cumul, natoms, n), ns2 = cumul_states(pr.j, cumul, natoms, n);
+    const long r0 = cumul[pr.i], c0 = cumul[pr.j];
     const long nn = (long)n * n;
-    if (COMPLEX) {
-        const double2 v = reinterpret_cast<const double2 *>(rho)[(size_t)k * nn + row + col * n];
-        double2 o = row == col ? make_double2(v.x, 0.0) : make_double2(2.0 * v.x, 2.0 * v.y);
-        reinterpret_cast<double2 *>(rw)[t] = o;
-    } else {
-        const double v = rho[(size_t)k * nn + row + col * n];
-        rw[t] = row == col ? v : 2.0 * v;
+    for (int e = threadIdx.x; e < ns1 * ns2; e += blockDim.x) {
+        const long row = r0 + e % ns1, col = c0 + e / ns1;
+        const size_t o = (size_t)k * nbe + pr.blk + e;
+        if (COMPLEX) {
+            double2 w = make_double2(0.0, 0.0);
+            if (row <= col) {
+                const double2 v = reinterpret_cast<const double2 *>(rho)[(size_t)k * nn + row + col * n];
+                w = row == col ? make_double2(v.x, 0.0) : make_double2(2.0 * v.x, 2.0 * v.y);
+            }
+            reinterpret_cast<double2 *>(rw)[o] = w;
+        } else {
+            double w = 0.0;
+            if (row <= col) {
+                const double v = rho[(size_t)k * nn + row + col * n];
+                w = row == col ? v : 2.0 * v;
+            }
+            rw[o] = w;
+        }
     }
 }
 
-// partial[term][blk] = sum over a chunk of the term's panel of M . rw for a
-// group of TG terms at once, so each chunk of the weighted density rw is
-// read once per group instead of once per term (the panel is the only
-// stream that has to come from HBM).
+// partial[term][blk] = sum over a chunk of the term's panel of M . rw
+// (matrixElementMult summed over k-points); TG terms per work-group share
+// each chunk of the weighted density rw.
 template <int TG>
 __global__ __launch_bounds__(256) void k_trace_partial(const double *__restrict__ M, long len,
                                                        int nterms, const double *__restrict__ rw,
@@ -1212,7 +1440,6 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     }
     HIPCHK(hipSetDevice(c->device));
     const long nn = (long)h.n * h.n;
-    const long npack = (long)h.n * (h.n + 1) / 2;
     const long per = panel_doubles(c);
     const bool cx = !h.gamma;
     const int dsz = cx ? 2 : 1;
@@ -1249,30 +1476,39 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
         BLASCHK(rocblas_dsyrk(c->blas, rocblas_fill_upper, rocblas_operation_none, h.n, nocc, &one,
                               W, h.n, &zero, c->rho, h.n));
     }
-    const long pt = npack * h.nk;
     if (cx)
-        hipLaunchKernelGGL(k_pack_weighted<true>, dim3((unsigned)((pt + 255) / 256)), dim3(256), 0,
-                           c->stream, c->rho, h.n, h.nk, npack, c->rhow);
+        hipLaunchKernelGGL(k_pack_weighted<true>, dim3(c->npairs, h.nk), dim3(256), 0, c->stream,
+                           c->rho, h.n, c->nbe, c->pairs, c->d.cumul, h.natoms, c->rhow);
     else
-        hipLaunchKernelGGL(k_pack_weighted<false>, dim3((unsigned)((pt + 255) / 256)), dim3(256), 0,
-                           c->stream, c->rho, h.n, 1, npack, c->rhow);
+        hipLaunchKernelGGL(k_pack_weighted<false>, dim3(c->npairs, 1), dim3(256), 0, c->stream,
+                           c->rho, h.n, c->nbe, c->pairs, c->d.cumul, h.natoms, c->rhow);
     HIPCHK(hipGetLastError());
     const int nblk = 64;
     const int nterms = h.pot_dim + 3;
     if (!c->partial) RCHK(dev_alloc(c, &c->partial, (size_t)nterms * nblk));
     double *outd = nullptr;
     RCHK(dev_alloc(c, &outd, (size_t)nterms));
-    // S, NP, KE (one term each), then the potential panel in groups of 16
+    // S, NP, KE (one term each), then the potential terms: from the stored
+    // panel, or directly from the primitive blocks
+    const bool direct = !c->assembled[3];
     const int order3[3] = {0, 2, 1};
     for (int q = 0; q < 3; ++q)
         hipLaunchKernelGGL(k_trace_partial<1>, dim3(nblk, 1), dim3(256), 0, c->stream,
                            c->panel[order3[q]], per, 1, c->rhow, nblk, c->partial + (size_t)q * nblk);
-    if (h.pot_dim > 0)
+    if (h.pot_dim > 0 && !direct)
         hipLaunchKernelGGL(k_trace_partial<1>, dim3(nblk, h.pot_dim), dim3(256), 0, c->stream,
                            c->panel[3], per, h.pot_dim, c->rhow, nblk,
                            c->partial + (size_t)3 * nblk);
+    if (h.pot_dim > 0 && direct) {
+        if (!c->tpart) RCHK(dev_alloc(c, &c->tpart, (size_t)c->npairs * h.pot_dim));
+        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg, c->ep_blocks, nullptr,
+                    c->rhow, c->type_term0, c->pair_order, c->npairs, 0, nullptr, 0, c->tpart,
+                    c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
+        HIPCHK(launch_epd_trace(L));
+        HIPCHK(launch_epd_trace_final(c->tpart, c->npairs, h.pot_dim, outd + 3, c->stream));
+    }
     hipLaunchKernelGGL(k_trace_final, dim3((nterms + 255) / 256), dim3(256), 0, c->stream, c->partial,
-                       nblk, nterms, outd);
+                       nblk, direct ? 3 : nterms, outd);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1345,7 +1581,7 @@ extern "C" int imago_dsygv(imago_ctx *c, int n, double *A, double *B, double *w,
 
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 3) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 5) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

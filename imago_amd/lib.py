@@ -23,8 +23,8 @@ from ._desc import make_desc
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libimago_hip.so")
 
-OL, KE, NP, EP = 1, 2, 3, 4
-KIND_CODES = {"ol": OL, "ke": KE, "np": NP, "ep": EP}
+OL, KE, NP, EP, EP_DIRECT = 1, 2, 3, 4, 5
+KIND_CODES = {"ol": OL, "ke": KE, "np": NP, "ep": EP, "ep_direct": EP_DIRECT}
 
 ERRORS = {
     -1: "IMAGO_E_ARG", -2: "IMAGO_E_HIP", -3: "IMAGO_E_LATTICE", -4: "IMAGO_E_NOTPD",

@@ -49,6 +49,7 @@ extern "C" {
 #define IMAGO_KE 2   /* kinetic            gaussOverlapKE  integrals.F90:656  */
 #define IMAGO_NP 3   /* nuclear potential  gaussOverlapNP  integrals.F90:1531 */
 #define IMAGO_EP 4   /* electronic pot.    elecPotGaussOverlap :2357          */
+#define IMAGO_EP_DIRECT 5 /* EP without panels: see imago_assemble           */
 
 /* Boys-function modes for the nuclear integral */
 #define IMAGO_BOYS_REFERENCE 0  /* gaussIntegrals.f90:16080-16221 verbatim */
@@ -138,9 +139,16 @@ int imago_boys_batch(imago_ctx *ctx, int n, const double *T, double *F7);
 
 /* ---- assembly: gaussOverlapOL/KE/NP, elecPotGaussOverlap -------------- *
  * Computes the lattice-summed, Bloch-phased matrices of one kind into
- * device-resident packed panels (per k for the complex build; one real
- * panel for gamma).  For IMAGO_EP all potDim terms are built, term j in the
+ * device-resident panels (per k for the complex build; one real panel for
+ * gamma); imago_get_packed returns any of them in the reference's packed
+ * layout.  For IMAGO_EP all potDim terms are built, term j in the
  * reference's order (potential type, then alpha; integrals.F90:2411-2476).
+ * IMAGO_EP_DIRECT builds no EP panels: it screens the EP tuples and stores
+ * their primitive blocks once, and imago_secular / imago_density_traces
+ * then form sum_j c_j V_j and Tr(rho V_j) from those blocks each call (the
+ * fused accumulation of the reference's post-SCF path, integrals.F90:
+ * 2860-3605); it needs gamma or at most 4 k-points.  When both are
+ * assembled the stored panels are used.
  * coreDim must be 0, so orthoOL/ortho reduce to packing (intgOrtho.F90). */
 int imago_assemble(imago_ctx *ctx, int kind);
 
@@ -186,7 +194,10 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
 /* ---- timing ----------------------------------------------------------- *
  * Milliseconds of the last launch of the named stage measured with HIP
  * events on the library's stream: 0 = last imago_assemble, 1 = H build of
- * the last imago_secular, 2 = its eigensolve, 3 = last density+traces. */
+ * the last imago_secular, 2 = its eigensolve, 3 = last density+traces,
+ * 4 = the EP panel store of the last IMAGO_EP assembly, 5 = the EP
+ * screening and primitive blocks of the last EP assembly that built them
+ * (0 when they were already built for the system). */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 #ifdef __cplusplus

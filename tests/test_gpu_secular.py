@@ -142,3 +142,70 @@ def test_repeated_secular_reuses_overlap_factor(gpu_ctx, oracle_mod):
         assert tr[0] == pytest.approx(ref, rel=1e-10)
         # Tr(rho S) = sum of the occupations: 3 states x 2/nk at nk k-points
         assert tr[0] == pytest.approx(6.0, rel=1e-9)
+
+
+def _secular_traces_oracle(oracle_mod, s, coeffs, occ_fn, ns):
+    """Eigenvalues of every k and the traces (S, NP, KE, V_j) from the
+    oracle's matrices for the GPU eigenvectors passed in via occ_fn."""
+    n = s.vale_dim
+    S = oracle_mod.assemble(s, "ol")
+    K = oracle_mod.assemble(s, "ke")
+    N = oracle_mod.assemble(s, "np")
+    V = oracle_mod.assemble(s, "ep")
+    evs = []
+    for k in range(s.num_kpoints):
+        H = oracle_mod.unpack(oracle_mod.build_hamiltonian(N[k], K[k], V[:, k], coeffs), n)
+        w, _, info = oracle_mod.hegv(H, oracle_mod.unpack(S[k], n))
+        assert info == 0
+        evs.append(w[:ns])
+    return np.array(evs), (S, N, K, V)
+
+
+@pytest.mark.parametrize("name", ["small_complex", "small_gamma", "si8_cubic_d"])
+def test_direct_ep_path_matches_oracle(gpu_ctx, oracle_mod, name):
+    """IMAGO_EP_DIRECT: H(k) and Tr(rho V_j) from the stored primitive
+    blocks instead of the potDim panels, against the oracle's panels."""
+    s = {"small_complex": lambda: configs.small_test_system(seed=9),
+         "small_gamma": lambda: configs.small_test_system(seed=9, gamma=True),
+         # cutoff 16 bohr: at 12 the screened overlap of this cell is not
+         # positive definite
+         "si8_cubic_d": lambda: configs.si64_supercell(reps=(1, 1, 1), mesh=(2, 2, 2),
+                                                       cutoff_bohr=16.0)}[name]()
+    print("[gpu] direct EP", name, flush=True)
+    gpu_ctx.set_system(s)
+    for kind in ("ol", "ke", "np", "ep_direct"):
+        gpu_ctx.assemble(kind)
+    n = s.vale_dim
+    coeffs = configs.pot_coefficients(s, seed=3)
+    coeffs[1] = 0.0   # readPackedMatrixAccum: factor 0.0 adds unscaled
+    ev = gpu_ctx.secular(coeffs, n)
+    ref, (S, N, K, V) = _secular_traces_oracle(oracle_mod, s, coeffs, None, n)
+    assert np.max(np.abs(ev - ref)) < 1e-8, np.max(np.abs(ev - ref))
+    nel = 6 if name != "si8_cubic_d" else 14 * s.num_atoms
+    occ = oracle_mod.populate(ev, s.kpoint_weights, nel)
+    tr, prho = gpu_ctx.density_traces(occ)
+    t_ref = np.zeros(3)
+    p_ref = np.zeros(s.pot_dim)
+    cols = np.concatenate([np.full(c + 1, c) for c in range(n)])
+    rows = np.concatenate([np.arange(c + 1) for c in range(n)])
+    for k in range(s.num_kpoints):
+        X = gpu_ctx.eigenvectors(k, n)
+        rp = ((X * occ[k]) @ X.conj().T)[rows, cols]
+        if s.gamma:
+            rp = rp.real
+        for q, M in enumerate((S[k], N[k], K[k])):
+            t_ref[q] += oracle_mod.matrix_element_mult(M, rp, n)
+        for j in range(s.pot_dim):
+            p_ref[j] += oracle_mod.matrix_element_mult(V[j, k], rp, n)
+    assert np.allclose(tr, t_ref, rtol=1e-10, atol=1e-12)
+    scale = np.max(np.abs(p_ref))
+    assert np.max(np.abs(prho - p_ref)) <= 1e-10 * scale, np.max(np.abs(prho - p_ref)) / scale
+    assert tr[0] == pytest.approx(nel, rel=1e-9)
+
+
+def test_direct_ep_path_refuses_many_kpoints(gpu_ctx):
+    s = configs.si2_primitive()   # 10 k-points
+    gpu_ctx.set_system(s)
+    with pytest.raises(ImagoError) as e:
+        gpu_ctx.assemble("ep_direct")
+    assert e.value.name == "IMAGO_E_ARG" and "4 k-points" in str(e.value)
