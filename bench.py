@@ -77,7 +77,7 @@ def gpu_step(ctx, system, coeffs, num_states, oracle_populate, ep_kind):
     t0 = time.perf_counter()
     ctx.set_system(system)
     t1 = time.perf_counter()
-    stage = {}
+    stage = {"set_system_lib": ctx.timing(6), "geometry": ctx.timing(7)}
     for kind in ("ol", "ke", "np", ep_kind):
         ctx.assemble(kind)
         stage["assemble_" + kind] = ctx.timing(0)

@@ -16,4 +16,9 @@ hipError_t ep_exclusive_scan(const long *in, long *out, long n, void *tmp, size_
 {
     return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)n, stream);
 }
+hipError_t int_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t *tmp_bytes,
+                              hipStream_t stream)
+{
+    return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)n, stream);
+}
 }  // namespace imago

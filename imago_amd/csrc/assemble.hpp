@@ -212,8 +212,8 @@ __global__ __launch_bounds__(64) void k_assemble(DevSys S, const PairRec *__rest
                         ocz = c * (A.z - Bp.z) + Bp.z;
                         tad_base = 1 - sep / ad[a1 * MA + a2];
                     }
-                    const int clo = cand_off[(size_t)img * (S.num_pot_sites + 1) + site_lo];
-                    const int chi = cand_off[(size_t)img * (S.num_pot_sites + 1) + site_hi];
+                    const int clo = cand_off[(size_t)img * S.num_pot_sites + site_lo];
+                    const int chi = cand_off[(size_t)img * S.num_pot_sites + site_hi];
                     for (int ci = clo; ci < chi; ++ci) {
                         const CandRec cr = cands[ci];
                         const int pt = S.pot_site_type[cr.site];

@@ -73,6 +73,8 @@ hipError_t launch_epd_trace_final(const double *tpart, int npairs, int pot_dim, 
 // exclusive prefix sum of n longs (tmp == nullptr: only sets *tmp_bytes)
 hipError_t ep_exclusive_scan(const long *in, long *out, long n, void *tmp, size_t *tmp_bytes,
                              hipStream_t stream);
+hipError_t int_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t *tmp_bytes,
+                              hipStream_t stream);
 
 template <int KIND, int LMAX, int NE, bool G>
 hipError_t launch_one(const AssembleLaunch &L)

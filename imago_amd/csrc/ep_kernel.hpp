@@ -116,7 +116,9 @@ __global__ __launch_bounds__(64) void k_ep_screen(DevSys S, const PairRec *__res
     // alphaPotDist of exponent a: apd0[a * aj_stride + a1 * MA + a2]
     const size_t aj_stride = (size_t)NEL * NEL * MA * MA;
     const double *apd0 = S.apdist + (size_t)pel * S.max_pot_alphas * aj_stride + eoff;
-    const size_t csz = (size_t)S.num_pot_sites + 1;
+    // candidates of (image, site) at cand_off[image * sites + site]; the
+    // entry after the last site of an image is the next image's first
+    const size_t csz = (size_t)S.num_pot_sites;
 
     long tpos = FILL ? tseg[task] : 0;
     long ipos = FILL ? iseg[task] : 0;

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <thread>
 #include <cmath>
 #include <cstdarg>
@@ -24,6 +25,7 @@
 #include "common.hpp"
 #include "assemble.hpp"
 #include "assemble_launch.hpp"
+#include "geom_kernel.hpp"
 
 using namespace imago;
 
@@ -68,12 +70,48 @@ struct HostSys {
     int n = 0, pot_dim = 0;
 };
 
+// Device buffers of the current system.  Released buffers are kept and
+// handed out again to later requests of a similar size (set_system is
+// called once per SCF run and repeatedly in benchmarks; returning GBs to
+// the driver and mapping them again costs milliseconds).  The cache is
+// emptied when an allocation fails and when the context is destroyed.
 struct DevBuf {
-    std::vector<void *> ptrs;
+    struct Blk {
+        void *p;
+        size_t bytes;
+    };
+    std::vector<Blk> live, cache;
+    void *take(size_t bytes)
+    {
+        int best = -1;
+        for (int i = 0; i < (int)cache.size(); ++i)
+            if (cache[i].bytes >= bytes && cache[i].bytes <= 2 * bytes + (1u << 20) &&
+                (best < 0 || cache[i].bytes < cache[best].bytes))
+                best = i;
+        if (best < 0) return nullptr;
+        const Blk b = cache[best];
+        cache.erase(cache.begin() + best);
+        live.push_back(b);
+        return b.p;
+    }
     void release()
     {
-        for (void *p : ptrs) (void)hipFree(p);
-        ptrs.clear();
+        cache.insert(cache.end(), live.begin(), live.end());
+        live.clear();
+    }
+    void release_one(void *p)
+    {
+        for (size_t i = 0; i < live.size(); ++i)
+            if (live[i].p == p) {
+                cache.push_back(live[i]);
+                live.erase(live.begin() + i);
+                return;
+            }
+    }
+    void purge_cache()
+    {
+        for (const Blk &b : cache) (void)hipFree(b.p);
+        cache.clear();
     }
 };
 
@@ -136,7 +174,7 @@ struct imago_ctx {
     double *coef_eff = nullptr;   // [pot_dim]
     double *hpartial = nullptr;   // [groups][panel]
     double *tpart = nullptr;      // [pairs][pot_dim]
-    double timing[6] = {0, 0, 0, 0, 0, 0};
+    double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 };
 
@@ -181,16 +219,25 @@ static void ctx_free_system(imago_ctx *c)
 template <class T>
 static int dev_alloc(imago_ctx *c, T **p, size_t count)
 {
-    void *q = nullptr;
-    size_t bytes = std::max<size_t>(count * sizeof(T), 16);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess)
-        return fail(IMAGO_E_NOMEM, "hipMalloc of %zu bytes failed: %s", bytes,
-                    hipGetErrorString(e));
-    c->sysbuf.ptrs.push_back(q);
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    void *q = c->sysbuf.take(bytes);
+    if (!q) {
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            c->sysbuf.purge_cache();
+            e = hipMalloc(&q, bytes);
+        }
+        if (e != hipSuccess)
+            return fail(IMAGO_E_NOMEM, "hipMalloc of %zu bytes failed: %s", bytes,
+                        hipGetErrorString(e));
+        c->sysbuf.live.push_back({q, bytes});
+    }
     *p = (T *)q;
     return IMAGO_OK;
 }
+
+static void dev_release(imago_ctx *c, void *p) { c->sysbuf.release_one(p); }
 
 template <class T>
 static int dev_upload(imago_ctx *c, const T **dst, const std::vector<T> &v)
@@ -202,33 +249,9 @@ static int dev_upload(imago_ctx *c, const T **dst, const std::vector<T> &v)
     return IMAGO_OK;
 }
 
-// findLatticeVector (lattice.f90:784-890): nearest lattice point to v.
-// The fractional coordinates are rounded half away from zero, then the
-// closest of the nearby sorted cells is taken (the reference's Wigner-cell
-// tables answer the same question).
 static void find_lattice_vector(const HostSys &h, const double *v, double *L)
 {
-    double L0[3] = {0, 0, 0};
-    for (int i = 0; i < 3; ++i) {
-        double f = h.inv[3 * i] * v[0] + h.inv[3 * i + 1] * v[1] + h.inv[3 * i + 2] * v[2];
-        double rnd = (double)(long)(std::fabs(f) + 0.5);
-        if (f < 0.0) rnd = -rnd;
-        for (int x = 0; x < 3; ++x) L0[x] += h.rv[3 * i + x] * rnd;
-    }
-    double rem[3] = {v[0] - L0[0], v[1] - L0[1], v[2] - L0[2]};
-    double best = 1e300;
-    int bi = 0;
-    int lim = std::min(h.ncells, 125);
-    for (int k = 0; k < lim; ++k) {
-        const double *R = &h.cell_dims[3 * k];
-        double d = 0;
-        for (int x = 0; x < 3; ++x) d += (rem[x] - R[x]) * (rem[x] - R[x]);
-        if (d < best) {
-            best = d;
-            bi = k;
-        }
-    }
-    for (int x = 0; x < 3; ++x) L[x] = L0[x] + h.cell_dims[3 * bi + x];
+    imago::find_lattice_vector(h.rv, h.inv, h.cell_dims.data(), h.cell_sizes.data(), h.ncells, v, L);
 }
 
 static size_t adx(const HostSys &h, int e1, int e2, int a1, int a2)
@@ -455,15 +478,15 @@ static int build_host(HostSys &h, const imago_system_desc *s)
     return IMAGO_OK;
 }
 
-// Pair/image table (integrals.F90:323-430) and 3-centre candidate images,
-// built for the pairs (i, j >= i) of one atom i; atoms run in parallel and
-// their parts are concatenated in atom order.
+// Pair/image table (integrals.F90:323-430) with the Bloch phases of every
+// image, built for the pairs (i, j >= i) of one atom i; atoms run in
+// parallel and their parts are concatenated in atom order.  The 3-centre
+// candidate images are listed on the device (k_geom_cands).
 struct GeomPart {
     std::vector<PairRec> pairs;
     std::vector<ImgRec> imgs;
     std::vector<double2> phase;
-    std::vector<CandRec> cands;
-    std::vector<int> cand_off;   // relative to this part's cands
+    std::vector<int> img_pair;   // pair of each image, relative to this part's pairs
     int rc = IMAGO_OK;
     std::string err;
 };
@@ -474,8 +497,7 @@ static void build_geometry_atom(const HostSys &h, int i, GeomPart &g)
     auto &pairs = g.pairs;
     auto &imgs = g.imgs;
     auto &phase = g.phase;
-    auto &cands = g.cands;
-    auto &cand_off = g.cand_off;
+    auto &img_pair = g.img_pair;
     const double lastsz = h.cell_sizes[h.ncells - 1];
     {
         const int t1 = h.atom_type[i], e1 = h.type_el[t1];
@@ -533,70 +555,7 @@ static void build_geometry_atom(const HostSys &h, int i, GeomPart &g)
                     double2 s = shift[q];
                     phase.push_back(make_double2(pf.x * s.x - pf.y * s.y, pf.x * s.y + pf.y * s.x));
                 }
-                // candidate potential images of every site for this image
-                const double ad00 = h.adist[adx(h, e1, e2, 0, 0)];
-                const double fac = 1.0 - sep / ad00;
-                double mid[3], half = 0, seg[3];
-                for (int x = 0; x < 3; ++x) {
-                    mid[x] = 0.5 * (A[x] + (&ir.bx)[x]);
-                    seg[x] = A[x] - (&ir.bx)[x];
-                    half += seg[x] * seg[x];
-                }
-                half = 0.5 * std::sqrt(half);
-                for (int s2 = 0; s2 < h.nsites; ++s2) {
-                    cand_off.push_back((int)cands.size());
-                    const int pt = h.site_type[s2], pel = h.pot_el[pt];
-                    double bound = 0.0;
-                    const size_t base = adx(h, e1, e2, 0, 0);
-                    if (h.el_nuc_alpha[pel] > 0.0)
-                        bound = std::max(bound, h.andist[(size_t)pel * h.nel * h.nel * h.maxa * h.maxa + base]);
-                    if (h.el_npot[pel] > 0)
-                        bound = std::max(bound, h.apdist[((size_t)pel * h.maxpa) * h.nel * h.nel * h.maxa * h.maxa + base]);
-                    bound = bound * std::max(fac, 0.0) * (1.0 + 1e-9) + 1e-12;
-                    const double rb = std::sqrt(bound);
-                    const double *C = &h.site_pos[3 * s2];
-                    double mv[3] = {mid[0] - C[0], mid[1] - C[1], mid[2] - C[2]}, L2[3];
-                    find_lattice_vector(h, mv, L2);
-                    double cm[3] = {mv[0] - L2[0], mv[1] - L2[1], mv[2] - L2[2]};
-                    double rreq = std::sqrt(cm[0] * cm[0] + cm[1] * cm[1] + cm[2] * cm[2]) + half + rb;
-                    if (rreq * rreq > lastsz) {
-                        char buf[256];
-                        snprintf(buf, sizeof buf,
-                                 "More lattice points needed for potential site %d near atom "
-                                 "pair (%d,%d)", s2 + 1, i + 1, j + 1);
-                        g.rc = IMAGO_E_LATTICE;
-                        g.err = buf;
-                        return;
-                    }
-                    for (int n = 0; n < h.ncells; ++n) {
-                        if (h.cell_sizes[n] > rreq * rreq) break;
-                        const double *Rn = &h.cell_dims[3 * n];
-                        double P[3];
-                        for (int x = 0; x < 3; ++x) P[x] = C[x] + L2[x] + Rn[x];
-                        // squared distance from P to the segment [B', A]
-                        double w[3] = {P[0] - ir.bx, P[1] - ir.by, P[2] - ir.bz};
-                        double ss = seg[0] * seg[0] + seg[1] * seg[1] + seg[2] * seg[2];
-                        double tpar = ss > 0 ? (w[0] * seg[0] + w[1] * seg[1] + w[2] * seg[2]) / ss : 0.0;
-                        tpar = std::min(1.0, std::max(0.0, tpar));
-                        double dd = 0;
-                        for (int x = 0; x < 3; ++x) {
-                            double q2 = w[x] - tpar * seg[x];
-                            dd += q2 * q2;
-                        }
-                        if (dd > bound) continue;
-                        CandRec cr;
-                        cr.lx = L2[0];
-                        cr.ly = L2[1];
-                        cr.lz = L2[2];
-                        cr.px = P[0];
-                        cr.py = P[1];
-                        cr.pz = P[2];
-                        cr.site = s2;
-                        cr.cell = n;
-                        cands.push_back(cr);
-                    }
-                }
-                cand_off.push_back((int)cands.size());
+                img_pair.push_back((int)pairs.size());
                 pr.img_cnt++;
                 if (pr.img_cnt >= 65536) {
                     g.rc = IMAGO_E_LATTICE;
@@ -607,6 +566,91 @@ static void build_geometry_atom(const HostSys &h, int i, GeomPart &g)
             pairs.push_back(pr);
         }
     }
+}
+
+// 3-centre candidate images of every (image, potential site) on the
+// device (k_geom_cands): count, scan into cand_off[image * sites + site],
+// fill.  The largest threshold of each (potential element, element pair)
+// bounds the distance of a passing image from the segment [B', A].
+static int build_candidates(imago_ctx *c, const std::vector<int> &img_pair)
+{
+    const HostSys &h = c->h;
+    const long nimg = c->nimgs, ntask = nimg * h.nsites;
+    if (ntask >= (1L << 31)) return fail(IMAGO_E_ARG, "%ld (image, site) pairs", ntask);
+    std::vector<double> bound0((size_t)h.nel * h.nel * h.nel, 0.0);
+    for (int pel = 0; pel < h.nel; ++pel)
+        for (int e1 = 0; e1 < h.nel; ++e1)
+            for (int e2 = 0; e2 < h.nel; ++e2) {
+                const size_t base = adx(h, e1, e2, 0, 0);
+                double b = 0.0;
+                if (h.el_nuc_alpha[pel] > 0.0)
+                    b = std::max(b, h.andist[(size_t)pel * h.nel * h.nel * h.maxa * h.maxa + base]);
+                if (h.el_npot[pel] > 0)
+                    b = std::max(b, h.apdist[((size_t)pel * h.maxpa) * h.nel * h.nel * h.maxa * h.maxa + base]);
+                bound0[((size_t)pel * h.nel + e1) * h.nel + e2] = b;
+            }
+    const int *dimg_pair = nullptr;
+    const double *dbound0 = nullptr, *dsizes = nullptr;
+    RCHK(dev_upload(c, &dimg_pair, img_pair));
+    RCHK(dev_upload(c, &dbound0, bound0));
+    RCHK(dev_upload(c, &dsizes, h.cell_sizes));
+    GeomArgs G{};
+    G.pairs = c->pairs;
+    G.imgs = c->imgs;
+    G.img_pair = dimg_pair;
+    G.atom_pos = c->d.atom_pos;
+    G.atom_type = c->d.atom_type;
+    G.type_element = c->d.type_element;
+    G.site_pos = c->d.pot_pos;
+    G.site_type = c->d.pot_site_type;
+    G.pot_el = c->d.pot_el;
+    G.adist = c->d.adist;
+    G.bound0 = dbound0;
+    G.cell_dims = c->d.cell_dims;
+    G.cell_sizes = dsizes;
+    std::memcpy(G.rv, h.rv, sizeof G.rv);
+    std::memcpy(G.inv, h.inv, sizeof G.inv);
+    G.ncells = h.ncells;
+    G.nsites = h.nsites;
+    G.nel = h.nel;
+    G.maxa = h.maxa;
+    G.nimgs = nimg;
+    int *cnt = nullptr, *err = nullptr;
+    RCHK(dev_alloc(c, &cnt, (size_t)ntask + 1));
+    RCHK(dev_alloc(c, &c->cand_off, (size_t)ntask + 1));
+    RCHK(dev_alloc(c, &err, 1));
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(cnt + ntask, 0, sizeof(int), c->stream));
+    const unsigned g = (unsigned)((ntask + 255) / 256);
+    if (ntask > 0)
+        hipLaunchKernelGGL(k_geom_cands<false>, dim3(g), dim3(256), 0, c->stream, G, cnt,
+                           (const int *)nullptr, (CandRec *)nullptr, err);
+    HIPCHK(hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIPCHK(int_exclusive_scan(cnt, c->cand_off, ntask + 1, nullptr, &tmp_bytes, c->stream));
+    char *tmp = nullptr;
+    RCHK(dev_alloc(c, &tmp, tmp_bytes));
+    HIPCHK(int_exclusive_scan(cnt, c->cand_off, ntask + 1, tmp, &tmp_bytes, c->stream));
+    int herr = 0, total = 0;
+    HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&total, c->cand_off + ntask, sizeof(int), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, cnt);
+    dev_release(c, tmp);
+    dev_release(c, err);
+    if (herr)
+        return fail(IMAGO_E_LATTICE, "More lattice points needed for the potential sites near "
+                                     "some atom pair: extend the cell list");
+    if (total < 0) return fail(IMAGO_E_ARG, "more than 2^31 candidate images");
+    c->ncands = total;
+    RCHK(dev_alloc(c, &c->cands, (size_t)std::max(total, 1)));
+    if (ntask > 0)
+        hipLaunchKernelGGL(k_geom_cands<true>, dim3(g), dim3(256), 0, c->stream, G, (int *)nullptr,
+                           c->cand_off, c->cands, (int *)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IMAGO_OK;
 }
 
 static int build_geometry(imago_ctx *c)
@@ -625,19 +669,17 @@ static int build_geometry(imago_ctx *c)
     std::vector<PairRec> pairs;
     std::vector<ImgRec> imgs;
     std::vector<double2> phase;
-    std::vector<CandRec> cands;
-    std::vector<int> cand_off;
+    std::vector<int> img_pair;
     for (auto &g : parts) {
         if (g.rc != IMAGO_OK) return fail(g.rc, "%s", g.err.c_str());
-        const int ib = (int)imgs.size(), cb = (int)cands.size();
+        const int ib = (int)imgs.size(), pb = (int)pairs.size();
         for (PairRec pr : g.pairs) {
             pr.img_off += ib;
             pairs.push_back(pr);
         }
         imgs.insert(imgs.end(), g.imgs.begin(), g.imgs.end());
         phase.insert(phase.end(), g.phase.begin(), g.phase.end());
-        cands.insert(cands.end(), g.cands.begin(), g.cands.end());
-        for (int o : g.cand_off) cand_off.push_back(o + cb);
+        for (int q : g.img_pair) img_pair.push_back(q + pb);
     }
     c->npairs = (int)pairs.size();
     // pairs ordered by column atom j, then row atom i (the packed layout's
@@ -680,21 +722,15 @@ static int build_geometry(imago_ctx *c)
     }
     c->hpairs = pairs;
     c->nimgs = (long)imgs.size();
-    c->ncands = (long)cands.size();
     RCHK(dev_alloc(c, &c->pairs, pairs.size()));
     RCHK(dev_alloc(c, &c->imgs, imgs.size()));
     RCHK(dev_alloc(c, &c->phase, phase.size()));
-    RCHK(dev_alloc(c, &c->cands, cands.size()));
-    RCHK(dev_alloc(c, &c->cand_off, cand_off.size()));
     HIPCHK(hipMemcpy(c->pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
     if (!imgs.empty()) {
         HIPCHK(hipMemcpy(c->imgs, imgs.data(), imgs.size() * sizeof(ImgRec), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->phase, phase.data(), phase.size() * sizeof(double2), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->cand_off, cand_off.data(), cand_off.size() * sizeof(int),
-                         hipMemcpyHostToDevice));
     }
-    if (!cands.empty())
-        HIPCHK(hipMemcpy(c->cands, cands.data(), cands.size() * sizeof(CandRec), hipMemcpyHostToDevice));
+    RCHK(build_candidates(c, img_pair));
     return IMAGO_OK;
 }
 
@@ -738,6 +774,7 @@ extern "C" int imago_ctx_destroy(imago_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     ctx_free_system(c);
+    c->sysbuf.purge_cache();
     if (c->blas) rocblas_destroy_handle(c->blas);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -751,6 +788,8 @@ extern "C" int imago_set_system(imago_ctx *c, const imago_system_desc *s, int *v
                                 int *pot_dim)
 {
     if (!c) return fail(IMAGO_E_ARG, "null context");
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     ctx_free_system(c);
@@ -803,13 +842,17 @@ extern "C" int imago_set_system(imago_ctx *c, const imago_system_desc *s, int *v
         RCHK(dev_upload(c, &tt0, h.type_term0));
         c->type_term0 = tt0;
     }
+    const auto t1 = clk::now();
     RCHK(build_geometry(c));
+    const auto t2 = clk::now();
+    c->timing[7] = std::chrono::duration<double, std::milli>(t2 - t1).count();
     int lmax = 0;
     for (int t = 0; t < h.nt; ++t)
         for (int a = 0; a < h.noa[4 * t]; ++a) lmax = std::max(lmax, lma_to_l_host(h.lma[t * h.maxa + a]));
     c->lmax = lmax;
     c->ne = (h.maxs * h.maxs + 63) / 64;
     c->has_system = true;
+    c->timing[6] = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
     if (vale_dim) *vale_dim = h.n;
     if (pot_dim) *pot_dim = h.pot_dim;
     return IMAGO_OK;
@@ -829,12 +872,6 @@ static AssembleLaunch make_launch(imago_ctx *c, double *out, long term_stride, i
                           c->ep_nitems, h.nk};
 }
 
-static void dev_release(imago_ctx *c, void *p)
-{
-    auto &v = c->sysbuf.ptrs;
-    v.erase(std::remove(v.begin(), v.end(), p), v.end());
-    (void)hipFree(p);
-}
 
 // Screening of the EP terms (k_ep_screen) and their primitive blocks
 // (k_ep_prims): count the tuples and items of every (potential type, pair),
@@ -1521,12 +1558,8 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
         for (int q = 0; q < 3; ++q) traces3[q] = res[q];
     if (pot_rho)
         for (int j = 0; j < h.pot_dim; ++j) pot_rho[j] = res[3 + j];
-    // release the temporaries (last two allocations)
-    for (void *p : {(void *)W, (void *)outd}) {
-        auto &v = c->sysbuf.ptrs;
-        v.erase(std::remove(v.begin(), v.end(), p), v.end());
-        (void)hipFree(p);
-    }
+    dev_release(c, W);
+    dev_release(c, outd);
     return IMAGO_OK;
 }
 
@@ -1581,7 +1614,7 @@ extern "C" int imago_dsygv(imago_ctx *c, int n, double *A, double *B, double *w,
 
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 5) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 7) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

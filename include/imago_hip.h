@@ -197,7 +197,8 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * the last imago_secular, 2 = its eigensolve, 3 = last density+traces,
  * 4 = the EP panel store of the last IMAGO_EP assembly, 5 = the EP
  * screening and primitive blocks of the last EP assembly that built them
- * (0 when they were already built for the system). */
+ * (0 when they were already built for the system), 6 = host wall time of
+ * the last imago_set_system, 7 = its lattice-image / candidate part. */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 #ifdef __cplusplus
