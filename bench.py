@@ -56,6 +56,20 @@ def parse():
     return p.parse_args()
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of kernel from its committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's counts), or
+    None when there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", "%s_si64_pmc.md" % kernel)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def calibrate_copy():
     import torch
     n = 1 << 29   # 2^29 doubles = 4 GiB
@@ -88,10 +102,14 @@ def gpu_step(ctx, system, coeffs, num_states, oracle_populate, ep_kind):
     ev = ctx.secular(coeffs, num_states)
     stage["hbuild"] = ctx.timing(1)
     stage["eigensolve"] = ctx.timing(2)
+    if ep_kind == "ep_direct":
+        stage["k_epd_hsum"] = ctx.timing(8)
     t3 = time.perf_counter()
     occ = oracle_populate(ev, system.kpoint_weights, electrons(system))
     tr, prho = ctx.density_traces(occ)
     stage["traces"] = ctx.timing(3)
+    if ep_kind == "ep_direct":
+        stage["k_epd_trace"] = ctx.timing(9)
     t4 = time.perf_counter()
     stage["set_system_wall"] = (t1 - t0) * 1e3
     stage["assemble_wall"] = (t2 - t1) * 1e3
@@ -238,24 +256,43 @@ def main():
     step_s = wall / args.steps
     st = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
     n = system.vale_dim
-    npack = n * (n + 1) // 2
-    ep_bytes = system.pot_dim * system.num_kpoints * npack * 16.0
-    ep_ms = st["assemble_" + ep_kind]
-    hb_bytes = (system.pot_dim + 3) * system.num_kpoints * npack * 16.0
-    eig_flop = 32.0 * n ** 3 * system.num_kpoints
-    kernels = {
-        "assemble_ep": {"bound": "hbm", "achieved": ep_bytes / (ep_ms * 1e-3) / 1e9,
-                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "ms": ep_ms},
-        "hbuild": {"bound": "hbm", "achieved": hb_bytes / (st["hbuild"] * 1e-3) / 1e9,
-                   "peak": PEAK_HBM_GBS, "unit": "GB/s", "ms": st["hbuild"]},
-        "eigensolve": {"bound": "mfma", "achieved": eig_flop / (st["eigensolve"] * 1e-3) / 1e12,
-                       "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "ms": st["eigensolve"]},
-    }
+    nk = system.num_kpoints
+    ns = n // system.num_atoms
+    npairs = system.num_atoms * (system.num_atoms + 1) // 2
+    nbe = npairs * ns * ns                       # block-layout entries per k
+    per = 2 * nbe * nk                           # doubles of one complex panel
+    ntup, nitems, bsz = ctx.ep_counts()
+    ng = -(-len(system.pot_types) // 8)          # groups of 8 potential types in the H sum
+    kernels = {}
+    if ep_kind == "ep_direct":
+        kernels["k_epd_hsum"] = {
+            "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "ms": st["k_epd_hsum"],
+            # every primitive block and tuple read once, the groups' partial H written
+            "bytes": nitems * bsz * 8.0 + ntup * 24.0 + ng * per * 8.0}
+        kernels["k_epd_trace"] = {
+            "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "ms": st["k_epd_trace"],
+            "bytes": nitems * bsz * 8.0 + ntup * 24.0 + per * 8.0}
+    else:
+        kernels["k_ep_store"] = {
+            "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "ms": st["ep_store"],
+            "bytes": system.pot_dim * per * 8.0 + nitems * bsz * 8.0 + ntup * 24.0}
+        kernels["k_hbuild"] = {
+            "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "ms": st["hbuild"],
+            "bytes": (system.pot_dim + 3) * per * 8.0}
+    for v in kernels.values():
+        v["achieved"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+    stage_flops = {"eigensolve": {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS,
+                                  "ms": st["eigensolve"], "flop": 32.0 * n ** 3 * nk}}
+    for v in stage_flops.values():
+        v["achieved"] = v["flop"] / (v["ms"] * 1e-3) / 1e12
+    # the roofline line is the kernel of this library that takes the most
+    # time per step; the eigensolve (a rocSOLVER/rocBLAS pipeline of many
+    # launches) is reported as a stage beside it
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     d = kernels[dom]
     roofline = {"kernel": dom, "bound": d["bound"], "achieved": round(d["achieved"], 3),
                 "peak": d["peak"], "unit": d["unit"], "frac": round(d["achieved"] / d["peak"], 5),
-                "traffic": None}
+                "traffic": profiled_traffic(dom)}
     cpu = None
     if not args.no_cpu:
         log("cpu baseline leg (%d EP terms)" % args.cpu_terms)
@@ -283,10 +320,10 @@ def main():
         "data": "synthetic (seeded Si64 supercell, random-normalised contractions)",
         "config": {"workload": "si64_supercell: 64 atoms, n=896, 4 k-points, potDim=1024, "
                                "cutoff 16 bohr, one hot-path pass per step",
-                   "parallelism": "single GPU"},
+                   "ep_mode": args.ep_mode, "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 3) for k, v in st.items()},
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
-                        for kk, vv in v.items()} for k, v in kernels.items()},
+                        for kk, vv in v.items()} for k, v in {**kernels, **stage_flops}.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "trace_check": {"tr_rho_S": float(tr[0]), "electrons": electrons(system)},

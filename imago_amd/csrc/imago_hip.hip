@@ -138,6 +138,7 @@ struct imago_ctx {
     // block layout of the panels: each pair's block (row states x column
     // states, column-major) back to back; nbe entries per k-point
     long nbe = 0;
+    int max_img = 0;   // largest lattice-image count of a pair
     std::vector<PairRec> hpairs;
     const int *atom_of_state = nullptr;   // [n]
     const int *pair_index = nullptr;      // [natoms][natoms]
@@ -164,6 +165,7 @@ struct imago_ctx {
     // block of every (tuple, exponent) item, built once per system
     EpTuple *ep_tuples = nullptr;
     long *ep_tseg = nullptr;     // [types * pairs + 1] offsets into ep_tuples
+    long *ep_iseg = nullptr;     // [types * pairs + 1] first item of each task
     int *ep_item_tuple = nullptr;
     double *ep_blocks = nullptr;
     long ep_ntuples = 0, ep_nitems = 0;
@@ -174,8 +176,8 @@ struct imago_ctx {
     double *coef_eff = nullptr;   // [pot_dim]
     double *hpartial = nullptr;   // [groups][panel]
     double *tpart = nullptr;      // [pairs][pot_dim]
-    double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    double timing[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -188,6 +190,7 @@ static void ctx_free_system(imago_ctx *c)
     c->chol_valid = false;
     c->ep_tuples = nullptr;
     c->ep_tseg = nullptr;
+    c->ep_iseg = nullptr;
     c->ep_item_tuple = nullptr;
     c->ep_blocks = nullptr;
     c->ep_ntuples = c->ep_nitems = 0;
@@ -721,6 +724,8 @@ static int build_geometry(imago_ctx *c)
         c->pair_index = pi;
     }
     c->hpairs = pairs;
+    c->max_img = 0;
+    for (const PairRec &pr : pairs) c->max_img = std::max(c->max_img, pr.img_cnt);
     c->nimgs = (long)imgs.size();
     RCHK(dev_alloc(c, &c->pairs, pairs.size()));
     RCHK(dev_alloc(c, &c->imgs, imgs.size()));
@@ -754,6 +759,7 @@ extern "C" int imago_ctx_create(int device, imago_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev3);
     if (e != hipSuccess) {
         delete c;
         return fail(IMAGO_E_HIP, "context set-up failed: %s", hipGetErrorString(e));
@@ -779,6 +785,7 @@ extern "C" int imago_ctx_destroy(imago_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return IMAGO_OK;
@@ -920,7 +927,7 @@ static int ep_prepare(imago_ctx *c)
     HIPCHK(launch_ep_prims(L));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
-    dev_release(c, ioff);
+    c->ep_iseg = ioff;
     c->ep_screened = true;
     return IMAGO_OK;
 }
@@ -1252,10 +1259,12 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     if (direct && !c->hpartial) RCHK(dev_alloc(c, &c->hpartial, (size_t)per * ngroups));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (direct) {
-        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg, c->ep_blocks, c->coef_eff,
+        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg,
+                    c->ep_blocks, c->coef_eff,
                     nullptr, c->type_term0, c->pair_order, c->npairs, ngroups, c->hpartial, per,
                     nullptr, c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
         HIPCHK(launch_epd_hsum(L));
+        HIPCHK(hipEventRecord(c->ev2, c->stream));
         HIPCHK(launch_epd_hfinal(c->panel[2], c->panel[1], c->hpartial, ngroups, per, c->blockH,
                                  c->stream));
     } else {
@@ -1307,6 +1316,11 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms1 = 0, ms2 = 0;
     HIPCHK(hipEventElapsedTime(&ms1, c->ev0, c->ev1));
+    if (direct) {
+        float ms8 = 0;
+        HIPCHK(hipEventElapsedTime(&ms8, c->ev0, c->ev2));
+        c->timing[8] = ms8;
+    }
     HIPCHK(hipEventElapsedTime(&ms2, e2, e3));
     (void)hipEventDestroy(e2);
     (void)hipEventDestroy(e3);
@@ -1538,10 +1552,13 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
                            c->partial + (size_t)3 * nblk);
     if (h.pot_dim > 0 && direct) {
         if (!c->tpart) RCHK(dev_alloc(c, &c->tpart, (size_t)c->npairs * h.pot_dim));
-        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg, c->ep_blocks, nullptr,
+        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg,
+                    c->ep_blocks, nullptr,
                     c->rhow, c->type_term0, c->pair_order, c->npairs, 0, nullptr, 0, c->tpart,
                     c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
+        HIPCHK(hipEventRecord(c->ev2, c->stream));
         HIPCHK(launch_epd_trace(L));
+        HIPCHK(hipEventRecord(c->ev3, c->stream));
         HIPCHK(launch_epd_trace_final(c->tpart, c->npairs, h.pot_dim, outd + 3, c->stream));
     }
     hipLaunchKernelGGL(k_trace_final, dim3((nterms + 255) / 256), dim3(256), 0, c->stream, c->partial,
@@ -1552,6 +1569,11 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[3] = ms;
+    if (h.pot_dim > 0 && direct) {
+        float ms9 = 0;
+        HIPCHK(hipEventElapsedTime(&ms9, c->ev2, c->ev3));
+        c->timing[9] = ms9;
+    }
     std::vector<double> res(nterms);
     HIPCHK(hipMemcpy(res.data(), outd, sizeof(double) * nterms, hipMemcpyDeviceToHost));
     if (traces3)
@@ -1612,9 +1634,20 @@ extern "C" int imago_dsygv(imago_ctx *c, int n, double *A, double *B, double *w,
     return single_gv(c, false, n, A, B, w, info);
 }
 
+extern "C" int imago_ep_counts(imago_ctx *c, long *tuples, long *items, int *block_doubles)
+{
+    if (!c || !c->has_system) return fail(IMAGO_E_STATE, "no system");
+    if (!c->ep_screened) return fail(IMAGO_E_STATE, "EP terms not assembled");
+    const int sh = (c->lmax + 1) * (c->lmax + 1);
+    if (tuples) *tuples = c->ep_ntuples;
+    if (items) *items = c->ep_nitems;
+    if (block_doubles) *block_doubles = sh * sh;
+    return IMAGO_OK;
+}
+
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 7) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 9) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

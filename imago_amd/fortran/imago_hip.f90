@@ -14,7 +14,7 @@ module O_ImagoHIP
 
    integer(c_int), parameter :: IMAGO_OK = 0
    integer(c_int), parameter :: IMAGO_OL = 1, IMAGO_KE = 2, IMAGO_NP = 3, &
-         & IMAGO_EP = 4
+         & IMAGO_EP = 4, IMAGO_EP_DIRECT = 5
 
    ! imago_system_desc (include/imago_hip.h), field for field
    type, bind(C) :: imago_system_desc
@@ -153,6 +153,14 @@ module O_ImagoHIP
          integer(c_int), value :: stage
          real(c_double), intent(out) :: ms
       end function imago_last_timing
+
+      integer(c_int) function imago_ep_counts(ctx, tuples, items, block_doubles) &
+            & bind(C, name="imago_ep_counts")
+         import :: c_int, c_ptr, c_long
+         type(c_ptr), value :: ctx
+         integer(c_long), intent(out) :: tuples, items
+         integer(c_int), intent(out) :: block_doubles
+      end function imago_ep_counts
    end interface
 
 contains

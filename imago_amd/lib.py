@@ -38,7 +38,7 @@ EXPORTS = [
     "imago_ctx_create", "imago_ctx_destroy", "imago_last_error", "imago_set_system",
     "imago_primitive_batch", "imago_boys_batch", "imago_assemble", "imago_get_packed",
     "imago_secular", "imago_get_eigenvectors", "imago_density_traces", "imago_zhegv",
-    "imago_dsygv", "imago_last_timing",
+    "imago_dsygv", "imago_last_timing", "imago_ep_counts",
 ]
 
 
@@ -77,6 +77,8 @@ def load():
     L.imago_zhegv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _ip]
     L.imago_dsygv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _ip]
     L.imago_last_timing.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.imago_ep_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long),
+                                  ctypes.POINTER(ctypes.c_long), _ip]
     _lib = L
     return L
 
@@ -201,6 +203,12 @@ class Context:
         v = ctypes.c_double()
         _check(load().imago_last_timing(self._h, stage, ctypes.byref(v)))
         return v.value
+
+    def ep_counts(self):
+        """(tuples, items, doubles per primitive block) of the EP screening."""
+        t, i, b = ctypes.c_long(), ctypes.c_long(), ctypes.c_int()
+        _check(load().imago_ep_counts(self._h, ctypes.byref(t), ctypes.byref(i), ctypes.byref(b)))
+        return t.value, i.value, b.value
 
     # -- single problems ----------------------------------------------------
     def hegv(self, H, S):

@@ -198,8 +198,16 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * 4 = the EP panel store of the last IMAGO_EP assembly, 5 = the EP
  * screening and primitive blocks of the last EP assembly that built them
  * (0 when they were already built for the system), 6 = host wall time of
- * the last imago_set_system, 7 = its lattice-image / candidate part. */
+ * the last imago_set_system, 7 = its lattice-image / candidate part,
+ * 8 = the EP sum of the last direct H build, 9 = the EP traces of the last
+ * direct imago_density_traces. */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
+
+/* Sizes of the EP screening of the current system (after an IMAGO_EP or
+ * IMAGO_EP_DIRECT assembly): screened (image, alpha pair, candidate)
+ * tuples, (tuple, exponent) primitive items, and doubles per stored
+ * primitive block.  For measurement (bytes moved per launch). */
+int imago_ep_counts(imago_ctx *ctx, long *tuples, long *items, int *block_doubles);
 
 #ifdef __cplusplus
 }
