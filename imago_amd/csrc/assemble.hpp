@@ -48,7 +48,7 @@ struct CandRec {
 
 struct DevSys {
     int num_types, max_alphas, max_states, num_elements, max_pot_alphas;
-    int num_atoms, num_pot_sites, num_pot_types, num_cells, nk, vale_dim, pot_dim;
+    int num_atoms, num_pot_sites, num_pot_types, num_cells, nk, vale_dim, pot_dim, gamma;
     long nbe;         // block-layout entries per k-point
     const double *type_alphas;    // [t][a]
     const double *type_basis;     // [t][s][a]

@@ -41,35 +41,50 @@ hipError_t launch_ep_screen(const AssembleLaunch &L, bool fill, long *tseg, long
                             size_t lds);
 hipError_t launch_ep_prims(const AssembleLaunch &L);
 
-// direct (panel-free) EP contributions, asm_epd.hip; H is summed over groups
-// of this many potential types before the final addition
-constexpr int kEpdTypesPerGroup = 8;
+// direct (panel-free) EP contributions, asm_epd.hip
 
+// A run of items of one atom pair: potential types [t_lo, t_hi), starting
+// at item i_lo within type t_lo and ending at item i_hi within type
+// t_hi - 1.  The host cuts the pairs' items into runs of at most
+// kEpdRunItems, so an on-site pair with thousands of items does not hold
+// one wavefront for the length of the whole launch.
+constexpr int kEpdRunItems = 128;
+struct EpdSub {
+    int pair, t_lo, t_hi, pad;
+    long i_lo, i_hi;
+    long out;   // H sum: offset (doubles) of the run's partial block; traces: its row
+};
 struct EpdLaunch {
     DevSys S;
     const PairRec *pairs;
     const double2 *phase;
     const EpTuple *tuples;
-    const long *tseg;
+    const long *tseg;         // [types * pairs + 1] first tuple of each task
+    const long *iseg;         // [types * pairs + 1] first item of each task
     const double *blocks;
     const double *coef_eff;   // hsum: per term, 0.0 replaced by 1.0
     const double *rw;         // trace: weighted density, block layout
     const int *type_term0;
-    const int *order;
-    int npairs, ngroups;
-    double *partial;          // hsum: [ngroups][group_stride]
-    long group_stride;
-    double *tpart;            // trace: [npairs][pot_dim]
-    int lmax, ne;
+    const EpdSub *subs;       // runs of items, one wavefront each
+    long nsubs;
+    double *out;              // hsum: run partial blocks; trace: [runs][max_pot_alphas]
+    int npairs, lmax, ne;
     bool gamma;
     hipStream_t stream;
 };
 hipError_t launch_epd_hsum(const EpdLaunch &L);
 hipError_t launch_epd_trace(const EpdLaunch &L);
-hipError_t launch_epd_hfinal(const double *np, const double *ke, const double *partial, int ngroups,
-                             long len, double *H, hipStream_t stream);
-hipError_t launch_epd_trace_final(const double *tpart, int npairs, int pot_dim, double *out,
-                                  hipStream_t stream);
+hipError_t launch_epd_hfinal(const EpdLaunch &L, const double *np, const double *ke,
+                             const long *psub, double *H);
+hipError_t launch_epd_trace_final(const double *tsub, const long *roff, int npairs,
+                                  const int *term_type, const int *term_alpha, int max_pot_alphas,
+                                  int pot_dim, double *out, hipStream_t stream);
+hipError_t launch_epd_trace_runs(bool fill, const long *iseg, long ntask, int npairs, long C,
+                                 long *rcnt, const long *roff, EpdSub *runs, hipStream_t stream);
+hipError_t launch_epd_hsum_runs(bool fill, const long *iseg, int npairs, int ntypes, long C,
+                                const PairRec *pairs, const DevSys &S, long *rcnt, long *wcnt,
+                                const long *roff, const long *woff, EpdSub *runs,
+                                hipStream_t stream);
 // exclusive prefix sum of n longs (tmp == nullptr: only sets *tmp_bytes)
 hipError_t ep_exclusive_scan(const long *in, long *out, long n, void *tmp, size_t *tmp_bytes,
                              hipStream_t stream);

@@ -10,16 +10,16 @@
 // (integrals.F90:2860-3605).  Here both per-iteration quantities come from
 // the primitive blocks of k_ep_prims, once per item:
 //
-//   k_epd_hsum   one wavefront per (pair, group of potential types):
-//                sum_items c'_j(item) * C1 C2 block, summed per image and
-//                Bloch-phased, written as the group's partial block;
-//                k_epd_hfinal adds NP, KE and the groups' partials.
-//   k_epd_trace  one wavefront per (pair, potential type): for each image
-//                X_e = sum_k Re(phase_k conj(rw_k,e)) with rw the weighted
-//                density of the block layout, and per item
-//                sum_e C1 C2 block_e X_e added to its term; per-(pair, term)
-//                partials are summed over pairs in a fixed order by
-//                k_epd_trace_final.
+//   k_epd_hsum   one wavefront per run of at most a fixed number of a
+//                pair's items (EpdSub): sum_items c'_j(item) * C1 C2 block,
+//                summed per image and Bloch-phased, written as the run's
+//                partial block; k_epd_hfinal adds NP, KE and the pair's
+//                partials in run order.
+//   k_epd_trace  one wavefront per run of one (pair, potential type): for
+//                each image X_e = sum_k Re(phase_k conj(rw_k,e)) with rw the
+//                weighted density of the block layout, and per item
+//                sum_e C1 C2 block_e X_e added to its term; the runs' sums
+//                are added per term in a fixed order by k_epd_trace_final.
 // Both require at most 4 k-points (or Gamma) so Q(k) and rw(k) of a lane's
 // entries stay in registers.
 #pragma once
@@ -58,28 +58,131 @@ __device__ __forceinline__ void epd_tuple_fields(const EpTuple &t, int src, int 
     mask = ((unsigned long long)hi << 32) | lo;
 }
 
-// partial[grp][k][nbe] (complex, or real for Gamma) of one pair's block:
-// sum over the group's potential terms of c'_j V_j(k).
+// Run tables on the device.  Traces: every (type, pair) task cut into
+// runs of at most C items, runs in task order (type-major), so the runs of
+// type t start at roff[t * npairs].  count: rcnt[task]; fill: the runs.
+template <bool FILL>
+__global__ void k_epd_trace_runs(const long *__restrict__ iseg, long ntask, int npairs, long C,
+                                 long *__restrict__ rcnt, const long *__restrict__ roff,
+                                 EpdSub *__restrict__ runs)
+{
+    const long task = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (task >= ntask) return;
+    const long a = iseg[task], b = iseg[task + 1];
+    const long n = (b - a + C - 1) / C;
+    if (!FILL) {
+        rcnt[task] = n;
+        return;
+    }
+    const int p = (int)(task % npairs), t = (int)(task / npairs);
+    for (long r = 0; r < n; ++r) {
+        const long q = roff[task] + r;
+        runs[q] = EpdSub{p, t, t + 1, 0, a + r * C, min(b, a + (r + 1) * C), q};
+    }
+}
+
+// H sum: each pair's items over all types cut in order into runs of at
+// most C items (a pair without items gets one empty run).  count:
+// rcnt[p] = runs, wcnt[p] = runs * block doubles; fill: the runs with
+// their partial offsets from woff.
+template <bool FILL>
+__global__ void k_epd_hsum_runs(const long *__restrict__ iseg, int npairs, int ntypes, long C,
+                                const PairRec *__restrict__ pairs, const DevSys S,
+                                long *__restrict__ rcnt, long *__restrict__ wcnt,
+                                const long *__restrict__ roff, const long *__restrict__ woff,
+                                EpdSub *__restrict__ runs)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const PairRec pr = pairs[p];
+    const long blk = (long)S.type_nstates[S.atom_type[pr.i]] * S.type_nstates[S.atom_type[pr.j]] *
+                     (S.nk) * (S.gamma ? 1 : 2);
+    long nr = 0, acc = 0;
+    bool open = false;
+    EpdSub cur{p, 0, 0, 0, 0, 0, 0};
+    const long r0 = FILL ? roff[p] : 0, w0 = FILL ? woff[p] : 0;
+    for (int t = 0; t < ntypes; ++t) {
+        long a = iseg[(long)t * npairs + p];
+        const long b = iseg[(long)t * npairs + p + 1];
+        while (a < b) {
+            if (!open) {
+                cur = EpdSub{p, t, t + 1, 0, a, b, w0 + nr * blk};
+                open = true;
+                acc = 0;
+            }
+            const long take = min(b - a, C - acc);
+            cur.t_hi = t + 1;
+            cur.i_hi = a + take;
+            acc += take;
+            a += take;
+            if (acc == C) {
+                if (FILL) runs[r0 + nr] = cur;
+                ++nr;
+                open = false;
+            }
+        }
+    }
+    if (open) {
+        if (FILL) runs[r0 + nr] = cur;
+        ++nr;
+    }
+    if (nr == 0) {
+        if (FILL) runs[r0] = EpdSub{p, 0, 0, 0, 0, 0, w0};
+        nr = 1;
+    }
+    if (!FILL) {
+        rcnt[p] = nr;
+        wcnt[p] = nr * blk;
+    }
+}
+
+// Walks, in order, the items of task [t0, t1) tuples that fall in [a, b):
+// f(image, a1, a2, exponent, item).  Tuples are in image order and their
+// items are numbered in tuple order, exponents ascending.
+template <class F>
+__device__ __forceinline__ void epd_walk(const EpTuple *__restrict__ tuples, long t0, long t1,
+                                         long a, long b, F &&f)
+{
+    const int lane = threadIdx.x & 63;
+    for (long tb = t0; tb < t1; tb += 64) {
+        EpTuple t{};
+        if (tb + lane < t1) t = tuples[tb + lane];
+        unsigned long long m = __ballot(tb + lane < t1);
+        while (m) {
+            const int src = __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            int key, item0;
+            unsigned long long mm;
+            epd_tuple_fields(t, src, key, item0, mm);
+            if (item0 >= b) return;
+            if (item0 + __popcll(mm) <= a) continue;
+            f(key >> 16, (key >> 8) & 255, key & 255, item0, mm);
+        }
+    }
+}
+
+// part[sub.out ...] = [k][nent] (complex, or real for Gamma): the run's
+// sum of c'_j V_j(k) over its items, on its pair's block.
 template <int LMAX, int NE, bool GAMMA>
 __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__restrict__ pairs,
                                                  const double2 *__restrict__ phase,
                                                  const EpTuple *__restrict__ tuples,
                                                  const long *__restrict__ tseg,
+                                                 const long *__restrict__ iseg,
                                                  const double *__restrict__ blocks,
                                                  const double *__restrict__ coef_eff,
                                                  const int *__restrict__ type_term0,
-                                                 const int *__restrict__ order, int npairs,
-                                                 int ngroups, double *__restrict__ partial,
-                                                 long group_stride)
+                                                 const EpdSub *__restrict__ subs, long nsubs,
+                                                 int npairs, double *__restrict__ part)
 {
     constexpr int SH = (LMAX + 1) * (LMAX + 1);
     constexpr int BS = SH * SH;
     constexpr int NQ = GAMMA ? 1 : 4;
     const int lane = threadIdx.x;
-    const long logical = blockIdx.x;
-    if (logical >= (long)npairs * ngroups) return;
-    const int grp = (int)(logical % ngroups);
-    const int pidx = order[logical / ngroups];
+    const long sidx = blockIdx.x;
+    if (sidx >= nsubs) return;
+    const EpdSub sub = subs[sidx];
+    const int pidx = sub.pair;
     const PairRec pr = pairs[pidx];
     const int t1 = S.atom_type[pr.i], t2 = S.atom_type[pr.j];
     const int ns1 = S.type_nstates[t1], ns2 = S.type_nstates[t2];
@@ -121,85 +224,95 @@ __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__rest
         cur_img = -1;
     };
 
-    const int pt_lo = grp * kEpdTypesPerGroup;
-    const int pt_hi = min(S.num_pot_types, pt_lo + kEpdTypesPerGroup);
-    for (int pt = pt_lo; pt < pt_hi; ++pt) {
+    for (int pt = sub.t_lo; pt < sub.t_hi; ++pt) {
         const long task = (long)pt * npairs + pidx;
-        const long toff = tseg[task];
-        const int cnt = (int)(tseg[task + 1] - toff);
+        long a = iseg[task], b = iseg[task + 1];
+        if (pt == sub.t_lo && sub.i_lo > a) a = sub.i_lo;
+        if (pt == sub.t_hi - 1 && sub.i_hi < b) b = sub.i_hi;
+        if (a >= b) continue;
         const double *cf = coef_eff + type_term0[pt];
         flush();
-        for (int b = 0; b < cnt; b += 64) {
-            EpTuple t{};
-            if (b + lane < cnt) t = tuples[toff + b + lane];
-            unsigned long long m = __ballot(b + lane < cnt);
-            while (m) {
-                const int src = __ffsll((long long)m) - 1;
-                m &= m - 1ull;
-                int key, item;
-                unsigned long long mm;
-                epd_tuple_fields(t, src, key, item, mm);
-                const int im = key >> 16, a1 = (key >> 8) & 255, a2 = key & 255;
-                if (im != cur_img) {
-                    flush();
-                    cur_img = im;
-                }
-                double cc[NE];
+        epd_walk(tuples, tseg[task], tseg[task + 1], a, b,
+                 [&](int im, int a1, int a2, int item0, unsigned long long mm) {
+                     if (im != cur_img) {
+                         flush();
+                         cur_img = im;
+                     }
+                     double cc[NE];
 #pragma unroll
-                for (int e = 0; e < NE; ++e) cc[e] = B1[E.em1[e] * MA + a1] * B2[E.em2[e] * MA + a2];
-                // two items at a time so their block loads overlap
-                while (mm) {
-                    const int ja = __ffsll((long long)mm) - 1;
-                    mm &= mm - 1ull;
-                    const bool two = mm != 0ull;
-                    const int jb = two ? __ffsll((long long)mm) - 1 : ja;
-                    if (two) mm &= mm - 1ull;
-                    const double *ba = blocks + (size_t)item * BS;
-                    const double *bb = blocks + (size_t)(item + (two ? 1 : 0)) * BS;
-                    const double ca = cf[ja], cb = two ? cf[jb] : 0.0;
-                    double va[NE], vb[NE];
+                     for (int e = 0; e < NE; ++e)
+                         cc[e] = B1[E.em1[e] * MA + a1] * B2[E.em2[e] * MA + a2];
+                     // items of this tuple inside [a, b), two at a time so
+                     // their block loads overlap
+                     long it = item0;
+                     while (mm) {
+                         const int ja = __ffsll((long long)mm) - 1;
+                         mm &= mm - 1ull;
+                         const long ia = it++;
+                         if (ia < a) continue;
+                         if (ia >= b) break;
+                         const bool two = mm != 0ull && it < b;
+                         const int jb = two ? __ffsll((long long)mm) - 1 : ja;
+                         if (two) {
+                             mm &= mm - 1ull;
+                             ++it;
+                         }
+                         const double *ba = blocks + (size_t)ia * BS;
+                         const double *bb = blocks + (size_t)(ia + (two ? 1 : 0)) * BS;
+                         const double ca = cf[ja], cb = two ? cf[jb] : 0.0;
+                         double va[NE], vb[NE];
 #pragma unroll
-                    for (int e = 0; e < NE; ++e) {
-                        va[e] = ba[E.eo[e]];
-                        vb[e] = bb[E.eo[e]];
-                    }
+                         for (int e = 0; e < NE; ++e) {
+                             va[e] = ba[E.eo[e]];
+                             vb[e] = bb[E.eo[e]];
+                         }
 #pragma unroll
-                    for (int e = 0; e < NE; ++e) P[e] += cc[e] * (ca * va[e] + cb * vb[e]);
-                    item += two ? 2 : 1;
-                }
-            }
-        }
+                         for (int e = 0; e < NE; ++e) P[e] += cc[e] * (ca * va[e] + cb * vb[e]);
+                     }
+                 });
     }
     flush();
-    double *o = partial + (size_t)grp * group_stride;
+    double *o = part + sub.out;
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
         if (E.est[e] == 0) continue;
         const bool zero = E.est[e] == 2;
-        const long idx = (long)pr.blk + lane + 64 * e;
+        const int ee = lane + 64 * e;
         if (GAMMA) {
-            o[idx] = zero ? 0.0 : Q[e][0].x;
+            o[ee] = zero ? 0.0 : Q[e][0].x;
         } else {
 #pragma unroll
             for (int k = 0; k < NQ; ++k)
                 if (k < nk)
-                    reinterpret_cast<double2 *>(o)[(size_t)k * S.nbe + idx] =
+                    reinterpret_cast<double2 *>(o)[(size_t)k * nent + ee] =
                         zero ? make_double2(0.0, 0.0) : Q[e][k];
         }
     }
 }
 
-// H = NP + KE + sum of the groups' partials, elementwise over the block layout
-__global__ __launch_bounds__(256) void k_epd_hfinal(const double *__restrict__ np,
+// H = NP + KE + the pair's run partials, one work-group per pair; the runs
+// of pair p are subs[psub[p] .. psub[p + 1]) and are added in that order.
+template <bool GAMMA>
+__global__ __launch_bounds__(256) void k_epd_hfinal(const DevSys S, const PairRec *__restrict__ pairs,
+                                                    const double *__restrict__ np,
                                                     const double *__restrict__ ke,
-                                                    const double *__restrict__ partial, int ngroups,
-                                                    long len, double *__restrict__ H)
+                                                    const double *__restrict__ part,
+                                                    const EpdSub *__restrict__ subs,
+                                                    const long *__restrict__ psub,
+                                                    double *__restrict__ H)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= len) return;
-    double a = np[e] + ke[e];
-    for (int g = 0; g < ngroups; ++g) a += partial[(size_t)g * len + e];
-    H[e] = a;
+    const int p = blockIdx.x;
+    const PairRec pr = pairs[p];
+    const int ns1 = S.type_nstates[S.atom_type[pr.i]], ns2 = S.type_nstates[S.atom_type[pr.j]];
+    const int nent = ns1 * ns2, nk = GAMMA ? 1 : S.nk, dz = GAMMA ? 1 : 2;
+    const long s0 = psub[p], s1 = psub[p + 1];
+    for (int idx = threadIdx.x; idx < nk * nent * dz; idx += blockDim.x) {
+        const int k = idx / (nent * dz), r = idx % (nent * dz);
+        const size_t o = ((size_t)k * S.nbe + pr.blk) * dz + r;
+        double a = np[o] + ke[o];
+        for (long s = s0; s < s1; ++s) a += part[subs[s].out + idx];
+        H[o] = a;
+    }
 }
 
 // wave sum of a double (every lane gets the total)
@@ -210,28 +323,29 @@ __device__ __forceinline__ double epd_wave_sum(double v)
     return v;
 }
 
-// tpart[pair][term] = sum over the items of (pair, term) of
-// sum_e C1 C2 block_e X_e(image), X from the weighted density rw.
+// tsub[sub.out][a] = sum over the run's items of exponent a of
+// sum_e C1 C2 block_e X_e(image), X from the weighted density rw
+// (runs of one potential type).
 template <int LMAX, int NE, bool GAMMA>
 __global__ __launch_bounds__(64) void k_epd_trace(DevSys S, const PairRec *__restrict__ pairs,
                                                   const double2 *__restrict__ phase,
                                                   const EpTuple *__restrict__ tuples,
                                                   const long *__restrict__ tseg,
+                                                  const long *__restrict__ iseg,
                                                   const double *__restrict__ blocks,
                                                   const double *__restrict__ rw,
-                                                  const int *__restrict__ type_term0,
-                                                  const int *__restrict__ order, int npairs,
-                                                  double *__restrict__ tpart)
+                                                  const EpdSub *__restrict__ subs, long nsubs,
+                                                  int npairs, double *__restrict__ tsub)
 {
     constexpr int SH = (LMAX + 1) * (LMAX + 1);
     constexpr int BS = SH * SH;
     constexpr int NQ = GAMMA ? 1 : 4;
     extern __shared__ double smem[];   // [max_pot_alphas][64] per-lane term sums
     const int lane = threadIdx.x;
-    const long logical = blockIdx.x;
-    if (logical >= (long)npairs * S.num_pot_types) return;
-    const int pt = (int)(logical % S.num_pot_types);
-    const int pidx = order[logical / S.num_pot_types];
+    const long sidx = blockIdx.x;
+    if (sidx >= nsubs) return;
+    const EpdSub sub = subs[sidx];
+    const int pidx = sub.pair, pt = sub.t_lo;
     const PairRec pr = pairs[pidx];
     const int t1 = S.atom_type[pr.i], t2 = S.atom_type[pr.j];
     const int ns1 = S.type_nstates[t1], ns2 = S.type_nstates[t2];
@@ -262,74 +376,75 @@ __global__ __launch_bounds__(64) void k_epd_trace(DevSys S, const PairRec *__res
     int cur_img = -1;
 
     const long task = (long)pt * npairs + pidx;
-    const long toff = tseg[task];
-    const int cnt = (int)(tseg[task + 1] - toff);
-    for (int b = 0; b < cnt; b += 64) {
-        EpTuple t{};
-        if (b + lane < cnt) t = tuples[toff + b + lane];
-        unsigned long long m = __ballot(b + lane < cnt);
-        while (m) {
-            const int src = __ffsll((long long)m) - 1;
-            m &= m - 1ull;
-            int key, item;
-            unsigned long long mm;
-            epd_tuple_fields(t, src, key, item, mm);
-            const int im = key >> 16, a1 = (key >> 8) & 255, a2 = key & 255;
-            if (im != cur_img) {
-                cur_img = im;
-                const double2 *ph = phase + (size_t)(pr.img_off + im) * nk;
+    const long a = sub.i_lo, b = sub.i_hi;
+    epd_walk(tuples, tseg[task], tseg[task + 1], a, b,
+             [&](int im, int a1, int a2, int item0, unsigned long long mm) {
+                 if (im != cur_img) {
+                     cur_img = im;
+                     const double2 *ph = phase + (size_t)(pr.img_off + im) * nk;
 #pragma unroll
-                for (int e = 0; e < NE; ++e) {
-                    double x = 0.0;
+                     for (int e = 0; e < NE; ++e) {
+                         double x = 0.0;
 #pragma unroll
-                    for (int k = 0; k < NQ; ++k)
-                        if (k < nk) {
-                            const double2 f = GAMMA ? make_double2(1.0, 0.0) : ph[k];
-                            x += f.x * R[e][k].x + f.y * R[e][k].y;
-                        }
-                    X[e] = x;
-                }
-            }
-            double cx[NE];
+                         for (int k = 0; k < NQ; ++k)
+                             if (k < nk) {
+                                 const double2 f = GAMMA ? make_double2(1.0, 0.0) : ph[k];
+                                 x += f.x * R[e][k].x + f.y * R[e][k].y;
+                             }
+                         X[e] = x;
+                     }
+                 }
+                 double cx[NE];
 #pragma unroll
-            for (int e = 0; e < NE; ++e)
-                cx[e] = B1[E.em1[e] * MA + a1] * B2[E.em2[e] * MA + a2] * X[e];
-            while (mm) {
-                const int ja = __ffsll((long long)mm) - 1;
-                mm &= mm - 1ull;
-                const bool two = mm != 0ull;
-                const int jb = two ? __ffsll((long long)mm) - 1 : ja;
-                if (two) mm &= mm - 1ull;
-                const double *ba = blocks + (size_t)item * BS;
-                const double *bb = blocks + (size_t)(item + (two ? 1 : 0)) * BS;
-                double sa = 0.0, sb = 0.0;
+                 for (int e = 0; e < NE; ++e)
+                     cx[e] = B1[E.em1[e] * MA + a1] * B2[E.em2[e] * MA + a2] * X[e];
+                 long it = item0;
+                 while (mm) {
+                     const int ja = __ffsll((long long)mm) - 1;
+                     mm &= mm - 1ull;
+                     const long ia = it++;
+                     if (ia < a) continue;
+                     if (ia >= b) break;
+                     const bool two = mm != 0ull && it < b;
+                     const int jb = two ? __ffsll((long long)mm) - 1 : ja;
+                     if (two) {
+                         mm &= mm - 1ull;
+                         ++it;
+                     }
+                     const double *ba = blocks + (size_t)ia * BS;
+                     const double *bb = blocks + (size_t)(ia + (two ? 1 : 0)) * BS;
+                     double sa = 0.0, sb = 0.0;
 #pragma unroll
-                for (int e = 0; e < NE; ++e) {
-                    sa += cx[e] * ba[E.eo[e]];
-                    sb += cx[e] * bb[E.eo[e]];
-                }
-                T[ja * 64 + lane] += sa;
-                if (two) T[jb * 64 + lane] += sb;
-                item += two ? 2 : 1;
-            }
-        }
-    }
-    const int term0 = type_term0[pt];
-    for (int a = 0; a < na3; ++a) {
-        const double v = epd_wave_sum(T[a * 64 + lane]);
-        if (lane == 0) tpart[(size_t)pidx * S.pot_dim + term0 + a] = v;
+                     for (int e = 0; e < NE; ++e) {
+                         sa += cx[e] * ba[E.eo[e]];
+                         sb += cx[e] * bb[E.eo[e]];
+                     }
+                     T[ja * 64 + lane] += sa;
+                     if (two) T[jb * 64 + lane] += sb;
+                 }
+             });
+    for (int x = 0; x < na3; ++x) {
+        const double v = epd_wave_sum(T[x * 64 + lane]);
+        if (lane == 0) tsub[(size_t)sub.out * S.max_pot_alphas + x] = v;
     }
 }
 
-// out[term] = sum over pairs of tpart[pair][term], a fixed-order tree per term
-__global__ __launch_bounds__(256) void k_epd_trace_final(const double *__restrict__ tpart,
-                                                         int npairs, int pot_dim,
+// out[term] = sum over the runs of the term's potential type (runs
+// roff[type * npairs] .. roff[(type + 1) * npairs]) of their sums, a
+// fixed-order tree
+__global__ __launch_bounds__(256) void k_epd_trace_final(const double *__restrict__ tsub,
+                                                         const long *__restrict__ roff, int npairs,
+                                                         const int *__restrict__ term_type,
+                                                         const int *__restrict__ term_alpha,
+                                                         int max_pot_alphas,
                                                          double *__restrict__ out)
 {
     __shared__ double red[256];
     const int term = blockIdx.x;
+    const int pt = term_type[term], aj = term_alpha[term];
+    const long s0 = roff[(long)pt * npairs], s1 = roff[(long)(pt + 1) * npairs];
     double s = 0.0;
-    for (int p = threadIdx.x; p < npairs; p += 256) s += tpart[(size_t)p * pot_dim + term];
+    for (long q = s0 + threadIdx.x; q < s1; q += 256) s += tsub[(size_t)q * max_pot_alphas + aj];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {

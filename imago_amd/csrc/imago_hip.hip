@@ -173,9 +173,16 @@ struct imago_ctx {
     // direct EP path (IMAGO_EP_DIRECT): H(k) and the traces from the
     // primitive blocks each iteration, no potDim panels
     bool ep_direct = false;
+    bool ep_runs = false;         // run tables below built
     double *coef_eff = nullptr;   // [pot_dim]
-    double *hpartial = nullptr;   // [groups][panel]
-    double *tpart = nullptr;      // [pairs][pot_dim]
+    EpdSub *hruns = nullptr;      // H sum: runs in pair order
+    long nhruns = 0;
+    long *hrun_lo = nullptr;      // [pairs + 1] first run of each pair
+    double *hpart = nullptr;      // run partial blocks
+    EpdSub *truns = nullptr;      // traces: runs in potential-type order
+    long ntruns = 0;
+    long *trun_lo = nullptr;      // [types * pairs + 1] first run of each (type, pair) task
+    double *tpart = nullptr;      // [runs][max_pot_alphas]
     double timing[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
 };
@@ -196,7 +203,11 @@ static void ctx_free_system(imago_ctx *c)
     c->ep_ntuples = c->ep_nitems = 0;
     c->ep_screened = false;
     c->ep_direct = false;
-    c->coef_eff = c->hpartial = c->tpart = nullptr;
+    c->ep_runs = false;
+    c->coef_eff = c->hpart = c->tpart = nullptr;
+    c->hruns = c->truns = nullptr;
+    c->hrun_lo = c->trun_lo = nullptr;
+    c->nhruns = c->ntruns = 0;
     c->pairs = nullptr;
     c->type_term0 = nullptr;
     c->pair_order = nullptr;
@@ -816,6 +827,7 @@ extern "C" int imago_set_system(imago_ctx *c, const imago_system_desc *s, int *v
     d.nk = h.nk;
     d.vale_dim = h.n;
     d.pot_dim = h.pot_dim;
+    d.gamma = h.gamma;
     std::vector<int> nal(h.nt);
     for (int t = 0; t < h.nt; ++t) nal[t] = h.noa[4 * t];
     RCHK(dev_upload(c, &d.type_alphas, h.alphas));
@@ -932,6 +944,67 @@ static int ep_prepare(imago_ctx *c)
     return IMAGO_OK;
 }
 
+// Runs of at most kEpdRunItems items for the direct EP kernels, built on
+// the device from the per-task item offsets (k_epd_hsum_runs,
+// k_epd_trace_runs: count, scan, fill).  For the H sum each pair's items
+// over all potential types are cut in order (partials in pair order); for
+// the traces each (type, pair) task is cut separately (runs in type order,
+// those of type t from trun_lo[t * npairs]).
+static int ep_build_runs(imago_ctx *c)
+{
+    if (c->ep_runs) return IMAGO_OK;
+    const HostSys &h = c->h;
+    const int np = c->npairs;
+    const long ntask = (long)np * h.npt, C = kEpdRunItems;
+    long *rcnt = nullptr, *wcnt = nullptr, *woff = nullptr, *hlo = nullptr, *tcnt = nullptr,
+         *tlo = nullptr;
+    RCHK(dev_alloc(c, &rcnt, (size_t)np + 1));
+    RCHK(dev_alloc(c, &wcnt, (size_t)np + 1));
+    RCHK(dev_alloc(c, &woff, (size_t)np + 1));
+    RCHK(dev_alloc(c, &hlo, (size_t)np + 1));
+    RCHK(dev_alloc(c, &tcnt, (size_t)ntask + 1));
+    RCHK(dev_alloc(c, &tlo, (size_t)ntask + 1));
+    HIPCHK(hipMemsetAsync(rcnt + np, 0, sizeof(long), c->stream));
+    HIPCHK(hipMemsetAsync(wcnt + np, 0, sizeof(long), c->stream));
+    HIPCHK(hipMemsetAsync(tcnt + ntask, 0, sizeof(long), c->stream));
+    HIPCHK(launch_epd_hsum_runs(false, c->ep_iseg, np, h.npt, C, c->pairs, c->d, rcnt, wcnt,
+                                nullptr, nullptr, nullptr, c->stream));
+    HIPCHK(launch_epd_trace_runs(false, c->ep_iseg, ntask, np, C, tcnt, nullptr, nullptr,
+                                 c->stream));
+    size_t tb = 0, t2 = 0, t3 = 0;
+    HIPCHK(ep_exclusive_scan(rcnt, hlo, np + 1, nullptr, &tb, c->stream));
+    HIPCHK(ep_exclusive_scan(wcnt, woff, np + 1, nullptr, &t2, c->stream));
+    HIPCHK(ep_exclusive_scan(tcnt, tlo, ntask + 1, nullptr, &t3, c->stream));
+    tb = std::max(tb, std::max(t2, t3));
+    char *tmp = nullptr;
+    RCHK(dev_alloc(c, &tmp, tb));
+    HIPCHK(ep_exclusive_scan(rcnt, hlo, np + 1, tmp, &tb, c->stream));
+    HIPCHK(ep_exclusive_scan(wcnt, woff, np + 1, tmp, &tb, c->stream));
+    HIPCHK(ep_exclusive_scan(tcnt, tlo, ntask + 1, tmp, &tb, c->stream));
+    long tot[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(&tot[0], hlo + np, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&tot[1], woff + np, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&tot[2], tlo + ntask, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    RCHK(dev_alloc(c, &c->hruns, (size_t)std::max(tot[0], 1L)));
+    RCHK(dev_alloc(c, &c->truns, (size_t)std::max(tot[2], 1L)));
+    RCHK(dev_alloc(c, &c->hpart, (size_t)std::max(tot[1], 1L)));
+    RCHK(dev_alloc(c, &c->tpart, (size_t)std::max(tot[2], 1L) * h.maxpa));
+    HIPCHK(launch_epd_hsum_runs(true, c->ep_iseg, np, h.npt, C, c->pairs, c->d, nullptr, nullptr,
+                                hlo, woff, c->hruns, c->stream));
+    HIPCHK(launch_epd_trace_runs(true, c->ep_iseg, ntask, np, C, nullptr, tlo, c->truns,
+                                 c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (void *p : {(void *)rcnt, (void *)wcnt, (void *)woff, (void *)tcnt, (void *)tmp})
+        dev_release(c, p);
+    c->hrun_lo = hlo;
+    c->trun_lo = tlo;
+    c->nhruns = tot[0];
+    c->ntruns = tot[2];
+    c->ep_runs = true;
+    return IMAGO_OK;
+}
+
 static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride, int term0,
                            int nterms)
 {
@@ -974,6 +1047,7 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         RCHK(ep_prepare(c));
+        RCHK(ep_build_runs(c));
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         float ms = 0;
@@ -1255,18 +1329,15 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
         HIPCHK(hipMemcpyAsync(c->coef_eff, ce.data(), sizeof(double) * h.pot_dim,
                               hipMemcpyHostToDevice, c->stream));
     }
-    const int ngroups = (h.npt + kEpdTypesPerGroup - 1) / kEpdTypesPerGroup;
-    if (direct && !c->hpartial) RCHK(dev_alloc(c, &c->hpartial, (size_t)per * ngroups));
+    if (direct) RCHK(ep_build_runs(c));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (direct) {
-        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg,
-                    c->ep_blocks, c->coef_eff,
-                    nullptr, c->type_term0, c->pair_order, c->npairs, ngroups, c->hpartial, per,
-                    nullptr, c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
+        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg, c->ep_iseg,
+                    c->ep_blocks, c->coef_eff, nullptr, c->type_term0, c->hruns, c->nhruns,
+                    c->hpart, c->npairs, c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
         HIPCHK(launch_epd_hsum(L));
         HIPCHK(hipEventRecord(c->ev2, c->stream));
-        HIPCHK(launch_epd_hfinal(c->panel[2], c->panel[1], c->hpartial, ngroups, per, c->blockH,
-                                 c->stream));
+        HIPCHK(launch_epd_hfinal(L, c->panel[2], c->panel[1], c->hrun_lo, c->blockH));
     } else {
         hipLaunchKernelGGL(k_hbuild, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, c->stream,
                            c->panel[2], c->panel[1], c->panel[3], c->coef, h.pot_dim, per,
@@ -1551,15 +1622,15 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
                            c->panel[3], per, h.pot_dim, c->rhow, nblk,
                            c->partial + (size_t)3 * nblk);
     if (h.pot_dim > 0 && direct) {
-        if (!c->tpart) RCHK(dev_alloc(c, &c->tpart, (size_t)c->npairs * h.pot_dim));
-        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg,
-                    c->ep_blocks, nullptr,
-                    c->rhow, c->type_term0, c->pair_order, c->npairs, 0, nullptr, 0, c->tpart,
-                    c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
+        RCHK(ep_build_runs(c));
+        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg, c->ep_iseg,
+                    c->ep_blocks, nullptr, c->rhow, c->type_term0, c->truns, c->ntruns,
+                    c->tpart, c->npairs, c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
         HIPCHK(hipEventRecord(c->ev2, c->stream));
         HIPCHK(launch_epd_trace(L));
         HIPCHK(hipEventRecord(c->ev3, c->stream));
-        HIPCHK(launch_epd_trace_final(c->tpart, c->npairs, h.pot_dim, outd + 3, c->stream));
+        HIPCHK(launch_epd_trace_final(c->tpart, c->trun_lo, c->npairs, c->d.term_type,
+                                      c->d.term_alpha, h.maxpa, h.pot_dim, outd + 3, c->stream));
     }
     hipLaunchKernelGGL(k_trace_final, dim3((nterms + 255) / 256), dim3(256), 0, c->stream, c->partial,
                        nblk, direct ? 3 : nterms, outd);
