@@ -163,7 +163,8 @@ hipError_t launch_ep_prims_pass(const AssembleLaunch &L)
     const unsigned g = (unsigned)((L.ep_nitems + 63) / 64);
 #define IMAGO_EPP_CASE(LM)                                                                    \
     case LM:                                                                                  \
-        hipLaunchKernelGGL(k_ep_prims<LM>, dim3(g), dim3(64), 0, L.stream, L.S, L.pairs,       \
+        hipLaunchKernelGGL(k_ep_prims<LM>, dim3(g), dim3(64), ep_prims_lds_bytes(LM),         \
+                           L.stream, L.S, L.pairs,                                            \
                            L.imgs, L.cands, L.ep_tuples, L.ep_item_tuple, L.ep_nitems,         \
                            L.npairs, L.ep_blocks);                                            \
         return hipGetLastError();

@@ -192,12 +192,24 @@ __global__ __launch_bounds__(64) void k_ep_screen(DevSys S, const PairRec *__res
 // k_ep_prims: one lane per item (tuple, exponent): the primitive block of
 // electron3CIntg (gaussIntegrals.f90:11624) for the tuple's alpha pair,
 // image and candidate site image with the exponent's alpha, written whole
-// (SH x SH, lm1-major) to blocks + item * SH * SH.
+// (SH x SH, lm1-major) to blocks + item * SH * SH.  A wavefront's 64 items
+// are consecutive, so their blocks are one contiguous range: a quarter of
+// the lanes at a time evaluates into an LDS tile, which the whole
+// wavefront then copies out with coalesced stores (a block written
+// directly by its lane would be 81 stores 648 B apart).
 template <int SH>
 struct BlockAcc {
     double *o;
     __device__ __forceinline__ void add(int i, int j, double x) { o[i * SH + j] = x; }
 };
+
+constexpr int kEpPrimsTile = 16;   // blocks per LDS tile
+
+__host__ __device__ inline size_t ep_prims_lds_bytes(int lmax)
+{
+    const int sh = (lmax + 1) * (lmax + 1);
+    return (size_t)kEpPrimsTile * sh * sh * sizeof(double);
+}
 
 template <int LMAX>
 __global__ __launch_bounds__(64) void k_ep_prims(DevSys S, const PairRec *__restrict__ pairs,
@@ -208,31 +220,47 @@ __global__ __launch_bounds__(64) void k_ep_prims(DevSys S, const PairRec *__rest
                                                  int npairs, double *__restrict__ blocks)
 {
     constexpr int SH = (LMAX + 1) * (LMAX + 1);
-    const long it = (long)blockIdx.x * 64 + threadIdx.x;
-    if (it >= nitems) return;
-    const EpTuple t = tuples[item_tuple[it]];
-    // exponent: the (it - t.item)-th set bit of the mask
-    unsigned long long m = t.mask;
-    for (long r = it - t.item; r > 0; --r) m &= m - 1ull;
-    const int aj = __ffsll((long long)m) - 1;
-    const int pidx = t.task % npairs, pt = t.task / npairs;
-    const PairRec pr = pairs[pidx];
-    const int t1 = S.atom_type[pr.i], t2 = S.atom_type[pr.j];
-    const int MA = S.max_alphas;
-    const int im = t.key >> 16, a1 = (t.key >> 8) & 255, a2 = t.key & 255;
-    const CandRec cr = cands[t.cand];
-    const ImgRec ir = imgs[pr.img_off + im];
-    const Vec3 A = {S.atom_pos[3 * pr.i], S.atom_pos[3 * pr.i + 1], S.atom_pos[3 * pr.i + 2]};
-    const Vec3 Bp = {ir.bx, ir.by, ir.bz};
-    const Vec3 Cp = {cr.px, cr.py, cr.pz};
-    const int la = lma_to_l(S.type_lma[t1 * MA + a1]);
-    const int lb = lma_to_l(S.type_lma[t2 * MA + a2]);
-    double *o = blocks + (size_t)it * SH * SH;
-    for (int z = 0; z < SH * SH; ++z) o[z] = 0.0;
-    BlockAcc<SH> acc{o};
-    prim_dispatch<K_EP, LMAX>(la, lb, S.type_alphas[t1 * MA + a1], S.type_alphas[t2 * MA + a2],
-                              S.pot_alphas[(size_t)pt * S.max_pot_alphas + aj], A, Bp, Cp, 1.0,
-                              acc);
+    constexpr int BS = SH * SH;
+    extern __shared__ double tile[];   // [kEpPrimsTile][BS]
+    const int lane = threadIdx.x;
+    const long base = (long)blockIdx.x * 64;
+    const long it = base + lane;
+    for (int q = 0; q < 64 / kEpPrimsTile; ++q) {
+        const long t0 = base + q * kEpPrimsTile;
+        if (t0 >= nitems) break;
+        if (lane / kEpPrimsTile == q && it < nitems) {
+            const EpTuple t = tuples[item_tuple[it]];
+            // exponent: the (it - t.item)-th set bit of the mask
+            unsigned long long m = t.mask;
+            for (long r = it - t.item; r > 0; --r) m &= m - 1ull;
+            const int aj = __ffsll((long long)m) - 1;
+            const int pidx = t.task % npairs, pt = t.task / npairs;
+            const PairRec pr = pairs[pidx];
+            const int t1 = S.atom_type[pr.i], t2 = S.atom_type[pr.j];
+            const int MA = S.max_alphas;
+            const int im = t.key >> 16, a1 = (t.key >> 8) & 255, a2 = t.key & 255;
+            const CandRec cr = cands[t.cand];
+            const ImgRec ir = imgs[pr.img_off + im];
+            const Vec3 A = {S.atom_pos[3 * pr.i], S.atom_pos[3 * pr.i + 1],
+                            S.atom_pos[3 * pr.i + 2]};
+            const Vec3 Bp = {ir.bx, ir.by, ir.bz};
+            const Vec3 Cp = {cr.px, cr.py, cr.pz};
+            const int la = lma_to_l(S.type_lma[t1 * MA + a1]);
+            const int lb = lma_to_l(S.type_lma[t2 * MA + a2]);
+            double *o = tile + (lane % kEpPrimsTile) * BS;
+            for (int z = 0; z < BS; ++z) o[z] = 0.0;
+            BlockAcc<SH> acc{o};
+            prim_dispatch<K_EP, LMAX>(la, lb, S.type_alphas[t1 * MA + a1],
+                                      S.type_alphas[t2 * MA + a2],
+                                      S.pot_alphas[(size_t)pt * S.max_pot_alphas + aj], A, Bp, Cp,
+                                      1.0, acc);
+        }
+        __syncthreads();
+        const long nb = min((long)kEpPrimsTile, nitems - t0);
+        double *dst = blocks + (size_t)t0 * BS;
+        for (int z = lane; z < nb * BS; z += 64) dst[z] = tile[z];
+        __syncthreads();
+    }
 }
 
 // Work-group -> (potential type, pair) mapping.  Work-groups are dealt to
