@@ -45,15 +45,15 @@ hipError_t launch_epd_trace(const EpdLaunch &L) { IMAGO_EPD_DISPATCH(trace_one) 
 #undef IMAGO_EPD_LM
 
 hipError_t launch_epd_hfinal(const EpdLaunch &L, const double *np, const double *ke,
-                             const long *psub, double *H)
+                             const long *psub, const long *woff, double *H)
 {
     if (L.npairs <= 0) return hipSuccess;
     if (L.gamma)
         hipLaunchKernelGGL(k_epd_hfinal<true>, dim3(L.npairs), dim3(256), 0, L.stream, L.S,
-                           L.pairs, np, ke, L.out, L.subs, psub, H);
+                           L.pairs, np, ke, L.out, psub, woff, H);
     else
         hipLaunchKernelGGL(k_epd_hfinal<false>, dim3(L.npairs), dim3(256), 0, L.stream, L.S,
-                           L.pairs, np, ke, L.out, L.subs, psub, H);
+                           L.pairs, np, ke, L.out, psub, woff, H);
     return hipGetLastError();
 }
 

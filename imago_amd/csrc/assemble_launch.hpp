@@ -75,7 +75,7 @@ struct EpdLaunch {
 hipError_t launch_epd_hsum(const EpdLaunch &L);
 hipError_t launch_epd_trace(const EpdLaunch &L);
 hipError_t launch_epd_hfinal(const EpdLaunch &L, const double *np, const double *ke,
-                             const long *psub, double *H);
+                             const long *psub, const long *woff, double *H);
 hipError_t launch_epd_trace_final(const double *tsub, const long *roff, int npairs,
                                   const int *term_type, const int *term_alpha, int max_pot_alphas,
                                   int pot_dim, double *out, hipStream_t stream);

@@ -291,26 +291,30 @@ __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__rest
 }
 
 // H = NP + KE + the pair's run partials, one work-group per pair; the runs
-// of pair p are subs[psub[p] .. psub[p + 1]) and are added in that order.
+// of pair p are psub[p] .. psub[p + 1] and their partial blocks lie back to
+// back from woff[p], added in run order.
 template <bool GAMMA>
 __global__ __launch_bounds__(256) void k_epd_hfinal(const DevSys S, const PairRec *__restrict__ pairs,
                                                     const double *__restrict__ np,
                                                     const double *__restrict__ ke,
                                                     const double *__restrict__ part,
-                                                    const EpdSub *__restrict__ subs,
                                                     const long *__restrict__ psub,
+                                                    const long *__restrict__ woff,
                                                     double *__restrict__ H)
 {
     const int p = blockIdx.x;
     const PairRec pr = pairs[p];
     const int ns1 = S.type_nstates[S.atom_type[pr.i]], ns2 = S.type_nstates[S.atom_type[pr.j]];
-    const int nent = ns1 * ns2, nk = GAMMA ? 1 : S.nk, dz = GAMMA ? 1 : 2;
-    const long s0 = psub[p], s1 = psub[p + 1];
-    for (int idx = threadIdx.x; idx < nk * nent * dz; idx += blockDim.x) {
-        const int k = idx / (nent * dz), r = idx % (nent * dz);
+    const int nk = GAMMA ? 1 : S.nk, dz = GAMMA ? 1 : 2;
+    const int blk = ns1 * ns2 * nk * dz;
+    const int nr = (int)(psub[p + 1] - psub[p]);
+    const double *src = part + woff[p];
+    for (int idx = threadIdx.x; idx < blk; idx += blockDim.x) {
+        const int k = idx / (ns1 * ns2 * dz), r = idx % (ns1 * ns2 * dz);
         const size_t o = ((size_t)k * S.nbe + pr.blk) * dz + r;
         double a = np[o] + ke[o];
-        for (long s = s0; s < s1; ++s) a += part[subs[s].out + idx];
+#pragma unroll 4
+        for (int q = 0; q < nr; ++q) a += src[(size_t)q * blk + idx];
         H[o] = a;
     }
 }

@@ -178,6 +178,7 @@ struct imago_ctx {
     EpdSub *hruns = nullptr;      // H sum: runs in pair order
     long nhruns = 0;
     long *hrun_lo = nullptr;      // [pairs + 1] first run of each pair
+    long *hrun_woff = nullptr;    // [pairs + 1] first partial double of each pair
     double *hpart = nullptr;      // run partial blocks
     EpdSub *truns = nullptr;      // traces: runs in potential-type order
     long ntruns = 0;
@@ -206,7 +207,7 @@ static void ctx_free_system(imago_ctx *c)
     c->ep_runs = false;
     c->coef_eff = c->hpart = c->tpart = nullptr;
     c->hruns = c->truns = nullptr;
-    c->hrun_lo = c->trun_lo = nullptr;
+    c->hrun_lo = c->trun_lo = c->hrun_woff = nullptr;
     c->nhruns = c->ntruns = 0;
     c->pairs = nullptr;
     c->type_term0 = nullptr;
@@ -995,9 +996,9 @@ static int ep_build_runs(imago_ctx *c)
     HIPCHK(launch_epd_trace_runs(true, c->ep_iseg, ntask, np, C, nullptr, tlo, c->truns,
                                  c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (void *p : {(void *)rcnt, (void *)wcnt, (void *)woff, (void *)tcnt, (void *)tmp})
-        dev_release(c, p);
+    for (void *p : {(void *)rcnt, (void *)wcnt, (void *)tcnt, (void *)tmp}) dev_release(c, p);
     c->hrun_lo = hlo;
+    c->hrun_woff = woff;
     c->trun_lo = tlo;
     c->nhruns = tot[0];
     c->ntruns = tot[2];
@@ -1337,7 +1338,7 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
                     c->hpart, c->npairs, c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
         HIPCHK(launch_epd_hsum(L));
         HIPCHK(hipEventRecord(c->ev2, c->stream));
-        HIPCHK(launch_epd_hfinal(L, c->panel[2], c->panel[1], c->hrun_lo, c->blockH));
+        HIPCHK(launch_epd_hfinal(L, c->panel[2], c->panel[1], c->hrun_lo, c->hrun_woff, c->blockH));
     } else {
         hipLaunchKernelGGL(k_hbuild, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, c->stream,
                            c->panel[2], c->panel[1], c->panel[3], c->coef, h.pot_dim, per,
