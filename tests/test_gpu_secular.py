@@ -209,3 +209,22 @@ def test_direct_ep_path_refuses_many_kpoints(gpu_ctx):
     with pytest.raises(ImagoError) as e:
         gpu_ctx.assemble("ep_direct")
     assert e.value.name == "IMAGO_E_ARG" and "4 k-points" in str(e.value)
+
+
+@pytest.mark.parametrize("n", [2, 3, 33, 64, 97, 300])
+def test_single_hegv_orders(gpu_ctx, oracle_mod, n):
+    """The batched Hermitian eigensolver (tridiagonal reduction in 32-column
+    panels, the panel-edge cases included) against LAPACK zhegv."""
+    rng = np.random.default_rng(100 + n)
+    M = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = M + M.conj().T
+    R = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    S = R @ R.conj().T + n * np.eye(n)
+    print("[gpu] hegv n=%d" % n, flush=True)
+    w, V = gpu_ctx.hegv(H, S)
+    w2, V2, info = oracle_mod.hegv(H, S)
+    assert info == 0
+    scale = max(1.0, np.max(np.abs(w2)))
+    assert np.max(np.abs(w - w2)) < 1e-10 * scale, np.max(np.abs(w - w2))
+    assert np.allclose(V.conj().T @ S @ V, np.eye(n), atol=1e-10)
+    assert np.allclose(H @ V, S @ V * w, atol=1e-8 * scale)
