@@ -2,6 +2,7 @@
 // integral kind (asm_ol.hip, asm_ke.hip, asm_np.hip, asm_ep.hip) so the
 // template instances compile in parallel.
 #pragma once
+#include <algorithm>
 #include "assemble.hpp"
 #include "ep_kernel.hpp"
 
@@ -105,8 +106,10 @@ hipError_t launch_ep_one(const AssembleLaunch &L)
 {
     const long total = (long)L.npairs * L.nterms;
     const long per = (total + kNumXcd - 1) / kNumXcd;
-    if (per * kNumXcd >= (1L << 32)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_ep_store<LMAX, NE, NKR, G>), dim3((unsigned)(per * kNumXcd)), dim3(64),
+    // at most 2^24 work-groups of 64 (a multiple of kNumXcd); the kernel
+    // strides over the rest
+    const long groups = std::min(per * kNumXcd, 1L << 24);
+    hipLaunchKernelGGL((k_ep_store<LMAX, NE, NKR, G>), dim3((unsigned)groups), dim3(64),
                        L.lds, L.stream, L.S, L.pairs, L.phase, L.ep_tuples, L.ep_tseg,
                        L.ep_blocks, L.order, L.npairs, L.out, L.term_stride);
     return hipGetLastError();

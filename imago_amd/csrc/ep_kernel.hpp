@@ -310,9 +310,12 @@ __global__ __launch_bounds__(64) void k_ep_store(
     const int lane = threadIdx.x;
     const long total = (long)S.pot_dim * npairs;
     const long per = (total + kNumXcd - 1) / kNumXcd;
-    const long g = blockIdx.x;
+    // grid-stride over (term, pair) tasks: a dispatch describes at most
+    // 2^32 - 1 work-items; the stride is a multiple of kNumXcd, so each
+    // work-group keeps its XCD's contiguous range
+    for (long g = blockIdx.x; g < per * kNumXcd; g += gridDim.x) {
     const long logical = (g % kNumXcd) * per + g / kNumXcd;
-    if (logical >= total) return;
+    if (logical >= total) continue;
     const int term = (int)(logical / npairs);
     const int pidx = order[logical % npairs];
     const int pt = S.term_type[term], aj = S.term_alpha[term];
@@ -441,6 +444,7 @@ __global__ __launch_bounds__(64) void k_ep_store(
                     zero ? make_double2(0.0, 0.0) : make_double2(q[0], q[1]);
             }
         }
+    }
     }
 }
 

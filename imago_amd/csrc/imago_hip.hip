@@ -905,7 +905,9 @@ static int ep_prepare(imago_ctx *c)
     if (lds > 64 * 1024)
         return fail(IMAGO_E_ARG, "EP screening needs %zu bytes of LDS (max_alphas %d)", lds, h.maxa);
     const long ntask = (long)c->npairs * h.npt;
-    if (ntask >= (1L << 31)) return fail(IMAGO_E_ARG, "%ld (pair, potential type) tasks", ntask);
+    // one 64-lane work-group per task; a dispatch counts its work-items in 32 bits
+    if (ntask * 64 > 0xffffffffL)
+        return fail(IMAGO_E_ARG, "%ld (pair, potential type) tasks exceed one dispatch", ntask);
     long *tcnt = nullptr, *icnt = nullptr, *toff = nullptr, *ioff = nullptr;
     RCHK(dev_alloc(c, &tcnt, (size_t)ntask + 1));
     RCHK(dev_alloc(c, &icnt, (size_t)ntask + 1));
@@ -1002,6 +1004,8 @@ static int ep_build_runs(imago_ctx *c)
     c->trun_lo = tlo;
     c->nhruns = tot[0];
     c->ntruns = tot[2];
+    if (std::max(tot[0], tot[2]) * 64 > 0xffffffffL)
+        return fail(IMAGO_E_ARG, "%ld / %ld EP runs exceed one dispatch", tot[0], tot[2]);
     c->ep_runs = true;
     return IMAGO_OK;
 }
@@ -1347,6 +1351,8 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     const long nn = (long)h.n * h.n;
+    if (nn * h.nk > 0xffffffffL - 255)
+        return fail(IMAGO_E_ARG, "%ld matrix entries exceed one dispatch", nn * h.nk);
     const unsigned ug = (unsigned)((nn * h.nk + 255) / 256);
     const bool cx = !h.gamma;
     void *Hm = cx ? (void *)c->Hfull : (void *)c->Hreal;
