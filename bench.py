@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--cpu-terms", type=int, default=48,
                    help="EP terms the CPU baseline evaluates (sample, extrapolated)")
     p.add_argument("--small", action="store_true", help="tiny workload for smoke runs")
+    p.add_argument("--workload", choices=("si64", "sio2"), default="si64",
+                   help="si64: the 64-atom Si supercell (the benchmark line); sio2: the "
+                        "216-atom amorphous SiO2 cell, Gamma only")
     p.add_argument("--ep-mode", choices=("direct", "panel"), default="direct",
                    help="direct: EP terms from the stored primitive blocks each iteration "
                         "(IMAGO_EP_DIRECT); panel: all potDim EP panels stored (IMAGO_EP)")
@@ -56,11 +59,11 @@ def parse():
     return p.parse_args()
 
 
-def profiled_traffic(kernel):
+def profiled_traffic(kernel, workload):
     """HBM bytes per launch of kernel from its committed rocprofv3 PMC
-    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's counts), or
-    None when there is none for this workload."""
-    path = os.path.join(ROOT, "profiles", "%s_si64_pmc.md" % kernel)
+    summary for this workload (FETCH_SIZE and WRITE_SIZE scaled by a plain
+    copy's counts), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_pmc.md" % (kernel, workload))
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -83,8 +86,10 @@ def calibrate_copy():
 
 
 def electrons(system):
-    # all-electron Si: 14 electrons per atom
-    return 14 * system.num_atoms
+    """All-electron count: Z of every atom's element."""
+    from imago_amd import configs
+    return int(round(sum(configs.Z_OF[system.atom_types[t].element]
+                         for t in system.atom_type_of)))
 
 
 def gpu_step(ctx, system, coeffs, num_states, oracle_populate, ep_kind):
@@ -189,7 +194,7 @@ def cpu_baseline(system, coeffs, num_states, cpu_terms):
     t["hbuild_sample"] = time.perf_counter() - t0
     t["hbuild_full_est"] = t["hbuild_sample"] * (pot_dim / nt) * nk
     # generalized eigensolve (LAPACK zhegv, one k-point) and scale by nk
-    H = oracle.unpack(h[0::2] + 1j * h[1::2], n)
+    H = oracle.unpack(h if system.gamma else h[0::2] + 1j * h[1::2], n)
     Sf = oracle.unpack(S[0], n)
     t0 = time.perf_counter()
     w, vec, info = oracle.hegv(H, Sf)
@@ -205,7 +210,7 @@ def cpu_baseline(system, coeffs, num_states, cpu_terms):
     cols = np.concatenate([np.full(c + 1, c) for c in range(n)])
     rows = np.concatenate([np.arange(c + 1) for c in range(n)])
     rp = rho[rows, cols]
-    rpr = np.ascontiguousarray(np.stack([rp.real, rp.imag], -1).ravel())
+    rpr = np.ascontiguousarray(rp.real if system.gamma else np.stack([rp.real, rp.imag], -1).ravel())
     t0 = time.perf_counter()
     for j in range(nt):
         oracle.matrix_element_mult_raw(V[j, 0], rpr, n)
@@ -233,6 +238,8 @@ def main():
     from imago_amd.lib import Context
     if args.small:
         system = configs.si64_supercell(reps=(1, 1, 1), cutoff_bohr=16.0)
+    elif args.workload == "sio2":
+        system = configs.sio2_amorphous()
     else:
         system = configs.si64_supercell()
     coeffs = configs.pot_coefficients(system, seed=7)
@@ -257,10 +264,9 @@ def main():
     st = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
     n = system.vale_dim
     nk = system.num_kpoints
-    ns = n // system.num_atoms
-    npairs = system.num_atoms * (system.num_atoms + 1) // 2
-    nbe = npairs * ns * ns                       # block-layout entries per k
-    per = 2 * nbe * nk                           # doubles of one complex panel
+    ns = np.array([len(system.atom_types[t].states()) for t in system.atom_type_of])
+    nbe = int((np.sum(ns) ** 2 + np.sum(ns ** 2)) // 2)   # block-layout entries per k
+    per = (1 if system.gamma else 2) * nbe * nk  # doubles of one panel
     ntup, nitems, bsz = ctx.ep_counts()
     ng = -(-len(system.pot_types) // 8)          # groups of 8 potential types in the H sum
     kernels = {}
@@ -282,7 +288,9 @@ def main():
     for v in kernels.values():
         v["achieved"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
     stage_flops = {"eigensolve": {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS,
-                                  "ms": st["eigensolve"], "flop": 32.0 * n ** 3 * nk}}
+                                  "ms": st["eigensolve"],
+                                  # zhegv ~ 32 n^3 per k, dsygv ~ 8 n^3 (LAWN 41)
+                                  "flop": (8.0 if system.gamma else 32.0) * n ** 3 * nk}}
     for v in stage_flops.values():
         v["achieved"] = v["flop"] / (v["ms"] * 1e-3) / 1e12
     # the roofline line is the kernel of this library that takes the most
@@ -292,7 +300,7 @@ def main():
     d = kernels[dom]
     roofline = {"kernel": dom, "bound": d["bound"], "achieved": round(d["achieved"], 3),
                 "peak": d["peak"], "unit": d["unit"], "frac": round(d["achieved"] / d["peak"], 5),
-                "traffic": profiled_traffic(dom)}
+                "traffic": profiled_traffic(dom, args.workload)}
     cpu = None
     if not args.no_cpu:
         log("cpu baseline leg (%d EP terms)" % args.cpu_terms)
@@ -308,7 +316,8 @@ def main():
     line = {
         "metric": METRIC,
         "value": round(step_s, 6),
-        "unit": "s per step (integral setup + 1 SCF secular/density iteration, Si64 n=896 nk=4)",
+        "unit": "s per step (integral setup + 1 SCF secular/density iteration, %s n=%d nk=%d)"
+                % (system.name, n, nk),
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -317,9 +326,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded Si64 supercell, random-normalised contractions)",
-        "config": {"workload": "si64_supercell: 64 atoms, n=896, 4 k-points, potDim=1024, "
-                               "cutoff 16 bohr, one hot-path pass per step",
+        "data": "synthetic (seeded %s, random-normalised contractions)" % system.name,
+        "config": {"workload": "%s: %d atoms, n=%d, %d k-point(s)%s, potDim=%d, cutoff 16 bohr, "
+                               "one hot-path pass per step"
+                               % (system.name, system.num_atoms, n, nk,
+                                  " (Gamma)" if system.gamma else "", system.pot_dim),
                    "ep_mode": args.ep_mode, "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 3) for k, v in st.items()},
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)

@@ -145,8 +145,14 @@ def _rsa(rng, box, n_si, n_o, d_sio, d_same, max_tries=200000):
 
 
 def sio2_amorphous(seed: int = 3, n_si: int = 72, n_o: int = 144,
-                   density_g_cm3: float = 2.2, log_thresh: float = -math.log(1e-16)
-                   ) -> System:
+                   density_g_cm3: float = 2.2, cutoff_bohr: float | None = 16.0,
+                   log_thresh: float | None = None) -> System:
+    """Amorphous SiO2 by random sequential insertion (1.45 A Si-O, 2.3 A
+    O-O / Si-Si minimum distances) in a cube at the given density; Si s,p,d
+    with 12 primitives, O s,p,d with 10; one potential type per site;
+    Gamma only.  The lattice cutoff is mapped onto -ln(basisFnConvg) with
+    the most diffuse exponent (0.12), as for the Si cells; cutoff_bohr=None
+    keeps the reference default 1e-16."""
     rng = np.random.default_rng(seed)
     mass = n_si * 28.0855 + n_o * 15.999
     vol_a3 = mass / 6.02214076e23 / density_g_cm3 * 1e24
@@ -163,6 +169,9 @@ def sio2_amorphous(seed: int = 3, n_si: int = 72, n_o: int = 144,
                             nuc_charge=Z_OF[int(sp)], nuc_alpha=NUC_ALPHA[int(sp)],
                             sites=[i]))
     rv = np.diag([box, box, box])
+    if log_thresh is None:
+        log_thresh = (log_thresh_for_cutoff(cutoff_bohr, 0.12) if cutoff_bohr is not None
+                      else -math.log(1e-16))
     return System(real_vectors=rv, atom_types=[t_si, t_o], atom_type_of=types,
                   atom_pos=pos, pot_types=pots, pot_site_pos=pos.copy(),
                   kpoints=np.zeros((1, 3)), kpoint_weights=np.array([2.0]),
