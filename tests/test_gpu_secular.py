@@ -228,3 +228,45 @@ def test_single_hegv_orders(gpu_ctx, oracle_mod, n):
     assert np.max(np.abs(w - w2)) < 1e-10 * scale, np.max(np.abs(w - w2))
     assert np.allclose(V.conj().T @ S @ V, np.eye(n), atol=1e-10)
     assert np.allclose(H @ V, S @ V * w, atol=1e-8 * scale)
+
+
+@pytest.mark.parametrize("n", [2, 17, 97, 300, 1024])
+def test_coop_tridiagonal_path_orders(gpu_ctx, oracle_mod, monkeypatch, n):
+    """The cooperative tridiagonal reduction (IMAGO_HEEVD=coop: k_hetrd_coop,
+    dstedc, zunmtr) against LAPACK zhegv, up to its largest order."""
+    monkeypatch.setenv("IMAGO_HEEVD", "coop")
+    rng = np.random.default_rng(300 + n)
+    M = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = M + M.conj().T
+    R = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    S = R @ R.conj().T + n * np.eye(n)
+    print("[gpu] coop hegv n=%d" % n, flush=True)
+    w, V = gpu_ctx.hegv(H, S)
+    w2, _, info = oracle_mod.hegv(H, S)
+    assert info == 0
+    scale = max(1.0, np.max(np.abs(w2)))
+    assert np.max(np.abs(w - w2)) < 1e-10 * scale, np.max(np.abs(w - w2))
+    assert np.allclose(V.conj().T @ S @ V, np.eye(n), atol=1e-10)
+    assert np.allclose(H @ V, S @ V * w, atol=1e-8 * scale)
+
+
+def test_coop_tridiagonal_path_batched_secular(gpu_ctx, monkeypatch):
+    """A batch of k-points through the cooperative path gives the default
+    path's eigenvalues and density traces."""
+    s = configs.small_test_system(seed=12)
+    n = s.vale_dim
+    coeffs = configs.pot_coefficients(s, seed=5)
+    occ = np.zeros((s.num_kpoints, n))
+    occ[:, :4] = 2.0 / s.num_kpoints
+    runs = []
+    for mode in ("rocsolver", "coop"):
+        monkeypatch.setenv("IMAGO_HEEVD", mode)
+        print("[gpu] batched secular via", mode, flush=True)
+        _setup(gpu_ctx, s)
+        ev = gpu_ctx.secular(coeffs, n)
+        tr, prho = gpu_ctx.density_traces(occ)
+        runs.append((ev, tr, prho))
+    (e1, t1, p1), (e2, t2, p2) = runs
+    assert np.max(np.abs(e1 - e2)) < 1e-10 * max(1.0, np.max(np.abs(e1)))
+    assert np.allclose(t1, t2, rtol=1e-10, atol=1e-12)
+    assert np.max(np.abs(p1 - p2)) <= 1e-10 * max(1.0, np.max(np.abs(p1)))
