@@ -68,16 +68,17 @@ hipError_t launch_epd_trace_final(const double *tsub, const long *roff, int npai
 }
 
 hipError_t launch_epd_trace_runs(bool fill, const long *iseg, long ntask, int npairs, long C,
-                                 long *rcnt, const long *roff, EpdSub *runs, hipStream_t stream)
+                                 long *rcnt, long *rcnt2, const long *roff, const long *roff2,
+                                 EpdSub *runs, hipStream_t stream)
 {
     if (ntask <= 0) return hipSuccess;
     const unsigned g = (unsigned)((ntask + 255) / 256);
     if (fill)
         hipLaunchKernelGGL(k_epd_trace_runs<true>, dim3(g), dim3(256), 0, stream, iseg, ntask,
-                           npairs, C, rcnt, roff, runs);
+                           npairs, C, rcnt, rcnt2, roff, roff2, runs);
     else
         hipLaunchKernelGGL(k_epd_trace_runs<false>, dim3(g), dim3(256), 0, stream, iseg, ntask,
-                           npairs, C, rcnt, roff, runs);
+                           npairs, C, rcnt, rcnt2, roff, roff2, runs);
     return hipGetLastError();
 }
 

@@ -81,7 +81,8 @@ hipError_t launch_epd_trace_final(const double *tsub, const long *roff, int npai
                                   const int *term_type, const int *term_alpha, int max_pot_alphas,
                                   int pot_dim, double *out, hipStream_t stream);
 hipError_t launch_epd_trace_runs(bool fill, const long *iseg, long ntask, int npairs, long C,
-                                 long *rcnt, const long *roff, EpdSub *runs, hipStream_t stream);
+                                 long *rcnt, long *rcnt2, const long *roff, const long *roff2,
+                                 EpdSub *runs, hipStream_t stream);
 hipError_t launch_epd_hsum_runs(bool fill, const long *iseg, int npairs, int ntypes, long C,
                                 const PairRec *pairs, const DevSys &S, long *rcnt, long *wcnt,
                                 const long *roff, const long *woff, EpdSub *runs,

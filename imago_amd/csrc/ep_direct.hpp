@@ -58,27 +58,32 @@ __device__ __forceinline__ void epd_tuple_fields(const EpTuple &t, int src, int 
     mask = ((unsigned long long)hi << 32) | lo;
 }
 
-// Run tables on the device.  Traces: every (type, pair) task cut into
-// runs of at most C items, runs in task order (type-major), so the runs of
-// type t start at roff[t * npairs].  count: rcnt[task]; fill: the runs.
+// Trace runs: each (type, pair) task's items cut into runs of at most C
+// items.  The run's partial goes to slot roff[task] + r (type-major task =
+// type * npairs + pair, the order k_epd_trace_final sums per term); the
+// runs are stored for execution in pair-major order (roff2[pair * ntypes +
+// type] + r), so consecutive work-groups share the pair's rho block in L2.
+// count: rcnt[task] = rcnt2[pair * ntypes + type] = runs of the task.
 template <bool FILL>
 __global__ void k_epd_trace_runs(const long *__restrict__ iseg, long ntask, int npairs, long C,
-                                 long *__restrict__ rcnt, const long *__restrict__ roff,
+                                 long *__restrict__ rcnt, long *__restrict__ rcnt2,
+                                 const long *__restrict__ roff, const long *__restrict__ roff2,
                                  EpdSub *__restrict__ runs)
 {
     const long task = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (task >= ntask) return;
     const long a = iseg[task], b = iseg[task + 1];
     const long n = (b - a + C - 1) / C;
+    const int p = (int)(task % npairs), t = (int)(task / npairs);
+    const long ntypes = ntask / npairs;
+    const long pm = (long)p * ntypes + t;
     if (!FILL) {
         rcnt[task] = n;
+        rcnt2[pm] = n;
         return;
     }
-    const int p = (int)(task % npairs), t = (int)(task / npairs);
-    for (long r = 0; r < n; ++r) {
-        const long q = roff[task] + r;
-        runs[q] = EpdSub{p, t, t + 1, 0, a + r * C, min(b, a + (r + 1) * C), q};
-    }
+    for (long r = 0; r < n; ++r)
+        runs[roff2[pm] + r] = EpdSub{p, t, t + 1, 0, a + r * C, min(b, a + (r + 1) * C), roff[task] + r};
 }
 
 // H sum: each pair's items over all types cut in order into runs of at
