@@ -267,17 +267,21 @@ def main():
     ns = np.array([len(system.atom_types[t].states()) for t in system.atom_type_of])
     nbe = int((np.sum(ns) ** 2 + np.sum(ns ** 2)) // 2)   # block-layout entries per k
     per = (1 if system.gamma else 2) * nbe * nk  # doubles of one panel
-    ntup, nitems, bsz = ctx.ep_counts()
-    ng = -(-len(system.pot_types) // 8)          # groups of 8 potential types in the H sum
+    ntup, nitems, bsz, npart = ctx.ep_counts()
     kernels = {}
     if ep_kind == "ep_direct":
+        # every primitive block and tuple read once, plus each item's sorted
+        # index and tuple index (4 + 4 bytes)
+        item_bytes = nitems * bsz * 8.0 + ntup * 24.0 + nitems * 8.0
         kernels["k_epd_hsum"] = {
             "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "ms": st["k_epd_hsum"],
-            # every primitive block and tuple read once, the groups' partial H written
-            "bytes": nitems * bsz * 8.0 + ntup * 24.0 + ng * per * 8.0}
+            # ... and the runs' partial blocks written
+            "bytes": item_bytes + npart * 8.0}
         kernels["k_epd_trace"] = {
             "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "ms": st["k_epd_trace"],
-            "bytes": nitems * bsz * 8.0 + ntup * 24.0 + per * 8.0}
+            # ... the density (folded per image for k-points) read once, one
+            # value per item written
+            "bytes": item_bytes + per * 8.0 + nitems * 8.0}
     else:
         kernels["k_ep_store"] = {
             "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "ms": st["ep_store"],

@@ -21,4 +21,10 @@ hipError_t int_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t
 {
     return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)n, stream);
 }
+hipError_t int_sort_pairs(const int *kin, int *kout, const int *vin, int *vout, long n, int bits,
+                          void *tmp, size_t *tmp_bytes, hipStream_t stream)
+{
+    return hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, kin, kout, vin, vout, (int)n, 0,
+                                              bits, stream);
+}
 }  // namespace imago

@@ -4,13 +4,19 @@
 
 namespace imago {
 
+static size_t epd_basis_lds(const EpdLaunch &L)
+{
+    return (size_t)2 * L.S.max_states * L.S.max_alphas * sizeof(double);
+}
+
 template <int LMAX, int NE, bool G>
 static hipError_t hsum_one(const EpdLaunch &L)
 {
     if (L.nsubs == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_epd_hsum<LMAX, NE, G>), dim3((unsigned)L.nsubs), dim3(64), 0, L.stream,
-                       L.S, L.pairs, L.phase, L.tuples, L.tseg, L.iseg, L.blocks, L.coef_eff,
-                       L.type_term0, L.subs, L.nsubs, L.npairs, L.out);
+    hipLaunchKernelGGL((k_epd_hsum<LMAX, NE, G>), dim3((unsigned)L.nsubs), dim3(64),
+                       epd_basis_lds(L), L.stream, L.S, L.pairs, L.phase, L.tuples, L.item_tuple,
+                       L.pitem, L.blocks, L.coef_eff, L.type_term0, L.subs, L.nsubs, L.npairs,
+                       L.out);
     return hipGetLastError();
 }
 
@@ -19,9 +25,8 @@ static hipError_t trace_one(const EpdLaunch &L)
 {
     if (L.nsubs == 0) return hipSuccess;
     hipLaunchKernelGGL((k_epd_trace<LMAX, NE, G>), dim3((unsigned)L.nsubs), dim3(64),
-                       (size_t)L.S.max_pot_alphas * 64 * sizeof(double), L.stream, L.S, L.pairs,
-                       L.phase, L.tuples, L.tseg, L.iseg, L.blocks, L.rw, L.subs, L.nsubs,
-                       L.npairs, L.out);
+                       epd_basis_lds(L), L.stream, L.S, L.pairs, L.tuples, L.item_tuple, L.pitem,
+                       L.blocks, L.X, L.ent, L.subs, L.nsubs, L.npairs, L.out);
     return hipGetLastError();
 }
 
@@ -57,44 +62,52 @@ hipError_t launch_epd_hfinal(const EpdLaunch &L, const double *np, const double 
     return hipGetLastError();
 }
 
-hipError_t launch_epd_trace_final(const double *tsub, const long *roff, int npairs,
-                                  const int *term_type, const int *term_alpha, int max_pot_alphas,
-                                  int pot_dim, double *out, hipStream_t stream)
+hipError_t launch_epd_fold(const EpdLaunch &L, const double *rw, double *X)
 {
-    if (pot_dim <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_epd_trace_final, dim3(pot_dim), dim3(256), 0, stream, tsub, roff, npairs,
-                       term_type, term_alpha, max_pot_alphas, out);
+    if (L.npairs <= 0 || L.gamma) return hipSuccess;
+    hipLaunchKernelGGL(k_epd_fold, dim3(L.npairs), dim3(256), 0, L.stream, L.S, L.pairs, L.phase,
+                       reinterpret_cast<const double2 *>(rw), L.ent, X);
     return hipGetLastError();
 }
 
-hipError_t launch_epd_trace_runs(bool fill, const long *iseg, long ntask, int npairs, long C,
-                                 long *rcnt, long *rcnt2, const long *roff, const long *roff2,
-                                 EpdSub *runs, hipStream_t stream)
+hipError_t launch_epd_trace_final(const double *ival, const unsigned char *jexp,
+                                  const long *iseg, int npairs, int ntypes, int max_pot_alphas,
+                                  const int *pot_nalpha, const int *term_type,
+                                  const int *term_alpha, int pot_dim, double *part, double *out,
+                                  hipStream_t stream)
 {
-    if (ntask <= 0) return hipSuccess;
-    const unsigned g = (unsigned)((ntask + 255) / 256);
-    if (fill)
-        hipLaunchKernelGGL(k_epd_trace_runs<true>, dim3(g), dim3(256), 0, stream, iseg, ntask,
-                           npairs, C, rcnt, rcnt2, roff, roff2, runs);
-    else
-        hipLaunchKernelGGL(k_epd_trace_runs<false>, dim3(g), dim3(256), 0, stream, iseg, ntask,
-                           npairs, C, rcnt, rcnt2, roff, roff2, runs);
+    if (ntypes <= 0 || pot_dim <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_epd_trace_chunks, dim3((unsigned)ntypes * kEpdTraceChunks), dim3(64),
+                       (size_t)max_pot_alphas * 64 * sizeof(double), stream, ival, jexp, iseg,
+                       npairs, pot_nalpha, max_pot_alphas, part);
+    hipLaunchKernelGGL(k_epd_trace_final, dim3((unsigned)((pot_dim + 255) / 256)), dim3(256), 0,
+                       stream, part, term_type, term_alpha, pot_dim, max_pot_alphas, out);
     return hipGetLastError();
 }
 
-hipError_t launch_epd_hsum_runs(bool fill, const long *iseg, int npairs, int ntypes, long C,
-                                const PairRec *pairs, const DevSys &S, long *rcnt, long *wcnt,
-                                const long *roff, const long *woff, EpdSub *runs,
-                                hipStream_t stream)
+hipError_t launch_epd_item_keys(const EpTuple *tuples, const int *item_tuple,
+                                const PairRec *pairs, long nitems, int npairs, int *key, int *val,
+                                unsigned char *jexp, hipStream_t stream)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_epd_item_keys, dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0,
+                       stream, tuples, item_tuple, pairs, nitems, npairs, key, val, jexp);
+    return hipGetLastError();
+}
+
+hipError_t launch_epd_pair_runs(bool fill, const int *skey, long nitems, const PairRec *pairs,
+                                int npairs, const DevSys &S, long C, long *hcnt, long *wcnt,
+                                long *tcnt, const long *hoff, const long *woff, const long *toff,
+                                EpdSub *hruns, EpdSub *truns, hipStream_t stream)
 {
     if (npairs <= 0) return hipSuccess;
     const unsigned g = (unsigned)((npairs + 63) / 64);
     if (fill)
-        hipLaunchKernelGGL(k_epd_hsum_runs<true>, dim3(g), dim3(64), 0, stream, iseg, npairs,
-                           ntypes, C, pairs, S, rcnt, wcnt, roff, woff, runs);
+        hipLaunchKernelGGL(k_epd_pair_runs<true>, dim3(g), dim3(64), 0, stream, skey, nitems,
+                           pairs, npairs, S, C, hcnt, wcnt, tcnt, hoff, woff, toff, hruns, truns);
     else
-        hipLaunchKernelGGL(k_epd_hsum_runs<false>, dim3(g), dim3(64), 0, stream, iseg, npairs,
-                           ntypes, C, pairs, S, rcnt, wcnt, roff, woff, runs);
+        hipLaunchKernelGGL(k_epd_pair_runs<false>, dim3(g), dim3(64), 0, stream, skey, nitems,
+                           pairs, npairs, S, C, hcnt, wcnt, tcnt, hoff, woff, toff, hruns, truns);
     return hipGetLastError();
 }
 

@@ -44,31 +44,31 @@ hipError_t launch_ep_prims(const AssembleLaunch &L);
 
 // direct (panel-free) EP contributions, asm_epd.hip
 
-// A run of items of one atom pair: potential types [t_lo, t_hi), starting
-// at item i_lo within type t_lo and ending at item i_hi within type
-// t_hi - 1.  The host cuts the pairs' items into runs of at most
-// kEpdRunItems, so an on-site pair with thousands of items does not hold
-// one wavefront for the length of the whole launch.
+// A run of items of one atom pair: positions [i_lo, i_hi) of the pair's
+// image-sorted items (pitem).  The host cuts every pair's items into runs
+// of at most kEpdRunItems, so an on-site pair with thousands of items does
+// not hold one wavefront for the length of the whole launch.
 constexpr int kEpdRunItems = 128;
 struct EpdSub {
     int pair, t_lo, t_hi, pad;
     long i_lo, i_hi;
-    long out;   // H sum: offset (doubles) of the run's partial block; traces: its row
+    long out;   // H sum: offset (doubles) of the run's partial block
 };
 struct EpdLaunch {
     DevSys S;
     const PairRec *pairs;
     const double2 *phase;
     const EpTuple *tuples;
-    const long *tseg;         // [types * pairs + 1] first tuple of each task
-    const long *iseg;         // [types * pairs + 1] first item of each task
+    const int *item_tuple;    // [items] tuple of each item
+    const int *pitem;         // [items] items sorted by (pair, lattice image)
     const double *blocks;
     const double *coef_eff;   // hsum: per term, 0.0 replaced by 1.0
-    const double *rw;         // trace: weighted density, block layout
+    const double *X;          // trace: folded density [image][ent] (rw for Gamma)
+    int ent;                  // entries per image of X (64 * ne)
     const int *type_term0;
     const EpdSub *subs;       // runs of items, one wavefront each
     long nsubs;
-    double *out;              // hsum: run partial blocks; trace: [runs][max_pot_alphas]
+    double *out;              // hsum: run partial blocks; trace: one value per item
     int npairs, lmax, ne;
     bool gamma;
     hipStream_t stream;
@@ -77,16 +77,23 @@ hipError_t launch_epd_hsum(const EpdLaunch &L);
 hipError_t launch_epd_trace(const EpdLaunch &L);
 hipError_t launch_epd_hfinal(const EpdLaunch &L, const double *np, const double *ke,
                              const long *psub, const long *woff, double *H);
-hipError_t launch_epd_trace_final(const double *tsub, const long *roff, int npairs,
-                                  const int *term_type, const int *term_alpha, int max_pot_alphas,
-                                  int pot_dim, double *out, hipStream_t stream);
-hipError_t launch_epd_trace_runs(bool fill, const long *iseg, long ntask, int npairs, long C,
-                                 long *rcnt, long *rcnt2, const long *roff, const long *roff2,
-                                 EpdSub *runs, hipStream_t stream);
-hipError_t launch_epd_hsum_runs(bool fill, const long *iseg, int npairs, int ntypes, long C,
-                                const PairRec *pairs, const DevSys &S, long *rcnt, long *wcnt,
-                                const long *roff, const long *woff, EpdSub *runs,
-                                hipStream_t stream);
+hipError_t launch_epd_fold(const EpdLaunch &L, const double *rw, double *X);
+// part: [types][kEpdTraceChunks][max_pot_alphas] scratch of the per-term sums
+hipError_t launch_epd_trace_final(const double *ival, const unsigned char *jexp,
+                                  const long *iseg, int npairs, int ntypes, int max_pot_alphas,
+                                  const int *pot_nalpha, const int *term_type,
+                                  const int *term_alpha, int pot_dim, double *part, double *out,
+                                  hipStream_t stream);
+hipError_t launch_epd_item_keys(const EpTuple *tuples, const int *item_tuple,
+                                const PairRec *pairs, long nitems, int npairs, int *key, int *val,
+                                unsigned char *jexp, hipStream_t stream);
+hipError_t launch_epd_pair_runs(bool fill, const int *skey, long nitems, const PairRec *pairs,
+                                int npairs, const DevSys &S, long C, long *hcnt, long *wcnt,
+                                long *tcnt, const long *hoff, const long *woff, const long *toff,
+                                EpdSub *hruns, EpdSub *truns, hipStream_t stream);
+// stable sort of n (key, value) int pairs on keys[0, 2^bits)
+hipError_t int_sort_pairs(const int *kin, int *kout, const int *vin, int *vout, long n, int bits,
+                          void *tmp, size_t *tmp_bytes, hipStream_t stream);
 // exclusive prefix sum of n longs (tmp == nullptr: only sets *tmp_bytes)
 hipError_t ep_exclusive_scan(const long *in, long *out, long n, void *tmp, size_t *tmp_bytes,
                              hipStream_t stream);

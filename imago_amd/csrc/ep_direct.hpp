@@ -8,24 +8,31 @@
 // matrixSubs.F90:66-112, in makeValenceRho valeCharge.F90:230-393).  Its
 // post-SCF path folds the coefficients into one accumulation instead
 // (integrals.F90:2860-3605).  Here both per-iteration quantities come from
-// the primitive blocks of k_ep_prims, once per item:
+// the primitive blocks of k_ep_prims, once per item ((tuple, exponent)
+// pair of one potential type and atom pair):
 //
-//   k_epd_hsum   one wavefront per run of at most a fixed number of a
-//                pair's items (EpdSub): sum_items c'_j(item) * C1 C2 block,
-//                summed per image and Bloch-phased, written as the run's
-//                partial block; k_epd_hfinal adds NP, KE and the pair's
-//                partials in run order.
-//   k_epd_trace  one wavefront per run of one (pair, potential type): for
-//                each image X_e = sum_k Re(phase_k conj(rw_k,e)) with rw the
-//                weighted density of the block layout, and per item
-//                sum_e C1 C2 block_e X_e added to its term; the runs' sums
-//                are added per term in a fixed order by k_epd_trace_final.
-// Both require at most 4 k-points (or Gamma) so Q(k) and rw(k) of a lane's
-// entries stay in registers.
+//   set-up (once per system)  every pair's items sorted by lattice image
+//                (stable, so types stay in order within an image) into
+//                pitem, cut into runs of at most kEpdRunItems (EpdSub).
+//   k_epd_hsum   one wavefront per run: sum over its items of c'_j C1 C2
+//                block, summed per image and Bloch-phased, written as the
+//                run's partial block; k_epd_hfinal adds NP, KE and the
+//                pair's partials in run order.
+//   k_epd_fold   (k-points) X(image)_e = sum_k Re(conj(phase_k) rw_k,e), the
+//                weighted density folded onto each lattice image.
+//   k_epd_trace  one wavefront per run: per item sum_e C1 C2 block_e X_e,
+//                one value per item; k_epd_trace_final adds them per term
+//                in item order.
+// A wavefront decodes 64 items at once (item -> tuple -> image, exponents,
+// type) and then streams their blocks a few items at a time, so the block
+// loads of several items are in flight together.  The hsum accumulators of
+// at most 4 k-points (or Gamma) stay in registers.
 #pragma once
 #include "ep_kernel.hpp"
 
 namespace imago {
+
+constexpr int kEpdGroup = 4;   // items whose block loads are issued together (NE <= 4)
 
 // per-lane entry tables of one pair block: em1/em2 row/column states, eo
 // the offset in a primitive block, est 0 outside / 1 stored / 2 stored zero
@@ -48,123 +55,124 @@ struct EpdEntries {
     }
 };
 
-__device__ __forceinline__ void epd_tuple_fields(const EpTuple &t, int src, int &key, int &item0,
-                                                 unsigned long long &mask)
+// The pair's two contraction tables in LDS: B1[state][alpha] of atom i then
+// B2 of atom j (2 * max_states * max_alphas doubles per wavefront).
+__device__ __forceinline__ void epd_stage_basis(const DevSys &S, int t1, int t2, double *lds,
+                                                int lane)
 {
-    key = __builtin_amdgcn_readlane(t.key, src);
-    item0 = __builtin_amdgcn_readlane(t.item, src);
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)t.mask, src);
-    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(t.mask >> 32), src);
-    mask = ((unsigned long long)hi << 32) | lo;
-}
-
-// Trace runs: each (type, pair) task's items cut into runs of at most C
-// items.  The run's partial goes to slot roff[task] + r (type-major task =
-// type * npairs + pair, the order k_epd_trace_final sums per term); the
-// runs are stored for execution in pair-major order (roff2[pair * ntypes +
-// type] + r), so consecutive work-groups share the pair's rho block in L2.
-// count: rcnt[task] = rcnt2[pair * ntypes + type] = runs of the task.
-template <bool FILL>
-__global__ void k_epd_trace_runs(const long *__restrict__ iseg, long ntask, int npairs, long C,
-                                 long *__restrict__ rcnt, long *__restrict__ rcnt2,
-                                 const long *__restrict__ roff, const long *__restrict__ roff2,
-                                 EpdSub *__restrict__ runs)
-{
-    const long task = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (task >= ntask) return;
-    const long a = iseg[task], b = iseg[task + 1];
-    const long n = (b - a + C - 1) / C;
-    const int p = (int)(task % npairs), t = (int)(task / npairs);
-    const long ntypes = ntask / npairs;
-    const long pm = (long)p * ntypes + t;
-    if (!FILL) {
-        rcnt[task] = n;
-        rcnt2[pm] = n;
-        return;
+    const int n = S.max_states * S.max_alphas;
+    const double *B1 = S.type_basis + (size_t)t1 * n, *B2 = S.type_basis + (size_t)t2 * n;
+    for (int x = lane; x < n; x += 64) {
+        lds[x] = B1[x];
+        lds[n + x] = B2[x];
     }
-    for (long r = 0; r < n; ++r)
-        runs[roff2[pm] + r] = EpdSub{p, t, t + 1, 0, a + r * C, min(b, a + (r + 1) * C), roff[task] + r};
+    __syncthreads();
 }
 
-// H sum: each pair's items over all types cut in order into runs of at
-// most C items (a pair without items gets one empty run).  count:
-// rcnt[p] = runs, wcnt[p] = runs * block doubles; fill: the runs with
-// their partial offsets from woff.
+// Decode of item pitem[q] for this lane: the item, its lattice image, its
+// exponent pair key (a1 << 8 | a2), its potential exponent j and type.
+struct EpdItem {
+    int item, img, a12, j, type;
+};
+__device__ __forceinline__ EpdItem epd_decode(const EpTuple *__restrict__ tuples,
+                                              const int *__restrict__ item_tuple,
+                                              const int *__restrict__ pitem, long q, int npairs)
+{
+    EpdItem d;
+    d.item = pitem[q];
+    const EpTuple t = tuples[item_tuple[d.item]];
+    unsigned long long m = t.mask;
+    for (int r = d.item - t.item; r > 0; --r) m &= m - 1ull;
+    d.j = __ffsll((long long)m) - 1;
+    d.img = t.key >> 16;
+    d.a12 = t.key & 0xffff;
+    d.type = t.task / npairs;
+    return d;
+}
+
+__device__ __forceinline__ double epd_readlane(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// wave sum of a double (every lane gets the total)
+__device__ __forceinline__ double epd_wave_sum(double v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// set-up: image keys, pair ranges, runs
+
+// key[item] = global lattice image of the item (pairs[p].img_off + image),
+// val[item] = item, jexp[item] = its potential exponent.
+__global__ void k_epd_item_keys(const EpTuple *__restrict__ tuples,
+                                const int *__restrict__ item_tuple,
+                                const PairRec *__restrict__ pairs, long nitems, int npairs,
+                                int *__restrict__ key, int *__restrict__ val,
+                                unsigned char *__restrict__ jexp)
+{
+    const long it = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= nitems) return;
+    const EpTuple t = tuples[item_tuple[it]];
+    unsigned long long m = t.mask;
+    for (int r = (int)(it - t.item); r > 0; --r) m &= m - 1ull;
+    key[it] = pairs[t.task % npairs].img_off + (t.key >> 16);
+    val[it] = (int)it;
+    jexp[it] = (unsigned char)(__ffsll((long long)m) - 1);
+}
+
+__device__ __forceinline__ long epd_lower_bound(const int *__restrict__ a, long n, int x)
+{
+    long lo = 0, hi = n;
+    while (lo < hi) {
+        const long mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Runs of every pair's image-sorted items [lo, hi) of pitem, C items each.
+// H sum: at least one run per pair (an empty one for a pair without items),
+// partial blocks of blk doubles from woff[p]; traces: ceil(n / C) runs.
+// count: hcnt[p], wcnt[p] = hcnt[p] * blk, tcnt[p]; fill: the runs.
 template <bool FILL>
-__global__ void k_epd_hsum_runs(const long *__restrict__ iseg, int npairs, int ntypes, long C,
-                                const PairRec *__restrict__ pairs, const DevSys S,
-                                long *__restrict__ rcnt, long *__restrict__ wcnt,
-                                const long *__restrict__ roff, const long *__restrict__ woff,
-                                EpdSub *__restrict__ runs)
+__global__ void k_epd_pair_runs(const int *__restrict__ skey, long nitems,
+                                const PairRec *__restrict__ pairs, int npairs, const DevSys S,
+                                long C, long *__restrict__ hcnt, long *__restrict__ wcnt,
+                                long *__restrict__ tcnt, const long *__restrict__ hoff,
+                                const long *__restrict__ woff, const long *__restrict__ toff,
+                                EpdSub *__restrict__ hruns, EpdSub *__restrict__ truns)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npairs) return;
     const PairRec pr = pairs[p];
+    const long lo = epd_lower_bound(skey, nitems, pr.img_off);
+    const long hi = epd_lower_bound(skey, nitems, pr.img_off + pr.img_cnt);
     const long blk = (long)S.type_nstates[S.atom_type[pr.i]] * S.type_nstates[S.atom_type[pr.j]] *
-                     (S.nk) * (S.gamma ? 1 : 2);
-    long nr = 0, acc = 0;
-    bool open = false;
-    EpdSub cur{p, 0, 0, 0, 0, 0, 0};
-    const long r0 = FILL ? roff[p] : 0, w0 = FILL ? woff[p] : 0;
-    for (int t = 0; t < ntypes; ++t) {
-        long a = iseg[(long)t * npairs + p];
-        const long b = iseg[(long)t * npairs + p + 1];
-        while (a < b) {
-            if (!open) {
-                cur = EpdSub{p, t, t + 1, 0, a, b, w0 + nr * blk};
-                open = true;
-                acc = 0;
-            }
-            const long take = min(b - a, C - acc);
-            cur.t_hi = t + 1;
-            cur.i_hi = a + take;
-            acc += take;
-            a += take;
-            if (acc == C) {
-                if (FILL) runs[r0 + nr] = cur;
-                ++nr;
-                open = false;
-            }
-        }
-    }
-    if (open) {
-        if (FILL) runs[r0 + nr] = cur;
-        ++nr;
-    }
-    if (nr == 0) {
-        if (FILL) runs[r0] = EpdSub{p, 0, 0, 0, 0, 0, w0};
-        nr = 1;
-    }
+                     S.nk * (S.gamma ? 1 : 2);
+    const long nt = (hi - lo + C - 1) / C, nh = nt > 0 ? nt : 1;
     if (!FILL) {
-        rcnt[p] = nr;
-        wcnt[p] = nr * blk;
+        hcnt[p] = nh;
+        wcnt[p] = nh * blk;
+        tcnt[p] = nt;
+        return;
+    }
+    for (long r = 0; r < nh; ++r) {
+        const long a = lo + r * C, b = min(hi, a + C);
+        hruns[hoff[p] + r] = EpdSub{p, 0, 0, 0, a, a < b ? b : a, woff[p] + r * blk};
+        if (r < nt) truns[toff[p] + r] = EpdSub{p, 0, 0, 0, a, b, 0};
     }
 }
 
-// Walks, in order, the items of task [t0, t1) tuples that fall in [a, b):
-// f(image, a1, a2, exponent, item).  Tuples are in image order and their
-// items are numbered in tuple order, exponents ascending.
-template <class F>
-__device__ __forceinline__ void epd_walk(const EpTuple *__restrict__ tuples, long t0, long t1,
-                                         long a, long b, F &&f)
-{
-    const int lane = threadIdx.x & 63;
-    for (long tb = t0; tb < t1; tb += 64) {
-        EpTuple t{};
-        if (tb + lane < t1) t = tuples[tb + lane];
-        unsigned long long m = __ballot(tb + lane < t1);
-        while (m) {
-            const int src = __ffsll((long long)m) - 1;
-            m &= m - 1ull;
-            int key, item0;
-            unsigned long long mm;
-            epd_tuple_fields(t, src, key, item0, mm);
-            if (item0 >= b) return;
-            if (item0 + __popcll(mm) <= a) continue;
-            f(key >> 16, (key >> 8) & 255, key & 255, item0, mm);
-        }
-    }
-}
+// ---------------------------------------------------------------------------
+// H sum
 
 // part[sub.out ...] = [k][nent] (complex, or real for Gamma): the run's
 // sum of c'_j V_j(k) over its items, on its pair's block.
@@ -172,8 +180,8 @@ template <int LMAX, int NE, bool GAMMA>
 __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__restrict__ pairs,
                                                  const double2 *__restrict__ phase,
                                                  const EpTuple *__restrict__ tuples,
-                                                 const long *__restrict__ tseg,
-                                                 const long *__restrict__ iseg,
+                                                 const int *__restrict__ item_tuple,
+                                                 const int *__restrict__ pitem,
                                                  const double *__restrict__ blocks,
                                                  const double *__restrict__ coef_eff,
                                                  const int *__restrict__ type_term0,
@@ -183,21 +191,21 @@ __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__rest
     constexpr int SH = (LMAX + 1) * (LMAX + 1);
     constexpr int BS = SH * SH;
     constexpr int NQ = GAMMA ? 1 : 4;
+    constexpr int G = NE <= 4 ? kEpdGroup : 1;   // large blocks: one item at a time
+    extern __shared__ double bas[];   // B1, B2 of the pair
     const int lane = threadIdx.x;
     const long sidx = blockIdx.x;
     if (sidx >= nsubs) return;
     const EpdSub sub = subs[sidx];
-    const int pidx = sub.pair;
-    const PairRec pr = pairs[pidx];
+    const PairRec pr = pairs[sub.pair];
     const int t1 = S.atom_type[pr.i], t2 = S.atom_type[pr.j];
     const int ns1 = S.type_nstates[t1], ns2 = S.type_nstates[t2];
-    const int MA = S.max_alphas, MS = S.max_states;
+    const int MA = S.max_alphas, nb2 = S.max_states * MA;
     const int nk = GAMMA ? 1 : S.nk;
     const int nent = ns1 * ns2;
-    const double *B1 = S.type_basis + (size_t)t1 * MS * MA;
-    const double *B2 = S.type_basis + (size_t)t2 * MS * MA;
     EpdEntries<NE, SH> E;
     E.init(S, pr, t1, t2, ns1, nent, lane);
+    epd_stage_basis(S, t1, t2, bas, lane);
 
     double P[NE];
     double2 Q[NE][NQ];
@@ -226,55 +234,55 @@ __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__rest
             }
             P[e] = 0.0;
         }
-        cur_img = -1;
     };
 
-    for (int pt = sub.t_lo; pt < sub.t_hi; ++pt) {
-        const long task = (long)pt * npairs + pidx;
-        long a = iseg[task], b = iseg[task + 1];
-        if (pt == sub.t_lo && sub.i_lo > a) a = sub.i_lo;
-        if (pt == sub.t_hi - 1 && sub.i_hi < b) b = sub.i_hi;
-        if (a >= b) continue;
-        const double *cf = coef_eff + type_term0[pt];
-        flush();
-        epd_walk(tuples, tseg[task], tseg[task + 1], a, b,
-                 [&](int im, int a1, int a2, int item0, unsigned long long mm) {
-                     if (im != cur_img) {
-                         flush();
-                         cur_img = im;
-                     }
-                     double cc[NE];
+    for (long q0 = sub.i_lo; q0 < sub.i_hi; q0 += 64) {
+        const int nb = (int)min(64L, sub.i_hi - q0);
+        EpdItem d{0, 0, 0, 0, 0};
+        double cf = 0.0;
+        if (lane < nb) {
+            d = epd_decode(tuples, item_tuple, pitem, q0 + lane, npairs);
+            cf = coef_eff[type_term0[d.type] + d.j];
+        }
+        for (int u = 0; u < nb;) {
+            int it[G], im[G], a12[G];
+            double c[G];
 #pragma unroll
-                     for (int e = 0; e < NE; ++e)
-                         cc[e] = B1[E.em1[e] * MA + a1] * B2[E.em2[e] * MA + a2];
-                     // items of this tuple inside [a, b), two at a time so
-                     // their block loads overlap
-                     long it = item0;
-                     while (mm) {
-                         const int ja = __ffsll((long long)mm) - 1;
-                         mm &= mm - 1ull;
-                         const long ia = it++;
-                         if (ia < a) continue;
-                         if (ia >= b) break;
-                         const bool two = mm != 0ull && it < b;
-                         const int jb = two ? __ffsll((long long)mm) - 1 : ja;
-                         if (two) {
-                             mm &= mm - 1ull;
-                             ++it;
-                         }
-                         const double *ba = blocks + (size_t)ia * BS;
-                         const double *bb = blocks + (size_t)(ia + (two ? 1 : 0)) * BS;
-                         const double ca = cf[ja], cb = two ? cf[jb] : 0.0;
-                         double va[NE], vb[NE];
+            for (int g = 0; g < G; ++g) {
+                const int src = min(u + g, nb - 1);
+                it[g] = __builtin_amdgcn_readlane(d.item, src);
+                im[g] = __builtin_amdgcn_readlane(d.img, src);
+                a12[g] = __builtin_amdgcn_readlane(d.a12, src);
+                c[g] = epd_readlane(cf, src);
+            }
+            // items are image-ordered: this group is the prefix of m items
+            // on the first item's image
+            int m = 1;
 #pragma unroll
-                         for (int e = 0; e < NE; ++e) {
-                             va[e] = ba[E.eo[e]];
-                             vb[e] = bb[E.eo[e]];
-                         }
+            for (int g = 1; g < G; ++g)
+                if (u + g < nb && im[g] == im[0]) m = g + 1;
+            if (im[0] != cur_img) {
+                flush();
+                cur_img = im[0];
+            }
+            double v[G][NE];
 #pragma unroll
-                         for (int e = 0; e < NE; ++e) P[e] += cc[e] * (ca * va[e] + cb * vb[e]);
-                     }
-                 });
+            for (int g = 0; g < G; ++g) {
+                const double *bl = blocks + (size_t)it[g] * BS;
+#pragma unroll
+                for (int e = 0; e < NE; ++e) v[g][e] = bl[E.eo[e]];
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int a1 = a12[g] >> 8, a2 = a12[g] & 255;
+                const double cg = g < m ? c[g] : 0.0;
+#pragma unroll
+                for (int e = 0; e < NE; ++e)
+                    P[e] += cg * (bas[E.em1[e] * MA + a1] * bas[nb2 + E.em2[e] * MA + a2]) *
+                            v[g][e];
+            }
+            u += m;
+        }
     }
     flush();
     double *o = part + sub.out;
@@ -324,143 +332,190 @@ __global__ __launch_bounds__(256) void k_epd_hfinal(const DevSys S, const PairRe
     }
 }
 
-// wave sum of a double (every lane gets the total)
-__device__ __forceinline__ double epd_wave_sum(double v)
+// ---------------------------------------------------------------------------
+// traces
+
+// X[(img_off + image) * ent + e] = sum_k Re(conj(phase_k(image)) rw_k,e) for
+// every image of the pair (one work-group per pair); rw is the weighted
+// density in the block layout, [k][nbe] complex, at most 4 k-points.
+__global__ __launch_bounds__(256) void k_epd_fold(const DevSys S, const PairRec *__restrict__ pairs,
+                                                  const double2 *__restrict__ phase,
+                                                  const double2 *__restrict__ rw, int ent,
+                                                  double *__restrict__ X)
 {
+    const PairRec pr = pairs[blockIdx.x];
+    const int nent = S.type_nstates[S.atom_type[pr.i]] * S.type_nstates[S.atom_type[pr.j]];
+    const int nk = S.nk;
+    for (int e = threadIdx.x; e < nent; e += blockDim.x) {
+        double2 R[4];
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
+        for (int k = 0; k < 4; ++k)
+            R[k] = k < nk ? rw[(size_t)k * S.nbe + pr.blk + e] : make_double2(0.0, 0.0);
+        for (int im = 0; im < pr.img_cnt; ++im) {
+            const double2 *ph = phase + (size_t)(pr.img_off + im) * nk;
+            double x = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < nk) x += ph[k].x * R[k].x + ph[k].y * R[k].y;
+            X[(size_t)(pr.img_off + im) * ent + e] = x;
+        }
+    }
 }
 
-// tsub[sub.out][a] = sum over the run's items of exponent a of
-// sum_e C1 C2 block_e X_e(image), X from the weighted density rw
-// (runs of one potential type).
+// ival[item] = sum_e C1 C2 block_e X_e(image of the item) for every item of
+// the run; X from k_epd_fold ([image][ent]), or for Gamma the weighted
+// density rw itself (block layout, the same for every image).
 template <int LMAX, int NE, bool GAMMA>
 __global__ __launch_bounds__(64) void k_epd_trace(DevSys S, const PairRec *__restrict__ pairs,
-                                                  const double2 *__restrict__ phase,
                                                   const EpTuple *__restrict__ tuples,
-                                                  const long *__restrict__ tseg,
-                                                  const long *__restrict__ iseg,
+                                                  const int *__restrict__ item_tuple,
+                                                  const int *__restrict__ pitem,
                                                   const double *__restrict__ blocks,
-                                                  const double *__restrict__ rw,
+                                                  const double *__restrict__ X, int ent,
                                                   const EpdSub *__restrict__ subs, long nsubs,
-                                                  int npairs, double *__restrict__ tsub)
+                                                  int npairs, double *__restrict__ ival)
 {
     constexpr int SH = (LMAX + 1) * (LMAX + 1);
     constexpr int BS = SH * SH;
-    constexpr int NQ = GAMMA ? 1 : 4;
-    extern __shared__ double smem[];   // [max_pot_alphas][64] per-lane term sums
+    constexpr int G = NE <= 4 ? kEpdGroup : 1;   // large blocks: one item at a time
+    extern __shared__ double bas[];
     const int lane = threadIdx.x;
     const long sidx = blockIdx.x;
     if (sidx >= nsubs) return;
     const EpdSub sub = subs[sidx];
-    const int pidx = sub.pair, pt = sub.t_lo;
-    const PairRec pr = pairs[pidx];
+    const PairRec pr = pairs[sub.pair];
     const int t1 = S.atom_type[pr.i], t2 = S.atom_type[pr.j];
     const int ns1 = S.type_nstates[t1], ns2 = S.type_nstates[t2];
-    const int MA = S.max_alphas, MS = S.max_states;
-    const int nk = GAMMA ? 1 : S.nk;
+    const int MA = S.max_alphas, nb2 = S.max_states * MA;
     const int nent = ns1 * ns2;
-    const int na3 = S.pot_nalpha_of(pt);
-    const double *B1 = S.type_basis + (size_t)t1 * MS * MA;
-    const double *B2 = S.type_basis + (size_t)t2 * MS * MA;
     EpdEntries<NE, SH> E;
     E.init(S, pr, t1, t2, ns1, nent, lane);
-    double *T = smem;
-    for (int a = 0; a < na3; ++a) T[a * 64 + lane] = 0.0;
-
-    // the pair's weighted density, every k
-    double2 R[NE][NQ];
-#pragma unroll
-    for (int e = 0; e < NE; ++e)
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-            R[e][k] = make_double2(0.0, 0.0);
-            if (E.est[e] != 0 && k < nk) {
-                const size_t o = (size_t)k * S.nbe + pr.blk + lane + 64 * e;
-                R[e][k] = GAMMA ? make_double2(rw[o], 0.0) : reinterpret_cast<const double2 *>(rw)[o];
-            }
-        }
-    double X[NE];
+    epd_stage_basis(S, t1, t2, bas, lane);
+    // the density of the current image (for Gamma the same for all images)
+    double Xc[NE];
     int cur_img = -1;
+    if (GAMMA) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) Xc[e] = E.est[e] != 0 ? X[pr.blk + lane + 64 * e] : 0.0;
+    }
 
-    const long task = (long)pt * npairs + pidx;
-    const long a = sub.i_lo, b = sub.i_hi;
-    epd_walk(tuples, tseg[task], tseg[task + 1], a, b,
-             [&](int im, int a1, int a2, int item0, unsigned long long mm) {
-                 if (im != cur_img) {
-                     cur_img = im;
-                     const double2 *ph = phase + (size_t)(pr.img_off + im) * nk;
+    for (long q0 = sub.i_lo; q0 < sub.i_hi; q0 += 64) {
+        const int nb = (int)min(64L, sub.i_hi - q0);
+        EpdItem d{0, 0, 0, 0, 0};
+        if (lane < nb) d = epd_decode(tuples, item_tuple, pitem, q0 + lane, npairs);
+        for (int u = 0; u < nb;) {
+            int it[G], im[G], a12[G];
 #pragma unroll
-                     for (int e = 0; e < NE; ++e) {
-                         double x = 0.0;
+            for (int g = 0; g < G; ++g) {
+                const int src = min(u + g, nb - 1);
+                it[g] = __builtin_amdgcn_readlane(d.item, src);
+                im[g] = __builtin_amdgcn_readlane(d.img, src);
+                a12[g] = __builtin_amdgcn_readlane(d.a12, src);
+            }
+            // items are image-ordered: this group is the prefix of m items
+            // on the first item's image
+            int m = 1;
 #pragma unroll
-                         for (int k = 0; k < NQ; ++k)
-                             if (k < nk) {
-                                 const double2 f = GAMMA ? make_double2(1.0, 0.0) : ph[k];
-                                 x += f.x * R[e][k].x + f.y * R[e][k].y;
-                             }
-                         X[e] = x;
-                     }
-                 }
-                 double cx[NE];
+            for (int g = 1; g < G; ++g)
+                if (u + g < nb && im[g] == im[0]) m = g + 1;
+            if (!GAMMA && im[0] != cur_img) {
+                cur_img = im[0];
+                const double *xi = X + (size_t)(pr.img_off + cur_img) * ent;
 #pragma unroll
-                 for (int e = 0; e < NE; ++e)
-                     cx[e] = B1[E.em1[e] * MA + a1] * B2[E.em2[e] * MA + a2] * X[e];
-                 long it = item0;
-                 while (mm) {
-                     const int ja = __ffsll((long long)mm) - 1;
-                     mm &= mm - 1ull;
-                     const long ia = it++;
-                     if (ia < a) continue;
-                     if (ia >= b) break;
-                     const bool two = mm != 0ull && it < b;
-                     const int jb = two ? __ffsll((long long)mm) - 1 : ja;
-                     if (two) {
-                         mm &= mm - 1ull;
-                         ++it;
-                     }
-                     const double *ba = blocks + (size_t)ia * BS;
-                     const double *bb = blocks + (size_t)(ia + (two ? 1 : 0)) * BS;
-                     double sa = 0.0, sb = 0.0;
+                for (int e = 0; e < NE; ++e) Xc[e] = E.est[e] != 0 ? xi[lane + 64 * e] : 0.0;
+            }
+            double v[G][NE];
 #pragma unroll
-                     for (int e = 0; e < NE; ++e) {
-                         sa += cx[e] * ba[E.eo[e]];
-                         sb += cx[e] * bb[E.eo[e]];
-                     }
-                     T[ja * 64 + lane] += sa;
-                     if (two) T[jb * 64 + lane] += sb;
-                 }
-             });
-    for (int x = 0; x < na3; ++x) {
-        const double v = epd_wave_sum(T[x * 64 + lane]);
-        if (lane == 0) tsub[(size_t)sub.out * S.max_pot_alphas + x] = v;
+            for (int g = 0; g < G; ++g) {
+                const double *bl = blocks + (size_t)it[g] * BS;
+#pragma unroll
+                for (int e = 0; e < NE; ++e) v[g][e] = bl[E.eo[e]];
+            }
+            double s[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int a1 = a12[g] >> 8, a2 = a12[g] & 255;
+                s[g] = 0.0;
+#pragma unroll
+                for (int e = 0; e < NE; ++e)
+                    s[g] += bas[E.em1[e] * MA + a1] * bas[nb2 + E.em2[e] * MA + a2] * Xc[e] *
+                            v[g][e];
+            }
+            double tot;
+            int slot;
+            if (G == 4) {
+                // four wave sums at once: halve the values per lane at the
+                // xor-32 and xor-16 steps, then sum within 16 lanes; lane
+                // 16 g ends with item g's total
+                const bool hi32 = lane >= 32, hi16 = (lane & 16) != 0;
+                const double x0 = __shfl_xor(hi32 ? s[0] : s[2 % G], 32, 64);
+                const double x1 = __shfl_xor(hi32 ? s[1 % G] : s[3 % G], 32, 64);
+                const double a0 = (hi32 ? s[2 % G] : s[0]) + x0;
+                const double a1v = (hi32 ? s[3 % G] : s[1 % G]) + x1;
+                const double y = __shfl_xor(hi16 ? a0 : a1v, 16, 64);
+                tot = (hi16 ? a1v : a0) + y;
+#pragma unroll
+                for (int dd = 8; dd > 0; dd >>= 1) tot += __shfl_xor(tot, dd, 64);
+                slot = (lane >> 4) & 3;
+            } else {
+                tot = epd_wave_sum(s[0]);
+                slot = 0;
+            }
+            const bool writer = G == 4 ? (lane & 15) == 0 : lane == 0;
+            if (writer && slot < m) {
+                int iv = it[0];
+#pragma unroll
+                for (int g = 1; g < G; ++g)
+                    if (slot == g) iv = it[g];
+                ival[iv] = tot;
+            }
+            u += m;
+        }
     }
 }
 
-// out[term] = sum over the runs of the term's potential type (runs
-// roff[type * npairs] .. roff[(type + 1) * npairs]) of their sums, a
-// fixed-order tree
-__global__ __launch_bounds__(256) void k_epd_trace_final(const double *__restrict__ tsub,
-                                                         const long *__restrict__ roff, int npairs,
-                                                         const int *__restrict__ term_type,
-                                                         const int *__restrict__ term_alpha,
+// Sum of ival per term in two fixed-order passes.  Pass 1: work-group
+// (type, chunk) of 64 lanes takes chunk c of kEpdTraceChunks equal parts of
+// the type's items (iseg[type * npairs] .. iseg[(type + 1) * npairs]), each
+// lane a fixed stride, T[j][lane] in LDS, then a wave sum per exponent j:
+// part[(type * chunks + c) * max_pot_alphas + j].  Pass 2 (one thread per
+// term) adds the chunks in order: out[type_term0[type] + j].
+__global__ __launch_bounds__(64) void k_epd_trace_chunks(const double *__restrict__ ival,
+                                                         const unsigned char *__restrict__ jexp,
+                                                         const long *__restrict__ iseg,
+                                                         int npairs,
+                                                         const int *__restrict__ pot_nalpha,
                                                          int max_pot_alphas,
-                                                         double *__restrict__ out)
+                                                         double *__restrict__ part)
 {
-    __shared__ double red[256];
-    const int term = blockIdx.x;
-    const int pt = term_type[term], aj = term_alpha[term];
-    const long s0 = roff[(long)pt * npairs], s1 = roff[(long)(pt + 1) * npairs];
-    double s = 0.0;
-    for (long q = s0 + threadIdx.x; q < s1; q += 256) s += tsub[(size_t)q * max_pot_alphas + aj];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    extern __shared__ double T[];   // [na][64]
+    const int type = blockIdx.x / kEpdTraceChunks, ch = blockIdx.x % kEpdTraceChunks;
+    const int lane = threadIdx.x;
+    const int na = pot_nalpha[type];
+    for (int j = 0; j < na; ++j) T[j * 64 + lane] = 0.0;
+    const long a0 = iseg[(long)type * npairs], b0 = iseg[(long)(type + 1) * npairs];
+    const long len = (b0 - a0 + kEpdTraceChunks - 1) / kEpdTraceChunks;
+    const long a = a0 + ch * len, b = min(b0, a + len);
+    for (long it = a + lane; it < b; it += 64) T[jexp[it] * 64 + lane] += ival[it];
+    double *o = part + ((size_t)type * kEpdTraceChunks + ch) * max_pot_alphas;
+    for (int j = 0; j < na; ++j) {
+        const double v = epd_wave_sum(T[j * 64 + lane]);
+        if (lane == 0) o[j] = v;
     }
-    if (threadIdx.x == 0) out[term] = red[0];
+}
+
+__global__ void k_epd_trace_final(const double *__restrict__ part,
+                                  const int *__restrict__ term_type,
+                                  const int *__restrict__ term_alpha, int pot_dim,
+                                  int max_pot_alphas, double *__restrict__ out)
+{
+    const int term = blockIdx.x * blockDim.x + threadIdx.x;
+    if (term >= pot_dim) return;
+    const int type = term_type[term], j = term_alpha[term];
+    const double *p = part + (size_t)type * kEpdTraceChunks * max_pot_alphas + j;
+    double s = 0.0;
+    for (int c = 0; c < kEpdTraceChunks; ++c) s += p[(size_t)c * max_pot_alphas];
+    out[term] = s;
 }
 
 }  // namespace imago

@@ -41,6 +41,7 @@ struct EpTuple {
 };
 
 constexpr int kEpMaxPotAlphas = 64;
+constexpr int kEpdTraceChunks = 8;   // parts of a potential type's items in the trace sums
 constexpr int kNumXcd = 8;
 
 __host__ __device__ inline size_t ep_screen_lds_bytes(int ma)

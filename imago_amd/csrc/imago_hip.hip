@@ -176,15 +176,18 @@ struct imago_ctx {
     bool ep_direct = false;
     bool ep_runs = false;         // run tables below built
     double *coef_eff = nullptr;   // [pot_dim]
+    int *pitem = nullptr;         // [items] items sorted by (pair, lattice image)
+    unsigned char *item_jexp = nullptr;   // [items] potential exponent of each item
     EpdSub *hruns = nullptr;      // H sum: runs in pair order
     long nhruns = 0;
     long *hrun_lo = nullptr;      // [pairs + 1] first run of each pair
     long *hrun_woff = nullptr;    // [pairs + 1] first partial double of each pair
     double *hpart = nullptr;      // run partial blocks
-    EpdSub *truns = nullptr;      // traces: runs in potential-type order
+    EpdSub *truns = nullptr;      // traces: runs in pair order
     long ntruns = 0;
-    long *trun_lo = nullptr;      // [types * pairs + 1] first run of each (type, pair) task
-    double *tpart = nullptr;      // [runs][max_pot_alphas]
+    double *ival = nullptr;       // [items] trace value of each item
+    double *foldX = nullptr;      // [images][64 * ne] density folded per image
+    double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
     double timing[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
 };
@@ -206,9 +209,11 @@ static void ctx_free_system(imago_ctx *c)
     c->ep_screened = false;
     c->ep_direct = false;
     c->ep_runs = false;
-    c->coef_eff = c->hpart = c->tpart = nullptr;
+    c->coef_eff = c->hpart = c->ival = c->foldX = c->tchunk = nullptr;
+    c->pitem = nullptr;
+    c->item_jexp = nullptr;
     c->hruns = c->truns = nullptr;
-    c->hrun_lo = c->trun_lo = c->hrun_woff = nullptr;
+    c->hrun_lo = c->hrun_woff = nullptr;
     c->nhruns = c->ntruns = 0;
     c->pairs = nullptr;
     c->type_term0 = nullptr;
@@ -948,71 +953,79 @@ static int ep_prepare(imago_ctx *c)
     return IMAGO_OK;
 }
 
-// Runs of at most kEpdRunItems items for the direct EP kernels, built on
-// the device from the per-task item offsets (k_epd_hsum_runs,
-// k_epd_trace_runs: count, scan, fill).  For the H sum each pair's items
-// over all potential types are cut in order (partials in pair order); for
-// the traces each (type, pair) task is cut separately (runs in type order,
-// those of type t from trun_lo[t * npairs]).
+// Run tables of the direct EP path, once per system: every item's lattice
+// image as a key, a stable radix sort by it (so each pair's items are
+// contiguous and image-ordered, types in order within an image), and the
+// runs of at most kEpdRunItems over each pair's items: H-sum runs with
+// their partial offsets, trace runs, and every item's potential exponent.
 static int ep_build_runs(imago_ctx *c)
 {
     if (c->ep_runs) return IMAGO_OK;
     const HostSys &h = c->h;
     const int np = c->npairs;
-    const long ntask = (long)np * h.npt, C = kEpdRunItems;
-    long *rcnt = nullptr, *wcnt = nullptr, *woff = nullptr, *hlo = nullptr, *tcnt = nullptr,
-         *tlo = nullptr, *tcnt2 = nullptr, *tlo2 = nullptr;
-    RCHK(dev_alloc(c, &rcnt, (size_t)np + 1));
+    const long ni = c->ep_nitems, C = kEpdRunItems;
+    if (c->nimgs >= (1L << 30)) return fail(IMAGO_E_ARG, "%ld lattice images", c->nimgs);
+    int bits = 1;
+    while ((1L << bits) <= c->nimgs) ++bits;
+    int *key = nullptr, *skey = nullptr, *val = nullptr;
+    long *hcnt = nullptr, *wcnt = nullptr, *tcnt = nullptr, *hoff = nullptr, *woff = nullptr,
+         *toff = nullptr;
+    const size_t nis = (size_t)std::max(ni, 1L);
+    RCHK(dev_alloc(c, &key, nis));
+    RCHK(dev_alloc(c, &skey, nis));
+    RCHK(dev_alloc(c, &val, nis));
+    RCHK(dev_alloc(c, &c->pitem, nis));
+    RCHK(dev_alloc(c, &c->item_jexp, nis));
+    RCHK(dev_alloc(c, &c->ival, nis));
+    RCHK(dev_alloc(c, &hcnt, (size_t)np + 1));
     RCHK(dev_alloc(c, &wcnt, (size_t)np + 1));
+    RCHK(dev_alloc(c, &tcnt, (size_t)np + 1));
+    RCHK(dev_alloc(c, &hoff, (size_t)np + 1));
     RCHK(dev_alloc(c, &woff, (size_t)np + 1));
-    RCHK(dev_alloc(c, &hlo, (size_t)np + 1));
-    RCHK(dev_alloc(c, &tcnt, (size_t)ntask + 1));
-    RCHK(dev_alloc(c, &tlo, (size_t)ntask + 1));
-    RCHK(dev_alloc(c, &tcnt2, (size_t)ntask + 1));
-    RCHK(dev_alloc(c, &tlo2, (size_t)ntask + 1));
-    HIPCHK(hipMemsetAsync(tcnt2 + ntask, 0, sizeof(long), c->stream));
-    HIPCHK(hipMemsetAsync(rcnt + np, 0, sizeof(long), c->stream));
-    HIPCHK(hipMemsetAsync(wcnt + np, 0, sizeof(long), c->stream));
-    HIPCHK(hipMemsetAsync(tcnt + ntask, 0, sizeof(long), c->stream));
-    HIPCHK(launch_epd_hsum_runs(false, c->ep_iseg, np, h.npt, C, c->pairs, c->d, rcnt, wcnt,
-                                nullptr, nullptr, nullptr, c->stream));
-    HIPCHK(launch_epd_trace_runs(false, c->ep_iseg, ntask, np, C, tcnt, tcnt2, nullptr, nullptr,
-                                 nullptr, c->stream));
-    size_t tb = 0, t2 = 0, t3 = 0;
-    HIPCHK(ep_exclusive_scan(rcnt, hlo, np + 1, nullptr, &tb, c->stream));
+    RCHK(dev_alloc(c, &toff, (size_t)np + 1));
+    HIPCHK(launch_epd_item_keys(c->ep_tuples, c->ep_item_tuple, c->pairs, ni, np, key, val,
+                                c->item_jexp, c->stream));
+    size_t sb = 0, t1 = 0, t2 = 0, t3 = 0;
+    HIPCHK(int_sort_pairs(key, skey, val, c->pitem, ni, bits, nullptr, &sb, c->stream));
+    HIPCHK(ep_exclusive_scan(hcnt, hoff, np + 1, nullptr, &t1, c->stream));
     HIPCHK(ep_exclusive_scan(wcnt, woff, np + 1, nullptr, &t2, c->stream));
-    HIPCHK(ep_exclusive_scan(tcnt, tlo, ntask + 1, nullptr, &t3, c->stream));
-    tb = std::max(tb, std::max(t2, t3));
+    HIPCHK(ep_exclusive_scan(tcnt, toff, np + 1, nullptr, &t3, c->stream));
+    const size_t tb = std::max(std::max(sb, t1), std::max(t2, t3));
     char *tmp = nullptr;
     RCHK(dev_alloc(c, &tmp, tb));
-    HIPCHK(ep_exclusive_scan(rcnt, hlo, np + 1, tmp, &tb, c->stream));
-    HIPCHK(ep_exclusive_scan(wcnt, woff, np + 1, tmp, &tb, c->stream));
-    HIPCHK(ep_exclusive_scan(tcnt, tlo, ntask + 1, tmp, &tb, c->stream));
-    HIPCHK(ep_exclusive_scan(tcnt2, tlo2, ntask + 1, tmp, &tb, c->stream));
+    size_t tbx = tb;
+    if (ni > 0) HIPCHK(int_sort_pairs(key, skey, val, c->pitem, ni, bits, tmp, &tbx, c->stream));
+    HIPCHK(hipMemsetAsync(hcnt + np, 0, sizeof(long), c->stream));
+    HIPCHK(hipMemsetAsync(wcnt + np, 0, sizeof(long), c->stream));
+    HIPCHK(hipMemsetAsync(tcnt + np, 0, sizeof(long), c->stream));
+    HIPCHK(launch_epd_pair_runs(false, skey, ni, c->pairs, np, c->d, C, hcnt, wcnt, tcnt, nullptr,
+                                nullptr, nullptr, nullptr, nullptr, c->stream));
+    tbx = tb;
+    HIPCHK(ep_exclusive_scan(hcnt, hoff, np + 1, tmp, &tbx, c->stream));
+    tbx = tb;
+    HIPCHK(ep_exclusive_scan(wcnt, woff, np + 1, tmp, &tbx, c->stream));
+    tbx = tb;
+    HIPCHK(ep_exclusive_scan(tcnt, toff, np + 1, tmp, &tbx, c->stream));
     long tot[3] = {0, 0, 0};
-    HIPCHK(hipMemcpyAsync(&tot[0], hlo + np, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&tot[0], hoff + np, sizeof(long), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&tot[1], woff + np, sizeof(long), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(&tot[2], tlo + ntask, sizeof(long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&tot[2], toff + np, sizeof(long), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (std::max(tot[0], tot[2]) * 64 > 0xffffffffL)
+        return fail(IMAGO_E_ARG, "%ld / %ld EP runs exceed one dispatch", tot[0], tot[2]);
     RCHK(dev_alloc(c, &c->hruns, (size_t)std::max(tot[0], 1L)));
     RCHK(dev_alloc(c, &c->truns, (size_t)std::max(tot[2], 1L)));
     RCHK(dev_alloc(c, &c->hpart, (size_t)std::max(tot[1], 1L)));
-    RCHK(dev_alloc(c, &c->tpart, (size_t)std::max(tot[2], 1L) * h.maxpa));
-    HIPCHK(launch_epd_hsum_runs(true, c->ep_iseg, np, h.npt, C, c->pairs, c->d, nullptr, nullptr,
-                                hlo, woff, c->hruns, c->stream));
-    HIPCHK(launch_epd_trace_runs(true, c->ep_iseg, ntask, np, C, nullptr, nullptr, tlo, tlo2,
-                                 c->truns, c->stream));
+    HIPCHK(launch_epd_pair_runs(true, skey, ni, c->pairs, np, c->d, C, nullptr, nullptr, nullptr,
+                                hoff, woff, toff, c->hruns, c->truns, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (void *p : {(void *)rcnt, (void *)wcnt, (void *)tcnt, (void *)tcnt2, (void *)tlo2,
-                    (void *)tmp})
+    for (void *p : {(void *)key, (void *)skey, (void *)val, (void *)hcnt, (void *)wcnt,
+                    (void *)tcnt, (void *)toff, (void *)tmp})
         dev_release(c, p);
-    c->hrun_lo = hlo;
+    c->hrun_lo = hoff;
     c->hrun_woff = woff;
-    c->trun_lo = tlo;
     c->nhruns = tot[0];
     c->ntruns = tot[2];
-    if (std::max(tot[0], tot[2]) * 64 > 0xffffffffL)
-        return fail(IMAGO_E_ARG, "%ld / %ld EP runs exceed one dispatch", tot[0], tot[2]);
     c->ep_runs = true;
     return IMAGO_OK;
 }
@@ -1411,9 +1424,11 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     if (direct) RCHK(ep_build_runs(c));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (direct) {
-        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg, c->ep_iseg,
-                    c->ep_blocks, c->coef_eff, nullptr, c->type_term0, c->hruns, c->nhruns,
-                    c->hpart, c->npairs, c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
+        EpdLaunch L{c->d,          c->pairs,     c->phase,       c->ep_tuples,
+                    c->ep_item_tuple, c->pitem, c->ep_blocks,   c->coef_eff,
+                    nullptr,       0,            c->type_term0,  c->hruns,
+                    c->nhruns,     c->hpart,     c->npairs,      c->lmax,
+                    c->ne <= 4 ? 4 : 16,         h.gamma != 0,   c->stream};
         HIPCHK(launch_epd_hsum(L));
         HIPCHK(hipEventRecord(c->ev2, c->stream));
         HIPCHK(launch_epd_hfinal(L, c->panel[2], c->panel[1], c->hrun_lo, c->hrun_woff, c->blockH));
@@ -1626,6 +1641,24 @@ __global__ void k_trace_final(const double *__restrict__ partial, int nblk, int 
     out[t] = s;
 }
 
+// the same sum for few terms with many chunks: one work-group per term,
+// 256 fixed strides then a tree
+__global__ __launch_bounds__(256) void k_trace_final_wide(const double *__restrict__ partial,
+                                                          int nblk, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const int t = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(size_t)t * nblk + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[t] = red[0];
+}
+
 extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, double *traces3,
                                     double *pot_rho)
 {
@@ -1669,11 +1702,14 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     if (nocc == 0) {
         HIPCHK(hipMemsetAsync(c->rho, 0, sizeof(double) * nn * h.nk * dsz, c->stream));
     } else if (cx) {
-        const double one = 1.0, zero = 0.0;
-        BLASCHK(rocblas_zherk_strided_batched(
-            c->blas, rocblas_fill_upper, rocblas_operation_none, h.n, nocc, &one,
-            (const rocblas_double_complex *)W, h.n, (rocblas_stride)h.n * std::max(nocc, 1), &zero,
-            (rocblas_double_complex *)c->rho, h.n, (rocblas_stride)nn, h.nk));
+        // rho = W W^H as a full product: rocBLAS's zherk runs at ~1 TFLOP/s
+        // here (n = 896, k = 130), its zgemm near the FP64 matrix rate
+        const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+        const rocblas_stride sw = (rocblas_stride)h.n * std::max(nocc, 1);
+        BLASCHK(rocblas_zgemm_strided_batched(
+            c->blas, rocblas_operation_none, rocblas_operation_conjugate_transpose, h.n, h.n, nocc,
+            &one, (const rocblas_double_complex *)W, h.n, sw, (const rocblas_double_complex *)W,
+            h.n, sw, &zero, (rocblas_double_complex *)c->rho, h.n, (rocblas_stride)nn, h.nk));
     } else {
         const double one = 1.0, zero = 0.0;
         BLASCHK(rocblas_dsyrk(c->blas, rocblas_fill_upper, rocblas_operation_none, h.n, nocc, &one,
@@ -1686,9 +1722,12 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
         hipLaunchKernelGGL(k_pack_weighted<false>, dim3(c->npairs, 1), dim3(256), 0, c->stream,
                            c->rho, h.n, c->nbe, c->pairs, c->d.cumul, h.natoms, c->rhow);
     HIPCHK(hipGetLastError());
-    const int nblk = 64;
+    // chunks per term: the three single-panel traces get enough work-groups
+    // to fill the device on their own; the potDim panel terms 64 each
+    const int nblk3 = 1024, nblk = 64;
     const int nterms = h.pot_dim + 3;
-    if (!c->partial) RCHK(dev_alloc(c, &c->partial, (size_t)nterms * nblk));
+    if (!c->partial) RCHK(dev_alloc(c, &c->partial, (size_t)3 * nblk3 + (size_t)h.pot_dim * nblk));
+    double *part_ep = c->partial + (size_t)3 * nblk3;
     double *outd = nullptr;
     RCHK(dev_alloc(c, &outd, (size_t)nterms));
     // S, NP, KE (one term each), then the potential terms: from the stored
@@ -1696,25 +1735,36 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     const bool direct = !c->assembled[3];
     const int order3[3] = {0, 2, 1};
     for (int q = 0; q < 3; ++q)
-        hipLaunchKernelGGL(k_trace_partial<1>, dim3(nblk, 1), dim3(256), 0, c->stream,
-                           c->panel[order3[q]], per, 1, c->rhow, nblk, c->partial + (size_t)q * nblk);
+        hipLaunchKernelGGL(k_trace_partial<1>, dim3(nblk3, 1), dim3(256), 0, c->stream,
+                           c->panel[order3[q]], per, 1, c->rhow, nblk3,
+                           c->partial + (size_t)q * nblk3);
     if (h.pot_dim > 0 && !direct)
         hipLaunchKernelGGL(k_trace_partial<1>, dim3(nblk, h.pot_dim), dim3(256), 0, c->stream,
-                           c->panel[3], per, h.pot_dim, c->rhow, nblk,
-                           c->partial + (size_t)3 * nblk);
+                           c->panel[3], per, h.pot_dim, c->rhow, nblk, part_ep);
     if (h.pot_dim > 0 && direct) {
         RCHK(ep_build_runs(c));
-        EpdLaunch L{c->d, c->pairs, c->phase, c->ep_tuples, c->ep_tseg, c->ep_iseg,
-                    c->ep_blocks, nullptr, c->rhow, c->type_term0, c->truns, c->ntruns,
-                    c->tpart, c->npairs, c->lmax, c->ne <= 4 ? 4 : 16, h.gamma != 0, c->stream};
+        const int ne = c->ne <= 4 ? 4 : 16, ent = 64 * ne;
+        if (!h.gamma && !c->foldX) RCHK(dev_alloc(c, &c->foldX, (size_t)c->nimgs * ent));
+        EpdLaunch L{c->d,          c->pairs,     c->phase,       c->ep_tuples,
+                    c->ep_item_tuple, c->pitem, c->ep_blocks,   nullptr,
+                    h.gamma ? c->rhow : c->foldX, ent, c->type_term0, c->truns,
+                    c->ntruns,     c->ival,      c->npairs,      c->lmax,
+                    ne,            h.gamma != 0, c->stream};
         HIPCHK(hipEventRecord(c->ev2, c->stream));
+        HIPCHK(launch_epd_fold(L, c->rhow, c->foldX));
         HIPCHK(launch_epd_trace(L));
         HIPCHK(hipEventRecord(c->ev3, c->stream));
-        HIPCHK(launch_epd_trace_final(c->tpart, c->trun_lo, c->npairs, c->d.term_type,
-                                      c->d.term_alpha, h.maxpa, h.pot_dim, outd + 3, c->stream));
+        if (!c->tchunk)
+            RCHK(dev_alloc(c, &c->tchunk, (size_t)h.npt * kEpdTraceChunks * h.maxpa));
+        HIPCHK(launch_epd_trace_final(c->ival, c->item_jexp, c->ep_iseg, c->npairs, h.npt, h.maxpa,
+                                      c->d.pot_nalpha, c->d.term_type, c->d.term_alpha,
+                                      h.pot_dim, c->tchunk, outd + 3, c->stream));
     }
-    hipLaunchKernelGGL(k_trace_final, dim3((nterms + 255) / 256), dim3(256), 0, c->stream, c->partial,
-                       nblk, direct ? 3 : nterms, outd);
+    hipLaunchKernelGGL(k_trace_final_wide, dim3(3), dim3(256), 0, c->stream, c->partial, nblk3,
+                       outd);
+    if (h.pot_dim > 0 && !direct)
+        hipLaunchKernelGGL(k_trace_final, dim3((h.pot_dim + 255) / 256), dim3(256), 0, c->stream,
+                           part_ep, nblk, h.pot_dim, outd + 3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1786,7 +1836,8 @@ extern "C" int imago_dsygv(imago_ctx *c, int n, double *A, double *B, double *w,
     return single_gv(c, false, n, A, B, w, info);
 }
 
-extern "C" int imago_ep_counts(imago_ctx *c, long *tuples, long *items, int *block_doubles)
+extern "C" int imago_ep_counts(imago_ctx *c, long *tuples, long *items, int *block_doubles,
+                               long *partial_doubles)
 {
     if (!c || !c->has_system) return fail(IMAGO_E_STATE, "no system");
     if (!c->ep_screened) return fail(IMAGO_E_STATE, "EP terms not assembled");
@@ -1794,6 +1845,12 @@ extern "C" int imago_ep_counts(imago_ctx *c, long *tuples, long *items, int *blo
     if (tuples) *tuples = c->ep_ntuples;
     if (items) *items = c->ep_nitems;
     if (block_doubles) *block_doubles = sh * sh;
+    if (partial_doubles) {
+        long w = 0;
+        if (c->ep_runs)
+            HIPCHK(hipMemcpy(&w, c->hrun_woff + c->npairs, sizeof(long), hipMemcpyDeviceToHost));
+        *partial_doubles = w;
+    }
     return IMAGO_OK;
 }
 

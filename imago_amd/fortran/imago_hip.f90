@@ -154,11 +154,11 @@ module O_ImagoHIP
          real(c_double), intent(out) :: ms
       end function imago_last_timing
 
-      integer(c_int) function imago_ep_counts(ctx, tuples, items, block_doubles) &
-            & bind(C, name="imago_ep_counts")
+      integer(c_int) function imago_ep_counts(ctx, tuples, items, block_doubles, &
+            & partial_doubles) bind(C, name="imago_ep_counts")
          import :: c_int, c_ptr, c_long
          type(c_ptr), value :: ctx
-         integer(c_long), intent(out) :: tuples, items
+         integer(c_long), intent(out) :: tuples, items, partial_doubles
          integer(c_int), intent(out) :: block_doubles
       end function imago_ep_counts
    end interface

@@ -78,7 +78,8 @@ def load():
     L.imago_dsygv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _ip]
     L.imago_last_timing.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
     L.imago_ep_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long),
-                                  ctypes.POINTER(ctypes.c_long), _ip]
+                                  ctypes.POINTER(ctypes.c_long), _ip,
+                                  ctypes.POINTER(ctypes.c_long)]
     _lib = L
     return L
 
@@ -205,10 +206,12 @@ class Context:
         return v.value
 
     def ep_counts(self):
-        """(tuples, items, doubles per primitive block) of the EP screening."""
-        t, i, b = ctypes.c_long(), ctypes.c_long(), ctypes.c_int()
-        _check(load().imago_ep_counts(self._h, ctypes.byref(t), ctypes.byref(i), ctypes.byref(b)))
-        return t.value, i.value, b.value
+        """(tuples, items, doubles per primitive block, doubles of the direct
+        H-sum run partials) of the EP screening."""
+        t, i, b, w = ctypes.c_long(), ctypes.c_long(), ctypes.c_int(), ctypes.c_long()
+        _check(load().imago_ep_counts(self._h, ctypes.byref(t), ctypes.byref(i), ctypes.byref(b),
+                                      ctypes.byref(w)))
+        return t.value, i.value, b.value, w.value
 
     # -- single problems ----------------------------------------------------
     def hegv(self, H, S):

@@ -205,9 +205,12 @@ int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
  * IMAGO_EP_DIRECT assembly): screened (image, alpha pair, candidate)
- * tuples, (tuple, exponent) primitive items, and doubles per stored
- * primitive block.  For measurement (bytes moved per launch). */
-int imago_ep_counts(imago_ctx *ctx, long *tuples, long *items, int *block_doubles);
+ * tuples, (tuple, exponent) primitive items, doubles per stored primitive
+ * block, and (IMAGO_EP_DIRECT, else 0) doubles of the H-sum run partial
+ * blocks written per H build.  Any output may be NULL.  For measurement
+ * (bytes moved per launch). */
+int imago_ep_counts(imago_ctx *ctx, long *tuples, long *items, int *block_doubles,
+                    long *partial_doubles);
 
 #ifdef __cplusplus
 }
