@@ -6,16 +6,17 @@
 // Each matrix lives in the registers of P = ceil(n / 16) work-groups for the
 // whole reduction: work-group w holds rows 16w .. 16w+15, thread t of the
 // work-group row 16w + (t & 15) at the columns (t >> 4) + 32 s, s < S.  One
-// column step is two per-matrix barriers:
-//   A. the work-group holding row j builds the reflector from conj(row j)
-//      (zlarfg) and publishes v and tau;
-//   B. every work-group forms p = tau A v on its rows and the partial
-//      p^H v, and publishes both;
-//   C. every work-group forms w = p - tau/2 (p^H v) v and applies
-//      A -= v w^H + w v^H to its rows.
-// Only v, p and the dot partials go through memory; the matrix does not.
-// The tridiagonal (d, e), tau and the reflectors (LAPACK's lower layout,
-// consumed by zunmtr) are written as they are produced.
+// column step is one exchange between the work-groups of a matrix:
+//   every work-group holds the pivot row j of the current matrix in LDS and
+//   builds the reflector from conj(row j) (zlarfg) itself, bit-identically;
+//   forms p = tau A v on its own rows and the partial p^H v;
+//   publishes p, the partial and (the owner of row j+1) row j+1;
+//   after the barrier reads p, the partials and row j+1, forms
+//   w = p - tau/2 (p^H v) v, applies A -= v w^H + w v^H to its rows and
+//   the same update to row j+1, the next pivot row.
+// Only p, the dot partials and one row per step go through memory, double
+// buffered by step parity.  The tridiagonal (d, e), tau and the reflectors
+// (LAPACK's lower layout, consumed by zunmtr) are written by work-group 0.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -27,14 +28,16 @@ constexpr int kTrdGroups = 32;
 
 struct TrdArgs {
     int n, P;             // order; work-groups per matrix (ceil(n / 16))
+    int nb_launch;        // matrices of this launch
     const double2 *A;     // [nb][n * n] column-major, full Hermitian
     double2 *V;           // [nb][n * n] reflectors below the subdiagonal
     double *d, *e;        // [nb][n] diagonal, subdiagonal (e[n-1] = 0)
     double2 *tau;         // [nb][n] (tau[n-1] = 0)
-    double2 *vbuf;        // [nb][n] published v (and tau at index j)
-    double2 *pbuf;        // [nb][n] published p
-    double2 *dbuf;        // [nb][P] published dot partials
-    unsigned *bar;        // [nb] barrier counters, zero at launch
+    double2 *vbuf;        // [nb][2][n] published pivot rows
+    double2 *pbuf;        // [nb][2][n] published p
+    double2 *dbuf;        // [nb][2][64] published dot partials
+    unsigned *bar;        // [nb][P][kTrdFlagStride] barrier flags, then one abort
+                          // word; zero at launch
     int *err;             // set to 1 when a barrier wait times out
     int spin_limit;       // polls before a barrier wait gives up
 };
@@ -75,26 +78,37 @@ __device__ __forceinline__ double2 pub_load(const double2 *p)
     return make_double2(__longlong_as_double((long long)x), __longlong_as_double((long long)y));
 }
 
-// Barrier of the P work-groups of one matrix: monotonic counter, target =
-// P * (barriers passed so far).  Every thread first waits until its
-// published stores have completed; one thread per work-group then counts
-// in and polls.  A wait that exceeds spin_limit polls sets the counter's top
-// bit (every later wait of that matrix then passes at once) and *err, so
-// the kernel always ends.
-__device__ __forceinline__ void trd_barrier(unsigned *cnt, unsigned target, int *err, int limit)
+// Barrier of the P work-groups of one matrix without a shared counter:
+// work-group w stores the number of the step it has published into its own
+// flag (one 64-byte line each) after every thread's published stores have
+// completed; lanes 0 .. P-1 of wave 0 poll the P flags of the matrix in
+// parallel until all have reached the step.  A wait longer than limit
+// polls sets the abort word (every later wait of the launch then passes at
+// once) and *err, so the kernel always ends.
+constexpr int kTrdFlagStride = 16;   // unsigned per flag: one 64-byte line
+__device__ __forceinline__ void trd_barrier(unsigned *flags, int w, int P, unsigned step,
+                                            unsigned *abortw, int *err, int limit)
 {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int t = threadIdx.x;
+    if (t == 0)
+        __hip_atomic_store(flags + (size_t)w * kTrdFlagStride, step, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    if (t < 64) {
+        unsigned *f = flags + (size_t)(t < P ? t : 0) * kTrdFlagStride;
         int polls = 0;
-        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (true) {
+            const unsigned v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned ab = __hip_atomic_load(abortw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__all(v >= step || ab != 0u)) break;
             __builtin_amdgcn_s_sleep(1);
             if (++polls > limit) {
-                __hip_atomic_fetch_or(cnt, 0x80000000u, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t == 0) {
+                    __hip_atomic_store(abortw, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
                 break;
             }
         }
@@ -103,16 +117,26 @@ __device__ __forceinline__ void trd_barrier(unsigned *cnt, unsigned target, int 
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// wave sum of a double; every lane gets the total
+__device__ __forceinline__ double trd_wave_sum(double v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
 // grid = nb * P work-groups of kTrdThreads, all co-resident (cooperative
 // launch); n <= 32 * S.
 template <int S>
 __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
 {
-    __shared__ double2 vl[kTrdGroups * S];   // v (tau at index j)
-    __shared__ double2 pl[kTrdGroups * S];   // p
-    __shared__ double2 red[kTrdThreads / 64][kTrdRows];
+    constexpr int NW = kTrdThreads / 64;
+    __shared__ double2 vl[kTrdGroups * S];   // v of this step
+    __shared__ double2 pl[kTrdGroups * S];   // p of this step
+    __shared__ double2 rl[kTrdGroups * S];   // the pivot row
+    __shared__ double2 red[NW][kTrdRows];
     __shared__ double2 prow[kTrdRows];
-    __shared__ double2 bcast;
+    __shared__ double2 bc[4];                // tau, (beta, d), scale, dot
     const int n = T.n, P = T.P;
     const int m = blockIdx.x / P, w = blockIdx.x % P;
     const int t = threadIdx.x, r = t & (kTrdRows - 1), g = t >> 4;
@@ -121,9 +145,11 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
     const size_t nn = (size_t)n * n;
     const double2 *A = T.A + m * nn;
     double2 *V = T.V + m * nn;
-    double2 *vbuf = T.vbuf + (size_t)m * n, *pbuf = T.pbuf + (size_t)m * n;
-    double2 *dbuf = T.dbuf + (size_t)m * P;
-    unsigned *bar = T.bar + m;
+    double2 *rbuf = T.vbuf + (size_t)m * 2 * n, *pbuf = T.pbuf + (size_t)m * 2 * n;
+    double2 *dbuf = T.dbuf + (size_t)m * 2 * 64;
+    unsigned *flags = T.bar + (size_t)m * P * kTrdFlagStride;
+    unsigned *abortw = T.bar + (size_t)T.nb_launch * P * kTrdFlagStride;
+    const bool out = w == 0;   // work-group 0 writes d, e, tau and V
 
     double2 a[S];
 #pragma unroll
@@ -131,89 +157,59 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
         const int c = g + kTrdGroups * s;
         a[s] = (i < n && c < n) ? A[(size_t)c * n + i] : trd_zero();
     }
-    unsigned nbar = 0;
+    // pivot row 0: A(0, c) for every column
+    for (int c = t; c < n; c += kTrdThreads) rl[c] = A[(size_t)c * n];
+    __syncthreads();
+    unsigned nstep = 0;
     for (int j = 0; j + 1 < n; ++j) {
-        // ---- A: reflector from conj(row j) ----
-        if (w == (j >> 4)) {
-            const bool mine = r == (j & (kTrdRows - 1));
-            double2 nd = trd_zero();      // (|x(2:)|^2, A(j,j))
-            double2 al = trd_zero();      // alpha = conj(A(j, j+1))
-            if (mine) {
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int c = g + kTrdGroups * s;
-                    if (c >= j + 2 && c < n) nd.x += a[s].x * a[s].x + a[s].y * a[s].y;
-                    if (c == j + 1) al = make_double2(a[s].x, -a[s].y);
-                    if (c == j) nd.y = a[s].x;
-                }
+        const int slot = j & 1;
+        // ---- reflector from conj(row j): |x(2:)|^2 over the work-group ----
+        double x2 = 0.0;
+        for (int c = j + 2 + t; c < n; c += kTrdThreads) x2 += rl[c].x * rl[c].x + rl[c].y * rl[c].y;
+        x2 = trd_wave_sum(x2);
+        if (lane == 0) red[wave][0].x = x2;
+        __syncthreads();
+        if (t == 0) {
+            double xn2 = 0.0;
+            for (int q = 0; q < NW; ++q) xn2 += red[q][0].x;
+            const double2 alpha = make_double2(rl[j + 1].x, -rl[j + 1].y);
+            double beta, tr, ti;
+            double2 scale;
+            if (xn2 == 0.0 && alpha.y == 0.0) {
+                beta = alpha.x;
+                tr = ti = 0.0;
+                scale = trd_zero();
+            } else {
+                beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+                tr = (beta - alpha.x) / beta;
+                ti = -alpha.y / beta;
+                const double zr = alpha.x - beta, zi = alpha.y;
+                const double den = zr * zr + zi * zi;
+                scale = make_double2(zr / den, -zi / den);
             }
-            // the 32 threads of row j: lanes rj, rj+16, rj+32, rj+48 of each wave
-            nd.x += __shfl_xor(nd.x, 16);
-            nd.y += __shfl_xor(nd.y, 16);
-            al = make_double2(al.x + __shfl_xor(al.x, 16), al.y + __shfl_xor(al.y, 16));
-            nd.x += __shfl_xor(nd.x, 32);
-            nd.y += __shfl_xor(nd.y, 32);
-            al = make_double2(al.x + __shfl_xor(al.x, 32), al.y + __shfl_xor(al.y, 32));
-            if (mine && lane < kTrdRows) {
-                red[wave][0] = nd;
-                red[wave][1] = al;
+            bc[0] = make_double2(tr, ti);
+            bc[1] = make_double2(beta, rl[j].x);
+            bc[2] = scale;
+        }
+        __syncthreads();
+        const double2 tau = bc[0];
+        {
+            const double2 scale = bc[2];
+            for (int c = j + 1 + t; c < n; c += kTrdThreads) {
+                const double2 v = c == j + 1 ? make_double2(1.0, 0.0)
+                                             : trd_mul(scale, make_double2(rl[c].x, -rl[c].y));
+                vl[c] = v;
+                if (out) V[(size_t)j * n + c] = v;
             }
-            __syncthreads();
-            if (mine) {
-                double2 s2 = trd_zero(), alpha = trd_zero();
-                for (int q = 0; q < kTrdThreads / 64; ++q) {
-                    s2.x += red[q][0].x;
-                    s2.y += red[q][0].y;
-                    alpha.x += red[q][1].x;
-                    alpha.y += red[q][1].y;
-                }
-                const double xn2 = s2.x, djj = s2.y;
-                double beta, tr, ti;
-                double2 scale;
-                if (xn2 == 0.0 && alpha.y == 0.0) {
-                    beta = alpha.x;
-                    tr = ti = 0.0;
-                    scale = trd_zero();
-                } else {
-                    beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
-                    tr = (beta - alpha.x) / beta;
-                    ti = -alpha.y / beta;
-                    // 1 / (alpha - beta)
-                    const double zr = alpha.x - beta, zi = alpha.y;
-                    const double den = zr * zr + zi * zi;
-                    scale = make_double2(zr / den, -zi / den);
-                }
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int c = g + kTrdGroups * s;
-                    if (c >= j + 1 && c < n)
-                        vl[c] = c == j + 1 ? make_double2(1.0, 0.0)
-                                           : trd_mul(scale, make_double2(a[s].x, -a[s].y));
-                }
-                if (g == 0) {
-                    vl[j] = make_double2(tr, ti);
-                    bcast = make_double2(beta, djj);
-                }
-            }
-            __syncthreads();
-            // publish v and tau; reflector column j (zhetrd 'L' layout)
-            for (int c = j + t; c < n; c += kTrdThreads) {
-                const double2 v = vl[c];
-                pub_store(vbuf + c, v);
-                if (c > j) V[(size_t)j * n + c] = v;
-            }
-            if (t == 0) {
-                T.tau[(size_t)m * n + j] = vl[j];
-                T.e[(size_t)m * n + j] = bcast.x;
-                T.d[(size_t)m * n + j] = bcast.y;
+            if (out && t == 0) {
+                T.tau[(size_t)m * n + j] = tau;
+                T.e[(size_t)m * n + j] = bc[1].x;
+                T.d[(size_t)m * n + j] = bc[1].y;
             }
         }
-        trd_barrier(bar, (unsigned)P * ++nbar, T.err, T.spin_limit);
-
-        // ---- B: p = tau A v on this work-group's rows, partial p^H v ----
-        for (int c = j + t; c < n; c += kTrdThreads) vl[c] = pub_load(vbuf + c);
         __syncthreads();
-        const double2 tau = vl[j];
+
+        // ---- p = tau A v on this work-group's rows, partial p^H v ----
         const bool row_on = i > j && i < n;
         double2 sp = trd_zero();
         if (row_on) {
@@ -225,7 +221,6 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
                     sp.x += q.x;
                     sp.y += q.y;
                 }
-                // keep the scheduler from hoisting every LDS read at once
                 if ((s & 3) == 3) __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -235,38 +230,52 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
         __syncthreads();
         if (t < kTrdRows) {
             double2 p = trd_zero();
-            for (int q = 0; q < kTrdThreads / 64; ++q) {
+            for (int q = 0; q < NW; ++q) {
                 p.x += red[q][t].x;
                 p.y += red[q][t].y;
             }
             p = row_on ? trd_mul(tau, p) : trd_zero();
             prow[t] = p;
-            if (row_on) pub_store(pbuf + i, p);
-            // conj(p_i) v_i summed over the 16 rows (lanes 0..15 of wave 0),
-            // the same tree in every work-group
+            if (row_on) pub_store(pbuf + slot * n + i, p);
+            // conj(p_i) v_i over the 16 rows, the same tree in every work-group
             double2 dp = row_on ? trd_mulc(vl[i], p) : trd_zero();
             for (int o = 8; o >= 1; o >>= 1) {
                 const double2 x = trd_shfl_xor(dp, o);
                 dp = make_double2(dp.x + x.x, dp.y + x.y);
             }
-            if (t == 0) pub_store(dbuf + w, dp);
+            if (t == 0) pub_store(dbuf + slot * 64 + w, dp);
         }
-        trd_barrier(bar, (unsigned)P * ++nbar, T.err, T.spin_limit);
+        // the owner of row j+1 publishes it (columns j+1 ..), staged in LDS
+        // so that every thread issues a few of the stores
+        if (w == ((j + 1) >> 4)) {
+            if (r == ((j + 1) & (kTrdRows - 1))) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int c = g + kTrdGroups * s;
+                    if (c > j && c < n) pl[c] = a[s];
+                }
+            }
+            __syncthreads();
+            for (int c = j + 1 + t; c < n; c += kTrdThreads) pub_store(rbuf + slot * n + c, pl[c]);
+        }
+        trd_barrier(flags, w, P, ++nstep, abortw, T.err, T.spin_limit);
 
-        // ---- C: w = p - tau/2 (p^H v) v; A -= v w^H + w v^H ----
-        for (int c = j + 1 + t; c < n; c += kTrdThreads) pl[c] = pub_load(pbuf + c);
+        // ---- w = p - tau/2 (p^H v) v; A -= v w^H + w v^H; next pivot row ----
+        for (int c = j + 1 + t; c < n; c += kTrdThreads) {
+            pl[c] = pub_load(pbuf + slot * n + c);
+            rl[c] = pub_load(rbuf + slot * n + c);
+        }
         if (wave == 0) {
-            double2 x = lane < P ? pub_load(dbuf + lane) : trd_zero();
+            double2 x = lane < P ? pub_load(dbuf + slot * 64 + lane) : trd_zero();
             for (int o = 32; o >= 1; o >>= 1) {
                 const double2 y = trd_shfl_xor(x, o);
                 x = make_double2(x.x + y.x, x.y + y.y);
             }
-            if (lane == 0) bcast = x;
+            if (lane == 0) bc[3] = x;
         }
         __syncthreads();
+        const double2 a2 = trd_mul(make_double2(-0.5 * tau.x, -0.5 * tau.y), bc[3]);
         if (row_on) {
-            const double2 dot = bcast;
-            const double2 a2 = trd_mul(make_double2(-0.5 * tau.x, -0.5 * tau.y), dot);
             const double2 vi = vl[i];
             const double2 pi = prow[r];
             const double2 q0 = trd_mul(a2, vi);
@@ -285,17 +294,26 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
                 if ((s & 3) == 3) __builtin_amdgcn_sched_barrier(0);
             }
         }
+        // row j+1 of the updated matrix (v_{j+1} = 1): the next pivot row
+        {
+            const double2 q0 = a2;   // a2 * v_{j+1}
+            const double2 wj1 = make_double2(pl[j + 1].x + q0.x, pl[j + 1].y + q0.y);
+            __syncthreads();
+            for (int c = j + 1 + t; c < n; c += kTrdThreads) {
+                const double2 vc = vl[c];
+                const double2 q1 = trd_mul(a2, vc);
+                const double2 wc = make_double2(pl[c].x + q1.x, pl[c].y + q1.y);
+                const double2 u1 = make_double2(wc.x, -wc.y);   // 1 * conj(w_c)
+                const double2 u2 = trd_mulc(wj1, vc);
+                rl[c] = make_double2(rl[c].x - (u1.x + u2.x), rl[c].y - (u1.y + u2.y));
+            }
+        }
         __syncthreads();
     }
-    // last diagonal entry
-    if (i == n - 1) {
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-            if (g + kTrdGroups * s == n - 1) {
-                T.d[(size_t)m * n + n - 1] = a[s].x;
-                T.e[(size_t)m * n + n - 1] = 0.0;
-                T.tau[(size_t)m * n + n - 1] = make_double2(0.0, 0.0);
-            }
+    if (out && t == 0) {
+        T.d[(size_t)m * n + n - 1] = rl[n - 1].x;
+        T.e[(size_t)m * n + n - 1] = 0.0;
+        T.tau[(size_t)m * n + n - 1] = trd_zero();
     }
 }
 

@@ -1277,62 +1277,88 @@ static int factor_gv_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int 
     return IMAGO_OK;
 }
 
-__global__ void k_real_to_complex(const double *__restrict__ z, double2 *__restrict__ x, long count)
-{
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < count) x[t] = make_double2(z[t], 0.0);
-}
-
 // Hermitian eigendecomposition C = Y E Y^H of a batch of order n <= 1024,
-// the three stages of LAPACK zheevd: the cooperative reduction C = Q T Q^H
-// (k_hetrd_coop, T real tridiagonal), divide and conquer on T per matrix
-// (dstedc, T = Z E Z^T) and Y = Q Z (zunmtr).  A holds C on entry and Y on
-// exit; evals the eigenvalues in ascending order; dinfo dstedc's info.
-// Selected with IMAGO_HEEVD=coop.  rocSOLVER's batched zheevd stays the
-// default: at n = 896 x 4 the cooperative reduction takes 19.2 ms against
-// zhetrd's 20.5 ms, and the non-batched dstedc (15.0 ms) and zunmtr
-// (13.8 ms) calls make the whole solve slower than batched zheevd (29.9 ms).
+// the three stages of LAPACK zheevd:
+//   C = Q T Q^H, T real tridiagonal: k_hetrd_coop, one persistent
+//     cooperative kernel for the whole batch;
+//   T = Z E Z^T: the batch's tridiagonals as one block-diagonal problem
+//     (e[n-1] = 0 splits it) for divide and conquer (dstedc), its
+//     eigenvectors sorted back into their matrices (launch_trd_unmix);
+//   Y = Q Z: the reflectors applied in compact-WY blocks of kTrdBlock
+//     (zlarft factors by k_trd_larft, then W = V^H Y, W = T W, Y -= V W as
+//     batched zgemm), from the last block to the first.
+// A holds C on entry and Y on exit; evals the eigenvalues in ascending
+// order; dinfo[0] dstedc's info.
+// The default for orders up to kTrdMaxOrder: at 4 x 896 it takes 25 ms
+// where rocSOLVER's batched zheevd takes 30 (IMAGO_HEEVD=rocsolver selects
+// the latter).
 static bool use_coop_heevd(int n)
 {
     const char *env = getenv("IMAGO_HEEVD");
-    return hetrd_coop_supported(n) && env && strcmp(env, "coop") == 0;
+    return hetrd_coop_supported(n) && !(env && strcmp(env, "rocsolver") == 0);
 }
 
 static int heevd_coop(imago_ctx *c, int n, int nb, double2 *A, double *evals, double *ework,
                       int *dinfo)
 {
     const size_t nn = (size_t)n * n;
-    double2 *V = nullptr, *tau = nullptr, *vbuf = nullptr, *pbuf = nullptr, *dbuf = nullptr;
-    double *Z = nullptr;
+    const int N = n * nb, K = kTrdBlock, nblk = (n - 1 + K - 1) / K;
+    double2 *V = nullptr, *tau = nullptr, *vbuf = nullptr, *pbuf = nullptr, *dbuf = nullptr,
+            *Tf = nullptr, *W = nullptr, *W2 = nullptr;
+    double *Z = nullptr, *wall = nullptr;
+    int *owner = nullptr, *rank = nullptr;
     unsigned *bar = nullptr;
     int *err = nullptr;
     RCHK(dev_alloc(c, &V, nn * nb));
-    RCHK(dev_alloc(c, &Z, nn * nb));
+    RCHK(dev_alloc(c, &Z, (size_t)N * N));
+    RCHK(dev_alloc(c, &wall, (size_t)N));
+    RCHK(dev_alloc(c, &owner, (size_t)N));
+    RCHK(dev_alloc(c, &rank, (size_t)N));
     RCHK(dev_alloc(c, &tau, (size_t)n * nb));
-    RCHK(dev_alloc(c, &vbuf, (size_t)n * nb));
-    RCHK(dev_alloc(c, &pbuf, (size_t)n * nb));
-    RCHK(dev_alloc(c, &dbuf, (size_t)64 * nb));
-    RCHK(dev_alloc(c, &bar, (size_t)nb));
+    RCHK(dev_alloc(c, &vbuf, (size_t)2 * n * nb));
+    RCHK(dev_alloc(c, &pbuf, (size_t)2 * n * nb));
+    RCHK(dev_alloc(c, &dbuf, (size_t)2 * 64 * nb));
+    RCHK(dev_alloc(c, &Tf, (size_t)nb * std::max(nblk, 1) * K * K));
+    RCHK(dev_alloc(c, &W, (size_t)nb * K * n));
+    RCHK(dev_alloc(c, &W2, (size_t)nb * K * n));
+    RCHK(dev_alloc(c, &bar, trd_flag_words(n, nb)));
     RCHK(dev_alloc(c, &err, 1));
     HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int) * nb, c->stream));
+    HIPCHK(hipMemsetAsync(V, 0, sizeof(double2) * nn * nb, c->stream));
     TrdLaunch L{n, nb, A, V, evals, ework, tau, vbuf, pbuf, dbuf, bar, err, c->stream};
     HIPCHK(launch_hetrd_coop(L));
-    for (int k = 0; k < nb; ++k)
-        BLASCHK(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, n, evals + (size_t)k * n,
-                                 ework + (size_t)k * n, Z + k * nn, n, dinfo + k));
-    const long cnt = (long)(nn * nb);
-    hipLaunchKernelGGL(k_real_to_complex, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0,
-                       c->stream, Z, A, cnt);
-    HIPCHK(hipGetLastError());
-    for (int k = 0; k < nb; ++k)
-        BLASCHK(rocsolver_zunmtr(c->blas, rocblas_side_left, rocblas_fill_lower,
-                                 rocblas_operation_none, n, n, (rocblas_double_complex *)(V + k * nn),
-                                 n, (rocblas_double_complex *)(tau + (size_t)k * n),
-                                 (rocblas_double_complex *)(A + k * nn), n));
+    // d and e of the batch are back to back, e[n-1] = 0 between problems
+    HIPCHK(hipMemcpyAsync(wall, evals, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
+    BLASCHK(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, N, wall, ework, Z, N, dinfo));
+    HIPCHK(launch_trd_unmix(Z, wall, n, nb, owner, rank, evals, A, c->stream));
+    HIPCHK(launch_trd_larft(V, tau, n, nb, Tf, c->stream));
+    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0), mone(-1.0, 0.0);
+    auto *Vz = (rocblas_double_complex *)V, *Az = (rocblas_double_complex *)A;
+    auto *Wz = (rocblas_double_complex *)W, *W2z = (rocblas_double_complex *)W2;
+    auto *Tz = (rocblas_double_complex *)Tf;
+    const rocblas_stride snn = (rocblas_stride)nn, sw = (rocblas_stride)K * n,
+                         st = (rocblas_stride)std::max(nblk, 1) * K * K;
+    for (int b = nblk - 1; b >= 0; --b) {
+        const int j0 = b * K, kb = std::min(K, n - 1 - j0), r0 = j0 + 1, mb = n - r0;
+        const rocblas_double_complex *Vb = Vz + (size_t)j0 * n + r0;
+        rocblas_double_complex *Yb = Az + r0;
+        BLASCHK(rocblas_zgemm_strided_batched(c->blas, rocblas_operation_conjugate_transpose,
+                                              rocblas_operation_none, kb, n, mb, &one, Vb, n, snn,
+                                              Yb, n, snn, &zero, Wz, K, sw, nb));
+        BLASCHK(rocblas_zgemm_strided_batched(c->blas, rocblas_operation_none,
+                                              rocblas_operation_none, kb, n, kb, &one,
+                                              Tz + (size_t)b * K * K, K, st, Wz, K, sw, &zero,
+                                              W2z, K, sw, nb));
+        BLASCHK(rocblas_zgemm_strided_batched(c->blas, rocblas_operation_none,
+                                              rocblas_operation_none, mb, n, kb, &mone, Vb, n, snn,
+                                              W2z, K, sw, &one, Yb, n, snn, nb));
+    }
     int herr = 0;
     HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (void *p : {(void *)V, (void *)Z, (void *)tau, (void *)vbuf, (void *)pbuf, (void *)dbuf,
+    for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank, (void *)tau,
+                    (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf, (void *)W, (void *)W2,
                     (void *)bar, (void *)err})
         dev_release(c, p);
     if (herr) return fail(IMAGO_E_HIP, "tridiagonal reduction: a barrier wait timed out");

@@ -230,17 +230,20 @@ def test_single_hegv_orders(gpu_ctx, oracle_mod, n):
     assert np.allclose(H @ V, S @ V * w, atol=1e-8 * scale)
 
 
+@pytest.mark.parametrize("mode", ["coop", "rocsolver"])
 @pytest.mark.parametrize("n", [2, 17, 97, 300, 1024])
-def test_coop_tridiagonal_path_orders(gpu_ctx, oracle_mod, monkeypatch, n):
-    """The cooperative tridiagonal reduction (IMAGO_HEEVD=coop: k_hetrd_coop,
-    dstedc, zunmtr) against LAPACK zhegv, up to its largest order."""
-    monkeypatch.setenv("IMAGO_HEEVD", "coop")
+def test_eigensolver_paths_orders(gpu_ctx, oracle_mod, monkeypatch, mode, n):
+    """Both Hermitian eigensolvers against LAPACK zhegv up to the largest
+    order of the cooperative one: the default (cooperative tridiagonal
+    reduction, block-diagonal divide and conquer, compact-WY back-transform)
+    and rocSOLVER's zheevd (IMAGO_HEEVD=rocsolver)."""
+    monkeypatch.setenv("IMAGO_HEEVD", mode)
     rng = np.random.default_rng(300 + n)
     M = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     H = M + M.conj().T
     R = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     S = R @ R.conj().T + n * np.eye(n)
-    print("[gpu] coop hegv n=%d" % n, flush=True)
+    print("[gpu] %s hegv n=%d" % (mode, n), flush=True)
     w, V = gpu_ctx.hegv(H, S)
     w2, _, info = oracle_mod.hegv(H, S)
     assert info == 0
@@ -250,9 +253,9 @@ def test_coop_tridiagonal_path_orders(gpu_ctx, oracle_mod, monkeypatch, n):
     assert np.allclose(H @ V, S @ V * w, atol=1e-8 * scale)
 
 
-def test_coop_tridiagonal_path_batched_secular(gpu_ctx, monkeypatch):
-    """A batch of k-points through the cooperative path gives the default
-    path's eigenvalues and density traces."""
+def test_eigensolver_paths_batched_secular(gpu_ctx, monkeypatch):
+    """A batch of k-points through the cooperative eigensolver gives
+    rocSOLVER's eigenvalues and density traces."""
     s = configs.small_test_system(seed=12)
     n = s.vale_dim
     coeffs = configs.pot_coefficients(s, seed=5)
