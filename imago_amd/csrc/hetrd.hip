@@ -131,14 +131,14 @@ __global__ __launch_bounds__(256) void k_trd_gather(const double *__restrict__ Z
     if (threadIdx.x == 0) w[(size_t)b * n + p] = wall[q];
 }
 
-hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, int *owner,
-                            int *rank, double *w, double2 *X, hipStream_t stream)
+hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, int b0,
+                            int *owner, int *rank, double *w, double2 *X, hipStream_t stream)
 {
     const int N = n * nb;
     hipLaunchKernelGGL(k_trd_owner, dim3(N), dim3(256), 0, stream, Z, N, n, owner);
     hipLaunchKernelGGL(k_trd_rank, dim3(1), dim3(1024), 0, stream, owner, N, nb, rank);
-    hipLaunchKernelGGL(k_trd_gather, dim3(N), dim3(256), 0, stream, Z, wall, N, n, owner, rank, w,
-                       X);
+    hipLaunchKernelGGL(k_trd_gather, dim3(N), dim3(256), 0, stream, Z, wall, N, n, owner, rank,
+                       w + (size_t)b0 * n, X + (size_t)b0 * n * n);
     return hipGetLastError();
 }
 

@@ -82,9 +82,9 @@ __device__ __forceinline__ double2 pub_load(const double2 *p)
 // work-group w stores the number of the step it has published into its own
 // flag (one 64-byte line each) after every thread's published stores have
 // completed; lanes 0 .. P-1 of wave 0 poll the P flags of the matrix in
-// parallel until all have reached the step.  A wait longer than limit
-// polls sets the abort word (every later wait of the launch then passes at
-// once) and *err, so the kernel always ends.
+// parallel until all have reached the step.  Every 64 polls the abort word
+// is read; a wait longer than limit polls sets it (every later wait of the
+// launch then passes within 64 polls) and *err, so the kernel always ends.
 constexpr int kTrdFlagStride = 16;   // unsigned per flag: one 64-byte line
 __device__ __forceinline__ void trd_barrier(unsigned *flags, int w, int P, unsigned step,
                                             unsigned *abortw, int *err, int limit)
@@ -101,16 +101,19 @@ __device__ __forceinline__ void trd_barrier(unsigned *flags, int w, int P, unsig
         int polls = 0;
         while (true) {
             const unsigned v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned ab = __hip_atomic_load(abortw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__all(v >= step || ab != 0u)) break;
-            __builtin_amdgcn_s_sleep(1);
-            if (++polls > limit) {
-                if (t == 0) {
-                    __hip_atomic_store(abortw, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__all(v >= step)) break;
+            if ((++polls & 63) == 0) {
+                if (__hip_atomic_load(abortw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                    break;
+                if (polls > limit) {
+                    if (t == 0) {
+                        __hip_atomic_store(abortw, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    break;
                 }
-                break;
             }
+            __builtin_amdgcn_s_sleep(1);
         }
     }
     __syncthreads();
@@ -136,7 +139,8 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
     __shared__ double2 rl[kTrdGroups * S];   // the pivot row
     __shared__ double2 red[NW][kTrdRows];
     __shared__ double2 prow[kTrdRows];
-    __shared__ double2 bc[4];                // tau, (beta, d), scale, dot
+    __shared__ double nrm[kTrdThreads / 64];  // partial |x(2:)|^2 per wave
+    __shared__ double2 bc[4];                // [3]: the dot p^H v
     const int n = T.n, P = T.P;
     const int m = blockIdx.x / P, w = blockIdx.x % P;
     const int t = threadIdx.x, r = t & (kTrdRows - 1), g = t >> 4;
@@ -167,47 +171,41 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
         double x2 = 0.0;
         for (int c = j + 2 + t; c < n; c += kTrdThreads) x2 += rl[c].x * rl[c].x + rl[c].y * rl[c].y;
         x2 = trd_wave_sum(x2);
-        if (lane == 0) red[wave][0].x = x2;
+        if (lane == 0) nrm[wave] = x2;
         __syncthreads();
-        if (t == 0) {
+        // every thread forms the reflector from the same partials, so all
+        // work-groups and threads hold bit-identical beta, tau and scale
+        double2 tau, scale;
+        double beta;
+        {
             double xn2 = 0.0;
-            for (int q = 0; q < NW; ++q) xn2 += red[q][0].x;
+            for (int q = 0; q < NW; ++q) xn2 += nrm[q];
             const double2 alpha = make_double2(rl[j + 1].x, -rl[j + 1].y);
-            double beta, tr, ti;
-            double2 scale;
             if (xn2 == 0.0 && alpha.y == 0.0) {
                 beta = alpha.x;
-                tr = ti = 0.0;
+                tau = trd_zero();
                 scale = trd_zero();
             } else {
                 beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
-                tr = (beta - alpha.x) / beta;
-                ti = -alpha.y / beta;
+                tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
                 const double zr = alpha.x - beta, zi = alpha.y;
                 const double den = zr * zr + zi * zi;
                 scale = make_double2(zr / den, -zi / den);
             }
-            bc[0] = make_double2(tr, ti);
-            bc[1] = make_double2(beta, rl[j].x);
-            bc[2] = scale;
         }
-        __syncthreads();
-        const double2 tau = bc[0];
-        {
-            const double2 scale = bc[2];
-            for (int c = j + 1 + t; c < n; c += kTrdThreads) {
-                const double2 v = c == j + 1 ? make_double2(1.0, 0.0)
-                                             : trd_mul(scale, make_double2(rl[c].x, -rl[c].y));
-                vl[c] = v;
-                if (out) V[(size_t)j * n + c] = v;
-            }
-            if (out && t == 0) {
-                T.tau[(size_t)m * n + j] = tau;
-                T.e[(size_t)m * n + j] = bc[1].x;
-                T.d[(size_t)m * n + j] = bc[1].y;
-            }
+        // v into LDS for the dot, the update and the next pivot row (read
+        // after the next work-group barrier); the hemv forms v from rl
+        for (int c = j + 1 + t; c < n; c += kTrdThreads) {
+            const double2 v = c == j + 1 ? make_double2(1.0, 0.0)
+                                         : trd_mul(scale, make_double2(rl[c].x, -rl[c].y));
+            vl[c] = v;
+            if (out) V[(size_t)j * n + c] = v;
         }
-        __syncthreads();
+        if (out && t == 0) {
+            T.tau[(size_t)m * n + j] = tau;
+            T.e[(size_t)m * n + j] = beta;
+            T.d[(size_t)m * n + j] = rl[j].x;
+        }
 
         // ---- p = tau A v on this work-group's rows, partial p^H v ----
         const bool row_on = i > j && i < n;
@@ -217,7 +215,9 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs T)
             for (int s = 0; s < S; ++s) {
                 const int c = g + kTrdGroups * s;
                 if (c > j && c < n) {
-                    const double2 q = trd_mul(a[s], vl[c]);
+                    const double2 vc = c == j + 1 ? make_double2(1.0, 0.0)
+                                                  : trd_mul(scale, make_double2(rl[c].x, -rl[c].y));
+                    const double2 q = trd_mul(a[s], vc);
                     sp.x += q.x;
                     sp.y += q.y;
                 }

@@ -29,10 +29,10 @@ inline size_t trd_flag_words(int n, int nb) { return (size_t)nb * ((n + 15) / 16
 // Eigenvectors of nb tridiagonal problems solved as one block-diagonal
 // problem of order N = nb * n (eigenvalues ascending over all blocks, Z of
 // order N): owner[q] = the block holding eigenvector q (its largest
-// entry), then for every block its eigenvalues in order into w[b][0..n)
-// and its vectors, as complex, into X[b] (n x n, column-major).
-hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, int *owner,
-                            int *rank, double *w, double2 *X, hipStream_t stream);
+// entry), then for every block its eigenvalues in order into w[b0 + b][0..n)
+// and its vectors, as complex, into X[b0 + b] (n x n, column-major).
+hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, int b0,
+                            int *owner, int *rank, double *w, double2 *X, hipStream_t stream);
 // Compact-WY factors of the reflectors of every matrix: T[m][b] (kTrdBlock
 // x kTrdBlock, upper, column-major) with H(j0) ... H(j0 + nbk - 1) =
 // I - V_b T V_b^H for block b (j0 = kTrdBlock b), zlarft forward/columnwise.

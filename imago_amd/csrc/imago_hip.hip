@@ -190,6 +190,11 @@ struct imago_ctx {
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
     double timing[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    // second stream and handle: the two halves of the batched tridiagonal
+    // divide and conquer run side by side (created on first use)
+    hipStream_t stream2 = nullptr;
+    rocblas_handle blas2 = nullptr;
+    hipEvent_t evf = nullptr, evj = nullptr;
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -804,6 +809,13 @@ extern "C" int imago_ctx_destroy(imago_ctx *c)
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->ev3) (void)hipEventDestroy(c->ev3);
+    if (c->blas2) rocblas_destroy_handle(c->blas2);
+    if (c->evf) (void)hipEventDestroy(c->evf);
+    if (c->evj) (void)hipEventDestroy(c->evj);
+    if (c->stream2) {
+        (void)hipStreamSynchronize(c->stream2);
+        (void)hipStreamDestroy(c->stream2);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return IMAGO_OK;
@@ -1310,7 +1322,8 @@ static int heevd_coop(imago_ctx *c, int n, int nb, double2 *A, double *evals, do
     unsigned *bar = nullptr;
     int *err = nullptr;
     RCHK(dev_alloc(c, &V, nn * nb));
-    RCHK(dev_alloc(c, &Z, (size_t)N * N));
+    RCHK(dev_alloc(c, &Z, (size_t)((nb + 1) / 2) * n * ((nb + 1) / 2) * n +
+                              (size_t)(nb / 2) * n * (nb / 2) * n));
     RCHK(dev_alloc(c, &wall, (size_t)N));
     RCHK(dev_alloc(c, &owner, (size_t)N));
     RCHK(dev_alloc(c, &rank, (size_t)N));
@@ -1328,10 +1341,35 @@ static int heevd_coop(imago_ctx *c, int n, int nb, double2 *A, double *evals, do
     HIPCHK(hipMemsetAsync(V, 0, sizeof(double2) * nn * nb, c->stream));
     TrdLaunch L{n, nb, A, V, evals, ework, tau, vbuf, pbuf, dbuf, bar, err, c->stream};
     HIPCHK(launch_hetrd_coop(L));
-    // d and e of the batch are back to back, e[n-1] = 0 between problems
+    // d and e of the batch are back to back, e[n-1] = 0 between problems:
+    // each half of the batch is one block-diagonal problem, the two halves
+    // on two streams (one call on the whole batch merges across the blocks
+    // at its top levels for nothing)
     HIPCHK(hipMemcpyAsync(wall, evals, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
-    BLASCHK(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, N, wall, ework, Z, N, dinfo));
-    HIPCHK(launch_trd_unmix(Z, wall, n, nb, owner, rank, evals, A, c->stream));
+    const int h0 = (nb + 1) / 2, h1 = nb - h0;
+    const int N0 = h0 * n, N1 = h1 * n;
+    double *Z1 = Z + (size_t)N0 * N0;
+    if (h1 > 0) {
+        if (!c->stream2) {
+            HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+            if (rocblas_create_handle(&c->blas2) != rocblas_status_success ||
+                rocblas_set_stream(c->blas2, c->stream2) != rocblas_status_success)
+                return fail(IMAGO_E_HIP, "rocblas handle for the second stream");
+            HIPCHK(hipEventCreateWithFlags(&c->evf, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&c->evj, hipEventDisableTiming));
+        }
+        HIPCHK(hipEventRecord(c->evf, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->stream2, c->evf, 0));
+        BLASCHK(rocsolver_dstedc(c->blas2, rocblas_evect_tridiagonal, N1, wall + N0, ework + N0,
+                                 Z1, N1, dinfo + 1));
+        HIPCHK(hipEventRecord(c->evj, c->stream2));
+    }
+    BLASCHK(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, N0, wall, ework, Z, N0, dinfo));
+    if (h1 > 0) HIPCHK(hipStreamWaitEvent(c->stream, c->evj, 0));
+    HIPCHK(launch_trd_unmix(Z, wall, n, h0, 0, owner, rank, evals, A, c->stream));
+    if (h1 > 0)
+        HIPCHK(launch_trd_unmix(Z1, wall + N0, n, h1, h0, owner + N0, rank + N0, evals, A,
+                                c->stream));
     HIPCHK(launch_trd_larft(V, tau, n, nb, Tf, c->stream));
     const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0), mone(-1.0, 0.0);
     auto *Vz = (rocblas_double_complex *)V, *Az = (rocblas_double_complex *)A;
