@@ -7,10 +7,10 @@ namespace imago {
 
 namespace {
 
-template <int S>
+template <class T, int S>
 hipError_t trd_one(const TrdLaunch &L)
 {
-    const void *fn = reinterpret_cast<const void *>(&k_hetrd_coop<S>);
+    const void *fn = reinterpret_cast<const void *>(&k_hetrd_coop<T, S>);
     int dev = 0, ncu = 0, per_cu = 0;
     hipError_t st = hipGetDevice(&dev);
     if (st != hipSuccess) return st;
@@ -19,6 +19,7 @@ hipError_t trd_one(const TrdLaunch &L)
     st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTrdThreads, 0);
     if (st != hipSuccess) return st;
     const int P = (L.n + kTrdRows - 1) / kTrdRows;
+    if (P > kTrdMaxGroups) return hipErrorInvalidValue;
     const int fit = per_cu * ncu / P;   // matrices whose work-groups are co-resident
     if (fit < 1) return hipErrorCooperativeLaunchTooLarge;
     const size_t nn = (size_t)L.n * L.n;
@@ -27,22 +28,22 @@ hipError_t trd_one(const TrdLaunch &L)
         st = hipMemsetAsync(L.bar, 0, sizeof(unsigned) * ((size_t)nb * P * kTrdFlagStride + 1),
                             L.stream);
         if (st != hipSuccess) return st;
-        TrdArgs T;
-        T.n = L.n;
-        T.P = P;
-        T.nb_launch = nb;
-        T.A = L.A + m0 * nn;
-        T.V = L.V + m0 * nn;
-        T.d = L.d + (size_t)m0 * L.n;
-        T.e = L.e + (size_t)m0 * L.n;
-        T.tau = L.tau + (size_t)m0 * L.n;
-        T.vbuf = L.vbuf;
-        T.pbuf = L.pbuf;
-        T.dbuf = L.dbuf;
-        T.bar = L.bar;
-        T.err = L.err;
-        T.spin_limit = 2000000;
-        void *args[] = {&T};
+        TrdArgs<T> A;
+        A.n = L.n;
+        A.P = P;
+        A.nb_launch = nb;
+        A.A = static_cast<const T *>(L.A) + m0 * nn;
+        A.V = static_cast<T *>(L.V) + m0 * nn;
+        A.d = L.d + (size_t)m0 * L.n;
+        A.e = L.e + (size_t)m0 * L.n;
+        A.tau = static_cast<T *>(L.tau) + (size_t)m0 * L.n;
+        A.vbuf = static_cast<T *>(L.vbuf);
+        A.pbuf = static_cast<T *>(L.pbuf);
+        A.dbuf = static_cast<T *>(L.dbuf);
+        A.bar = L.bar;
+        A.err = L.err;
+        A.spin_limit = 2000000;
+        void *args[] = {&A};
         st = hipLaunchCooperativeKernel(fn, dim3((unsigned)(nb * P)), dim3(kTrdThreads), args, 0,
                                         L.stream);
         if (st != hipSuccess) return st;
@@ -114,71 +115,78 @@ __global__ __launch_bounds__(1024) void k_trd_rank(const int *__restrict__ owner
     }
 }
 
-// one work-group per eigenvector q: its block's part as complex column
-// rank[q] of X[owner[q]], its eigenvalue to w[owner[q]][rank[q]]
+// one work-group per eigenvector q: its block's part as column rank[q] of
+// X[owner[q]] (complex with zero imaginary part, or real), its eigenvalue
+// to w[owner[q]][rank[q]]
+template <class T>
 __global__ __launch_bounds__(256) void k_trd_gather(const double *__restrict__ Z,
                                                     const double *__restrict__ wall, int N, int n,
                                                     const int *__restrict__ owner,
                                                     const int *__restrict__ rank,
-                                                    double *__restrict__ w, double2 *__restrict__ X)
+                                                    double *__restrict__ w, T *__restrict__ X)
 {
     const int q = blockIdx.x;
     const int b = owner[q], p = rank[q];
     if (p >= n) return;
     const double *z = Z + (size_t)q * N + (size_t)b * n;
-    double2 *x = X + (size_t)b * n * n + (size_t)p * n;
-    for (int r = threadIdx.x; r < n; r += 256) x[r] = make_double2(z[r], 0.0);
+    T *x = X + (size_t)b * n * n + (size_t)p * n;
+    for (int r = threadIdx.x; r < n; r += 256) x[r] = sc_make<T>(z[r], 0.0);
     if (threadIdx.x == 0) w[(size_t)b * n + p] = wall[q];
 }
 
 hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, int b0,
-                            int *owner, int *rank, double *w, double2 *X, hipStream_t stream)
+                            int *owner, int *rank, double *w, void *X, bool real,
+                            hipStream_t stream)
 {
     const int N = n * nb;
     hipLaunchKernelGGL(k_trd_owner, dim3(N), dim3(256), 0, stream, Z, N, n, owner);
     hipLaunchKernelGGL(k_trd_rank, dim3(1), dim3(1024), 0, stream, owner, N, nb, rank);
-    hipLaunchKernelGGL(k_trd_gather, dim3(N), dim3(256), 0, stream, Z, wall, N, n, owner, rank,
-                       w + (size_t)b0 * n, X + (size_t)b0 * n * n);
+    if (real)
+        hipLaunchKernelGGL(k_trd_gather<double>, dim3(N), dim3(256), 0, stream, Z, wall, N, n,
+                           owner, rank, w + (size_t)b0 * n,
+                           static_cast<double *>(X) + (size_t)b0 * n * n);
+    else
+        hipLaunchKernelGGL(k_trd_gather<double2>, dim3(N), dim3(256), 0, stream, Z, wall, N, n,
+                           owner, rank, w + (size_t)b0 * n,
+                           static_cast<double2 *>(X) + (size_t)b0 * n * n);
     return hipGetLastError();
 }
 
-// zlarft (direct = 'F', storev = 'C') for block b of matrix m, one
-// work-group each: G = V_b^H V_b over the rows below the block's first
+// zlarft / dlarft (direct = 'F', storev = 'C') for block b of matrix m,
+// one work-group each: G = V_b^H V_b over the rows below the block's first
 // reflector (64-row tiles of V_b staged in LDS, four entries of G per
 // thread), then T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i.
 // Rows of a reflector above its unit entry are zero in V.
-__global__ __launch_bounds__(256) void k_trd_larft(const double2 *__restrict__ V,
-                                                   const double2 *__restrict__ tau, int n,
-                                                   int nblocks, double2 *__restrict__ T)
+template <class T>
+__global__ __launch_bounds__(256) void k_trd_larft(const T *__restrict__ V,
+                                                   const T *__restrict__ tau, int n, int nblocks,
+                                                   T *__restrict__ Tout)
 {
     constexpr int K = kTrdBlock, R = 64;
-    __shared__ double2 tile[R][K + 1];
-    __shared__ double2 G[K][K + 1];
-    __shared__ double2 Ts[K][K + 1];
+    __shared__ T tile[R][K + 1];
+    __shared__ T G[K][K + 1];
+    __shared__ T Ts[K][K + 1];
     const int m = blockIdx.x / nblocks, b = blockIdx.x % nblocks;
     const int j0 = b * K, nbk = min(K, n - 1 - j0);
     const size_t nn = (size_t)n * n;
-    const double2 *Vm = V + m * nn;
-    const double2 *tm = tau + (size_t)m * n;
-    double2 acc[K * K / 256];
+    const T *Vm = V + m * nn;
+    const T *tm = tau + (size_t)m * n;
+    T acc[K * K / 256];
 #pragma unroll
-    for (int q = 0; q < K * K / 256; ++q) acc[q] = trd_zero();
+    for (int q = 0; q < K * K / 256; ++q) acc[q] = sc_zero<T>();
     for (int r0 = j0 + 1; r0 < n; r0 += R) {
         for (int e = threadIdx.x; e < R * K; e += 256) {
             const int rr = e % R, cc = e / R, r = r0 + rr;
-            tile[rr][cc] = (r < n && cc < nbk) ? Vm[(size_t)(j0 + cc) * n + r] : trd_zero();
+            tile[rr][cc] = (r < n && cc < nbk) ? Vm[(size_t)(j0 + cc) * n + r] : sc_zero<T>();
         }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < K * K / 256; ++q) {
             const int e = threadIdx.x + 256 * q, kr = e % K, ic = e / K;
             if (kr < ic) {
-                double2 s = acc[q];
-                for (int rr = 0; rr < R; ++rr) {
-                    const double2 x = trd_mulc(tile[rr][ic], tile[rr][kr]);   // conj(vk) vi
-                    s.x += x.x;
-                    s.y += x.y;
-                }
+                T s = acc[q];
+                for (int rr = 0; rr < R; ++rr)
+                    s = sc_add(s, sc_mulc(tile[rr][ic], tile[rr][kr]));   // conj(vk) vi
                 acc[q] = s;
             }
         }
@@ -188,49 +196,60 @@ __global__ __launch_bounds__(256) void k_trd_larft(const double2 *__restrict__ V
     for (int q = 0; q < K * K / 256; ++q) {
         const int e = threadIdx.x + 256 * q, kr = e % K, ic = e / K;
         G[kr][ic] = acc[q];
-        Ts[kr][ic] = trd_zero();
+        Ts[kr][ic] = sc_zero<T>();
     }
     __syncthreads();
     for (int i = 0; i < nbk; ++i) {
-        const double2 ti = tm[j0 + i];
+        const T ti = tm[j0 + i];
         if ((int)threadIdx.x < i) {
             // -tau_i * sum_k T(row, k) G(k, i), k in [row, i)
             const int row = threadIdx.x;
-            double2 s = trd_zero();
-            for (int k = row; k < i; ++k) {
-                const double2 q = trd_mul(Ts[row][k], G[k][i]);
-                s.x += q.x;
-                s.y += q.y;
-            }
-            const double2 v = trd_mul(ti, s);
-            Ts[row][i] = make_double2(-v.x, -v.y);
+            T s = sc_zero<T>();
+            for (int k = row; k < i; ++k) s = sc_add(s, sc_mul(Ts[row][k], G[k][i]));
+            Ts[row][i] = sc_scale(-1.0, sc_mul(ti, s));
         }
         if (threadIdx.x == 0) Ts[i][i] = ti;
         __syncthreads();
     }
-    double2 *To = T + ((size_t)m * nblocks + b) * K * K;
+    T *To = Tout + ((size_t)m * nblocks + b) * K * K;
     for (int e = threadIdx.x; e < K * K; e += 256) To[e] = Ts[e % K][e / K];
 }
 
-hipError_t launch_trd_larft(const double2 *V, const double2 *tau, int n, int nb, double2 *T,
+hipError_t launch_trd_larft(const void *V, const void *tau, int n, int nb, bool real, void *T,
                             hipStream_t stream)
 {
     const int nblocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_trd_larft, dim3((unsigned)(nb * nblocks)), dim3(256), 0, stream, V, tau,
-                       n, nblocks, T);
+    if (real)
+        hipLaunchKernelGGL(k_trd_larft<double>, dim3((unsigned)(nb * nblocks)), dim3(256), 0,
+                           stream, static_cast<const double *>(V),
+                           static_cast<const double *>(tau), n, nblocks, static_cast<double *>(T));
+    else
+        hipLaunchKernelGGL(k_trd_larft<double2>, dim3((unsigned)(nb * nblocks)), dim3(256), 0,
+                           stream, static_cast<const double2 *>(V),
+                           static_cast<const double2 *>(tau), n, nblocks,
+                           static_cast<double2 *>(T));
     return hipGetLastError();
 }
 
-bool hetrd_coop_supported(int n) { return n >= 2 && n <= kTrdMaxOrder; }
+bool hetrd_coop_supported(int n, bool real)
+{
+    return n >= 2 && n <= (real ? kTrdMaxOrderReal : kTrdMaxOrderComplex);
+}
 
 hipError_t launch_hetrd_coop(const TrdLaunch &L)
 {
-    if (!hetrd_coop_supported(L.n) || L.nb < 1) return hipErrorInvalidValue;
-    if (L.n <= 128) return trd_one<4>(L);
-    if (L.n <= 256) return trd_one<8>(L);
-    if (L.n <= 512) return trd_one<16>(L);
-    return trd_one<32>(L);
+    if (!hetrd_coop_supported(L.n, L.real) || L.nb < 1) return hipErrorInvalidValue;
+    if (L.real) {
+        if (L.n <= 256) return trd_one<double, 8>(L);
+        if (L.n <= 512) return trd_one<double, 16>(L);
+        if (L.n <= 1024) return trd_one<double, 32>(L);
+        return trd_one<double, 64>(L);
+    }
+    if (L.n <= 128) return trd_one<double2, 4>(L);
+    if (L.n <= 256) return trd_one<double2, 8>(L);
+    if (L.n <= 512) return trd_one<double2, 16>(L);
+    return trd_one<double2, 32>(L);
 }
 
 }  // namespace imago

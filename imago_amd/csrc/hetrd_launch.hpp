@@ -1,45 +1,53 @@
 // hetrd_launch.hpp - host interface of the cooperative tridiagonal
-// reduction (k_hetrd_coop in hetrd_coop.hpp).
+// reduction (k_hetrd_coop in hetrd_coop.hpp) and of the kernels around it
+// in the eigensolver (eigenvector sort, compact-WY factors).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <algorithm>
 
 namespace imago {
 
-constexpr int kTrdMaxOrder = 1024;   // 32 column groups x 32 registers
+// largest orders: 32 column groups x 32 complex / 64 real registers a lane
+constexpr int kTrdMaxOrderComplex = 1024;
+constexpr int kTrdMaxOrderReal = 2048;
+constexpr int kTrdBlock = 32;   // reflectors per compact-WY block of the back-transform
 
+// Buffers of one batch; T = double2 (complex Hermitian) or double (real
+// symmetric) for every pointer typed void here.
 struct TrdLaunch {
     int n, nb;              // order, number of matrices
-    const double2 *A;       // [nb][n * n] full Hermitian, column-major
-    double2 *V;             // [nb][n * n] reflectors (zhetrd 'L' layout)
+    bool real;              // scalar type double instead of double2
+    const void *A;          // [nb][n * n] full Hermitian / symmetric, column-major
+    void *V;                // [nb][n * n] reflectors (LAPACK 'L' layout), zeroed by the caller
     double *d, *e;          // [nb][n]
-    double2 *tau;           // [nb][n]
-    double2 *vbuf, *pbuf;   // [nb][2][n] exchanged pivot rows and p
-    double2 *dbuf;          // [nb][2][64] exchanged dot partials
+    void *tau;              // [nb][n]
+    void *vbuf, *pbuf;      // [nb][2][n] exchanged pivot rows and p
+    void *dbuf;             // [nb][2][128] exchanged dot partials
     unsigned *bar;          // trd_flag_words(n, nb) barrier flags
     int *err;               // [1], zeroed by the caller
     hipStream_t stream;
 };
 
-constexpr int kTrdBlock = 32;   // reflectors per compact-WY block of the back-transform
-
-bool hetrd_coop_supported(int n);
+bool hetrd_coop_supported(int n, bool real);
 // barrier words the launch needs: one 64-byte flag per work-group, an abort word
 inline size_t trd_flag_words(int n, int nb) { return (size_t)nb * ((n + 15) / 16) * 16 + 1; }
+// A = Q T Q^H for every matrix of the batch; matrices are processed in
+// groups whose work-groups are co-resident on the device.
+hipError_t launch_hetrd_coop(const TrdLaunch &L);
 // Eigenvectors of nb tridiagonal problems solved as one block-diagonal
 // problem of order N = nb * n (eigenvalues ascending over all blocks, Z of
 // order N): owner[q] = the block holding eigenvector q (its largest
 // entry), then for every block its eigenvalues in order into w[b0 + b][0..n)
-// and its vectors, as complex, into X[b0 + b] (n x n, column-major).
+// and its vectors into X[b0 + b] (n x n, column-major; complex with zero
+// imaginary parts, or real when real is set).
 hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, int b0,
-                            int *owner, int *rank, double *w, double2 *X, hipStream_t stream);
+                            int *owner, int *rank, double *w, void *X, bool real,
+                            hipStream_t stream);
 // Compact-WY factors of the reflectors of every matrix: T[m][b] (kTrdBlock
 // x kTrdBlock, upper, column-major) with H(j0) ... H(j0 + nbk - 1) =
-// I - V_b T V_b^H for block b (j0 = kTrdBlock b), zlarft forward/columnwise.
-hipError_t launch_trd_larft(const double2 *V, const double2 *tau, int n, int nb, double2 *T,
+// I - V_b T V_b^H for block b (j0 = kTrdBlock b), zlarft / dlarft forward,
+// columnwise.
+hipError_t launch_trd_larft(const void *V, const void *tau, int n, int nb, bool real, void *T,
                             hipStream_t stream);
-// A = Q T Q^H for every matrix of the batch; matrices are processed in
-// groups whose work-groups are co-resident on the device.
-hipError_t launch_hetrd_coop(const TrdLaunch &L);
 
 }  // namespace imago

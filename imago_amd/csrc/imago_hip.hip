@@ -1289,65 +1289,89 @@ static int factor_gv_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int 
     return IMAGO_OK;
 }
 
-// Hermitian eigendecomposition C = Y E Y^H of a batch of order n <= 1024,
-// the three stages of LAPACK zheevd:
+// Hermitian (complex) or symmetric (real) eigendecomposition C = Y E Y^H of
+// a batch, the three stages of LAPACK zheevd / dsyevd:
 //   C = Q T Q^H, T real tridiagonal: k_hetrd_coop, one persistent
 //     cooperative kernel for the whole batch;
-//   T = Z E Z^T: the batch's tridiagonals as one block-diagonal problem
-//     (e[n-1] = 0 splits it) for divide and conquer (dstedc), its
-//     eigenvectors sorted back into their matrices (launch_trd_unmix);
+//   T = Z E Z^T: each half of the batch's tridiagonals as one
+//     block-diagonal problem (e[n-1] = 0 splits it) for divide and conquer
+//     (dstedc), the halves on two streams, the eigenvectors sorted back
+//     into their matrices (launch_trd_unmix);
 //   Y = Q Z: the reflectors applied in compact-WY blocks of kTrdBlock
-//     (zlarft factors by k_trd_larft, then W = V^H Y, W = T W, Y -= V W as
-//     batched zgemm), from the last block to the first.
+//     (factors by k_trd_larft, then W = V^H Y, W = T W, Y -= V W as batched
+//     gemm), from the last block to the first.
 // A holds C on entry and Y on exit; evals the eigenvalues in ascending
-// order; dinfo[0] dstedc's info.
-// The default for orders up to kTrdMaxOrder: at 4 x 896 it takes 25 ms
-// where rocSOLVER's batched zheevd takes 30 (IMAGO_HEEVD=rocsolver selects
-// the latter).
-static bool use_coop_heevd(int n)
+// order; dinfo[0], dinfo[1] dstedc's info of each half.
+// The default up to order 1024 (complex) / 2048 (real): at 4 x 896 complex
+// it takes 17 ms where rocSOLVER's batched zheevd takes 30
+// (IMAGO_HEEVD=rocsolver selects the latter).
+static bool use_coop_heevd(int n, bool cx)
 {
     const char *env = getenv("IMAGO_HEEVD");
-    return hetrd_coop_supported(n) && !(env && strcmp(env, "rocsolver") == 0);
+    return hetrd_coop_supported(n, !cx) && !(env && strcmp(env, "rocsolver") == 0);
 }
 
-static int heevd_coop(imago_ctx *c, int n, int nb, double2 *A, double *evals, double *ework,
-                      int *dinfo)
+// C = alpha op(A) op(B) + beta C over a strided batch, complex or real
+static rocblas_status gemm_sb(rocblas_handle hb, bool cx, rocblas_operation ta,
+                              rocblas_operation tb, int m, int n, int k, double alpha,
+                              const void *A, int lda, rocblas_stride sa, const void *B, int ldb,
+                              rocblas_stride sb, double beta, void *C, int ldc, rocblas_stride sc,
+                              int batch)
 {
-    const size_t nn = (size_t)n * n;
+    if (cx) {
+        const rocblas_double_complex al(alpha, 0.0), be(beta, 0.0);
+        return rocblas_zgemm_strided_batched(hb, ta, tb, m, n, k, &al,
+                                             (const rocblas_double_complex *)A, lda, sa,
+                                             (const rocblas_double_complex *)B, ldb, sb, &be,
+                                             (rocblas_double_complex *)C, ldc, sc, batch);
+    }
+    const rocblas_operation ra =
+        ta == rocblas_operation_conjugate_transpose ? rocblas_operation_transpose : ta;
+    const rocblas_operation rb =
+        tb == rocblas_operation_conjugate_transpose ? rocblas_operation_transpose : tb;
+    return rocblas_dgemm_strided_batched(hb, ra, rb, m, n, k, &alpha, (const double *)A, lda, sa,
+                                         (const double *)B, ldb, sb, &beta, (double *)C, ldc, sc,
+                                         batch);
+}
+
+static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *evals,
+                      double *ework, int *dinfo)
+{
+    const size_t nn = (size_t)n * n, es = cx ? sizeof(double2) : sizeof(double);
     const int N = n * nb, K = kTrdBlock, nblk = (n - 1 + K - 1) / K;
-    double2 *V = nullptr, *tau = nullptr, *vbuf = nullptr, *pbuf = nullptr, *dbuf = nullptr,
-            *Tf = nullptr, *W = nullptr, *W2 = nullptr;
+    char *V = nullptr, *tau = nullptr, *vbuf = nullptr, *pbuf = nullptr, *dbuf = nullptr,
+         *Tf = nullptr, *W = nullptr, *W2 = nullptr;
     double *Z = nullptr, *wall = nullptr;
     int *owner = nullptr, *rank = nullptr;
     unsigned *bar = nullptr;
     int *err = nullptr;
-    RCHK(dev_alloc(c, &V, nn * nb));
-    RCHK(dev_alloc(c, &Z, (size_t)((nb + 1) / 2) * n * ((nb + 1) / 2) * n +
-                              (size_t)(nb / 2) * n * (nb / 2) * n));
+    const int h0 = (nb + 1) / 2, h1 = nb - h0;
+    const int N0 = h0 * n, N1 = h1 * n;
+    RCHK(dev_alloc(c, &V, nn * nb * es));
+    RCHK(dev_alloc(c, &Z, (size_t)N0 * N0 + (size_t)N1 * N1));
     RCHK(dev_alloc(c, &wall, (size_t)N));
     RCHK(dev_alloc(c, &owner, (size_t)N));
     RCHK(dev_alloc(c, &rank, (size_t)N));
-    RCHK(dev_alloc(c, &tau, (size_t)n * nb));
-    RCHK(dev_alloc(c, &vbuf, (size_t)2 * n * nb));
-    RCHK(dev_alloc(c, &pbuf, (size_t)2 * n * nb));
-    RCHK(dev_alloc(c, &dbuf, (size_t)2 * 64 * nb));
-    RCHK(dev_alloc(c, &Tf, (size_t)nb * std::max(nblk, 1) * K * K));
-    RCHK(dev_alloc(c, &W, (size_t)nb * K * n));
-    RCHK(dev_alloc(c, &W2, (size_t)nb * K * n));
+    RCHK(dev_alloc(c, &tau, (size_t)n * nb * es));
+    RCHK(dev_alloc(c, &vbuf, (size_t)2 * n * nb * es));
+    RCHK(dev_alloc(c, &pbuf, (size_t)2 * n * nb * es));
+    RCHK(dev_alloc(c, &dbuf, (size_t)2 * 128 * nb * es));
+    RCHK(dev_alloc(c, &Tf, (size_t)nb * std::max(nblk, 1) * K * K * es));
+    RCHK(dev_alloc(c, &W, (size_t)nb * K * n * es));
+    RCHK(dev_alloc(c, &W2, (size_t)nb * K * n * es));
     RCHK(dev_alloc(c, &bar, trd_flag_words(n, nb)));
     RCHK(dev_alloc(c, &err, 1));
     HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int) * nb, c->stream));
-    HIPCHK(hipMemsetAsync(V, 0, sizeof(double2) * nn * nb, c->stream));
-    TrdLaunch L{n, nb, A, V, evals, ework, tau, vbuf, pbuf, dbuf, bar, err, c->stream};
+    HIPCHK(hipMemsetAsync(V, 0, es * nn * nb, c->stream));
+    TrdLaunch L{n,     nb,   !cx,  A,    V,   evals, ework, tau,
+                vbuf,  pbuf, dbuf, bar,  err, c->stream};
     HIPCHK(launch_hetrd_coop(L));
     // d and e of the batch are back to back, e[n-1] = 0 between problems:
     // each half of the batch is one block-diagonal problem, the two halves
     // on two streams (one call on the whole batch merges across the blocks
     // at its top levels for nothing)
     HIPCHK(hipMemcpyAsync(wall, evals, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
-    const int h0 = (nb + 1) / 2, h1 = nb - h0;
-    const int N0 = h0 * n, N1 = h1 * n;
     double *Z1 = Z + (size_t)N0 * N0;
     if (h1 > 0) {
         if (!c->stream2) {
@@ -1366,31 +1390,24 @@ static int heevd_coop(imago_ctx *c, int n, int nb, double2 *A, double *evals, do
     }
     BLASCHK(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, N0, wall, ework, Z, N0, dinfo));
     if (h1 > 0) HIPCHK(hipStreamWaitEvent(c->stream, c->evj, 0));
-    HIPCHK(launch_trd_unmix(Z, wall, n, h0, 0, owner, rank, evals, A, c->stream));
+    HIPCHK(launch_trd_unmix(Z, wall, n, h0, 0, owner, rank, evals, A, !cx, c->stream));
     if (h1 > 0)
-        HIPCHK(launch_trd_unmix(Z1, wall + N0, n, h1, h0, owner + N0, rank + N0, evals, A,
+        HIPCHK(launch_trd_unmix(Z1, wall + N0, n, h1, h0, owner + N0, rank + N0, evals, A, !cx,
                                 c->stream));
-    HIPCHK(launch_trd_larft(V, tau, n, nb, Tf, c->stream));
-    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0), mone(-1.0, 0.0);
-    auto *Vz = (rocblas_double_complex *)V, *Az = (rocblas_double_complex *)A;
-    auto *Wz = (rocblas_double_complex *)W, *W2z = (rocblas_double_complex *)W2;
-    auto *Tz = (rocblas_double_complex *)Tf;
+    HIPCHK(launch_trd_larft(V, tau, n, nb, !cx, Tf, c->stream));
     const rocblas_stride snn = (rocblas_stride)nn, sw = (rocblas_stride)K * n,
                          st = (rocblas_stride)std::max(nblk, 1) * K * K;
+    char *Ac = static_cast<char *>(A);
     for (int b = nblk - 1; b >= 0; --b) {
         const int j0 = b * K, kb = std::min(K, n - 1 - j0), r0 = j0 + 1, mb = n - r0;
-        const rocblas_double_complex *Vb = Vz + (size_t)j0 * n + r0;
-        rocblas_double_complex *Yb = Az + r0;
-        BLASCHK(rocblas_zgemm_strided_batched(c->blas, rocblas_operation_conjugate_transpose,
-                                              rocblas_operation_none, kb, n, mb, &one, Vb, n, snn,
-                                              Yb, n, snn, &zero, Wz, K, sw, nb));
-        BLASCHK(rocblas_zgemm_strided_batched(c->blas, rocblas_operation_none,
-                                              rocblas_operation_none, kb, n, kb, &one,
-                                              Tz + (size_t)b * K * K, K, st, Wz, K, sw, &zero,
-                                              W2z, K, sw, nb));
-        BLASCHK(rocblas_zgemm_strided_batched(c->blas, rocblas_operation_none,
-                                              rocblas_operation_none, mb, n, kb, &mone, Vb, n, snn,
-                                              W2z, K, sw, &one, Yb, n, snn, nb));
+        const char *Vb = V + ((size_t)j0 * n + r0) * es;
+        char *Yb = Ac + (size_t)r0 * es;
+        BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_conjugate_transpose, rocblas_operation_none,
+                        kb, n, mb, 1.0, Vb, n, snn, Yb, n, snn, 0.0, W, K, sw, nb));
+        BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_none, rocblas_operation_none, kb, n, kb,
+                        1.0, Tf + (size_t)b * K * K * es, K, st, W, K, sw, 0.0, W2, K, sw, nb));
+        BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_none, rocblas_operation_none, mb, n, kb,
+                        -1.0, Vb, n, snn, W2, K, sw, 1.0, Yb, n, snn, nb));
     }
     int herr = 0;
     HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1418,8 +1435,8 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));
-        if (use_coop_heevd(n)) {
-            RCHK(heevd_coop(c, n, nb, (double2 *)A, evals, ework, dinfo));
+        if (use_coop_heevd(n, true)) {
+            RCHK(heevd_coop(c, true, n, nb, A, evals, ework, dinfo));
         } else {
             BLASCHK(rocsolver_zheevd_strided_batched(c->blas, rocblas_evect_original,
                                                      rocblas_fill_upper, n, a, n, nn, evals, n,
@@ -1438,8 +1455,13 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));
-        BLASCHK(rocsolver_dsyevd_strided_batched(c->blas, rocblas_evect_original, rocblas_fill_upper,
-                                                 n, a, n, nn, evals, n, ework, n, dinfo, nb));
+        if (use_coop_heevd(n, false)) {
+            RCHK(heevd_coop(c, false, n, nb, A, evals, ework, dinfo));
+        } else {
+            BLASCHK(rocsolver_dsyevd_strided_batched(c->blas, rocblas_evect_original,
+                                                     rocblas_fill_upper, n, a, n, nn, evals, n,
+                                                     ework, n, dinfo, nb));
+        }
         BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));

@@ -273,3 +273,24 @@ def test_eigensolver_paths_batched_secular(gpu_ctx, monkeypatch):
     assert np.max(np.abs(e1 - e2)) < 1e-10 * max(1.0, np.max(np.abs(e1)))
     assert np.allclose(t1, t2, rtol=1e-10, atol=1e-12)
     assert np.max(np.abs(p1 - p2)) <= 1e-10 * max(1.0, np.max(np.abs(p1)))
+
+
+@pytest.mark.parametrize("mode", ["coop", "rocsolver"])
+@pytest.mark.parametrize("n", [2, 17, 300, 1024, 1500, 2048])
+def test_real_eigensolver_paths_orders(gpu_ctx, oracle_mod, monkeypatch, mode, n):
+    """Both real symmetric eigensolvers (the Gamma path) against LAPACK dsygv
+    up to the largest order of the cooperative one."""
+    monkeypatch.setenv("IMAGO_HEEVD", mode)
+    rng = np.random.default_rng(500 + n)
+    M = rng.normal(size=(n, n))
+    H = M + M.T
+    R = rng.normal(size=(n, n))
+    S = R @ R.T + n * np.eye(n)
+    print("[gpu] %s sygv n=%d" % (mode, n), flush=True)
+    w, V = gpu_ctx.hegv(H, S)
+    w2, _, info = oracle_mod.hegv(H, S)
+    assert info == 0
+    scale = max(1.0, np.max(np.abs(w2)))
+    assert np.max(np.abs(w - w2)) < 1e-10 * scale, np.max(np.abs(w - w2))
+    assert np.allclose(V.T @ S @ V, np.eye(n), atol=1e-10)
+    assert np.allclose(H @ V, S @ V * w, atol=1e-8 * scale)
