@@ -188,13 +188,14 @@ struct imago_ctx {
     double *ival = nullptr;       // [items] trace value of each item
     double *foldX = nullptr;      // [images][64 * ne] density folded per image
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
-    double timing[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double timing[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // second stream and handle: the two halves of the batched tridiagonal
     // divide and conquer run side by side (created on first use)
     hipStream_t stream2 = nullptr;
     rocblas_handle blas2 = nullptr;
     hipEvent_t evf = nullptr, evj = nullptr;
+    hipEvent_t evh0 = nullptr, evh1 = nullptr;   // around the tridiagonal reduction
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -812,6 +813,8 @@ extern "C" int imago_ctx_destroy(imago_ctx *c)
     if (c->blas2) rocblas_destroy_handle(c->blas2);
     if (c->evf) (void)hipEventDestroy(c->evf);
     if (c->evj) (void)hipEventDestroy(c->evj);
+    if (c->evh0) (void)hipEventDestroy(c->evh0);
+    if (c->evh1) (void)hipEventDestroy(c->evh1);
     if (c->stream2) {
         (void)hipStreamSynchronize(c->stream2);
         (void)hipStreamDestroy(c->stream2);
@@ -1366,7 +1369,13 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     HIPCHK(hipMemsetAsync(V, 0, es * nn * nb, c->stream));
     TrdLaunch L{n,     nb,   !cx,  A,    V,   evals, ework, tau,
                 vbuf,  pbuf, dbuf, bar,  err, c->stream};
+    if (!c->evh0) {
+        HIPCHK(hipEventCreate(&c->evh0));
+        HIPCHK(hipEventCreate(&c->evh1));
+    }
+    HIPCHK(hipEventRecord(c->evh0, c->stream));
     HIPCHK(launch_hetrd_coop(L));
+    HIPCHK(hipEventRecord(c->evh1, c->stream));
     // d and e of the batch are back to back, e[n-1] = 0 between problems:
     // each half of the batch is one block-diagonal problem, the two halves
     // on two streams (one call on the whole batch merges across the blocks
@@ -1412,6 +1421,9 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     int herr = 0;
     HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    float tms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&tms, c->evh0, c->evh1));
+    c->timing[10] = tms;
     for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank, (void *)tau,
                     (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf, (void *)W, (void *)W2,
                     (void *)bar, (void *)err})
@@ -1423,6 +1435,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
 static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, const void *B,
                              double *evals, double *ework, int *dinfo, int *hinfo)
 {
+    c->timing[10] = 0.0;
     const rocblas_stride nn = (rocblas_stride)n * n;
     if (cx) {
         const rocblas_double_complex one(1.0, 0.0);
@@ -1942,7 +1955,7 @@ extern "C" int imago_ep_counts(imago_ctx *c, long *tuples, long *items, int *blo
 
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 9) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 10) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

@@ -200,7 +200,8 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * (0 when they were already built for the system), 6 = host wall time of
  * the last imago_set_system, 7 = its lattice-image / candidate part,
  * 8 = the EP sum of the last direct H build, 9 = the EP traces of the last
- * direct imago_density_traces. */
+ * direct imago_density_traces, 10 = the cooperative tridiagonal reduction
+ * of the last eigensolve (0 when rocSOLVER's eigensolver ran). */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
