@@ -27,6 +27,7 @@
 #include "assemble_launch.hpp"
 #include "geom_kernel.hpp"
 #include "hetrd_launch.hpp"
+#include "elecstat_launch.hpp"
 
 using namespace imago;
 
@@ -188,7 +189,7 @@ struct imago_ctx {
     double *ival = nullptr;       // [items] trace value of each item
     double *foldX = nullptr;      // [images][64 * ne] density folded per image
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
-    double timing[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double timing[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // second stream and handle: the two halves of the batched tridiagonal
     // divide and conquer run side by side (created on first use)
@@ -1953,9 +1954,193 @@ extern "C" int imago_ep_counts(imago_ctx *c, long *tuples, long *items, int *blo
     return IMAGO_OK;
 }
 
+// ---- electrostatics of the potential-side Gaussians (elecStat.f90) ------
+
+static int es_args(imago_ctx *c, double th, imago::EsArgs &A, const double **dsizes)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (!c->has_system) return fail(IMAGO_E_STATE, "imago_set_system must come first");
+    if (!(th > 0.0) || !std::isfinite(th))
+        return fail(IMAGO_E_ARG, "log_elec_thresh must be positive and finite (got %g)", th);
+    const HostSys &h = c->h;
+    if (h.maxpa > 64) return fail(IMAGO_E_ARG, "more than 64 alphas in a potential type");
+    RCHK(dev_upload(c, dsizes, h.cell_sizes));
+    A = imago::EsArgs{};
+    A.nsites = h.nsites;
+    A.npt = h.npt;
+    A.pot_dim = h.pot_dim;
+    A.ncells = h.ncells;
+    A.maxpa = h.maxpa;
+    A.site_pos = c->d.pot_pos;
+    A.site_type = c->d.pot_site_type;
+    A.type_first = c->d.pot_first;
+    A.type_mult = c->d.pot_mult;
+    A.type_nalpha = c->d.pot_nalpha;
+    A.type_alphas = c->d.pot_alphas;
+    A.type_z = c->d.pot_z;
+    A.type_nuc = c->d.pot_nuc_alpha;
+    A.type_term0 = c->type_term0;
+    A.term_type = c->d.term_type;
+    A.term_alpha = c->d.term_alpha;
+    A.cell_dims = c->d.cell_dims;
+    A.cell_sizes = *dsizes;
+    std::memcpy(A.rv, h.rv, sizeof A.rv);
+    std::memcpy(A.inv, h.inv, sizeof A.inv);
+    A.th = th;
+    return IMAGO_OK;
+}
+
+extern "C" int imago_elecstat(imago_ctx *c, double log_elec_thresh, double *pot_alpha_overlap,
+                              double *non_local_neut_q_pot, double *local_neut_q_pot,
+                              double *non_local_nuc_q_pot, double *local_nuc_q_pot)
+{
+    if (!pot_alpha_overlap || !non_local_neut_q_pot || !local_neut_q_pot ||
+        !non_local_nuc_q_pot || !local_nuc_q_pot)
+        return fail(IMAGO_E_ARG, "null output array");
+    imago::EsArgs A;
+    const double *dsizes = nullptr;
+    if (c) HIPCHK(hipSetDevice(c->device));
+    RCHK(es_args(c, log_elec_thresh, A, &dsizes));
+    const size_t P = (size_t)c->h.pot_dim, PP = P * P;
+    double *buf = nullptr;
+    RCHK(dev_alloc(c, &buf, 3 * PP + 2 * P));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(imago::launch_es_neutral_nuclear(A, buf, buf + PP, buf + 2 * PP, buf + 3 * PP,
+                                            buf + 3 * PP + P, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(pot_alpha_overlap, buf, PP * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(non_local_neut_q_pot, buf + PP, PP * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(local_neut_q_pot, buf + 2 * PP, PP * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(non_local_nuc_q_pot, buf + 3 * PP, P * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(local_nuc_q_pot, buf + 3 * PP + P, P * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timing[11] = ms;
+    dev_release(c, buf);
+    dev_release(c, (void *)dsizes);
+    return IMAGO_OK;
+}
+
+extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_cells_recip,
+                                const double *cell_dims_recip, const double *cell_sizes_recip,
+                                double *non_local_residual_q)
+{
+    if (!cell_dims_recip || !cell_sizes_recip || !non_local_residual_q)
+        return fail(IMAGO_E_ARG, "null array");
+    if (num_cells_recip < 1) return fail(IMAGO_E_ARG, "no reciprocal cells");
+    for (int q = 1; q < num_cells_recip; ++q)
+        if (!(cell_sizes_recip[q] > 0.0))
+            return fail(IMAGO_E_ARG, "reciprocal cell %d has size %g: cell 0 must be the only "
+                                     "origin", q, cell_sizes_recip[q]);
+    imago::EsArgs A;
+    const double *dsizes = nullptr;
+    if (c) HIPCHK(hipSetDevice(c->device));
+    RCHK(es_args(c, log_elec_thresh, A, &dsizes));
+    const HostSys &h = c->h;
+    // element runs in site order: the reference recomputes its reciprocal
+    // tables when the element of a type's first site changes
+    // (elecStat.f90:814-858) and keeps them for the types that follow
+    std::vector<int> site_src(h.nsites), src_type, ord_type;
+    int cur_el = -1;
+    for (int i = 0; i < h.nsites; ++i) {
+        const int t = h.site_type[i];
+        if (i == 0 || h.site_type[i - 1] != t) {
+            ord_type.push_back(t);
+            if (h.pot_el[t] != cur_el) {
+                cur_el = h.pot_el[t];
+                src_type.push_back(t);
+            }
+            if (h.pot_nalpha[t] > h.pot_nalpha[src_type.back()]) {
+                dev_release(c, (void *)dsizes);
+                return fail(IMAGO_E_ARG, "potential type %d has more alphas than type %d, whose "
+                                         "tables its element run shares", t, src_type.back());
+            }
+        }
+        site_src[i] = (int)src_type.size() - 1;
+    }
+    if ((int)ord_type.size() != h.npt) {
+        dev_release(c, (void *)dsizes);
+        return fail(IMAGO_E_ARG, "potential types without sites");
+    }
+    imago::EsRecip R{};
+    R.nk = imago::es_recip_nk(h.maxpa);
+    const int bi = imago::es_recip_bi(R.nk);
+    std::vector<int> blk;
+    for (int i = 0; i < h.nsites;) {
+        int n = 1;
+        while (n < bi && i + n < h.nsites && site_src[i + n] == site_src[i]) ++n;
+        blk.push_back(i);
+        blk.push_back(n);
+        blk.push_back(site_src[i]);
+        i += n;
+    }
+    R.nblk = (int)blk.size() / 3;
+    R.ncr = num_cells_recip;
+    R.nsrc = (int)src_type.size();
+    R.chunks = imago::es_recip_chunks(h.nsites, R.nblk, R.ncr);
+    double min_pot = 10000.0;   // potTypes.f90:213-250
+    for (int t = 0; t < h.npt; ++t) min_pot = std::min(min_pot, h.pot_alphas[(size_t)t * h.maxpa]);
+    R.min_pot = min_pot;
+    {
+        // realCellVolume as lattice.f90:195-216 leaves it: (a x b) . c
+        const double *a = h.rv, *b = h.rv + 3, *cc = h.rv + 6;
+        double v = 0.0;
+        for (int j = 0; j < 3; ++j) {
+            const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            v = v + (a[j1] * b[j2] - a[j2] * b[j1]) * cc[j];
+        }
+        R.volume = std::fabs(v);
+    }
+    std::vector<double> cdr(cell_dims_recip, cell_dims_recip + 3 * (size_t)num_cells_recip);
+    std::vector<double> csr(cell_sizes_recip, cell_sizes_recip + num_cells_recip);
+    const double *dcdr = nullptr, *dcsr = nullptr;
+    const int *dsrc = nullptr, *dsite_src = nullptr, *dord = nullptr, *dblk = nullptr;
+    RCHK(dev_upload(c, &dcdr, cdr));
+    RCHK(dev_upload(c, &dcsr, csr));
+    RCHK(dev_upload(c, &dsrc, src_type));
+    RCHK(dev_upload(c, &dsite_src, site_src));
+    RCHK(dev_upload(c, &dord, ord_type));
+    RCHK(dev_upload(c, &dblk, blk));
+    R.cdr = dcdr;
+    R.csr = dcsr;
+    R.src_type = dsrc;
+    R.site_src = dsite_src;
+    R.ord_type = dord;
+    R.blk = dblk;
+    const size_t nout = (size_t)h.npt * h.pot_dim;
+    double *E = nullptr, *S = nullptr, *out = nullptr;
+    double2 *CS = nullptr;
+    RCHK(dev_alloc(c, &E, imago::es_recip_e_doubles(R.nsrc, R.ncr, R.nk)));
+    RCHK(dev_alloc(c, &CS, (size_t)R.ncr * h.nsites));
+    RCHK(dev_alloc(c, &S, imago::es_recip_s_doubles(R.chunks, h.nsites, R.nk)));
+    RCHK(dev_alloc(c, &out, nout));
+    R.E = E;
+    R.CS = CS;
+    R.S = S;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(imago::launch_es_residual_q(A, R, out, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(non_local_residual_q, out, nout * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timing[12] = ms;
+    for (const void *p : {(const void *)dcdr, (const void *)dcsr, (const void *)dsrc,
+                          (const void *)dsite_src, (const void *)dord, (const void *)dblk,
+                          (const void *)E, (const void *)CS, (const void *)S,
+                          (const void *)out, (const void *)dsizes})
+        dev_release(c, const_cast<void *>(p));
+    return IMAGO_OK;
+}
+
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 10) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 12) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

@@ -161,6 +161,30 @@ module O_ImagoHIP
          integer(c_long), intent(out) :: tuples, items, partial_doubles
          integer(c_int), intent(out) :: block_doubles
       end function imago_ep_counts
+
+      ! neutralAndNuclearQPot (elecStat.f90:59-664): the arrays the
+      ! reference writes to its electrostatics HDF5 datasets
+      integer(c_int) function imago_elecstat(ctx, logElecThresh, potAlphaOverlap, &
+            & nonLocalNeutQPot, localNeutQPot, nonLocalNucQPot, localNucQPot) &
+            & bind(C, name="imago_elecstat")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: logElecThresh
+         real(c_double), intent(out) :: potAlphaOverlap(*), nonLocalNeutQPot(*), &
+               & localNeutQPot(*), nonLocalNucQPot(*), localNucQPot(*)
+      end function imago_elecstat
+
+      ! residualQ (elecStat.f90:665-1043), with O_Lattice's reciprocal cells
+      integer(c_int) function imago_residual_q(ctx, logElecThresh, numCellsRecip, &
+            & cellDimsRecip, cellSizesRecip, nonLocalResidualQ) &
+            & bind(C, name="imago_residual_q")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: logElecThresh
+         integer(c_int), value :: numCellsRecip
+         real(c_double), intent(in) :: cellDimsRecip(3,*), cellSizesRecip(*)
+         real(c_double), intent(out) :: nonLocalResidualQ(*)
+      end function imago_residual_q
    end interface
 
 contains

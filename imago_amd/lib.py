@@ -38,7 +38,7 @@ EXPORTS = [
     "imago_ctx_create", "imago_ctx_destroy", "imago_last_error", "imago_set_system",
     "imago_primitive_batch", "imago_boys_batch", "imago_assemble", "imago_get_packed",
     "imago_secular", "imago_get_eigenvectors", "imago_density_traces", "imago_zhegv",
-    "imago_dsygv", "imago_last_timing", "imago_ep_counts",
+    "imago_dsygv", "imago_last_timing", "imago_ep_counts", "imago_elecstat", "imago_residual_q",
 ]
 
 
@@ -80,6 +80,9 @@ def load():
     L.imago_ep_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long),
                                   ctypes.POINTER(ctypes.c_long), _ip,
                                   ctypes.POINTER(ctypes.c_long)]
+    L.imago_elecstat.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
+    L.imago_residual_q.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                   _dp]
     _lib = L
     return L
 
@@ -212,6 +215,34 @@ class Context:
         _check(load().imago_ep_counts(self._h, ctypes.byref(t), ctypes.byref(i), ctypes.byref(b),
                                       ctypes.byref(w)))
         return t.value, i.value, b.value, w.value
+
+    # -- electrostatics (elecStat.f90) --------------------------------------
+    def elecstat(self, log_elec_thresh=None):
+        """neutralAndNuclearQPot of the current system: dict with
+        potAlphaOverlap, nonLocalNeutQPot, localNeutQPot (potDim x potDim,
+        [r, c] = (term of site j, term of site i) as the reference's arrays),
+        nonLocalNucQPot and localNucQPot (potDim)."""
+        th = self.system.log_elec_thresh if log_elec_thresh is None else log_elec_thresh
+        P = self.pot_dim
+        mats = [np.zeros((P, P), order="F") for _ in range(3)]
+        vecs = [np.zeros(P), np.zeros(P)]
+        _check(load().imago_elecstat(self._h, float(th), *[_p(m) for m in mats],
+                                     *[_p(v) for v in vecs]))
+        return dict(potAlphaOverlap=mats[0], nonLocalNeutQPot=mats[1], localNeutQPot=mats[2],
+                    nonLocalNucQPot=vecs[0], localNucQPot=vecs[1])
+
+    def residual_q(self, log_elec_thresh=None, recip_cells=None):
+        """residualQ of the current system: numPotTypes x potDim
+        nonLocalResidualQ.  recip_cells = (dims [n][3], sizes [n]); the
+        system's reciprocal lattice by default."""
+        th = self.system.log_elec_thresh if log_elec_thresh is None else log_elec_thresh
+        dims, sizes = (self.system.recip_cells(th) if recip_cells is None else recip_cells)
+        dims = np.ascontiguousarray(dims, dtype=np.float64)
+        sizes = np.ascontiguousarray(sizes, dtype=np.float64)
+        out = np.zeros((len(self.system.pot_types), self.pot_dim), order="F")
+        _check(load().imago_residual_q(self._h, float(th), len(sizes), _p(dims), _p(sizes),
+                                       _p(out)))
+        return out
 
     # -- single problems ----------------------------------------------------
     def hegv(self, H, S):

@@ -154,6 +154,25 @@ def negligible_limit_real(log_thresh: float, min_atomic_alpha: float,
     return max(negl, 2.0 * log_thresh / min_atomic_alpha)
 
 
+def reciprocal_vectors(real_vectors: np.ndarray):
+    """recipVectors (columns) and realCellVolume as lattice.f90:195-216
+    computes them: column i is 2 pi (a_{i+1} x a_{i+2}) / ((a_{i+1} x
+    a_{i+2}) . a_i); the volume is the last of those triple products,
+    made positive."""
+    rv = np.asarray(real_vectors, dtype=np.float64)
+    recip = np.zeros((3, 3))
+    vol = 0.0
+    for i in range(3):
+        i1, i2 = (i + 1) % 3, (i + 2) % 3
+        vol = 0.0
+        for j in range(3):
+            j1, j2 = (j + 1) % 3, (j + 2) % 3
+            recip[j, i] = rv[j1, i1] * rv[j2, i2] - rv[j2, i1] * rv[j1, i2]
+            vol = vol + recip[j, i] * rv[j, i]
+        recip[:, i] = 2.0 * math.pi * recip[:, i] / vol
+    return recip, abs(vol)
+
+
 def make_lattice(real_vectors: np.ndarray, negl: float):
     """Sorted real-space cell list (makeLattice, lattice.f90:571-696).
     real_vectors: 3x3 with a,b,c as columns.  Returns (dims[n,3], sizes[n])."""
@@ -259,6 +278,7 @@ class System:
     kpoint_weights: np.ndarray          # [nk]
     log_basis_fn_thresh: float
     gamma: bool = False
+    log_elec_thresh: float = -math.log(1.0e-16)   # -ln(electroConvg), lattice.f90:294
     cell_dims: np.ndarray = None
     cell_sizes: np.ndarray = None
     name: str = ""
@@ -302,6 +322,19 @@ class System:
         order = sorted(range(len(self.pot_types)),
                        key=lambda t: self.pot_types[t].sites[0])
         return [(t, a) for t in order for a in range(len(self.pot_types[t].alphas))]
+
+    # electrostatics ----------------------------------------------------------
+    @property
+    def min_pot_alpha(self) -> float:
+        """minPotAlpha (potTypes.f90:213-250)."""
+        return min([10000.0] + [float(p.alphas[0]) for p in self.pot_types])
+
+    def recip_cells(self, log_elec_thresh: float | None = None):
+        """Reciprocal-space cells of initializeLattice (lattice.f90:386-420):
+        |G|^2 <= 4 logElecThresh minPotAlpha, sorted by |G|^2."""
+        th = self.log_elec_thresh if log_elec_thresh is None else log_elec_thresh
+        recip, _ = reciprocal_vectors(self.real_vectors)
+        return make_lattice(recip, 4.0 * th * self.min_pot_alpha)
 
     # packed arrays for imago_system_desc -------------------------------------
     def desc_arrays(self) -> dict:

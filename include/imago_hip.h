@@ -201,7 +201,9 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * the last imago_set_system, 7 = its lattice-image / candidate part,
  * 8 = the EP sum of the last direct H build, 9 = the EP traces of the last
  * direct imago_density_traces, 10 = the cooperative tridiagonal reduction
- * of the last eigensolve (0 when rocSOLVER's eigensolver ran). */
+ * of the last eigensolve (0 when rocSOLVER's eigensolver ran), 11 = the
+ * kernels of the last imago_elecstat, 12 = those of the last
+ * imago_residual_q (both without the copies to the host). */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
@@ -212,6 +214,35 @@ int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
  * (bytes moved per launch). */
 int imago_ep_counts(imago_ctx *ctx, long *tuples, long *items, int *block_doubles,
                     long *partial_doubles);
+
+/* ---- electrostatics of the potential-side Gaussians -------------------- *
+ * Set-up stage of makeSCFPot's electrostatics (src/imago/elecStat.f90),
+ * for the system of the last imago_set_system.  log_elec_thresh is
+ * -ln(electroConvg) (logElecThresh, lattice.f90:294); the real-space cells
+ * are the system's.  potDim terms are ordered as the EP terms (potential
+ * type in the order of its first site, then alpha).
+ *
+ * neutralAndNuclearQPot (elecStat.f90:59-664): pot_alpha_overlap,
+ * non_local_neut_q_pot and local_neut_q_pot are potDim x potDim,
+ * column-major, exactly the reference's arrays (element (r, c) sums the
+ * pairs whose site-j term is r and site-i term is c; not symmetric);
+ * non_local_nuc_q_pot and local_nuc_q_pot are [potDim].  Every element is
+ * written. */
+int imago_elecstat(imago_ctx *ctx, double log_elec_thresh, double *pot_alpha_overlap,
+                   double *non_local_neut_q_pot, double *local_neut_q_pot,
+                   double *non_local_nuc_q_pot, double *local_nuc_q_pot);
+
+/* residualQ (elecStat.f90:665-1043): non_local_residual_q is numPotTypes x
+ * potDim, column-major (row = the ordinal of site j's type in site order,
+ * column = the term of site i), as the reference's nonLocalResidualQ.
+ * The reciprocal cells are the caller's (initializeLattice's
+ * cellDimsRecip [num_cells_recip][3] and cellSizesRecip, |G|^2 ascending
+ * with cell 0 the origin; lattice.f90:386-420).  IMAGO_E_ARG when a
+ * potential type has more alphas than the first type of its element run
+ * (the reference's tables would hold stale rows). */
+int imago_residual_q(imago_ctx *ctx, double log_elec_thresh, int num_cells_recip,
+                     const double *cell_dims_recip, const double *cell_sizes_recip,
+                     double *non_local_residual_q);
 
 #ifdef __cplusplus
 }

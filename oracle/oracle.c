@@ -19,6 +19,10 @@
  *   H build      secularEqn.F90:148-171, matrixSubs.F90:339-376
  *   traces       matrixSubs.F90:66-112
  *   lattice      lattice.f90:784-890 (findLatticeVector)
+ *   electrostatics  elecStat.f90:59-664 (neutralAndNuclearQPot),
+ *                665-1043 (residualQ, restated with its real-space loop
+ *                stopping at the first cell beyond range in |R| order);
+ *                error_fn mathSubs.f90:22-161
  */
 #include <math.h>
 #include <stdlib.h>
@@ -1016,4 +1020,336 @@ double orc_matrix_element_mult(int dim1, int n, const double *m1,
         t += m1[dim1 * (diag - 1)] * m2[dim1 * (diag - 1)] / 2.0;
     }
     return t * 2.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Electrostatics of the potential-side Gaussians (elecStat.f90).      */
+
+/* error_fn, mathSubs.f90:22-161: the Sandia rational approximation of
+ * erf (SAND77-1441; W. J. Cody, Math. Comp. 23 (1969) 631), three ranges
+ * |x| <= 0.46875, <= 4, <= 6, and sign(x) beyond; 11 significant digits.
+ * The coefficients are the published ones, which the reference uses. */
+double orc_error_fn(double x)
+{
+    const double ax = fabs(x);
+    if (ax > 6.0) return x / ax;
+    const double x2 = ax * ax;
+    if (ax <= 0.46875) {
+        double a = ax * (242.6679552305318 +
+                         x2 * (21.97926161829415 +
+                               x2 * (6.996383488619136 + x2 * -0.03560984370181539)));
+        a = a / (215.0588758698612 +
+                 x2 * (91.16490540451490 + x2 * (15.08279763040779 + x2 * 1.0)));
+        return x < 0.0 ? -a : a;
+    }
+    double a;
+    if (ax <= 4.0) {
+        a = exp(-x2) *
+            (22.898992851659 +
+             ax * (26.094746956075 +
+                   ax * (14.571898596926 +
+                         ax * (4.2677201070898 +
+                               ax * (0.56437160686381 + ax * -0.0000060858151959688)))));
+        a = a / (22.898985749891 +
+                 ax * (51.933570687552 +
+                       ax * (50.273202863803 +
+                             ax * (26.288795758761 + ax * (7.5688482293618 + ax * 1.0)))));
+    } else {
+        const double ix2 = 1.0 / x2;
+        a = ix2 * (-0.0121308276389978 +
+                   ix2 * (-0.1199039552681460 +
+                          ix2 * (-0.243911029488626 + ix2 * -0.0324319519277746))) /
+            (0.0430026643452770 +
+             ix2 * (0.489552441961437 + ix2 * (1.43771227937118 + ix2 * 1.0)));
+        a = exp(-x2) * (0.564189583547756 + a) / ax;
+    }
+    /* sign(1 - a, x): |1 - a| with the sign of x (1 - a > 0 here) */
+    const double v = fabs(1.0 - a);
+    return x < 0.0 ? -v : v;
+}
+
+/* First term (0-based, in potDim) of every potential type: types in the
+ * order of their first site, alphas in order (initAlphaIndex in
+ * elecStat.f90:319-331). */
+static int *es_type_term0(const imago_system_desc *s)
+{
+    int *t0 = calloc(s->num_pot_types, sizeof(int));
+    int acc = 0;
+    for (int i = 0; i < s->num_pot_sites; ++i) {
+        const int t = s->pot_site_type[i];
+        if (i == 0 || s->pot_site_type[i - 1] != t) {
+            t0[t] = acc;
+            acc += s->pot_type_num_alphas[t];
+        }
+    }
+    return t0;
+}
+
+static int es_pot_dim(const imago_system_desc *s)
+{
+    int p = 0;
+    for (int t = 0; t < s->num_pot_types; ++t) p += s->pot_type_num_alphas[t];
+    return p;
+}
+
+/* Adds the (a, b) alpha-pair contribution of one site pair and cell
+ * (the three branches of elecStat.f90:449-537 share this form). */
+static void es_neut_add(int P, int col, int row, double r2, double r, double nqc,
+                        double s1, double red, double sred, double poc, double x2isp,
+                        double *ovl, double *nl_neut, double *l_neut)
+{
+    const long at = (long)col * P + row;
+    if (fabs(r2) >= SMALL_THRESH)
+        nl_neut[at] = nl_neut[at] +
+                      nqc / r * (-orc_error_fn(s1 * r) + orc_error_fn(sred * r));
+    else
+        l_neut[at] = l_neut[at] + nqc * x2isp * sred;
+    ovl[at] = ovl[at] + poc * exp(-red * r2);
+}
+
+/* neutralAndNuclearQPot, elecStat.f90:59-664.  Column-major potDim x
+ * potDim outputs with element (term of site j, term of site i), as the
+ * reference's arrays; vectors indexed by the term of site i. */
+int orc_elecstat(const imago_system_desc *s, double th, double *ovl, double *nl_neut,
+                 double *l_neut, double *nl_nuc, double *l_nuc)
+{
+    ctx_t c;
+    ctx_init(&c, s);
+    const int P = es_pot_dim(s);
+    const int mpa = s->max_pot_alphas;
+    int *t0 = es_type_term0(s);
+    memset(ovl, 0, sizeof(double) * (size_t)P * P);
+    memset(nl_neut, 0, sizeof(double) * (size_t)P * P);
+    memset(l_neut, 0, sizeof(double) * (size_t)P * P);
+    memset(nl_nuc, 0, sizeof(double) * P);
+    memset(l_nuc, 0, sizeof(double) * P);
+    const double pi = PI, x2pi = 2.0 * PI, sqrt_pi = sqrt(PI);
+    const double pi3 = PI * PI * PI, pi32 = pow(PI, 1.5), x2isp = 2.0 / sqrt_pi;
+    double *red = malloc(sizeof(double) * mpa * mpa);
+    double *sred = malloc(sizeof(double) * mpa * mpa);
+    double *nqc = malloc(sizeof(double) * mpa * mpa);
+    double *poc = malloc(sizeof(double) * mpa * mpa);
+    double *s1 = malloc(sizeof(double) * mpa), *s2 = malloc(sizeof(double) * mpa);
+    (void)pi;
+    for (int i = 0; i < s->num_pot_sites; ++i) {
+        const int t1 = s->pot_site_type[i];
+        const int n1 = s->pot_type_num_alphas[t1];
+        const double *a1 = s->pot_type_alphas + (size_t)t1 * mpa;
+        for (int l = 0; l < n1; ++l) s1[l] = sqrt(a1[l]);
+        for (int j = 0; j < s->num_pot_sites; ++j) {
+            const int t2 = s->pot_site_type[j];
+            const int n2 = s->pot_type_num_alphas[t2];
+            const double *a2 = s->pot_type_alphas + (size_t)t2 * mpa;
+            for (int m = 0; m < n2; ++m) s2[m] = sqrt(a2[m]);
+            const int minN = n1 < n2 ? n1 : n2, maxN = n1 > n2 ? n1 : n2;
+            /* elecStat.f90:362-393, element (l, k) at [l * mpa + k] */
+            for (int k = 0; k < n2; ++k)
+                for (int l = 0; l < n1; ++l) {
+                    red[l * mpa + k] = a1[l] * a2[k] / (a1[l] + a2[k]);
+                    nqc[l * mpa + k] = pi3 / (a1[l] * s1[l] * a2[k] * s2[k]);
+                    poc[l * mpa + k] = pow(PI / (a1[l] + a2[k]), 1.5);
+                    sred[l * mpa + k] = sqrt(red[l * mpa + k]);
+                }
+            const double *pi_ = s->pot_site_pos + 3 * i, *pj = s->pot_site_pos + 3 * j;
+            const double sep[3] = {pi_[0] - pj[0], pi_[1] - pj[1], pi_[2] - pj[2]};
+            double L[3];
+            find_lattice_vector(&c, sep, L);
+            const double ms[3] = {sep[0] - L[0], sep[1] - L[1], sep[2] - L[2]};
+            const double nucA = s->pot_type_nuc_alpha[t2], Z2 = s->pot_type_nuc_charge[t2];
+            for (int k = 0; k < s->num_cells; ++k) {
+                const double *R = s->cell_dims + 3 * k;
+                const double d0 = ms[0] - R[0], d1 = ms[1] - R[1], d2 = ms[2] - R[2];
+                const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                const double r = sqrt(r2);
+                int above = 0, l;
+                for (l = 0; l < minN - 1; ++l) {
+                    if (red[l * mpa + l] * r2 > th) {
+                        above = 1;
+                        break;
+                    }
+                    for (int m = l; m < n2; ++m)
+                        es_neut_add(P, t0[t1] + l, t0[t2] + m, r2, r, nqc[l * mpa + m], s1[l],
+                                    red[l * mpa + m], sred[l * mpa + m], poc[l * mpa + m],
+                                    x2isp, ovl, nl_neut, l_neut);
+                    for (int m = l + 1; m < n1; ++m)
+                        es_neut_add(P, t0[t1] + m, t0[t2] + l, r2, r, nqc[m * mpa + l], s1[m],
+                                    red[m * mpa + l], sred[m * mpa + l], poc[m * mpa + l],
+                                    x2isp, ovl, nl_neut, l_neut);
+                }
+                if (!above) {
+                    /* l = minN - 1 here (the loop variable after a loop
+                     * that ran to completion, elecStat.f90:500-537) */
+                    for (int m = l; m < maxN; ++m) {
+                        const int la = n1 <= n2 ? l : m, lb = n1 <= n2 ? m : l;
+                        es_neut_add(P, t0[t1] + la, t0[t2] + lb, r2, r, nqc[la * mpa + lb],
+                                    s1[la], red[la * mpa + lb], sred[la * mpa + lb],
+                                    poc[la * mpa + lb], x2isp, ovl, nl_neut, l_neut);
+                    }
+                }
+                /* gaussian-screened nuclear charge, elecStat.f90:541-578 */
+                for (int m = 0; m < n1; ++m) {
+                    const int at = t0[t1] + m;
+                    const double nf = a1[m] + nucA;
+                    const double scr = r2 * a1[m] * nucA / nf;
+                    if (scr < th) {
+                        if (fabs(scr) >= SMALL_THRESH) {
+                            const double cf = a1[m] * r / sqrt(nf);
+                            nl_nuc[at] = nl_nuc[at] -
+                                         Z2 * pi32 * exp(-scr) * orc_error_fn(cf) / cf / nf;
+                        } else {
+                            nl_nuc[at] = nl_nuc[at] - Z2 * exp(-scr) * x2pi / nf;
+                        }
+                    } else {
+                        break;
+                    }
+                }
+            }
+            /* bare nuclear local part, elecStat.f90:583-596 */
+            if (sep[0] * sep[0] + sep[1] * sep[1] + sep[2] * sep[2] < SMALL_THRESH) {
+                const double Z1 = s->pot_type_nuc_charge[t1];
+                for (int m = 0; m < n1; ++m)
+                    l_nuc[t0[t1] + m] = l_nuc[t0[t1] + m] - x2pi * Z1 / a1[m];
+            }
+        }
+    }
+    free(red);
+    free(sred);
+    free(nqc);
+    free(poc);
+    free(s1);
+    free(s2);
+    free(t0);
+    ctx_free(&c);
+    return 0;
+}
+
+/* realCellVolume as initializeLattice leaves it (lattice.f90:195-216):
+ * (a x b) . c from the last cofactor column, made positive. */
+static double es_cell_volume(const double *rv)
+{
+    const double *a = rv, *b = rv + 3, *cc = rv + 6;
+    double v = 0;
+    for (int j = 0; j < 3; ++j) {
+        const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        v = v + (a[j1] * b[j2] - a[j2] * b[j1]) * cc[j];
+    }
+    return fabs(v);
+}
+
+/* residualQ, elecStat.f90:665-1043: the real-space and reciprocal-space
+ * (Ewald) sums of every (potential type of site j, term of site i) against
+ * the system's smallest potential alpha.  out is column-major numPotTypes x
+ * potDim; the row is the ordinal of site j's type in site order.
+ * Reciprocal cells [ncr][3] with sizes |G|^2 ascending, cell 0 the origin.
+ * Returns IMAGO_E_ARG when a type has more alphas than the type at the
+ * start of its element run (the reference would read stale table rows). */
+int orc_residual_q(const imago_system_desc *s, double th, int ncr, const double *cdr,
+                   const double *csr, double *out)
+{
+    return orc_residual_q_sites(s, th, ncr, cdr, csr, 0, s->num_pot_sites, out);
+}
+
+/* orc_residual_q restricted to the sites i in [i_begin, i_end) of the
+ * outer loop (the columns of their types; others are left 0).  The
+ * element-run tables are kept as the full loop would have them. */
+int orc_residual_q_sites(const imago_system_desc *s, double th, int ncr, const double *cdr,
+                         const double *csr, int i_begin, int i_end, double *out)
+{
+    ctx_t c;
+    ctx_init(&c, s);
+    const int P = es_pot_dim(s);
+    const int mpa = s->max_pot_alphas;
+    int npt_seen = 0;
+    for (int i = 0; i < s->num_pot_sites; ++i)
+        if (i == 0 || s->pot_site_type[i - 1] != s->pot_site_type[i]) ++npt_seen;
+    int *t0 = es_type_term0(s);
+    memset(out, 0, sizeof(double) * (size_t)npt_seen * P);
+    const double pi32 = pow(PI, 1.5), x2isp = 2.0 / sqrt(PI), sqth = sqrt(th);
+    double min_pot = 10000.0;
+    for (int t = 0; t < s->num_pot_types; ++t)
+        if (s->pot_type_alphas[(size_t)t * mpa] < min_pot)
+            min_pot = s->pot_type_alphas[(size_t)t * mpa];
+    const double vol = es_cell_volume(s->real_vectors);
+    double *caf1 = calloc(mpa, sizeof(double)), *caf2 = calloc(mpa, sizeof(double));
+    double *sred0 = calloc(mpa, sizeof(double));   /* reducedPotAlphaSqrt(:, 1) */
+    double *mras = calloc(mpa, sizeof(double));
+    double *rce = calloc((size_t)mpa * ncr, sizeof(double));
+    double *phase = calloc(ncr, sizeof(double));
+    int cur_el = -1, src_n = 0, rc = 0;
+    for (int i = 0; i < s->num_pot_sites && rc == 0; ++i) {
+        const int t1 = s->pot_site_type[i];
+        const int n1 = s->pot_type_num_alphas[t1];
+        const double *a1 = s->pot_type_alphas + (size_t)t1 * mpa;
+        if (i == 0 || s->pot_site_type[i - 1] != t1) {
+            for (int k = 0; k < n1; ++k) caf1[k] = pi32 / (a1[k] * sqrt(a1[k]));
+            const int last_el = cur_el;
+            cur_el = s->pot_type_element[t1];
+            if (last_el != cur_el) {
+                for (int k = 0; k < n1; ++k) {
+                    const double mra = a1[k] * min_pot / (a1[k] + min_pot);
+                    mras[k] = sqrt(mra);
+                    const double inv4 = 0.25 / mra;
+                    for (int q = 1; q < ncr; ++q)
+                        rce[(size_t)k * ncr + q] = exp(-csr[q] * inv4) / csr[q];
+                }
+                src_n = n1;
+            }
+            if (n1 > src_n) {
+                rc = IMAGO_E_ARG;
+                break;
+            }
+        }
+        if (i < i_begin || i >= i_end) continue;
+        int ord = -1;
+        for (int j = 0; j < s->num_pot_sites; ++j) {
+            const int t2 = s->pot_site_type[j];
+            const double *a2 = s->pot_type_alphas + (size_t)t2 * mpa;
+            if (j == 0 || s->pot_site_type[j - 1] != t2) {
+                ++ord;
+                const int n2 = s->pot_type_num_alphas[t2];
+                for (int k = 0; k < n2; ++k) caf2[k] = pi32 / (a2[k] * sqrt(a2[k]));
+                for (int k = 0; k < n1; ++k) sred0[k] = sqrt(a1[k] * a2[0] / (a1[k] + a2[0]));
+            }
+            const double *pi_ = s->pot_site_pos + 3 * i, *pj = s->pot_site_pos + 3 * j;
+            const double sep[3] = {pi_[0] - pj[0], pi_[1] - pj[1], pi_[2] - pj[2]};
+            for (int q = 0; q < ncr; ++q) {
+                const double *G = cdr + 3 * q;
+                phase[q] = cos(G[0] * sep[0] + G[1] * sep[1] + G[2] * sep[2]);
+            }
+            double L[3];
+            find_lattice_vector(&c, sep, L);
+            const double ms[3] = {sep[0] - L[0], sep[1] - L[1], sep[2] - L[2]};
+            for (int k = 0; k < n1; ++k) {
+                const long at = (long)(t0[t1] + k) * npt_seen + ord;
+                const double af = caf1[k] * caf2[0];
+                const double asf = x2isp * af * (sred0[k] - mras[k]);
+                const double avf = 4.0 * PI * af / vol;
+                for (int l = 0; l < s->num_cells; ++l) {
+                    const double *R = s->cell_dims + 3 * l;
+                    const double d0 = ms[0] - R[0], d1 = ms[1] - R[1], d2 = ms[2] - R[2];
+                    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                    if (r2 < SMALL_THRESH) {
+                        out[at] = out[at] + asf;
+                    } else {
+                        const double r = sqrt(r2);
+                        if (mras[k] * r > sqth && sred0[k] * r > sqth) break;
+                        out[at] = out[at] + af * (-orc_error_fn(mras[k] * r) +
+                                                  orc_error_fn(sred0[k] * r)) / r;
+                    }
+                }
+                double sum = 0.0;
+                for (int q = 1; q < ncr; ++q) sum = sum + rce[(size_t)k * ncr + q] * phase[q];
+                out[at] = out[at] + avf * (sum - 0.25 / min_pot);
+            }
+        }
+    }
+    free(caf1);
+    free(caf2);
+    free(sred0);
+    free(mras);
+    free(rce);
+    free(phase);
+    free(t0);
+    ctx_free(&c);
+    return rc;
 }

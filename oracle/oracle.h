@@ -76,6 +76,27 @@ double orc_matrix_element_mult(int dim1, int n, const double *m1,
 /* number of primitive tuples evaluated by the last orc_assemble call */
 long orc_last_tuple_count(void);
 
+/* error_fn (mathSubs.f90:22-161): the Sandia/Cody rational erf. */
+double orc_error_fn(double x);
+
+/* neutralAndNuclearQPot (elecStat.f90:59-664) at log threshold th
+ * (-ln electroConvg).  ovl, nl_neut, l_neut: potDim x potDim column-major,
+ * element (term of site j, term of site i); nl_nuc, l_nuc: [potDim]. */
+int orc_elecstat(const imago_system_desc *sys, double th, double *ovl,
+                 double *nl_neut, double *l_neut, double *nl_nuc,
+                 double *l_nuc);
+
+/* residualQ (elecStat.f90:665-1043).  Reciprocal cells cdr[ncr][3] with
+ * sizes csr[ncr] (|G|^2 ascending, cell 0 the origin); out numPotTypes x
+ * potDim column-major.  Returns 0 or IMAGO_E_ARG. */
+int orc_residual_q(const imago_system_desc *sys, double th, int ncr,
+                   const double *cdr, const double *csr, double *out);
+/* The same for the outer-loop sites [i_begin, i_end) only: the columns of
+ * their types, other columns 0. */
+int orc_residual_q_sites(const imago_system_desc *sys, double th, int ncr,
+                         const double *cdr, const double *csr, int i_begin,
+                         int i_end, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,13 @@ def lib():
         L.orc_matrix_element_mult.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_matrix_element_mult.restype = ctypes.c_double
         L.orc_use_external_primitives.argtypes = [ctypes.c_int]
+        L.orc_error_fn.argtypes = [ctypes.c_double]
+        L.orc_error_fn.restype = ctypes.c_double
+        L.orc_elecstat.argtypes = [ctypes.c_void_p, ctypes.c_double] + [_dp] * 5
+        L.orc_residual_q.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                     _dp]
+        L.orc_residual_q_sites.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp,
+                                           _dp, ctypes.c_int, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -66,6 +73,8 @@ def ref_lib():
             getattr(R, n).argtypes = [ctypes.c_double] * 3 + [_dp, _dp, _dp,
                                                               ctypes.c_int, _dp]
         R.ref_boys.argtypes = [ctypes.c_double, _dp]
+        R.ref_error_fn.argtypes = [ctypes.c_double]
+        R.ref_error_fn.restype = ctypes.c_double
         _ref = R
     return _ref
 
@@ -150,6 +159,50 @@ def assemble(system, kind, term_begin=0, term_end=None, raw=False):
         res = out.reshape(nterm, nk, npack, 2)
         res = res[..., 0] + 1j * res[..., 1]
     return res if kind == "ep" else res[0]
+
+
+def error_fn(x, source="oracle"):
+    """error_fn (mathSubs.f90:22-161) from the restatement or the
+    reference's compiled module (source='ref')."""
+    if source == "oracle":
+        return lib().orc_error_fn(float(x))
+    return ref_lib().ref_error_fn(float(x))
+
+
+def elecstat(system, log_elec_thresh=None):
+    """orc_elecstat (neutralAndNuclearQPot, elecStat.f90:59-664): the same
+    dict as imago_amd.lib.Context.elecstat."""
+    from imago_amd._desc import make_desc
+    th = system.log_elec_thresh if log_elec_thresh is None else log_elec_thresh
+    desc, keep = make_desc(system)
+    P = system.pot_dim
+    mats = [np.zeros((P, P), order="F") for _ in range(3)]
+    vecs = [np.zeros(P), np.zeros(P)]
+    rc = lib().orc_elecstat(ctypes.byref(desc), float(th), *[_p(m) for m in mats],
+                            *[_p(v) for v in vecs])
+    if rc != 0:
+        raise RuntimeError("oracle elecstat failed with code %d" % rc)
+    return dict(potAlphaOverlap=mats[0], nonLocalNeutQPot=mats[1], localNeutQPot=mats[2],
+                nonLocalNucQPot=vecs[0], localNucQPot=vecs[1])
+
+
+def residual_q(system, log_elec_thresh=None, recip_cells=None, sites=None):
+    """orc_residual_q (residualQ, elecStat.f90:665-1043): numPotTypes x
+    potDim.  sites = (begin, end) restricts the outer site loop (the other
+    columns stay 0)."""
+    from imago_amd._desc import make_desc
+    th = system.log_elec_thresh if log_elec_thresh is None else log_elec_thresh
+    dims, sizes = system.recip_cells(th) if recip_cells is None else recip_cells
+    dims = np.ascontiguousarray(dims, dtype=np.float64)
+    sizes = np.ascontiguousarray(sizes, dtype=np.float64)
+    desc, keep = make_desc(system)
+    out = np.zeros((len(system.pot_types), system.pot_dim), order="F")
+    i0, i1 = (0, len(system.pot_site_pos)) if sites is None else sites
+    rc = lib().orc_residual_q_sites(ctypes.byref(desc), float(th), len(sizes), _p(dims),
+                                    _p(sizes), int(i0), int(i1), _p(out))
+    if rc != 0:
+        raise RuntimeError("oracle residual_q failed with code %d" % rc)
+    return out
 
 
 def last_tuple_count():

@@ -1,8 +1,9 @@
-! C-callable entry points into the reference's primitive-integral module.
+! C-callable entry points into the reference's primitive-integral module
+! and its error function.
 !
 ! Checker only: this file is compiled together with the reference's own
-! sources (src/kinds.f90, src/constants.f90, src/imago/gaussIntegrals.f90)
-! into oracle/_ref/libgauss_ref.so, which tests/golden/make_golden.py uses
+! sources (src/kinds.f90, src/constants.f90, src/imago/gaussIntegrals.f90,
+! src/imago/mathSubs.f90) into oracle/_ref/libgauss_ref.so, which tests/golden/make_golden.py uses
 ! to write golden vectors.  Nothing in the product links it.
 !
 ! Each routine zeroes sh before the call: the reference routines only write
@@ -12,6 +13,7 @@ module imago_ref_bindc
    use iso_c_binding
    use O_Kinds
    use O_GaussianIntegrals
+   use O_MathSubs, only: error_fn
    implicit none
 contains
 
@@ -58,5 +60,13 @@ contains
       real(c_double), intent(out) :: F(7)
       call boys(T, F)
    end subroutine ref_boys
+
+   function ref_error_fn(x) bind(C, name="ref_error_fn") result(v)
+      real(c_double), value :: x
+      real(c_double) :: v
+      real(double) :: xx
+      xx = x
+      v = error_fn(xx)
+   end function ref_error_fn
 
 end module imago_ref_bindc
