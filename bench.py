@@ -18,6 +18,11 @@ host potential update (makeSCFPot) is not part of the path built here.
 
 Prints one JSON line.  --gpus N with N > 1 is refused: this build runs on
 one GPU (the path would shard by k-point and by potential term).
+
+--stage elecstat measures instead the potential-side electrostatics set-up
+(neutralAndNuclearQPot + residualQ of elecStat.f90: imago_elecstat and
+imago_residual_q) on the chosen workload, one set-up per step, and prints
+its own JSON line of the same shape.
 """
 from __future__ import annotations
 
@@ -53,6 +58,9 @@ def parse():
     p.add_argument("--ep-mode", choices=("direct", "panel"), default="direct",
                    help="direct: EP terms from the stored primitive blocks each iteration "
                         "(IMAGO_EP_DIRECT); panel: all potDim EP panels stored (IMAGO_EP)")
+    p.add_argument("--stage", choices=("step", "elecstat"), default="step",
+                   help="step: the hot-path step (the benchmark line); elecstat: the "
+                        "electrostatics set-up (imago_elecstat + imago_residual_q)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 4 GiB device-to-device copy kernel (scale factor "
                         "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -225,6 +233,98 @@ def cpu_baseline(system, coeffs, num_states, cpu_terms):
     return total, t, nt
 
 
+def recip_flop(system, dims):
+    """Algorithmic FP64 flops of the reciprocal-space sums of residualQ
+    (elecStat.f90:986-990) as summed here: +G and -G add the same term, so
+    one cell of each pair is summed (twice weighted); for every site pair
+    and summed cell the phase cos(G.(r_i - r_j)) as cos_i cos_j + sin_i
+    sin_j (3 flops) and one multiply-add per alpha of site i (2 flops)."""
+    keys = {tuple(d) for d in dims[1:]}
+    pairs = sum(1 for d in dims[1:] if tuple(-d) in keys) // 2
+    nsum = len(dims) - 1 - pairs
+    nsites = len(system.pot_site_pos)
+    nalpha_i = sum(len(p.alphas) * len(p.sites) for p in system.pot_types)
+    return nsum * (3.0 * nsites * nsites + 2.0 * nalpha_i * nsites)
+
+
+def elecstat_cpu_baseline(system, budget_s=10.0):
+    """Oracle neutralAndNuclearQPot in full and residualQ over its outer-loop
+    sites in order until about budget_s seconds have passed, scaled to all
+    sites; one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    t0 = time.perf_counter()
+    oracle.elecstat(system)
+    t_es = time.perf_counter() - t0
+    nsites = len(system.pot_site_pos)
+    dims, sizes = system.recip_cells()
+    ns, spent = 0, 0.0
+    while ns < nsites and spent < budget_s:
+        nxt = min(nsites, ns + max(1, nsites // 64))
+        t0 = time.perf_counter()
+        oracle.residual_q(system, recip_cells=(dims, sizes), sites=(ns, nxt))
+        spent += time.perf_counter() - t0
+        ns = nxt
+    return t_es, spent * nsites / ns, ns
+
+
+def bench_elecstat(args, system, ctx, log):
+    dims, sizes = system.recip_cells()
+    ncr = len(sizes)
+    P = system.pot_dim
+    run = lambda: (ctx.elecstat(), ctx.residual_q(recip_cells=(dims, sizes)))
+    for i in range(args.warmup):
+        run()
+    ms_es, ms_rq, ms_rc, walls = [], [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        run()
+        walls.append(time.perf_counter() - t0)
+        ms_es.append(ctx.timing(11))
+        ms_rq.append(ctx.timing(12))
+        ms_rc.append(ctx.timing(13))
+        log("step %d: elecstat %.3f ms, residualQ %.3f ms (reciprocal sum %.3f ms), wall with "
+            "host copies %.1f ms" % (i, ms_es[-1], ms_rq[-1], ms_rc[-1], walls[-1] * 1e3))
+    es, rq, rc = (float(np.mean(x)) for x in (ms_es, ms_rq, ms_rc))
+    flop = recip_flop(system, dims)
+    achieved = flop / (rc * 1e-3) / 1e12
+    roofline = {"kernel": "k_es_recip", "bound": "mfma", "achieved": round(achieved, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None,
+                "flop_per_launch": flop}
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline leg (oracle)")
+        t_es, t_rq, ns = elecstat_cpu_baseline(system)
+        cpu = {"value": round((t_es + t_rq) * 1e3, 1), "unit": "ms per set-up", "cores": 1,
+               "kind": "port",
+               "sample": ("oracle on %s: neutralAndNuclearQPot in full (%.2f s), residualQ "
+                          "over %d of %d outer-loop sites (%.2f s scaled to all)")
+                         % (system.name, t_es, ns, len(system.pot_site_pos), t_rq),
+               "host": platform.processor() or platform.machine()}
+    line = {
+        "metric": "potential-side electrostatics set-up time (neutralAndNuclearQPot + "
+                  "residualQ)",
+        "value": round(es + rq, 4),
+        "unit": "ms per set-up (kernels, results resident in HBM)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(es + rq, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded %s)" % system.name,
+        "config": {"workload": "%s: %d potential sites, potDim=%d, %d real-space and %d "
+                               "reciprocal cells, electroConvg 1e-16"
+                               % (system.name, len(system.pot_site_pos), P,
+                                  len(system.cell_sizes), ncr),
+                   "parallelism": "single GPU"},
+        "stages_ms": {"elecstat": round(es, 4), "residual_q": round(rq, 4),
+                      "k_es_recip": round(rc, 4),
+                      "wall_with_host_copies": round(float(np.mean(walls)) * 1e3, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -250,6 +350,10 @@ def main():
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     log("system %s n=%d nk=%d potDim=%d" % (system.name, system.vale_dim, system.num_kpoints,
                                           system.pot_dim))
+    if args.stage == "elecstat":
+        ctx.set_system(system)
+        bench_elecstat(args, system, ctx, log)
+        return
     for i in range(args.warmup):
         dt, st, _ = gpu_step(ctx, system, coeffs, num_states, populate, ep_kind)
         log("warmup %d: %.3f s %s" % (i, dt, {k: round(v, 2) for k, v in st.items()}))

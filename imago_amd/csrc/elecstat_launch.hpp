@@ -22,6 +22,7 @@ struct EsArgs {
     const double *cell_sizes;   // [cell]
     double rv[9], inv[9];       // real vectors (columns) and their inverse (rows), by value
     double th;                  // -ln(electroConvg)
+    double4 *msep;              // scratch [site][site]: minimal separation, |r_i - r_j|^2
 };
 
 // neutralAndNuclearQPot (elecStat.f90:59-664): ovl, nl_neut, l_neut are
@@ -33,9 +34,10 @@ hipError_t launch_es_neutral_nuclear(const EsArgs &A, double *ovl, double *nl_ne
 
 // residualQ (elecStat.f90:665-1043).
 struct EsRecip {
-    int ncr;                    // reciprocal cells, cell 0 the origin
+    int ncr;                    // reciprocal cells summed (the origin left out)
     const double *cdr;          // [ncr][3]
     const double *csr;          // [ncr] |G|^2
+    const double *cw;           // [ncr] 2 for a cell standing for +G and -G, else 1
     int nsrc;                   // element runs: the types whose alphas key the tables
     const int *src_type;        // [nsrc] type of each run's first site
     const int *site_src;        // [site] run of the site
@@ -69,8 +71,9 @@ inline size_t es_recip_s_doubles(int chunks, int nsites, int nk)
     return (size_t)chunks * nsites * nk * nsites;
 }
 
-// out: numPotTypes x potDim column-major, every element written.
+// out: numPotTypes x potDim column-major, every element written.  The
+// events (may be null) are recorded around the reciprocal-sum kernel.
 hipError_t launch_es_residual_q(const EsArgs &A, const EsRecip &R, double *out,
-                                hipStream_t stream);
+                                hipStream_t stream, hipEvent_t recip0, hipEvent_t recip1);
 
 }  // namespace imago

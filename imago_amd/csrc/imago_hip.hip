@@ -11,6 +11,9 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
+#include <cstdint>
+#include <map>
 #include <atomic>
 #include <chrono>
 #include <thread>
@@ -189,7 +192,7 @@ struct imago_ctx {
     double *ival = nullptr;       // [items] trace value of each item
     double *foldX = nullptr;      // [images][64 * ne] density folded per image
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
-    double timing[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double timing[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // second stream and handle: the two halves of the batched tridiagonal
     // divide and conquer run side by side (created on first use)
@@ -1987,6 +1990,7 @@ static int es_args(imago_ctx *c, double th, imago::EsArgs &A, const double **dsi
     std::memcpy(A.rv, h.rv, sizeof A.rv);
     std::memcpy(A.inv, h.inv, sizeof A.inv);
     A.th = th;
+    RCHK(dev_alloc(c, &A.msep, (size_t)h.nsites * h.nsites));
     return IMAGO_OK;
 }
 
@@ -2023,6 +2027,7 @@ extern "C" int imago_elecstat(imago_ctx *c, double log_elec_thresh, double *pot_
     c->timing[11] = ms;
     dev_release(c, buf);
     dev_release(c, (void *)dsizes);
+    dev_release(c, A.msep);
     return IMAGO_OK;
 }
 
@@ -2080,9 +2085,8 @@ extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_ce
         i += n;
     }
     R.nblk = (int)blk.size() / 3;
-    R.ncr = num_cells_recip;
     R.nsrc = (int)src_type.size();
-    R.chunks = imago::es_recip_chunks(h.nsites, R.nblk, R.ncr);
+
     double min_pot = 10000.0;   // potTypes.f90:213-250
     for (int t = 0; t < h.npt; ++t) min_pot = std::min(min_pot, h.pot_alphas[(size_t)t * h.maxpa]);
     R.min_pot = min_pot;
@@ -2096,18 +2100,57 @@ extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_ce
         }
         R.volume = std::fabs(v);
     }
-    std::vector<double> cdr(cell_dims_recip, cell_dims_recip + 3 * (size_t)num_cells_recip);
-    std::vector<double> csr(cell_sizes_recip, cell_sizes_recip + num_cells_recip);
-    const double *dcdr = nullptr, *dcsr = nullptr;
+    // The reciprocal sum leaves out cell 0, the origin (elecStat.f90:986-990).
+    // +G and -G add the same term (cos is even), so one cell of each exact
+    // pair is summed with weight 2; a cell without a partner keeps weight 1.
+    std::vector<double> cdr, csr, cw;
+    {
+        auto key = [](const double *v, double sg) {
+            std::array<uint64_t, 3> k;
+            for (int x = 0; x < 3; ++x) {
+                double t = sg * v[x];
+                if (t == 0.0) t = 0.0;
+                std::memcpy(&k[x], &t, sizeof t);
+            }
+            return k;
+        };
+        std::map<std::array<uint64_t, 3>, int> where;
+        for (int q = 1; q < num_cells_recip; ++q)
+            where.emplace(key(cell_dims_recip + 3 * (size_t)q, 1.0), q);
+        std::vector<char> used(num_cells_recip, 0);
+        for (int q = 1; q < num_cells_recip; ++q) {
+            if (used[q]) continue;
+            const auto it = where.find(key(cell_dims_recip + 3 * (size_t)q, -1.0));
+            double w = 1.0;
+            if (it != where.end() && it->second > q && !used[it->second]) {
+                used[it->second] = 1;
+                w = 2.0;
+            }
+            cdr.insert(cdr.end(), cell_dims_recip + 3 * (size_t)q,
+                       cell_dims_recip + 3 * (size_t)q + 3);
+            csr.push_back(cell_sizes_recip[q]);
+            cw.push_back(w);
+        }
+        if (csr.empty()) {   // no cell but the origin: an empty sum
+            cdr.assign(3, 0.0);
+            csr.push_back(1.0);
+            cw.push_back(0.0);
+        }
+    }
+    const double *dcdr = nullptr, *dcsr = nullptr, *dcw = nullptr;
     const int *dsrc = nullptr, *dsite_src = nullptr, *dord = nullptr, *dblk = nullptr;
+    R.ncr = (int)csr.size();
+    R.chunks = imago::es_recip_chunks(h.nsites, R.nblk, R.ncr);
     RCHK(dev_upload(c, &dcdr, cdr));
     RCHK(dev_upload(c, &dcsr, csr));
+    RCHK(dev_upload(c, &dcw, cw));
     RCHK(dev_upload(c, &dsrc, src_type));
     RCHK(dev_upload(c, &dsite_src, site_src));
     RCHK(dev_upload(c, &dord, ord_type));
     RCHK(dev_upload(c, &dblk, blk));
     R.cdr = dcdr;
     R.csr = dcsr;
+    R.cw = dcw;
     R.src_type = dsrc;
     R.site_src = dsite_src;
     R.ord_type = dord;
@@ -2123,24 +2166,27 @@ extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_ce
     R.CS = CS;
     R.S = S;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(imago::launch_es_residual_q(A, R, out, c->stream));
+    HIPCHK(imago::launch_es_residual_q(A, R, out, c->stream, c->ev2, c->ev3));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipMemcpyAsync(non_local_residual_q, out, nout * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
+    float ms = 0.f, ms_r = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms_r, c->ev2, c->ev3));
     c->timing[12] = ms;
-    for (const void *p : {(const void *)dcdr, (const void *)dcsr, (const void *)dsrc,
+    c->timing[13] = ms_r;
+    for (const void *p : {(const void *)dcdr, (const void *)dcsr, (const void *)dcw,
+                          (const void *)dsrc,
                           (const void *)dsite_src, (const void *)dord, (const void *)dblk,
                           (const void *)E, (const void *)CS, (const void *)S,
-                          (const void *)out, (const void *)dsizes})
+                          (const void *)out, (const void *)dsizes, (const void *)A.msep})
         dev_release(c, const_cast<void *>(p));
     return IMAGO_OK;
 }
 
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 12) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 13) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

@@ -203,7 +203,8 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * direct imago_density_traces, 10 = the cooperative tridiagonal reduction
  * of the last eigensolve (0 when rocSOLVER's eigensolver ran), 11 = the
  * kernels of the last imago_elecstat, 12 = those of the last
- * imago_residual_q (both without the copies to the host). */
+ * imago_residual_q (both without the copies to the host), 13 = the
+ * reciprocal-space sum kernel of the last imago_residual_q. */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
