@@ -1,4 +1,5 @@
-"""ctypes mirror of ``imago_system_desc`` (include/imago_hip.h)."""
+"""ctypes mirrors of ``imago_system_desc`` and ``imago_xc_mesh_params``
+(include/imago_hip.h)."""
 from __future__ import annotations
 
 import ctypes
@@ -45,6 +46,24 @@ class SystemDesc(ctypes.Structure):
         ("gamma", ctypes.c_int),
         ("log_basis_fn_thresh", ctypes.c_double),
     ]
+
+
+class XcMeshParams(ctypes.Structure):
+    _fields_ = [
+        ("num_sample_vectors", ctypes.c_int),
+        ("weight_in", ctypes.c_double),
+        ("weight_out", ctypes.c_double),
+        ("r_sample_in", ctypes.c_double),
+        ("r_sample_out", ctypes.c_double),
+        ("r_sample_space", ctypes.c_double),
+    ]
+
+
+def make_xc_params(mesh):
+    """XcMeshParams from an XcMesh (imago_amd.system)."""
+    return XcMeshParams(int(mesh.num_sample_vectors), float(mesh.weight_in),
+                        float(mesh.weight_out), float(mesh.r_sample_in),
+                        float(mesh.r_sample_out), float(mesh.r_sample_space))
 
 
 def make_desc(system):

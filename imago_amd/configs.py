@@ -32,6 +32,7 @@ from .system import (AtomType, PotType, System, BOHR_PER_ANGSTROM,
 SI, O, C = 0, 1, 2
 Z_OF = {SI: 14.0, O: 8.0, C: 6.0}
 NUC_ALPHA = {SI: 20.0, O: 20.0, C: 20.0}     # atomicPDB si/pot1 nucAlpha
+COV_RADIUS = {SI: 2.10, O: 1.25, C: 1.45}    # covalent radii in bohr (1.11, 0.66, 0.77 A)
 
 
 def log_thresh_for_cutoff(cutoff_bohr: float, alpha_min: float) -> float:
@@ -79,7 +80,8 @@ def si2_primitive(seed: int = 1, cutoff_bohr: float = 14.0, mesh=(4, 4, 4),
     pos = np.array([[0, 0, 0], [0.25 * a] * 3])
     at = make_atom_type(rng, SI, ls, nalpha=nalpha)
     pot = PotType(element=SI, alphas=geometric_alphas(0.1, 1.0e4, npot),
-                  nuc_charge=Z_OF[SI], nuc_alpha=NUC_ALPHA[SI], sites=[0, 1])
+                  nuc_charge=Z_OF[SI], nuc_alpha=NUC_ALPHA[SI], sites=[0, 1],
+                  covalent_radius=COV_RADIUS[SI])
     if log_thresh is None:
         log_thresh = log_thresh_for_cutoff(cutoff_bohr, at.alphas[0])
     kp, kw = kpoint_mesh(rv, mesh, ops=cubic_point_group())
@@ -106,7 +108,8 @@ def si64_supercell(seed: int = 2, cutoff_bohr: float = 16.0, mesh=(2, 2, 2),
     at = make_atom_type(rng, SI, ls)
     palphas = geometric_alphas(0.1, 1.0e4, 16)
     pots = [PotType(element=SI, alphas=palphas, nuc_charge=Z_OF[SI],
-                    nuc_alpha=NUC_ALPHA[SI], sites=[i]) for i in range(len(pos))]
+                    nuc_alpha=NUC_ALPHA[SI], sites=[i], covalent_radius=COV_RADIUS[SI])
+            for i in range(len(pos))]
     if log_thresh is None:
         log_thresh = log_thresh_for_cutoff(cutoff_bohr, at.alphas[0])
     kp, kw = kpoint_mesh(rv, mesh)          # no symmetry beyond time reversal
@@ -167,7 +170,7 @@ def sio2_amorphous(seed: int = 3, n_si: int = 72, n_o: int = 144,
     for i, sp in enumerate(species):
         pots.append(PotType(element=int(sp), alphas=geometric_alphas(0.1, 1.0e4, 16),
                             nuc_charge=Z_OF[int(sp)], nuc_alpha=NUC_ALPHA[int(sp)],
-                            sites=[i]))
+                            sites=[i], covalent_radius=COV_RADIUS[int(sp)]))
     rv = np.diag([box, box, box])
     if log_thresh is None:
         log_thresh = (log_thresh_for_cutoff(cutoff_bohr, 0.12) if cutoff_bohr is not None
@@ -202,16 +205,20 @@ def small_test_system(seed: int = 5, gamma: bool = False, lmax: int = 2,
         types.append(tb)
         tof = [0, 1, 0]
     pots = [PotType(element=SI, alphas=geometric_alphas(0.15, 200.0, 4), nuc_charge=4.0,
-                    nuc_alpha=5.0, sites=[0, 1] if not two_elements else [0]),
+                    nuc_alpha=5.0, sites=[0, 1] if not two_elements else [0],
+                    covalent_radius=COV_RADIUS[SI]),
             ]
     if two_elements:
         pots.append(PotType(element=O, alphas=geometric_alphas(0.2, 100.0, 3),
-                            nuc_charge=6.0, nuc_alpha=7.0, sites=[1]))
+                            nuc_charge=6.0, nuc_alpha=7.0, sites=[1],
+                            covalent_radius=COV_RADIUS[O]))
         pots.append(PotType(element=SI, alphas=geometric_alphas(0.15, 200.0, 4),
-                            nuc_charge=4.0, nuc_alpha=5.0, sites=[2]))
+                            nuc_charge=4.0, nuc_alpha=5.0, sites=[2],
+                            covalent_radius=COV_RADIUS[SI]))
     else:
         pots.append(PotType(element=SI, alphas=geometric_alphas(0.15, 200.0, 4),
-                            nuc_charge=4.0, nuc_alpha=5.0, sites=[2]))
+                            nuc_charge=4.0, nuc_alpha=5.0, sites=[2],
+                            covalent_radius=COV_RADIUS[SI]))
     kp, kw = kpoint_mesh(rv, (2, 2, 1), shift=(0.5, 0.0, 0.0))
     lt = log_thresh_for_cutoff(cutoff_bohr, 0.2)
     return System(real_vectors=rv, atom_types=types, atom_type_of=tof,

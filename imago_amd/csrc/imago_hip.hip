@@ -31,6 +31,7 @@
 #include "geom_kernel.hpp"
 #include "hetrd_launch.hpp"
 #include "elecstat_launch.hpp"
+#include "xcmesh_launch.hpp"
 
 using namespace imago;
 
@@ -192,7 +193,7 @@ struct imago_ctx {
     double *ival = nullptr;       // [items] trace value of each item
     double *foldX = nullptr;      // [images][64 * ne] density folded per image
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
-    double timing[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double timing[15] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // second stream and handle: the two halves of the batched tridiagonal
     // divide and conquer run side by side (created on first use)
@@ -200,6 +201,16 @@ struct imago_ctx {
     rocblas_handle blas2 = nullptr;
     hipEvent_t evf = nullptr, evj = nullptr;
     hipEvent_t evh0 = nullptr, evh1 = nullptr;   // around the tridiagonal reduction
+    // exchange-correlation mesh of the system (imago_xc_mesh)
+    bool xc_ready = false;
+    imago_xc_mesh_params xcp{};
+    double xc_th = 0.0;
+    std::vector<int> xc_npts, xc_off;     // per site: ray points, first point
+    int xc_maxpts = 0;
+    long xc_total = 0;
+    double *xc_pts = nullptr, *xc_w = nullptr;   // [points][3], [points]
+    double *xc_pls = nullptr, *xc_vec = nullptr, *xc_sitecov = nullptr;
+    double *xc_sizes = nullptr;                  // real-space cell sizes
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -244,6 +255,12 @@ static void ctx_free_system(imago_ctx *c)
     c->blockH = nullptr;
     c->rho = c->rhow = c->occ = c->partial = nullptr;
     c->have_vectors = false;
+    c->xc_ready = false;
+    c->xc_npts.clear();
+    c->xc_off.clear();
+    c->xc_maxpts = 0;
+    c->xc_total = 0;
+    c->xc_pts = c->xc_w = c->xc_pls = c->xc_vec = c->xc_sitecov = c->xc_sizes = nullptr;
     c->has_system = false;
 }
 
@@ -2184,9 +2201,286 @@ extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_ce
     return IMAGO_OK;
 }
 
+// ---- exchange-correlation mesh (exchCorr.f90) ---------------------------
+
+// makeSampleVectors (exchCorr.f90:920-980): the generalised spiral of n
+// points on the sphere (E. B. Saff and A. B. J. Kuijlaars, Math.
+// Intelligencer 19 (1997) 5), with the reference's constant 3.6.
+static void xc_sample_vectors(int n, std::vector<double> &v)
+{
+    std::vector<double> h(n), th(n), ph(n);
+    h[0] = -1.0;
+    h[n - 1] = 1.0;
+    for (int k = 2; k <= n - 1; ++k) h[k - 1] = -1.0 + 2.0 * (k - 1.0) / (n - 1.0);
+    for (int k = 0; k < n; ++k) th[k] = std::acos(h[k]);
+    ph[0] = 0.0;
+    ph[n - 1] = 0.0;
+    const double two_pi = 2.0 * 3.141592653589793;
+    for (int k = 2; k <= n - 1; ++k) {
+        ph[k - 1] = ph[k - 2] + 3.6 / std::sqrt((double)n) * 1.0 / std::sqrt(1.0 - h[k - 1] * h[k - 1]);
+        while (ph[k - 1] > two_pi) ph[k - 1] = ph[k - 1] - two_pi;
+    }
+    v.assign(3 * (size_t)n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        v[3 * k] = std::sin(th[k]) * std::cos(ph[k]);
+        v[3 * k + 1] = std::sin(th[k]) * std::sin(ph[k]);
+        v[3 * k + 2] = std::cos(th[k]);
+    }
+}
+
+static imago::XcArgs xc_args(imago_ctx *c)
+{
+    const HostSys &h = c->h;
+    imago::XcArgs X{};
+    X.nsites = h.nsites;
+    X.npt = h.npt;
+    X.pot_dim = h.pot_dim;
+    X.ncells = h.ncells;
+    X.maxpa = h.maxpa;
+    X.nreps = std::min(64, h.ncells);
+    X.nvec = c->xcp.num_sample_vectors;
+    X.site_pos = c->d.pot_pos;
+    X.site_type = c->d.pot_site_type;
+    X.site_cov = c->xc_sitecov;
+    X.type_first = c->d.pot_first;
+    X.type_mult = c->d.pot_mult;
+    X.type_nalpha = c->d.pot_nalpha;
+    X.type_alphas = c->d.pot_alphas;
+    X.type_term0 = c->type_term0;
+    X.cell_dims = c->d.cell_dims;
+    X.cell_sizes = c->xc_sizes;
+    X.pls = c->xc_pls;
+    X.vec = c->xc_vec;
+    std::memcpy(X.rv, h.rv, sizeof X.rv);
+    std::memcpy(X.inv, h.inv, sizeof X.inv);
+    X.th = c->xc_th;
+    double min_pot = 10000.0;   // potTypes.f90:213-250
+    for (int t = 0; t < h.npt; ++t) min_pot = std::min(min_pot, h.pot_alphas[(size_t)t * h.maxpa]);
+    X.max_radius = std::sqrt(c->xcp.r_sample_out / min_pot);
+    return X;
+}
+
+extern "C" int imago_xc_mesh(imago_ctx *c, const imago_xc_mesh_params *params,
+                             double log_elec_thresh, const double *covalent_radius,
+                             int *max_num_ray_points)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (!c->has_system) return fail(IMAGO_E_STATE, "imago_set_system must come first");
+    if (!params || !covalent_radius) return fail(IMAGO_E_ARG, "null argument");
+    const imago_xc_mesh_params &p = *params;
+    if (p.num_sample_vectors < 1 || p.num_sample_vectors > 100000)
+        return fail(IMAGO_E_ARG, "num_sample_vectors %d out of range", p.num_sample_vectors);
+    if (!(p.r_sample_space > 0.0 && p.r_sample_space < 1.0) || !(p.r_sample_in > 0.0) ||
+        !(p.r_sample_out > 0.0))
+        return fail(IMAGO_E_ARG, "radial sampling needs rIn > 0, rOut > 0, 0 < spacing < 1");
+    if (!(p.weight_in >= 0.0) || !(p.weight_out >= 0.0))
+        return fail(IMAGO_E_ARG, "angular weights must be >= 0");
+    if (!(log_elec_thresh > 0.0) || !std::isfinite(log_elec_thresh))
+        return fail(IMAGO_E_ARG, "log_elec_thresh must be positive and finite");
+    HIPCHK(hipSetDevice(c->device));
+    HostSys &h = c->h;
+    if (h.maxpa > 64) return fail(IMAGO_E_ARG, "more than 64 alphas in a potential type");
+    for (void *q : {(void *)c->xc_pts, (void *)c->xc_w, (void *)c->xc_pls, (void *)c->xc_vec,
+                    (void *)c->xc_sitecov, (void *)c->xc_sizes})
+        if (q) dev_release(c, q);
+    c->xc_pts = c->xc_w = c->xc_pls = c->xc_vec = c->xc_sitecov = c->xc_sizes = nullptr;
+    c->xc_ready = false;
+    c->xcp = p;
+    c->xc_th = log_elec_thresh;
+    const int ns = h.nsites, nv = p.num_sample_vectors;
+    std::vector<double> vec, cov(ns), pls(3 * (size_t)ns, 0.0);
+    xc_sample_vectors(nv, vec);
+    const double small = 1.0e-8;
+    for (int i = 0; i < ns; ++i) {
+        cov[i] = covalent_radius[h.site_type[i]];
+        if (cov[i] < small) continue;
+        double L[3];
+        find_lattice_vector(h, &h.site_pos[3 * (size_t)i], L);   // exchCorr.f90:137-148
+        for (int x = 0; x < 3; ++x) pls[3 * i + x] = h.site_pos[3 * i + x] - L[x];
+    }
+    {
+        const double *q = nullptr;
+        RCHK(dev_upload(c, &q, vec));
+        c->xc_vec = const_cast<double *>(q);
+        RCHK(dev_upload(c, &q, cov));
+        c->xc_sitecov = const_cast<double *>(q);
+        RCHK(dev_upload(c, &q, pls));
+        c->xc_pls = const_cast<double *>(q);
+        RCHK(dev_upload(c, &q, h.cell_sizes));
+        c->xc_sizes = const_cast<double *>(q);
+    }
+    const imago::XcArgs X = xc_args(c);
+    double *dmax = nullptr;
+    RCHK(dev_alloc(c, &dmax, (size_t)ns * nv));
+    HIPCHK(imago::launch_xc_maxrad(X, dmax, c->stream));
+    std::vector<double> maxrad((size_t)ns * nv);
+    HIPCHK(hipMemcpyAsync(maxrad.data(), dmax, maxrad.size() * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, dmax);
+    // ray points and radial weights of every site (exchCorr.f90:492-540)
+    const double angw[2] = {p.weight_in, p.weight_out};
+    const double wf = (1.0 - p.r_sample_space) / std::sqrt(p.r_sample_space);
+    const double sqsp = std::sqrt(p.r_sample_space);
+    std::vector<double> pts, w;
+    c->xc_npts.assign(ns, 0);
+    c->xc_off.assign(ns + 1, 0);
+    int most = 0;
+    for (int i = 0; i < ns; ++i) {
+        c->xc_off[i] = (int)w.size();
+        const int t = h.site_type[i];
+        if (cov[i] < small) continue;
+        const double cov_half = cov[i] / 2.0;
+        const double min_radius =
+            std::sqrt(p.r_sample_in / h.pot_alphas[(size_t)t * h.maxpa + h.pot_nalpha[t] - 1]);
+        const double *ls = &pls[3 * (size_t)i];
+        int np = 0;
+        for (int j = 0; j < nv; ++j) {
+            const double *v = &vec[3 * (size_t)j];
+            double r = maxrad[(size_t)i * nv + j];
+            int wi = 1;
+            double epf = 0.5;
+            for (int guard = 0;; ++guard) {
+                if (guard > 100000)
+                    return fail(IMAGO_E_ARG, "ray of site %d does not reach rSampleIn", i);
+                for (int x = 0; x < 3; ++x) pts.push_back(r * v[x] + ls[x]);
+                w.push_back(angw[wi] * wf * epf * (r * r * r));
+                ++np;
+                epf = 1.0;
+                if (r < cov_half || r < min_radius) wi = 0;
+                if (angw[wi] < small) break;
+                if (r < min_radius) {
+                    r = r * sqsp;
+                    for (int x = 0; x < 3; ++x) pts.push_back(0.75 * r * v[x] + ls[x]);
+                    w.push_back(angw[0] * (r * r * r) / 3.0);
+                    ++np;
+                    break;
+                }
+                r = r * p.r_sample_space;
+            }
+        }
+        c->xc_npts[i] = np;
+        most = std::max(most, np);
+    }
+    c->xc_off[ns] = (int)w.size();
+    c->xc_total = (long)w.size();
+    c->xc_maxpts = most;
+    if (w.empty()) {
+        pts.assign(3, 0.0);
+        w.assign(1, 0.0);
+    }
+    {
+        const double *q = nullptr;
+        RCHK(dev_upload(c, &q, pts));
+        c->xc_pts = const_cast<double *>(q);
+        RCHK(dev_upload(c, &q, w));
+        c->xc_w = const_cast<double *>(q);
+    }
+    c->xc_ready = true;
+    if (max_num_ray_points) *max_num_ray_points = most;
+    return IMAGO_OK;
+}
+
+extern "C" int imago_xc_rho_op(imago_ctx *c, int site, int gga, int *num_ray_points,
+                               double *radial_weight, double *exch_rho_op)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
+    if (site < 0 || site >= c->h.nsites) return fail(IMAGO_E_ARG, "site %d out of range", site);
+    if (!radial_weight || !exch_rho_op) return fail(IMAGO_E_ARG, "null output array");
+    HIPCHK(hipSetDevice(c->device));
+    const int P = c->h.pot_dim, mp = std::max(c->xc_maxpts, 1), np = c->xc_npts[site];
+    const int nop = gga ? 10 : 1;
+    const size_t nout = (size_t)nop * mp * P;
+    double *op = nullptr;
+    RCHK(dev_alloc(c, &op, nout));
+    HIPCHK(hipMemsetAsync(op, 0, nout * 8, c->stream));
+    const imago::XcArgs X = xc_args(c);
+    HIPCHK(imago::launch_xc_op(X, gga != 0, c->xc_pts + 3 * (size_t)c->xc_off[site], np, mp, op,
+                               nullptr, nullptr, c->stream));
+    std::fill(radial_weight, radial_weight + mp, 0.0);
+    if (np > 0)
+        HIPCHK(hipMemcpyAsync(radial_weight, c->xc_w + c->xc_off[site], (size_t)np * 8,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(exch_rho_op, op, nout * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, op);
+    if (num_ray_points) *num_ray_points = np;
+    return IMAGO_OK;
+}
+
+extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
+    if (!exch_corr_overlap) return fail(IMAGO_E_ARG, "null output array");
+    HIPCHK(hipSetDevice(c->device));
+    const int P = c->h.pot_dim, ns = c->h.nsites;
+    const int mp = std::max(c->xc_maxpts, 1);
+    const imago::XcArgs X = xc_args(c);
+    // sites in batches of about 1 GB of operator
+    const size_t per_site = (size_t)mp * P;
+    const int batch = (int)std::max<size_t>(1, std::min<size_t>(ns, (1ull << 27) / per_site));
+    double *C = nullptr, *op = nullptr, *A = nullptr, *Cs = nullptr;
+    unsigned char *nz = nullptr;
+    int *rows = nullptr, *cnt = nullptr, *slot = nullptr;
+    RCHK(dev_alloc(c, &C, (size_t)P * P));
+    RCHK(dev_alloc(c, &op, (size_t)batch * per_site));
+    RCHK(dev_alloc(c, &A, per_site));
+    RCHK(dev_alloc(c, &Cs, (size_t)P * P));
+    RCHK(dev_alloc(c, &nz, (size_t)batch * P));
+    RCHK(dev_alloc(c, &rows, (size_t)batch * P));
+    RCHK(dev_alloc(c, &cnt, (size_t)batch));
+    RCHK(dev_alloc(c, &slot, (size_t)batch * mp));
+    HIPCHK(hipMemsetAsync(C, 0, (size_t)P * P * 8, c->stream));
+    std::vector<int> hslot((size_t)batch * mp), hcnt(batch);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int s0 = 0; s0 < ns; s0 += batch) {
+        const int nb = std::min(batch, ns - s0);
+        const int p0 = c->xc_off[s0], p1 = c->xc_off[s0 + nb];
+        const int npb = p1 - p0;
+        if (npb == 0) continue;
+        for (int b = 0; b < nb; ++b)
+            for (int q = c->xc_off[s0 + b]; q < c->xc_off[s0 + b + 1]; ++q) hslot[q - p0] = b;
+        HIPCHK(hipMemcpyAsync(slot, hslot.data(), (size_t)npb * 4, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemsetAsync(op, 0, (size_t)npb * P * 8, c->stream));
+        HIPCHK(hipMemsetAsync(nz, 0, (size_t)nb * P, c->stream));
+        HIPCHK(imago::launch_xc_op(X, false, c->xc_pts + 3 * (size_t)p0, npb, npb, op, nz, slot,
+                                   c->stream));
+        HIPCHK(imago::launch_xc_compact(nz, nb, P, rows, cnt, c->stream));
+        HIPCHK(hipMemcpyAsync(hcnt.data(), cnt, (size_t)nb * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int b = 0; b < nb; ++b) {
+            const int m = hcnt[b], np = c->xc_npts[s0 + b];
+            if (m == 0 || np == 0) continue;
+            const int q0 = c->xc_off[s0 + b] - p0;
+            const int *rb = rows + (size_t)b * P;
+            HIPCHK(imago::launch_xc_gather(op + (size_t)q0 * P, c->xc_w + p0 + q0, P, rb, m, np,
+                                           A, c->stream));
+            const double one = 1.0, zero = 0.0;
+            BLASCHK(rocblas_dsyrk(c->blas, rocblas_fill_upper, rocblas_operation_none, m, np,
+                                  &one, A, m, &zero, Cs, m));
+            HIPCHK(imago::launch_xc_scatter(Cs, rb, m, P, C, c->stream));
+        }
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(exch_corr_overlap, C, (size_t)P * P * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timing[14] = ms;
+    for (void *q : {(void *)C, (void *)op, (void *)A, (void *)Cs, (void *)nz, (void *)rows,
+                    (void *)cnt, (void *)slot})
+        dev_release(c, q);
+    return IMAGO_OK;
+}
+
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 13) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 14) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

@@ -38,6 +38,13 @@ module O_ImagoHIP
       real(c_double) :: log_basis_fn_thresh
    end type imago_system_desc
 
+   ! readExchCorrMeshParameters input block (exchCorr.f90:876-913)
+   type, bind(C) :: imago_xc_mesh_params
+      integer(c_int) :: num_sample_vectors
+      real(c_double) :: weight_in, weight_out
+      real(c_double) :: r_sample_in, r_sample_out, r_sample_space
+   end type imago_xc_mesh_params
+
    type(c_ptr), save :: imagoCtx = c_null_ptr
 
    interface
@@ -185,6 +192,35 @@ module O_ImagoHIP
          real(c_double), intent(in) :: cellDimsRecip(3,*), cellSizesRecip(*)
          real(c_double), intent(out) :: nonLocalResidualQ(*)
       end function imago_residual_q
+
+      ! getECMeshParameters + makeSampleVectors (exchCorr.f90:63-342, 920-980)
+      integer(c_int) function imago_xc_mesh(ctx, params, logElecThresh, covalentRadius, &
+            & maxNumRayPoints) bind(C, name="imago_xc_mesh")
+         import :: c_int, c_ptr, c_double, imago_xc_mesh_params
+         type(c_ptr), value :: ctx
+         type(imago_xc_mesh_params), intent(in) :: params
+         real(c_double), value :: logElecThresh
+         real(c_double), intent(in) :: covalentRadius(*)
+         integer(c_int), intent(out) :: maxNumRayPoints
+      end function imago_xc_mesh
+
+      ! one site's numRayPoints, radialWeight and exchRhoOp (exchCorr.f90:492-790)
+      integer(c_int) function imago_xc_rho_op(ctx, site, gga, numRayPoints, radialWeight, &
+            & exchRhoOp) bind(C, name="imago_xc_rho_op")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: site, gga
+         integer(c_int), intent(out) :: numRayPoints
+         real(c_double), intent(out) :: radialWeight(*), exchRhoOp(*)
+      end function imago_xc_rho_op
+
+      ! exchCorrOverlap (exchCorr.f90:805-810)
+      integer(c_int) function imago_xc_overlap(ctx, exchCorrOverlap) &
+            & bind(C, name="imago_xc_overlap")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: exchCorrOverlap(*)
+      end function imago_xc_overlap
    end interface
 
 contains

@@ -39,6 +39,7 @@ EXPORTS = [
     "imago_primitive_batch", "imago_boys_batch", "imago_assemble", "imago_get_packed",
     "imago_secular", "imago_get_eigenvectors", "imago_density_traces", "imago_zhegv",
     "imago_dsygv", "imago_last_timing", "imago_ep_counts", "imago_elecstat", "imago_residual_q",
+    "imago_xc_mesh", "imago_xc_rho_op", "imago_xc_overlap",
 ]
 
 
@@ -83,6 +84,9 @@ def load():
     L.imago_elecstat.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
     L.imago_residual_q.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp,
                                    _dp]
+    L.imago_xc_mesh.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp, _ip]
+    L.imago_xc_rho_op.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
+    L.imago_xc_overlap.argtypes = [ctypes.c_void_p, _dp]
     _lib = L
     return L
 
@@ -242,6 +246,43 @@ class Context:
         out = np.zeros((len(self.system.pot_types), self.pot_dim), order="F")
         _check(load().imago_residual_q(self._h, float(th), len(sizes), _p(dims), _p(sizes),
                                        _p(out)))
+        return out
+
+    # -- exchange-correlation mesh (exchCorr.f90) ---------------------------
+    def xc_mesh(self, mesh=None, log_elec_thresh=None, covalent_radius=None):
+        """getECMeshParameters for the current system; returns
+        maxNumRayPoints.  mesh: an imago_amd.system.XcMesh (makeinput's
+        defaults when None); covalent radii from the potential types."""
+        from imago_amd._desc import make_xc_params
+        from imago_amd.system import XcMesh
+        self._xc_mesh = mesh or XcMesh()
+        params = make_xc_params(self._xc_mesh)
+        th = self.system.log_elec_thresh if log_elec_thresh is None else log_elec_thresh
+        cov = np.ascontiguousarray(
+            [p.covalent_radius for p in self.system.pot_types] if covalent_radius is None
+            else covalent_radius, dtype=np.float64)
+        mx = ctypes.c_int()
+        _check(load().imago_xc_mesh(self._h, ctypes.byref(params), float(th), _p(cov),
+                                    ctypes.byref(mx)))
+        self.xc_maxpts = mx.value
+        return mx.value
+
+    def xc_rho_op(self, site, gga=False):
+        """(num_ray_points, radialWeight[maxpts], exchRhoOp[potDim, maxpts,
+        numOp] in Fortran order) of one site."""
+        mp = max(self.xc_maxpts, 1)
+        nop = 10 if gga else 1
+        rw = np.zeros(mp)
+        op = np.zeros((self.pot_dim, mp, nop), order="F")
+        n = ctypes.c_int()
+        _check(load().imago_xc_rho_op(self._h, int(site), 1 if gga else 0, ctypes.byref(n),
+                                      _p(rw), _p(op)))
+        return n.value, rw, op
+
+    def xc_overlap(self):
+        """exchCorrOverlap, potDim x potDim (upper triangle)."""
+        out = np.zeros((self.pot_dim, self.pot_dim), order="F")
+        _check(load().imago_xc_overlap(self._h, _p(out)))
         return out
 
     # -- single problems ----------------------------------------------------

@@ -91,12 +91,26 @@ class AtomType:
 
 
 @dataclass
+class XcMesh:
+    """Exchange-correlation mesh input (readExchCorrMeshParameters,
+    exchCorr.f90:876-913); defaults are makeinput's (num_samp_vectors 100,
+    xc weights 0.5 0.5, radial sampling 0.1 3.5 0.8)."""
+    num_sample_vectors: int = 100
+    weight_in: float = 0.5
+    weight_out: float = 0.5
+    r_sample_in: float = 0.1
+    r_sample_out: float = 3.5
+    r_sample_space: float = 0.8
+
+
+@dataclass
 class PotType:
     element: int
     alphas: np.ndarray
     nuc_charge: float
     nuc_alpha: float
     sites: list             # indices into the potential-site list (contiguous)
+    covalent_radius: float = 0.0   # bohr (potTypes(:)%covalentRadius)
 
     def __post_init__(self):
         self.alphas = np.asarray(self.alphas, dtype=np.float64)

@@ -110,6 +110,18 @@ typedef struct imago_system_desc {
     double log_basis_fn_thresh;   /* -ln(basisFnConvg) (lattice.f90:239)    */
 } imago_system_desc;
 
+/*
+ * Exchange-correlation mesh parameters, the input block of
+ * readExchCorrMeshParameters (exchCorr.f90:876-913).
+ */
+typedef struct {
+    int num_sample_vectors;       /* NUM_ANGULAR_SAMPLE_VECTORS             */
+    double weight_in, weight_out; /* WTIN_WTOUT (angSampleWeights(1:2))     */
+    double r_sample_in;           /* RADIAL_SAMPLE-IN_OUT_SPACING (1)       */
+    double r_sample_out;          /*                              (2)       */
+    double r_sample_space;        /*                              (3), < 1  */
+} imago_xc_mesh_params;
+
 typedef struct imago_ctx imago_ctx;
 
 /* ---- context ---------------------------------------------------------- */
@@ -204,7 +216,8 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * of the last eigensolve (0 when rocSOLVER's eigensolver ran), 11 = the
  * kernels of the last imago_elecstat, 12 = those of the last
  * imago_residual_q (both without the copies to the host), 13 = the
- * reciprocal-space sum kernel of the last imago_residual_q. */
+ * reciprocal-space sum kernel of the last imago_residual_q, 14 = the
+ * device work of the last imago_xc_overlap. */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
@@ -244,6 +257,32 @@ int imago_elecstat(imago_ctx *ctx, double log_elec_thresh, double *pot_alpha_ove
 int imago_residual_q(imago_ctx *ctx, double log_elec_thresh, int num_cells_recip,
                      const double *cell_dims_recip, const double *cell_sizes_recip,
                      double *non_local_residual_q);
+
+/* ---- exchange-correlation mesh (src/imago/exchCorr.f90) ----------------- *
+ * imago_xc_mesh is getECMeshParameters (exchCorr.f90:63-342) with
+ * makeSampleVectors (:920-980): the angular sample vectors, the largest
+ * radius of every (site, vector) ray and the ray points of every
+ * potential site, kept on the device for the two calls below.
+ * covalent_radius[numPotTypes] is potTypes(:)%covalentRadius; a site whose
+ * type has a radius below 1e-8 gets no mesh, as in the reference.  Returns
+ * maxNumRayPoints. */
+int imago_xc_mesh(imago_ctx *ctx, const imago_xc_mesh_params *params, double log_elec_thresh,
+                  const double *covalent_radius, int *max_num_ray_points);
+
+/* The mesh of one potential site (0-based) as makeECMeshAndOverlap
+ * (exchCorr.f90:345-875) writes it: *num_ray_points, radial_weight
+ * [maxNumRayPoints] and exch_rho_op, potDim x maxNumRayPoints x numOp
+ * column-major (numOp = 1 for LDA, 10 for GGA: the reference's
+ * exchRhoOp(potDim, maxNumRayPoints, numOpValues)); entries beyond the
+ * site's points are 0. */
+int imago_xc_rho_op(imago_ctx *ctx, int site, int gga, int *num_ray_points,
+                    double *radial_weight, double *exch_rho_op);
+
+/* exchCorrOverlap (exchCorr.f90:805-810): potDim x potDim column-major,
+ * the sum over every site's ray points of radialWeight * op op^T of the
+ * LDA operator, upper triangle (row <= column) filled as the reference
+ * fills it, lower triangle 0. */
+int imago_xc_overlap(imago_ctx *ctx, double *exch_corr_overlap);
 
 #ifdef __cplusplus
 }

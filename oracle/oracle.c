@@ -23,6 +23,8 @@
  *                665-1043 (residualQ, restated with its real-space loop
  *                stopping at the first cell beyond range in |R| order);
  *                error_fn mathSubs.f90:22-161
+ *   XC mesh      exchCorr.f90:63-342 (getECMeshParameters), 345-875
+ *                (makeECMeshAndOverlap), 920-980 (makeSampleVectors)
  */
 #include <math.h>
 #include <stdlib.h>
@@ -1352,4 +1354,245 @@ int orc_residual_q_sites(const imago_system_desc *s, double th, int ncr, const d
     free(t0);
     ctx_free(&c);
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Exchange-correlation mesh (exchCorr.f90).                           */
+
+/* makeSampleVectors, exchCorr.f90:920-980: a spiral of n unit vectors. */
+void orc_xc_sample_vectors(int n, double *v)
+{
+    double *h = malloc(sizeof(double) * n), *th = malloc(sizeof(double) * n);
+    double *ph = malloc(sizeof(double) * n);
+    h[0] = -1.0;
+    h[n - 1] = 1.0;
+    for (int k = 2; k <= n - 1; ++k) h[k - 1] = -1.0 + 2.0 * (k - 1.0) / (n - 1.0);
+    for (int k = 0; k < n; ++k) th[k] = acos(h[k]);
+    ph[0] = 0.0;
+    ph[n - 1] = 0.0;
+    for (int k = 2; k <= n - 1; ++k) {
+        ph[k - 1] = ph[k - 2] + 3.6 / sqrt((double)n) * 1.0 / sqrt(1.0 - h[k - 1] * h[k - 1]);
+        while (ph[k - 1] > 2.0 * PI) ph[k - 1] = ph[k - 1] - 2.0 * PI;
+    }
+    for (int k = 0; k < n; ++k) {
+        v[3 * k] = sin(th[k]) * cos(ph[k]);
+        v[3 * k + 1] = sin(th[k]) * sin(ph[k]);
+        v[3 * k + 2] = cos(th[k]);
+    }
+    free(h);
+    free(th);
+    free(ph);
+}
+
+static double xc_min_pot_alpha(const imago_system_desc *s)
+{
+    double m = 10000.0;
+    for (int t = 0; t < s->num_pot_types; ++t)
+        if (s->pot_type_alphas[(size_t)t * s->max_pot_alphas] < m)
+            m = s->pot_type_alphas[(size_t)t * s->max_pot_alphas];
+    return m;
+}
+
+/* getECMeshParameters, exchCorr.f90:63-342.  vec[nvec][3] from
+ * orc_xc_sample_vectors; out: pls[site][3] (potLatticeSep), maxrad
+ * [site][nvec] (siteSampleMaxRadius), npts[site] (ray points).  Returns
+ * maxNumRayPoints. */
+int orc_xc_mesh(const imago_system_desc *s, const imago_xc_mesh_params *p,
+                const double *cov, const double *vec, double *pls, double *maxrad,
+                int *npts)
+{
+    ctx_t c;
+    ctx_init(&c, s);
+    const int ns = s->num_pot_sites, nv = p->num_sample_vectors;
+    const int mpa = s->max_pot_alphas;
+    const int nreps = s->num_cells < 64 ? s->num_cells : 64;
+    const double angw[2] = {p->weight_in, p->weight_out};
+    memset(pls, 0, sizeof(double) * 3 * ns);
+    memset(maxrad, 0, sizeof(double) * (size_t)ns * nv);
+    memset(npts, 0, sizeof(int) * ns);
+    for (int i = 0; i < ns; ++i) {
+        if (cov[s->pot_site_type[i]] < SMALL_THRESH) continue;
+        double L[3];
+        find_lattice_vector(&c, s->pot_site_pos + 3 * i, L);
+        for (int x = 0; x < 3; ++x) pls[3 * i + x] = s->pot_site_pos[3 * i + x] - L[x];
+    }
+    const double max_radius = sqrt(p->r_sample_out / xc_min_pot_alpha(s));
+    int most = 0;
+    for (int i = 0; i < ns; ++i) {
+        const int ti = s->pot_site_type[i];
+        if (cov[ti] < SMALL_THRESH) continue;
+        const double cov_half = cov[ti] / 2.0;
+        const double min_radius =
+            sqrt(p->r_sample_in / s->pot_type_alphas[(size_t)ti * mpa + s->pot_type_num_alphas[ti] - 1]);
+        int cur = 0;
+        for (int j = 0; j < nv; ++j) {
+            const double *v = vec + 3 * j;
+            double cmax = max_radius;
+            for (int k = 0; k < ns; ++k) {
+                const int tk = s->pot_site_type[k];
+                if (cov[tk] < SMALL_THRESH) continue;
+                const double f = cov[ti] / (cov[ti] + cov[tk]);
+                for (int l = 0; l < nreps; ++l) {
+                    double sep[3];
+                    for (int x = 0; x < 3; ++x) {
+                        const double xl = pls[3 * k + x] + s->cell_dims[3 * l + x];
+                        const double x1 = pls[3 * i + x] + s->cell_dims[x];
+                        sep[x] = f * (xl - x1);
+                    }
+                    const double dot = sep[0] * v[0] + sep[1] * v[1] + sep[2] * v[2];
+                    if (dot <= SMALL_THRESH) continue;
+                    const double q = (sep[0] * sep[0] + sep[1] * sep[1] + sep[2] * sep[2]) / dot;
+                    if (q < cmax) cmax = q;
+                }
+            }
+            maxrad[(size_t)i * nv + j] = cmax;
+            double r = cmax;
+            int w = 1;
+            for (;;) {
+                ++cur;
+                if (r < cov_half || r < min_radius) w = 0;
+                if (angw[w] < SMALL_THRESH) break;
+                if (r < min_radius) {
+                    ++cur;
+                    break;
+                }
+                r = r * p->r_sample_space;
+            }
+        }
+        npts[i] = cur;
+        if (cur > most) most = cur;
+    }
+    ctx_free(&c);
+    return most;
+}
+
+/* The operator of one ray point (exchCorr.f90:587-760): for every site l
+ * and cell within logElecThresh/alpha_1 of the point, exp(-alpha r^2) of
+ * the site's alphas up to the first beyond the threshold, added into
+ * op[o][k][term] (o = 0 .. numOp-1).  The GGA values follow the
+ * reference's recurrences as written. */
+static void xc_point_op(const ctx_t *c, const int *t0, double th, const double *pt, int gga,
+                        int maxpts, int k, int P, double *op)
+{
+    const imago_system_desc *s = c->s;
+    const int mpa = s->max_pot_alphas;
+    for (int l = 0; l < s->num_pot_sites; ++l) {
+        const int t = s->pot_site_type[l];
+        const int na = s->pot_type_num_alphas[t];
+        const double *a = s->pot_type_alphas + (size_t)t * mpa;
+        const double negl = th / a[0];
+        const double *pl = s->pot_site_pos + 3 * l;
+        const double sv[3] = {pt[0] - pl[0], pt[1] - pl[1], pt[2] - pl[2]};
+        double L[3];
+        find_lattice_vector(c, sv, L);
+        const double off[3] = {sv[0] - L[0], sv[1] - L[1], sv[2] - L[2]};
+        const double osq = off[0] * off[0] + off[1] * off[1] + off[2] * off[2];
+        const double negl_plus = negl + osq + 2.0 * sqrt(negl * osq);
+        for (int m = 0; m < s->num_cells; ++m) {
+            if (s->cell_sizes[m] > negl_plus) break;
+            const double *R = s->cell_dims + 3 * m;
+            const double dx = off[0] - R[0], dy = off[1] - R[1], dz = off[2] - R[2];
+            const double rad = dx * dx + dy * dy + dz * dz;
+            if (rad > negl) continue;
+            const double dd[9] = {dx, dy, dz, dx * dx, dx * dy, dx * dz, dy * dy, dy * dz, dz * dz};
+            for (int n = 0; n < na; ++n) {
+                const double am = a[n] * rad;
+                if (am > th) break;
+                const long at = (long)k * P + t0[t] + n;
+                op[at] = op[at] + exp(-am);
+                if (gga) {
+                    for (int o = 1; o < 10; ++o) {
+                        const long ao = ((long)o * maxpts + k) * P + t0[t] + n;
+                        const double f = o < 4 ? a[n] : a[n] * a[n];
+                        op[ao] = f * op[ao] + dd[o - 1] * exp(-am);
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* The mesh of one site as makeECMeshAndOverlap builds it (exchCorr.f90:
+ * 492-775): its ray points and radial weights, and the operator of every
+ * point.  rw[maxpts], op[numOp][maxpts][P], both zeroed first.  Returns
+ * the site's number of ray points. */
+int orc_xc_site(const imago_system_desc *s, const imago_xc_mesh_params *p, double th,
+                const double *cov, const double *vec, const double *pls,
+                const double *maxrad, int maxpts, int site, int gga, double *rw, double *op)
+{
+    ctx_t c;
+    ctx_init(&c, s);
+    const int P = es_pot_dim(s);
+    const int nop = gga ? 10 : 1;
+    const int mpa = s->max_pot_alphas, nv = p->num_sample_vectors;
+    int *t0 = es_type_term0(s);
+    memset(rw, 0, sizeof(double) * maxpts);
+    memset(op, 0, sizeof(double) * (size_t)nop * maxpts * P);
+    const int ti = s->pot_site_type[site];
+    int np = 0;
+    if (!(cov[ti] < SMALL_THRESH)) {
+        const double angw[2] = {p->weight_in, p->weight_out};
+        const double cov_half = cov[ti] / 2.0;
+        const double wf = (1.0 - p->r_sample_space) / sqrt(p->r_sample_space);
+        const double sqsp = sqrt(p->r_sample_space);
+        const double min_radius =
+            sqrt(p->r_sample_in / s->pot_type_alphas[(size_t)ti * mpa + s->pot_type_num_alphas[ti] - 1]);
+        const double *ls = pls + 3 * site;
+        double *pts = malloc(sizeof(double) * 3 * (maxpts + 1));
+        for (int j = 0; j < nv; ++j) {
+            const double *v = vec + 3 * j;
+            double r = maxrad[(size_t)site * nv + j];
+            int w = 1;
+            const int start = np;
+            double epf = 0.5;
+            for (;;) {
+                for (int x = 0; x < 3; ++x) pts[3 * np + x] = r * v[x] + ls[x];
+                rw[np] = angw[w] * wf * epf * (r * r * r);
+                ++np;
+                epf = 1.0;
+                if (r < cov_half || r < min_radius) w = 0;
+                if (angw[w] < SMALL_THRESH) break;
+                if (r < min_radius) {
+                    r = r * sqsp;
+                    for (int x = 0; x < 3; ++x) pts[3 * np + x] = 0.75 * r * v[x] + ls[x];
+                    rw[np] = angw[0] * (r * r * r) / 3.0;
+                    ++np;
+                    break;
+                }
+                r = r * p->r_sample_space;
+            }
+            for (int k = start; k < np; ++k)
+                xc_point_op(&c, t0, th, pts + 3 * k, gga, maxpts, k, P, op);
+        }
+        free(pts);
+    }
+    free(t0);
+    ctx_free(&c);
+    return np;
+}
+
+/* exchCorrOverlap, exchCorr.f90:805-810: over every site's ray points in
+ * order, column k gets op(k) op(1:k) radialWeight (upper triangle). */
+int orc_xc_overlap(const imago_system_desc *s, const imago_xc_mesh_params *p, double th,
+                   const double *cov, const double *vec, const double *pls,
+                   const double *maxrad, int maxpts, double *ovl)
+{
+    const int P = es_pot_dim(s);
+    memset(ovl, 0, sizeof(double) * (size_t)P * P);
+    double *rw = malloc(sizeof(double) * maxpts);
+    double *op = malloc(sizeof(double) * (size_t)maxpts * P);
+    for (int i = 0; i < s->num_pot_sites; ++i) {
+        const int np = orc_xc_site(s, p, th, cov, vec, pls, maxrad, maxpts, i, 0, rw, op);
+        for (int j = 0; j < np; ++j) {
+            const double *o = op + (size_t)j * P;
+            for (int k = 0; k < P; ++k) {
+                if (o[k] == 0.0) continue;   /* adds only zeros */
+                double *col = ovl + (size_t)k * P;
+                for (int r = 0; r <= k; ++r) col[r] = col[r] + o[k] * o[r] * rw[j];
+            }
+        }
+    }
+    free(rw);
+    free(op);
+    return 0;
 }

@@ -97,6 +97,29 @@ int orc_residual_q_sites(const imago_system_desc *sys, double th, int ncr,
                          const double *cdr, const double *csr, int i_begin,
                          int i_end, double *out);
 
+/* makeSampleVectors (exchCorr.f90:920-980): v[n][3]. */
+void orc_xc_sample_vectors(int n, double *v);
+
+/* getECMeshParameters (exchCorr.f90:63-342): pls[site][3], maxrad
+ * [site][nvec], npts[site]; returns maxNumRayPoints. */
+int orc_xc_mesh(const imago_system_desc *sys, const imago_xc_mesh_params *p,
+                const double *cov, const double *vec, double *pls,
+                double *maxrad, int *npts);
+
+/* One site's mesh of makeECMeshAndOverlap (exchCorr.f90:492-775):
+ * rw[maxpts], op[numOp][maxpts][potDim]; returns its ray points. */
+int orc_xc_site(const imago_system_desc *sys, const imago_xc_mesh_params *p,
+                double th, const double *cov, const double *vec,
+                const double *pls, const double *maxrad, int maxpts, int site,
+                int gga, double *rw, double *op);
+
+/* exchCorrOverlap (exchCorr.f90:805-810), potDim x potDim column-major,
+ * upper triangle. */
+int orc_xc_overlap(const imago_system_desc *sys, const imago_xc_mesh_params *p,
+                   double th, const double *cov, const double *vec,
+                   const double *pls, const double *maxrad, int maxpts,
+                   double *ovl);
+
 #ifdef __cplusplus
 }
 #endif

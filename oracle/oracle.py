@@ -54,6 +54,15 @@ def lib():
                                      _dp]
         L.orc_residual_q_sites.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp,
                                            _dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_xc_sample_vectors.argtypes = [ctypes.c_int, _dp]
+        L.orc_xc_mesh.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp, _dp,
+                                  ctypes.POINTER(ctypes.c_int)]
+        L.orc_xc_mesh.restype = ctypes.c_int
+        L.orc_xc_site.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp, _dp,
+                                  _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_xc_site.restype = ctypes.c_int
+        L.orc_xc_overlap.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp, _dp,
+                                     _dp, _dp, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -203,6 +212,53 @@ def residual_q(system, log_elec_thresh=None, recip_cells=None, sites=None):
     if rc != 0:
         raise RuntimeError("oracle residual_q failed with code %d" % rc)
     return out
+
+
+class XcOracleMesh:
+    """getECMeshParameters of the oracle for one system and mesh input."""
+
+    def __init__(self, system, mesh=None, covalent_radius=None):
+        from imago_amd._desc import make_desc, make_xc_params
+        from imago_amd.system import XcMesh
+        self.system = system
+        self.mesh = mesh or XcMesh()
+        self.params = make_xc_params(self.mesh)
+        self.cov = np.ascontiguousarray(
+            [p.covalent_radius for p in system.pot_types] if covalent_radius is None
+            else covalent_radius, dtype=np.float64)
+        self.desc, self._keep = make_desc(system)
+        nv = self.mesh.num_sample_vectors
+        ns = len(system.pot_site_pos)
+        self.vec = np.zeros((nv, 3))
+        lib().orc_xc_sample_vectors(nv, _p(self.vec))
+        self.pls = np.zeros((ns, 3))
+        self.maxrad = np.zeros((ns, nv))
+        self.npts = np.zeros(ns, dtype=np.int32)
+        self.maxpts = lib().orc_xc_mesh(ctypes.byref(self.desc), ctypes.byref(self.params),
+                                        _p(self.cov), _p(self.vec), _p(self.pls),
+                                        _p(self.maxrad),
+                                        self.npts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+
+    def site(self, site, gga=False):
+        """(num_ray_points, radialWeight[maxpts], exchRhoOp[potDim, maxpts,
+        numOp] in Fortran order)."""
+        P = self.system.pot_dim
+        nop = 10 if gga else 1
+        rw = np.zeros(max(self.maxpts, 1))
+        op = np.zeros((P, max(self.maxpts, 1), nop), order="F")
+        n = lib().orc_xc_site(ctypes.byref(self.desc), ctypes.byref(self.params),
+                              float(self.system.log_elec_thresh), _p(self.cov), _p(self.vec),
+                              _p(self.pls), _p(self.maxrad), max(self.maxpts, 1), int(site),
+                              1 if gga else 0, _p(rw), _p(op))
+        return n, rw, op
+
+    def overlap(self):
+        P = self.system.pot_dim
+        out = np.zeros((P, P), order="F")
+        lib().orc_xc_overlap(ctypes.byref(self.desc), ctypes.byref(self.params),
+                             float(self.system.log_elec_thresh), _p(self.cov), _p(self.vec),
+                             _p(self.pls), _p(self.maxrad), max(self.maxpts, 1), _p(out))
+        return out
 
 
 def last_tuple_count():
