@@ -22,7 +22,10 @@ one GPU (the path would shard by k-point and by potential term).
 --stage elecstat measures instead the potential-side electrostatics set-up
 (neutralAndNuclearQPot + residualQ of elecStat.f90: imago_elecstat and
 imago_residual_q) on the chosen workload, one set-up per step, and prints
-its own JSON line of the same shape.
+its own JSON line of the same shape; --stage xcmesh the exchange-
+correlation mesh and its overlap (getECMeshParameters +
+makeECMeshAndOverlap of exchCorr.f90: imago_xc_mesh and imago_xc_overlap,
+makeinput's default mesh of 100 angular vectors).
 """
 from __future__ import annotations
 
@@ -58,9 +61,11 @@ def parse():
     p.add_argument("--ep-mode", choices=("direct", "panel"), default="direct",
                    help="direct: EP terms from the stored primitive blocks each iteration "
                         "(IMAGO_EP_DIRECT); panel: all potDim EP panels stored (IMAGO_EP)")
-    p.add_argument("--stage", choices=("step", "elecstat"), default="step",
+    p.add_argument("--stage", choices=("step", "elecstat", "xcmesh"), default="step",
                    help="step: the hot-path step (the benchmark line); elecstat: the "
-                        "electrostatics set-up (imago_elecstat + imago_residual_q)")
+                        "electrostatics set-up (imago_elecstat + imago_residual_q); xcmesh: "
+                        "the exchange-correlation mesh and overlap (imago_xc_mesh + "
+                        "imago_xc_overlap)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 4 GiB device-to-device copy kernel (scale factor "
                         "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -325,6 +330,74 @@ def bench_elecstat(args, system, ctx, log):
     print(json.dumps(line), flush=True)
 
 
+def xc_cpu_baseline(system, budget_s=10.0):
+    """Oracle getECMeshParameters in full and the overlap sum over its sites
+    in order until about budget_s seconds have passed, scaled to all sites;
+    one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    t0 = time.perf_counter()
+    om = oracle.XcOracleMesh(system)
+    t_mesh = time.perf_counter() - t0
+    nsites = len(system.pot_site_pos)
+    ns, spent = 0, 0.0
+    while ns < nsites and spent < budget_s:
+        t0 = time.perf_counter()
+        om.overlap(sites=(ns, ns + 1))
+        spent += time.perf_counter() - t0
+        ns += 1
+    return t_mesh, spent * nsites / ns, ns, om
+
+
+def bench_xcmesh(args, system, ctx, log):
+    P = system.pot_dim
+    for i in range(args.warmup):
+        ctx.xc_mesh()
+        ctx.xc_overlap()
+    walls_mesh, ms_ov, walls_ov = [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        mx = ctx.xc_mesh()
+        walls_mesh.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        ctx.xc_overlap()
+        walls_ov.append(time.perf_counter() - t0)
+        ms_ov.append(ctx.timing(14))
+        log("step %d: mesh %.1f ms (wall), overlap %.3f ms (device), %.1f ms with the copy"
+            % (i, walls_mesh[-1] * 1e3, ms_ov[-1], walls_ov[-1] * 1e3))
+    mesh_ms = float(np.mean(walls_mesh)) * 1e3
+    ov_ms = float(np.mean(ms_ov))
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline leg (oracle)")
+        t_mesh, t_ov, ns, _ = xc_cpu_baseline(system)
+        cpu = {"value": round((t_mesh + t_ov) * 1e3, 1), "unit": "ms per set-up", "cores": 1,
+               "kind": "port",
+               "sample": ("oracle on %s: getECMeshParameters in full (%.2f s), the overlap sum "
+                          "over %d of %d sites (%.2f s scaled to all)")
+                         % (system.name, t_mesh, ns, len(system.pot_site_pos), t_ov),
+               "host": platform.processor() or platform.machine()}
+    line = {
+        "metric": "exchange-correlation mesh set-up time (getECMeshParameters + "
+                  "exchCorrOverlap)",
+        "value": round(mesh_ms + ov_ms, 3),
+        "unit": "ms per set-up (mesh call wall time + overlap device time, result in HBM)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(mesh_ms + ov_ms, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded %s)" % system.name,
+        "config": {"workload": "%s: %d potential sites, potDim=%d, 100 angular vectors, "
+                               "%d ray points per site at most"
+                               % (system.name, len(system.pot_site_pos), P, mx),
+                   "parallelism": "single GPU"},
+        "stages_ms": {"xc_mesh_wall": round(mesh_ms, 3), "xc_overlap": round(ov_ms, 3),
+                      "xc_overlap_wall_with_copy": round(float(np.mean(walls_ov)) * 1e3, 3)},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -353,6 +426,10 @@ def main():
     if args.stage == "elecstat":
         ctx.set_system(system)
         bench_elecstat(args, system, ctx, log)
+        return
+    if args.stage == "xcmesh":
+        ctx.set_system(system)
+        bench_xcmesh(args, system, ctx, log)
         return
     for i in range(args.warmup):
         dt, st, _ = gpu_step(ctx, system, coeffs, num_states, populate, ep_kind)

@@ -2418,20 +2418,24 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
     const int P = c->h.pot_dim, ns = c->h.nsites;
     const int mp = std::max(c->xc_maxpts, 1);
     const imago::XcArgs X = xc_args(c);
-    // sites in batches of about 1 GB of operator
+    // Sites in batches of about 4 GB of operator.  Per site the rows that
+    // any of its points touches are gathered (scaled by sqrt(w)) into a
+    // column block of A; one strided-batched GEMM forms A_b A_b^T for every
+    // site of the batch (zero-padded to the batch's largest row count and
+    // point count), and the upper triangles are added into C site by site.
     const size_t per_site = (size_t)mp * P;
-    const int batch = (int)std::max<size_t>(1, std::min<size_t>(ns, (1ull << 27) / per_site));
-    double *C = nullptr, *op = nullptr, *A = nullptr, *Cs = nullptr;
+    const int batch = (int)std::max<size_t>(1, std::min<size_t>(ns, (1ull << 29) / per_site));
+    double *C = nullptr, *op = nullptr;
     unsigned char *nz = nullptr;
     int *rows = nullptr, *cnt = nullptr, *slot = nullptr;
     RCHK(dev_alloc(c, &C, (size_t)P * P));
     RCHK(dev_alloc(c, &op, (size_t)batch * per_site));
-    RCHK(dev_alloc(c, &A, per_site));
-    RCHK(dev_alloc(c, &Cs, (size_t)P * P));
     RCHK(dev_alloc(c, &nz, (size_t)batch * P));
     RCHK(dev_alloc(c, &rows, (size_t)batch * P));
     RCHK(dev_alloc(c, &cnt, (size_t)batch));
     RCHK(dev_alloc(c, &slot, (size_t)batch * mp));
+    double *A = nullptr, *Cs = nullptr;
+    size_t a_cap = 0, c_cap = 0;
     HIPCHK(hipMemsetAsync(C, 0, (size_t)P * P * 8, c->stream));
     std::vector<int> hslot((size_t)batch * mp), hcnt(batch);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -2452,18 +2456,38 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
         HIPCHK(hipMemcpyAsync(hcnt.data(), cnt, (size_t)nb * 4, hipMemcpyDeviceToHost,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        int mm = 0, nn = 0;
         for (int b = 0; b < nb; ++b) {
-            const int m = hcnt[b], np = c->xc_npts[s0 + b];
-            if (m == 0 || np == 0) continue;
-            const int q0 = c->xc_off[s0 + b] - p0;
-            const int *rb = rows + (size_t)b * P;
-            HIPCHK(imago::launch_xc_gather(op + (size_t)q0 * P, c->xc_w + p0 + q0, P, rb, m, np,
-                                           A, c->stream));
-            const double one = 1.0, zero = 0.0;
-            BLASCHK(rocblas_dsyrk(c->blas, rocblas_fill_upper, rocblas_operation_none, m, np,
-                                  &one, A, m, &zero, Cs, m));
-            HIPCHK(imago::launch_xc_scatter(Cs, rb, m, P, C, c->stream));
+            mm = std::max(mm, hcnt[b]);
+            nn = std::max(nn, c->xc_npts[s0 + b]);
         }
+        if (mm == 0 || nn == 0) continue;
+        const size_t a_need = (size_t)nb * mm * nn, c_need = (size_t)nb * mm * mm;
+        if (a_need > a_cap) {
+            if (A) dev_release(c, A);
+            RCHK(dev_alloc(c, &A, a_need));
+            a_cap = a_need;
+        }
+        if (c_need > c_cap) {
+            if (Cs) dev_release(c, Cs);
+            RCHK(dev_alloc(c, &Cs, c_need));
+            c_cap = c_need;
+        }
+        HIPCHK(hipMemsetAsync(A, 0, a_need * 8, c->stream));
+        for (int b = 0; b < nb; ++b) {
+            const int q0 = c->xc_off[s0 + b] - p0;
+            HIPCHK(imago::launch_xc_gather(op + (size_t)q0 * P, c->xc_w + p0 + q0, P,
+                                           rows + (size_t)b * P, hcnt[b], c->xc_npts[s0 + b], mm,
+                                           A + (size_t)b * mm * nn, c->stream));
+        }
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(rocblas_dgemm_strided_batched(
+            c->blas, rocblas_operation_none, rocblas_operation_transpose, mm, mm, nn, &one, A, mm,
+            (rocblas_stride)mm * nn, A, mm, (rocblas_stride)mm * nn, &zero, Cs, mm,
+            (rocblas_stride)mm * mm, nb));
+        for (int b = 0; b < nb; ++b)
+            HIPCHK(imago::launch_xc_scatter(Cs + (size_t)b * mm * mm, mm, rows + (size_t)b * P,
+                                            hcnt[b], P, C, c->stream));
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipMemcpyAsync(exch_corr_overlap, C, (size_t)P * P * 8, hipMemcpyDeviceToHost,
@@ -2474,7 +2498,7 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
     c->timing[14] = ms;
     for (void *q : {(void *)C, (void *)op, (void *)A, (void *)Cs, (void *)nz, (void *)rows,
                     (void *)cnt, (void *)slot})
-        dev_release(c, q);
+        if (q) dev_release(c, q);
     return IMAGO_OK;
 }
 

@@ -3,7 +3,7 @@
 // operator of every ray point of makeECMeshAndOverlap (:345-875), plus the
 // pieces of its overlap sum exchCorrOverlap = sum_p w_p op_p op_p^T that
 // this library does on the device (row compaction, gather, scatter-add;
-// the rank-k updates are rocBLAS dsyrk).
+// the rank-k products are a rocBLAS strided-batched dgemm).
 #include <hip/hip_runtime.h>
 
 #include "geom_kernel.hpp"
@@ -170,15 +170,15 @@ __global__ __launch_bounds__(1024) void k_xc_compact(const unsigned char *__rest
 __global__ __launch_bounds__(256) void k_xc_gather(const double *__restrict__ op,
                                                    const double *__restrict__ w, int pot_dim,
                                                    const int *__restrict__ rows, int m, int npts,
-                                                   double *__restrict__ A)
+                                                   int lda, double *__restrict__ A)
 {
     const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (g >= (size_t)m * npts) return;
     const int c = (int)(g % m), p = (int)(g / m);
-    A[g] = op[(size_t)p * pot_dim + rows[c]] * sqrt(w[p]);
+    A[(size_t)p * lda + c] = op[(size_t)p * pot_dim + rows[c]] * sqrt(w[p]);
 }
 
-__global__ __launch_bounds__(256) void k_xc_scatter(const double *__restrict__ Cs,
+__global__ __launch_bounds__(256) void k_xc_scatter(const double *__restrict__ Cs, int ldc,
                                                     const int *__restrict__ rows, int m,
                                                     int pot_dim, double *__restrict__ C)
 {
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_xc_scatter(const double *__restrict__ C
     const int c1 = (int)(g % m), c2 = (int)(g / m);
     if (c1 > c2) return;
     double *at = C + (size_t)rows[c2] * pot_dim + rows[c1];
-    *at = *at + Cs[g];
+    *at = *at + Cs[(size_t)c2 * ldc + c1];
 }
 
 inline unsigned xc_blocks(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -230,23 +230,23 @@ hipError_t launch_xc_compact(const unsigned char *nz, int nslots, int pot_dim, i
 }
 
 hipError_t launch_xc_gather(const double *op, const double *w, int pot_dim, const int *rows,
-                            int m, int npts, double *A, hipStream_t stream)
+                            int m, int npts, int lda, double *A, hipStream_t stream)
 {
     const size_t n = (size_t)m * npts;
     if (n == 0) return hipSuccess;
-    if (!xc_grid_ok(n)) return hipErrorInvalidValue;
+    if (!xc_grid_ok(n) || lda < m) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_xc_gather, dim3(xc_blocks(n)), dim3(256), 0, stream, op, w, pot_dim, rows,
-                       m, npts, A);
+                       m, npts, lda, A);
     return hipGetLastError();
 }
 
-hipError_t launch_xc_scatter(const double *Cs, const int *rows, int m, int pot_dim, double *C,
-                             hipStream_t stream)
+hipError_t launch_xc_scatter(const double *Cs, int ldc, const int *rows, int m, int pot_dim,
+                             double *C, hipStream_t stream)
 {
     const size_t n = (size_t)m * m;
     if (n == 0) return hipSuccess;
-    if (!xc_grid_ok(n)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_xc_scatter, dim3(xc_blocks(n)), dim3(256), 0, stream, Cs, rows, m,
+    if (!xc_grid_ok(n) || ldc < m) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_xc_scatter, dim3(xc_blocks(n)), dim3(256), 0, stream, Cs, ldc, rows, m,
                        pot_dim, C);
     return hipGetLastError();
 }

@@ -42,14 +42,14 @@ hipError_t launch_xc_op(const XcArgs &X, bool gga, const double *pts, int npts, 
 hipError_t launch_xc_compact(const unsigned char *nz, int nslots, int pot_dim, int *rows,
                              int *count, hipStream_t stream);
 
-// A[p][c] = op[p][rows[c]] * sqrt(w[p]) for p < npts, c < m (A is m x npts
-// column-major, op has leading dimension pot_dim).
+// A(c, p) = op[p][rows[c]] * sqrt(w[p]) for p < npts, c < m (A column-major
+// with leading dimension lda >= m, op with leading dimension pot_dim).
 hipError_t launch_xc_gather(const double *op, const double *w, int pot_dim, const int *rows,
-                            int m, int npts, double *A, hipStream_t stream);
+                            int m, int npts, int lda, double *A, hipStream_t stream);
 
-// C[rows[c1]][rows[c2]] += Cs(c1, c2) for c1 <= c2 (C pot_dim x pot_dim and
-// Cs m x m, column-major).
-hipError_t launch_xc_scatter(const double *Cs, const int *rows, int m, int pot_dim, double *C,
-                             hipStream_t stream);
+// C(rows[c1], rows[c2]) += Cs(c1, c2) for c1 <= c2 < m (C pot_dim x pot_dim,
+// Cs with leading dimension ldc; both column-major).
+hipError_t launch_xc_scatter(const double *Cs, int ldc, const int *rows, int m, int pot_dim,
+                             double *C, hipStream_t stream);
 
 }  // namespace imago

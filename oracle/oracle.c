@@ -1577,11 +1577,20 @@ int orc_xc_overlap(const imago_system_desc *s, const imago_xc_mesh_params *p, do
                    const double *cov, const double *vec, const double *pls,
                    const double *maxrad, int maxpts, double *ovl)
 {
+    return orc_xc_overlap_sites(s, p, th, cov, vec, pls, maxrad, maxpts, 0, s->num_pot_sites,
+                                ovl);
+}
+
+/* The same sum over the sites [i_begin, i_end) only. */
+int orc_xc_overlap_sites(const imago_system_desc *s, const imago_xc_mesh_params *p, double th,
+                         const double *cov, const double *vec, const double *pls,
+                         const double *maxrad, int maxpts, int i_begin, int i_end, double *ovl)
+{
     const int P = es_pot_dim(s);
     memset(ovl, 0, sizeof(double) * (size_t)P * P);
     double *rw = malloc(sizeof(double) * maxpts);
     double *op = malloc(sizeof(double) * (size_t)maxpts * P);
-    for (int i = 0; i < s->num_pot_sites; ++i) {
+    for (int i = i_begin; i < i_end; ++i) {
         const int np = orc_xc_site(s, p, th, cov, vec, pls, maxrad, maxpts, i, 0, rw, op);
         for (int j = 0; j < np; ++j) {
             const double *o = op + (size_t)j * P;

@@ -119,6 +119,11 @@ int orc_xc_overlap(const imago_system_desc *sys, const imago_xc_mesh_params *p,
                    double th, const double *cov, const double *vec,
                    const double *pls, const double *maxrad, int maxpts,
                    double *ovl);
+int orc_xc_overlap_sites(const imago_system_desc *sys,
+                         const imago_xc_mesh_params *p, double th,
+                         const double *cov, const double *vec,
+                         const double *pls, const double *maxrad, int maxpts,
+                         int i_begin, int i_end, double *ovl);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,9 @@ def lib():
         L.orc_xc_site.restype = ctypes.c_int
         L.orc_xc_overlap.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp, _dp,
                                      _dp, _dp, ctypes.c_int, _dp]
+        L.orc_xc_overlap_sites.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp,
+                                           _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -252,12 +255,15 @@ class XcOracleMesh:
                               1 if gga else 0, _p(rw), _p(op))
         return n, rw, op
 
-    def overlap(self):
+    def overlap(self, sites=None):
+        """exchCorrOverlap; sites = (begin, end) sums those sites only."""
         P = self.system.pot_dim
         out = np.zeros((P, P), order="F")
-        lib().orc_xc_overlap(ctypes.byref(self.desc), ctypes.byref(self.params),
-                             float(self.system.log_elec_thresh), _p(self.cov), _p(self.vec),
-                             _p(self.pls), _p(self.maxrad), max(self.maxpts, 1), _p(out))
+        i0, i1 = (0, len(self.system.pot_site_pos)) if sites is None else sites
+        lib().orc_xc_overlap_sites(ctypes.byref(self.desc), ctypes.byref(self.params),
+                                   float(self.system.log_elec_thresh), _p(self.cov),
+                                   _p(self.vec), _p(self.pls), _p(self.maxrad),
+                                   max(self.maxpts, 1), int(i0), int(i1), _p(out))
         return out
 
 
