@@ -354,7 +354,7 @@ def bench_xcmesh(args, system, ctx, log):
     for i in range(args.warmup):
         ctx.xc_mesh()
         ctx.xc_overlap()
-    walls_mesh, ms_ov, walls_ov = [], [], []
+    walls_mesh, ms_ov, walls_ov, ms_gemm = [], [], [], []
     for i in range(args.steps):
         t0 = time.perf_counter()
         mx = ctx.xc_mesh()
@@ -363,10 +363,21 @@ def bench_xcmesh(args, system, ctx, log):
         ctx.xc_overlap()
         walls_ov.append(time.perf_counter() - t0)
         ms_ov.append(ctx.timing(14))
+        ms_gemm.append(ctx.timing(15))
         log("step %d: mesh %.1f ms (wall), overlap %.3f ms (device), %.1f ms with the copy"
             % (i, walls_mesh[-1] * 1e3, ms_ov[-1], walls_ov[-1] * 1e3))
     mesh_ms = float(np.mean(walls_mesh)) * 1e3
     ov_ms = float(np.mean(ms_ov))
+    gemm_ms = float(np.mean(ms_gemm))
+    npoints, gflop = ctx.xc_counts()
+    # the overlap's products B_b^T B_b, one strided-batched rocBLAS dgemm per
+    # batch of sites: its flops as executed (full squares, zero padding to
+    # the batch's largest row and point counts) over its HIP-event time
+    achieved = gflop / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    roofline = {"kernel": "rocblas_dgemm_strided_batched (B_b^T B_b)", "bound": "mfma",
+                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None,
+                "flop_per_launch": gflop}
     cpu = None
     if not args.no_cpu:
         log("cpu baseline leg (oracle)")
@@ -391,8 +402,10 @@ def bench_xcmesh(args, system, ctx, log):
                                % (system.name, len(system.pot_site_pos), P, mx),
                    "parallelism": "single GPU"},
         "stages_ms": {"xc_mesh_wall": round(mesh_ms, 3), "xc_overlap": round(ov_ms, 3),
+                      "xc_overlap_gemm": round(gemm_ms, 3),
                       "xc_overlap_wall_with_copy": round(float(np.mean(walls_ov)) * 1e3, 3)},
-        "roofline": None,
+        "ray_points": npoints,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -18,8 +18,9 @@ namespace imago {
 // findLatticeVector (lattice.f90:784-890): the lattice point nearest to v.
 // Fractional coordinates are rounded half away from zero, then the closest
 // of the cells (sorted by squared length, cell 0 the origin) to the
-// remainder r is taken.  No cell longer than 2|r| can be closer to r than
-// the origin, so the scan stops there (and after at most 125 cells).
+// remainder r is taken, the first of equals.  A cell R with |R| > |r| +
+// sqrt(best) is farther from r than the best so far, and so are all the
+// cells after it, so the scan stops there (and after at most 125 cells).
 IMAGO_HD void find_lattice_vector(const double *rv, const double *inv, const double *cell_dims,
                                   const double *cell_sizes, int ncells, const double *v,
                                   double *L)
@@ -34,7 +35,7 @@ IMAGO_HD void find_lattice_vector(const double *rv, const double *inv, const dou
     }
     const double rem[3] = {v[0] - L0[0], v[1] - L0[1], v[2] - L0[2]};
     const double rr = sqrt(rem[0] * rem[0] + rem[1] * rem[1] + rem[2] * rem[2]);
-    const double stop = (2.0 * rr + 1e-9) * (2.0 * rr + 1e-9);
+    double stop = (2.0 * rr + 1e-9) * (2.0 * rr + 1e-9);
     double best = 1e300;
     int bi = 0;
     const int lim = ncells < 125 ? ncells : 125;
@@ -46,6 +47,8 @@ IMAGO_HD void find_lattice_vector(const double *rv, const double *inv, const dou
         if (d < best) {
             best = d;
             bi = k;
+            const double s = rr + sqrt(best) + 1e-9;
+            if (s * s < stop) stop = s * s;
         }
     }
     for (int x = 0; x < 3; ++x) L[x] = L0[x] + cell_dims[3 * bi + x];

@@ -193,7 +193,7 @@ struct imago_ctx {
     double *ival = nullptr;       // [items] trace value of each item
     double *foldX = nullptr;      // [images][64 * ne] density folded per image
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
-    double timing[15] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double timing[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // second stream and handle: the two halves of the batched tridiagonal
     // divide and conquer run side by side (created on first use)
@@ -211,6 +211,8 @@ struct imago_ctx {
     double *xc_pts = nullptr, *xc_w = nullptr;   // [points][3], [points]
     double *xc_pls = nullptr, *xc_vec = nullptr, *xc_sitecov = nullptr;
     double *xc_sizes = nullptr;                  // real-space cell sizes
+    double *xc_dmax = nullptr;                   // [site] longest ray
+    double xc_gemm_flops = 0.0;                  // of the last imago_xc_overlap
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -261,6 +263,7 @@ static void ctx_free_system(imago_ctx *c)
     c->xc_maxpts = 0;
     c->xc_total = 0;
     c->xc_pts = c->xc_w = c->xc_pls = c->xc_vec = c->xc_sitecov = c->xc_sizes = nullptr;
+    c->xc_dmax = nullptr;
     c->has_system = false;
 }
 
@@ -2281,9 +2284,10 @@ extern "C" int imago_xc_mesh(imago_ctx *c, const imago_xc_mesh_params *params,
     HostSys &h = c->h;
     if (h.maxpa > 64) return fail(IMAGO_E_ARG, "more than 64 alphas in a potential type");
     for (void *q : {(void *)c->xc_pts, (void *)c->xc_w, (void *)c->xc_pls, (void *)c->xc_vec,
-                    (void *)c->xc_sitecov, (void *)c->xc_sizes})
+                    (void *)c->xc_sitecov, (void *)c->xc_sizes, (void *)c->xc_dmax})
         if (q) dev_release(c, q);
     c->xc_pts = c->xc_w = c->xc_pls = c->xc_vec = c->xc_sitecov = c->xc_sizes = nullptr;
+    c->xc_dmax = nullptr;
     c->xc_ready = false;
     c->xcp = p;
     c->xc_th = log_elec_thresh;
@@ -2365,6 +2369,14 @@ extern "C" int imago_xc_mesh(imago_ctx *c, const imago_xc_mesh_params *params,
     c->xc_off[ns] = (int)w.size();
     c->xc_total = (long)w.size();
     c->xc_maxpts = most;
+    {
+        std::vector<double> dm(ns, 0.0);
+        for (int i = 0; i < ns; ++i)
+            for (int j = 0; j < nv; ++j) dm[i] = std::max(dm[i], maxrad[(size_t)i * nv + j]);
+        const double *q = nullptr;
+        RCHK(dev_upload(c, &q, dm));
+        c->xc_dmax = const_cast<double *>(q);
+    }
     if (w.empty()) {
         pts.assign(3, 0.0);
         w.assign(1, 0.0);
@@ -2397,7 +2409,7 @@ extern "C" int imago_xc_rho_op(imago_ctx *c, int site, int gga, int *num_ray_poi
     HIPCHK(hipMemsetAsync(op, 0, nout * 8, c->stream));
     const imago::XcArgs X = xc_args(c);
     HIPCHK(imago::launch_xc_op(X, gga != 0, c->xc_pts + 3 * (size_t)c->xc_off[site], np, mp, op,
-                               nullptr, nullptr, c->stream));
+                               c->stream));
     std::fill(radial_weight, radial_weight + mp, 0.0);
     if (np > 0)
         HIPCHK(hipMemcpyAsync(radial_weight, c->xc_w + c->xc_off[site], (size_t)np * 8,
@@ -2418,46 +2430,54 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
     const int P = c->h.pot_dim, ns = c->h.nsites;
     const int mp = std::max(c->xc_maxpts, 1);
     const imago::XcArgs X = xc_args(c);
-    // Sites in batches of about 4 GB of operator.  Per site the rows that
-    // any of its points touches are gathered (scaled by sqrt(w)) into a
-    // column block of A; one strided-batched GEMM forms A_b A_b^T for every
-    // site of the batch (zero-padded to the batch's largest row count and
-    // point count), and the upper triangles are added into C site by site.
-    const size_t per_site = (size_t)mp * P;
-    const int batch = (int)std::max<size_t>(1, std::min<size_t>(ns, (1ull << 29) / per_site));
-    double *C = nullptr, *op = nullptr;
+    // Sites in batches.  For each site, the terms its points can touch
+    // (a superset from the geometry) are its rows; the operator of its
+    // points goes straight into a block B_b (points x rows, scaled by
+    // sqrt(w)), one strided-batched GEMM forms B_b^T B_b for every site of
+    // the batch (zero-padded to the batch's largest row and point counts),
+    // and the upper triangles are added into C site by site in order.
+    const int batch = std::max(1, std::min(ns, 64));
+    double *C = nullptr;
     unsigned char *nz = nullptr;
-    int *rows = nullptr, *cnt = nullptr, *slot = nullptr;
+    int *rows = nullptr, *cnt = nullptr, *slot = nullptr, *first = nullptr, *rank = nullptr;
+    int *err = nullptr;
     RCHK(dev_alloc(c, &C, (size_t)P * P));
-    RCHK(dev_alloc(c, &op, (size_t)batch * per_site));
     RCHK(dev_alloc(c, &nz, (size_t)batch * P));
     RCHK(dev_alloc(c, &rows, (size_t)batch * P));
+    RCHK(dev_alloc(c, &rank, (size_t)batch * P));
     RCHK(dev_alloc(c, &cnt, (size_t)batch));
     RCHK(dev_alloc(c, &slot, (size_t)batch * mp));
+    RCHK(dev_alloc(c, &first, (size_t)batch));
+    RCHK(dev_alloc(c, &err, 1));
     double *A = nullptr, *Cs = nullptr;
     size_t a_cap = 0, c_cap = 0;
     HIPCHK(hipMemsetAsync(C, 0, (size_t)P * P * 8, c->stream));
-    std::vector<int> hslot((size_t)batch * mp), hcnt(batch);
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    std::vector<int> hslot((size_t)batch * mp), hcnt(batch), hfirst(batch);
+    std::vector<hipEvent_t> gev;   // pairs around the batched GEMMs
+    double gflop = 0.0;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int s0 = 0; s0 < ns; s0 += batch) {
         const int nb = std::min(batch, ns - s0);
         const int p0 = c->xc_off[s0], p1 = c->xc_off[s0 + nb];
         const int npb = p1 - p0;
         if (npb == 0) continue;
-        for (int b = 0; b < nb; ++b)
+        for (int b = 0; b < nb; ++b) {
+            hfirst[b] = c->xc_off[s0 + b] - p0;
             for (int q = c->xc_off[s0 + b]; q < c->xc_off[s0 + b + 1]; ++q) hslot[q - p0] = b;
+        }
         HIPCHK(hipMemcpyAsync(slot, hslot.data(), (size_t)npb * 4, hipMemcpyHostToDevice,
                               c->stream));
-        HIPCHK(hipMemsetAsync(op, 0, (size_t)npb * P * 8, c->stream));
-        HIPCHK(hipMemsetAsync(nz, 0, (size_t)nb * P, c->stream));
-        HIPCHK(imago::launch_xc_op(X, false, c->xc_pts + 3 * (size_t)p0, npb, npb, op, nz, slot,
-                                   c->stream));
+        HIPCHK(hipMemcpyAsync(first, hfirst.data(), (size_t)nb * 4, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(imago::launch_xc_rows(X, s0, nb, c->xc_dmax, nz, c->stream));
         HIPCHK(imago::launch_xc_compact(nz, nb, P, rows, cnt, c->stream));
         HIPCHK(hipMemcpyAsync(hcnt.data(), cnt, (size_t)nb * 4, hipMemcpyDeviceToHost,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         int mm = 0, nn = 0;
         for (int b = 0; b < nb; ++b) {
+            if (c->xc_npts[s0 + b] == 0) continue;
             mm = std::max(mm, hcnt[b]);
             nn = std::max(nn, c->xc_npts[s0 + b]);
         }
@@ -2473,38 +2493,69 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
             RCHK(dev_alloc(c, &Cs, c_need));
             c_cap = c_need;
         }
+        HIPCHK(hipMemsetAsync(rank, 0xff, (size_t)nb * P * 4, c->stream));
+        HIPCHK(imago::launch_xc_rank(rows, cnt, nb, P, rank, c->stream));
         HIPCHK(hipMemsetAsync(A, 0, a_need * 8, c->stream));
-        for (int b = 0; b < nb; ++b) {
-            const int q0 = c->xc_off[s0 + b] - p0;
-            HIPCHK(imago::launch_xc_gather(op + (size_t)q0 * P, c->xc_w + p0 + q0, P,
-                                           rows + (size_t)b * P, hcnt[b], c->xc_npts[s0 + b], mm,
-                                           A + (size_t)b * mm * nn, c->stream));
-        }
+        HIPCHK(imago::launch_xc_op_rows(X, c->xc_pts + 3 * (size_t)p0, c->xc_w + p0, npb, slot,
+                                        first, rank, mm, nn, A, err, c->stream));
+        // A holds B_b = (sqrt(w) op)^T, points x rows, so C_b = B_b^T B_b
         const double one = 1.0, zero = 0.0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        gev.push_back(e0);
+        gev.push_back(e1);
+        HIPCHK(hipEventRecord(e0, c->stream));
         BLASCHK(rocblas_dgemm_strided_batched(
-            c->blas, rocblas_operation_none, rocblas_operation_transpose, mm, mm, nn, &one, A, mm,
-            (rocblas_stride)mm * nn, A, mm, (rocblas_stride)mm * nn, &zero, Cs, mm,
+            c->blas, rocblas_operation_transpose, rocblas_operation_none, mm, mm, nn, &one, A, nn,
+            (rocblas_stride)mm * nn, A, nn, (rocblas_stride)mm * nn, &zero, Cs, mm,
             (rocblas_stride)mm * mm, nb));
+        HIPCHK(hipEventRecord(e1, c->stream));
+        gflop += 2.0 * mm * (double)mm * nn * nb;
         for (int b = 0; b < nb; ++b)
-            HIPCHK(imago::launch_xc_scatter(Cs + (size_t)b * mm * mm, mm, rows + (size_t)b * P,
-                                            hcnt[b], P, C, c->stream));
+            if (c->xc_npts[s0 + b] > 0)
+                HIPCHK(imago::launch_xc_scatter(Cs + (size_t)b * mm * mm, mm, rows + (size_t)b * P,
+                                                hcnt[b], P, C, c->stream));
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(exch_corr_overlap, C, (size_t)P * P * 8, hipMemcpyDeviceToHost,
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[14] = ms;
-    for (void *q : {(void *)C, (void *)op, (void *)A, (void *)Cs, (void *)nz, (void *)rows,
-                    (void *)cnt, (void *)slot})
+    double gms = 0.0;
+    for (size_t q = 0; q + 1 < gev.size(); q += 2) {
+        float x = 0.f;
+        HIPCHK(hipEventElapsedTime(&x, gev[q], gev[q + 1]));
+        gms += x;
+    }
+    for (hipEvent_t e : gev) (void)hipEventDestroy(e);
+    c->timing[15] = gms;
+    c->xc_gemm_flops = gflop;
+    for (void *q : {(void *)C, (void *)A, (void *)Cs, (void *)nz, (void *)rows, (void *)rank,
+                    (void *)cnt, (void *)slot, (void *)first, (void *)err})
         if (q) dev_release(c, q);
+    if (herr)
+        return fail(IMAGO_E_HIP, "exchange-correlation overlap: a point touched a term outside "
+                                 "its site's row bound (internal error)");
+    return IMAGO_OK;
+}
+
+extern "C" int imago_xc_counts(imago_ctx *c, long *ray_points, double *gemm_flops)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
+    if (ray_points) *ray_points = c->xc_total;
+    if (gemm_flops) *gemm_flops = c->xc_gemm_flops;
     return IMAGO_OK;
 }
 
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 14) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 15) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

@@ -2,8 +2,9 @@
 // (exchCorr.f90): the ray lengths of getECMeshParameters (:63-342) and the
 // operator of every ray point of makeECMeshAndOverlap (:345-875), plus the
 // pieces of its overlap sum exchCorrOverlap = sum_p w_p op_p op_p^T that
-// this library does on the device (row compaction, gather, scatter-add;
-// the rank-k products are a rocBLAS strided-batched dgemm).
+// this library does on the device (the rows each site's points can touch,
+// the operator written straight into the product operands, scatter-add;
+// the products themselves are a rocBLAS strided-batched dgemm).
 #include <hip/hip_runtime.h>
 
 #include "geom_kernel.hpp"
@@ -23,6 +24,9 @@ constexpr double kXcSmall = 1.0e-8;   // smallThresh (constants.f90:49)
 __global__ __launch_bounds__(256) void k_xc_maxrad(XcArgs X, double *__restrict__ maxrad)
 {
 #pragma clang fp contract(off)
+    __shared__ double lreps[3 * 64];
+    for (int e = threadIdx.x; e < 3 * X.nreps; e += blockDim.x) lreps[e] = X.cell_dims[e];
+    __syncthreads();
     const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (g >= (size_t)X.nsites * X.nvec) return;
     const int i = (int)(g / X.nvec), j = (int)(g % X.nvec);
@@ -42,7 +46,7 @@ __global__ __launch_bounds__(256) void k_xc_maxrad(XcArgs X, double *__restrict_
         const double f = ci / (ci + ck);
         const double *pk = X.pls + 3 * k;
         for (int l = 0; l < X.nreps; ++l) {
-            const double *R = X.cell_dims + 3 * l;
+            const double *R = lreps + 3 * l;
             const double s0 = f * ((pk[0] + R[0]) - x10), s1 = f * ((pk[1] + R[1]) - x11),
                          s2 = f * ((pk[2] + R[2]) - x12);
             const double dot = s0 * v0 + s1 * v1 + s2 * v2;
@@ -54,56 +58,232 @@ __global__ __launch_bounds__(256) void k_xc_maxrad(XcArgs X, double *__restrict_
     maxrad[g] = cmax;
 }
 
+struct XcCells {
+    const double *dims, *sizes;
+};
+
+// Site l's offset from a point: (p - r_l) minus findLatticeVector of it,
+// its squared length, and the cell-list bound (sqrt(negl) + |offset|)^2
+// (exchCorr.f90:603-612).
+__device__ __forceinline__ void xc_offset(const XcArgs &X, const XcCells &C, double pt0,
+                                          double pt1, double pt2, int l, double negl,
+                                          double off[3], double *negl_plus)
+{
+#pragma clang fp contract(off)
+    const double *pl = X.site_pos + 3 * l;
+    const double sv[3] = {pt0 - pl[0], pt1 - pl[1], pt2 - pl[2]};
+    double L[3];
+    find_lattice_vector(X.rv, X.inv, C.dims, C.sizes, X.ncells, sv, L);
+    for (int x = 0; x < 3; ++x) off[x] = sv[x] - L[x];
+    const double osq = off[0] * off[0] + off[1] * off[1] + off[2] * off[2];
+    *negl_plus = negl + osq + 2.0 * sqrt(negl * osq);
+}
+
 // One thread per (ray point p, potential type t): the sites of type t in
 // order, their cells within (sqrt(th / alpha_1) + |offset|)^2 and within
 // th / alpha_1 of the point, the type's alphas up to the first with
 // alpha r^2 > th (exchCorr.f90:587-760).  Each thread owns its terms of
-// its point, so the sums keep the reference's order.  GGA: the
-// reference's recurrences op_o = alpha op_o + d_o e (o = x, y, z) and
-// alpha^2 op_o + d_o e (o = xx .. zz), as written.
-template <bool GGA>
+// its point and sums them in registers (NA >= the type's alphas) in the
+// reference's order.
+template <int NA>
 __global__ __launch_bounds__(256) void k_xc_op(XcArgs X, const double *__restrict__ pts,
-                                               int npts, int ld, double *__restrict__ op,
-                                               unsigned char *__restrict__ nz,
-                                               const int *__restrict__ pt_slot)
+                                               int npts, double *__restrict__ op)
 {
+    const XcCells C{X.cell_dims, X.cell_sizes};
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (size_t)npts * X.npt) return;
+    const int t = (int)(g % X.npt), p = (int)(g / X.npt);
+    const int na = X.type_nalpha[t];
+    const double *a = X.type_alphas + (size_t)t * X.maxpa;
+    double al[NA], acc[NA];
+#pragma unroll
+    for (int n = 0; n < NA; ++n) {
+        al[n] = n < na ? a[n] : 0.0;
+        acc[n] = 0.0;
+    }
+    const double negl = X.th / al[0];
+    const double pt0 = pts[3 * p], pt1 = pts[3 * p + 1], pt2 = pts[3 * p + 2];
+    const int l0 = X.type_first[t], l1 = l0 + X.type_mult[t];
+    for (int l = l0; l < l1; ++l) {
+        double off[3], negl_plus;
+        xc_offset(X, C, pt0, pt1, pt2, l, negl, off, &negl_plus);
+        for (int m = 0; m < X.ncells; ++m) {
+            if (C.sizes[m] > negl_plus) break;
+            double rad;
+            {
+#pragma clang fp contract(off)
+                const double *R = C.dims + 3 * m;
+                const double dx = off[0] - R[0], dy = off[1] - R[1], dz = off[2] - R[2];
+                rad = dx * dx + dy * dy + dz * dz;
+            }
+            if (rad > negl) continue;
+            bool go = true;
+#pragma unroll
+            for (int n = 0; n < NA; ++n) {
+                double am;
+                {
+#pragma clang fp contract(off)
+                    am = al[n] * rad;
+                }
+                go = go && n < na && !(am > X.th);
+                if (go) acc[n] = acc[n] + exp(-am);
+            }
+        }
+    }
+    double *o = op + (size_t)p * X.pot_dim + X.type_term0[t];
+#pragma unroll
+    for (int n = 0; n < NA; ++n)
+        if (n < na) o[n] = acc[n];
+}
+
+// Rows (terms) that the points of site i can touch, a superset: every
+// point lies within D_i = max_j maxrad[i][j] of pls_i, so a term (t, n) is
+// 0 at all of them when alpha_n (d - D_i)^2 > th for the nearest image, at
+// distance d, of every site of type t.  One thread per (batch slot, type).
+__global__ __launch_bounds__(256) void k_xc_rows(XcArgs X, int s0, int nb,
+                                                 const double *__restrict__ dmax,
+                                                 unsigned char *__restrict__ nz)
+{
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (size_t)nb * X.npt) return;
+    const int t = (int)(g % X.npt), b = (int)(g / X.npt), i = s0 + b;
+    const double *pi = X.pls + 3 * i;
+    double dmin = 1e300;
+    const int l0 = X.type_first[t], l1 = l0 + X.type_mult[t];
+    for (int l = l0; l < l1; ++l) {
+        const double *pl = X.site_pos + 3 * l;
+        const double v[3] = {pi[0] - pl[0], pi[1] - pl[1], pi[2] - pl[2]};
+        double L[3];
+        find_lattice_vector(X.rv, X.inv, X.cell_dims, X.cell_sizes, X.ncells, v, L);
+        const double d0 = v[0] - L[0], d1 = v[1] - L[1], d2 = v[2] - L[2];
+        dmin = fmin(dmin, sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+    }
+    // a little slack for the rounding of the points and of the distances
+    const double dd = fmax(0.0, dmin - dmax[i] * (1.0 + 1e-12) - 1e-9);
+    const int na = X.type_nalpha[t];
+    const double *a = X.type_alphas + (size_t)t * X.maxpa;
+    unsigned char *fl = nz + (size_t)b * X.pot_dim + X.type_term0[t];
+    for (int n = 0; n < na; ++n) fl[n] = a[n] * dd * dd <= X.th ? 1 : 0;
+}
+
+// rank[b][rows[b][c]] = c for c < count[b] (rank preset to -1)
+__global__ __launch_bounds__(256) void k_xc_rank(const int *__restrict__ rows,
+                                                 const int *__restrict__ count, int pot_dim,
+                                                 int *__restrict__ rank)
+{
+    const int b = blockIdx.y;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= count[b]) return;
+    rank[(size_t)b * pot_dim + rows[(size_t)b * pot_dim + c]] = c;
+}
+
+// The LDA operator of a batch of sites' points written straight into the
+// GEMM operands: B_b(p - first point of b, rank(term)) = op * sqrt(w_p)
+// (B_b ldn x lda, column-major, one block per batch slot b).  Threads run
+// over the points fastest: neighbouring lanes hold neighbouring points of
+// one ray and walk the same cells.  Terms outside the slot's rows must be
+// 0; err is set if one is not.
+template <int NA>
+__global__ __launch_bounds__(256) void k_xc_op_rows(XcArgs X, const double *__restrict__ pts,
+                                                    const double *__restrict__ w, int npts,
+                                                    const int *__restrict__ pt_slot,
+                                                    const int *__restrict__ slot_first,
+                                                    const int *__restrict__ rank, int lda,
+                                                    int ldn, double *__restrict__ B,
+                                                    int *__restrict__ err)
+{
+    const XcCells C{X.cell_dims, X.cell_sizes};
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (size_t)npts * X.npt) return;
+    const int p = (int)(g % npts), t = (int)(g / npts);
+    const int na = X.type_nalpha[t];
+    const double *a = X.type_alphas + (size_t)t * X.maxpa;
+    double al[NA], acc[NA];
+#pragma unroll
+    for (int n = 0; n < NA; ++n) {
+        al[n] = n < na ? a[n] : 0.0;
+        acc[n] = 0.0;
+    }
+    const double negl = X.th / al[0];
+    const double pt0 = pts[3 * p], pt1 = pts[3 * p + 1], pt2 = pts[3 * p + 2];
+    const int l0 = X.type_first[t], l1 = l0 + X.type_mult[t];
+    for (int l = l0; l < l1; ++l) {
+        double off[3], negl_plus;
+        xc_offset(X, C, pt0, pt1, pt2, l, negl, off, &negl_plus);
+        for (int m = 0; m < X.ncells; ++m) {
+            if (C.sizes[m] > negl_plus) break;
+            double rad;
+            {
+#pragma clang fp contract(off)
+                const double *R = C.dims + 3 * m;
+                const double dx = off[0] - R[0], dy = off[1] - R[1], dz = off[2] - R[2];
+                rad = dx * dx + dy * dy + dz * dz;
+            }
+            if (rad > negl) continue;
+            bool go = true;
+#pragma unroll
+            for (int n = 0; n < NA; ++n) {
+                double am;
+                {
+#pragma clang fp contract(off)
+                    am = al[n] * rad;
+                }
+                go = go && n < na && !(am > X.th);
+                if (go) acc[n] = acc[n] + exp(-am);
+            }
+        }
+    }
+    const int b = pt_slot[p];
+    const double sw = sqrt(w[p]);
+    const int *rk = rank + (size_t)b * X.pot_dim + X.type_term0[t];
+    double *Bb = B + (size_t)b * lda * ldn + (p - slot_first[b]);
+    bool bad = false;
+#pragma unroll
+    for (int n = 0; n < NA; ++n)
+        if (n < na) {
+            const int c = rk[n];
+            if (c >= 0)
+                Bb[(size_t)c * ldn] = acc[n] * sw;
+            else if (acc[n] != 0.0)
+                bad = true;
+        }
+    if (bad) *err = 1;
+}
+
+// The GGA operator of one site's points: as k_xc_op for the first value,
+// then the reference's recurrences op_o = alpha op_o + d_o e (o = x, y, z)
+// and alpha^2 op_o + d_o e (o = xx .. zz), as written, accumulated in
+// place (op zeroed by the caller; values o at stride ld * potDim).
+__global__ __launch_bounds__(256) void k_xc_op_gga(XcArgs X, const double *__restrict__ pts,
+                                                   int npts, int ld, double *__restrict__ op)
+{
+    const XcCells C{X.cell_dims, X.cell_sizes};
     const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (g >= (size_t)npts * X.npt) return;
     const int t = (int)(g % X.npt), p = (int)(g / X.npt);
     const int na = X.type_nalpha[t];
     const double *a = X.type_alphas + (size_t)t * X.maxpa;
     const double negl = X.th / a[0];
-    const int term0 = X.type_term0[t];
-    const long P = X.pot_dim;
     const double pt0 = pts[3 * p], pt1 = pts[3 * p + 1], pt2 = pts[3 * p + 2];
-    unsigned long long touched = 0;
-    double *o = op + (size_t)p * P + term0;
-    const size_t ostride = (size_t)ld * P;
+    double *o = op + (size_t)p * X.pot_dim + X.type_term0[t];
+    const size_t ostride = (size_t)ld * X.pot_dim;
     const int l0 = X.type_first[t], l1 = l0 + X.type_mult[t];
     for (int l = l0; l < l1; ++l) {
-        double off[3], osq, negl_plus;
-        {
-#pragma clang fp contract(off)
-            const double *pl = X.site_pos + 3 * l;
-            const double sv[3] = {pt0 - pl[0], pt1 - pl[1], pt2 - pl[2]};
-            double L[3];
-            find_lattice_vector(X.rv, X.inv, X.cell_dims, X.cell_sizes, X.ncells, sv, L);
-            for (int x = 0; x < 3; ++x) off[x] = sv[x] - L[x];
-            osq = off[0] * off[0] + off[1] * off[1] + off[2] * off[2];
-            negl_plus = negl + osq + 2.0 * sqrt(negl * osq);
-        }
+        double off[3], negl_plus;
+        xc_offset(X, C, pt0, pt1, pt2, l, negl, off, &negl_plus);
         for (int m = 0; m < X.ncells; ++m) {
-            if (X.cell_sizes[m] > negl_plus) break;
+            if (C.sizes[m] > negl_plus) break;
             double dx, dy, dz, rad;
             {
 #pragma clang fp contract(off)
-                const double *R = X.cell_dims + 3 * m;
+                const double *R = C.dims + 3 * m;
                 dx = off[0] - R[0];
                 dy = off[1] - R[1];
                 dz = off[2] - R[2];
                 rad = dx * dx + dy * dy + dz * dz;
             }
             if (rad > negl) continue;
+            const double dd[9] = {dx, dy, dz, dx * dx, dx * dy, dx * dz, dy * dy, dy * dz, dz * dz};
             for (int n = 0; n < na; ++n) {
                 double am;
                 {
@@ -113,26 +293,13 @@ __global__ __launch_bounds__(256) void k_xc_op(XcArgs X, const double *__restric
                 if (am > X.th) break;
                 const double e = exp(-am);
                 o[n] = o[n] + e;
-                touched |= 1ull << n;
-                if (GGA) {
-                    const double dd[9] = {dx, dy, dz, dx * dx, dx * dy, dx * dz,
-                                          dy * dy, dy * dz, dz * dz};
 #pragma unroll
-                    for (int q = 1; q < 10; ++q) {
-                        double *oq = o + q * ostride + n;
-                        const double f = q < 4 ? a[n] : a[n] * a[n];
-                        *oq = f * *oq + dd[q - 1] * e;
-                    }
+                for (int q = 1; q < 10; ++q) {
+                    double *oq = o + q * ostride + n;
+                    const double f = q < 4 ? a[n] : a[n] * a[n];
+                    *oq = f * *oq + dd[q - 1] * e;
                 }
             }
-        }
-    }
-    if (nz) {
-        unsigned char *fl = nz + (size_t)pt_slot[p] * P + term0;
-        while (touched) {
-            const int n = __ffsll((long long)touched) - 1;
-            touched &= touched - 1;
-            fl[n] = 1;
         }
     }
 }
@@ -167,17 +334,6 @@ __global__ __launch_bounds__(1024) void k_xc_compact(const unsigned char *__rest
     if (tid == 0) count[s] = base;
 }
 
-__global__ __launch_bounds__(256) void k_xc_gather(const double *__restrict__ op,
-                                                   const double *__restrict__ w, int pot_dim,
-                                                   const int *__restrict__ rows, int m, int npts,
-                                                   int lda, double *__restrict__ A)
-{
-    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (g >= (size_t)m * npts) return;
-    const int c = (int)(g % m), p = (int)(g / m);
-    A[(size_t)p * lda + c] = op[(size_t)p * pot_dim + rows[c]] * sqrt(w[p]);
-}
-
 __global__ __launch_bounds__(256) void k_xc_scatter(const double *__restrict__ Cs, int ldc,
                                                     const int *__restrict__ rows, int m,
                                                     int pot_dim, double *__restrict__ C)
@@ -206,17 +362,60 @@ hipError_t launch_xc_maxrad(const XcArgs &X, double *maxrad, hipStream_t stream)
 }
 
 hipError_t launch_xc_op(const XcArgs &X, bool gga, const double *pts, int npts, int ld,
-                        double *op, unsigned char *nz, const int *pt_slot, hipStream_t stream)
+                        double *op, hipStream_t stream)
 {
     const size_t n = (size_t)npts * X.npt;
     if (n == 0) return hipSuccess;
     if (!xc_grid_ok(n) || X.maxpa > 64 || ld < npts) return hipErrorInvalidValue;
-    if (gga)
-        hipLaunchKernelGGL(k_xc_op<true>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, npts,
-                           ld, op, nz, pt_slot);
+    if (gga) {
+        hipLaunchKernelGGL(k_xc_op_gga, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, npts, ld,
+                           op);
+        return hipGetLastError();
+    }
+    if (X.maxpa <= 16)
+        hipLaunchKernelGGL(k_xc_op<16>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, npts, op);
+    else if (X.maxpa <= 32)
+        hipLaunchKernelGGL(k_xc_op<32>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, npts, op);
     else
-        hipLaunchKernelGGL(k_xc_op<false>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, npts,
-                           ld, op, nz, pt_slot);
+        hipLaunchKernelGGL(k_xc_op<64>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, npts, op);
+    return hipGetLastError();
+}
+
+hipError_t launch_xc_rows(const XcArgs &X, int s0, int nb, const double *dmax,
+                          unsigned char *nz, hipStream_t stream)
+{
+    const size_t n = (size_t)nb * X.npt;
+    if (n == 0) return hipSuccess;
+    if (!xc_grid_ok(n)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_xc_rows, dim3(xc_blocks(n)), dim3(256), 0, stream, X, s0, nb, dmax, nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_xc_rank(const int *rows, const int *count, int nb, int pot_dim, int *rank,
+                          hipStream_t stream)
+{
+    if (nb <= 0 || nb > 65535) return nb <= 0 ? hipSuccess : hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_xc_rank, dim3((unsigned)((pot_dim + 255) / 256), (unsigned)nb), dim3(256),
+                       0, stream, rows, count, pot_dim, rank);
+    return hipGetLastError();
+}
+
+hipError_t launch_xc_op_rows(const XcArgs &X, const double *pts, const double *w, int npts,
+                             const int *pt_slot, const int *slot_first, const int *rank, int lda,
+                             int ldn, double *B, int *err, hipStream_t stream)
+{
+    const size_t n = (size_t)npts * X.npt;
+    if (n == 0) return hipSuccess;
+    if (!xc_grid_ok(n) || X.maxpa > 64) return hipErrorInvalidValue;
+    if (X.maxpa <= 16)
+        hipLaunchKernelGGL(k_xc_op_rows<16>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, w,
+                           npts, pt_slot, slot_first, rank, lda, ldn, B, err);
+    else if (X.maxpa <= 32)
+        hipLaunchKernelGGL(k_xc_op_rows<32>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, w,
+                           npts, pt_slot, slot_first, rank, lda, ldn, B, err);
+    else
+        hipLaunchKernelGGL(k_xc_op_rows<64>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, w,
+                           npts, pt_slot, slot_first, rank, lda, ldn, B, err);
     return hipGetLastError();
 }
 
@@ -226,17 +425,6 @@ hipError_t launch_xc_compact(const unsigned char *nz, int nslots, int pot_dim, i
     if (nslots <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_xc_compact, dim3((unsigned)nslots), dim3(1024), 0, stream, nz, pot_dim,
                        rows, count);
-    return hipGetLastError();
-}
-
-hipError_t launch_xc_gather(const double *op, const double *w, int pot_dim, const int *rows,
-                            int m, int npts, int lda, double *A, hipStream_t stream)
-{
-    const size_t n = (size_t)m * npts;
-    if (n == 0) return hipSuccess;
-    if (!xc_grid_ok(n) || lda < m) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_xc_gather, dim3(xc_blocks(n)), dim3(256), 0, stream, op, w, pot_dim, rows,
-                       m, npts, lda, A);
     return hipGetLastError();
 }
 

@@ -30,22 +30,35 @@ struct XcArgs {
 hipError_t launch_xc_maxrad(const XcArgs &X, double *maxrad, hipStream_t stream);
 
 // The operator of npts ray points (exchCorr.f90:587-760): op[o][p][term]
-// for o < (gga ? 10 : 1), p < npts, leading dimension ld points per
-// operator value; zeroed by the caller.  When nz is not null,
-// nz[pt_slot[p]][term] is set to 1 for every term a point adds to.
+// for o < (gga ? 10 : 1), p < npts, ld points per operator value.  LDA
+// writes every term of every point; GGA accumulates into op zeroed by the
+// caller.
 hipError_t launch_xc_op(const XcArgs &X, bool gga, const double *pts, int npts, int ld,
-                        double *op, unsigned char *nz, const int *pt_slot,
-                        hipStream_t stream);
+                        double *op, hipStream_t stream);
+
+// nz[b][term] = 1 for the terms the points of site s0 + b can touch (a
+// superset, from the nearest image of every site and the largest ray
+// length dmax[site]), b < nb.
+hipError_t launch_xc_rows(const XcArgs &X, int s0, int nb, const double *dmax,
+                          unsigned char *nz, hipStream_t stream);
+
+// rank[b][rows[b][c]] = c for c < count[b]; rank preset to -1 by the caller.
+hipError_t launch_xc_rank(const int *rows, const int *count, int nb, int pot_dim, int *rank,
+                          hipStream_t stream);
+
+// The LDA operator of npts points of a batch of sites written into the
+// product operands: B_b(p - slot_first[b], rank[b][term]) = op(term, p) *
+// sqrt(w[p]), b = pt_slot[p], B_b = B + b * lda * ldn (column-major, ldn
+// rows = points, lda columns = terms).  *err is set if a term outside a
+// slot's rows is not 0.
+hipError_t launch_xc_op_rows(const XcArgs &X, const double *pts, const double *w, int npts,
+                             const int *pt_slot, const int *slot_first, const int *rank, int lda,
+                             int ldn, double *B, int *err, hipStream_t stream);
 
 // Row lists of nslots flag rows: rows[s][0 .. count[s]) = the terms whose
 // flag nz[s][term] is set, ascending.
 hipError_t launch_xc_compact(const unsigned char *nz, int nslots, int pot_dim, int *rows,
                              int *count, hipStream_t stream);
-
-// A(c, p) = op[p][rows[c]] * sqrt(w[p]) for p < npts, c < m (A column-major
-// with leading dimension lda >= m, op with leading dimension pot_dim).
-hipError_t launch_xc_gather(const double *op, const double *w, int pot_dim, const int *rows,
-                            int m, int npts, int lda, double *A, hipStream_t stream);
 
 // C(rows[c1], rows[c2]) += Cs(c1, c2) for c1 <= c2 < m (C pot_dim x pot_dim,
 // Cs with leading dimension ldc; both column-major).

@@ -221,6 +221,14 @@ module O_ImagoHIP
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: exchCorrOverlap(*)
       end function imago_xc_overlap
+
+      integer(c_int) function imago_xc_counts(ctx, rayPoints, gemmFlops) &
+            & bind(C, name="imago_xc_counts")
+         import :: c_int, c_ptr, c_double, c_long
+         type(c_ptr), value :: ctx
+         integer(c_long), intent(out) :: rayPoints
+         real(c_double), intent(out) :: gemmFlops
+      end function imago_xc_counts
    end interface
 
 contains

@@ -39,7 +39,7 @@ EXPORTS = [
     "imago_primitive_batch", "imago_boys_batch", "imago_assemble", "imago_get_packed",
     "imago_secular", "imago_get_eigenvectors", "imago_density_traces", "imago_zhegv",
     "imago_dsygv", "imago_last_timing", "imago_ep_counts", "imago_elecstat", "imago_residual_q",
-    "imago_xc_mesh", "imago_xc_rho_op", "imago_xc_overlap",
+    "imago_xc_mesh", "imago_xc_rho_op", "imago_xc_overlap", "imago_xc_counts",
 ]
 
 
@@ -87,6 +87,7 @@ def load():
     L.imago_xc_mesh.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp, _ip]
     L.imago_xc_rho_op.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
     L.imago_xc_overlap.argtypes = [ctypes.c_void_p, _dp]
+    L.imago_xc_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), _dp]
     _lib = L
     return L
 
@@ -284,6 +285,12 @@ class Context:
         out = np.zeros((self.pot_dim, self.pot_dim), order="F")
         _check(load().imago_xc_overlap(self._h, _p(out)))
         return out
+
+    def xc_counts(self):
+        """(all ray points, flops of the last overlap's GEMMs)."""
+        n, f = ctypes.c_long(), ctypes.c_double()
+        _check(load().imago_xc_counts(self._h, ctypes.byref(n), ctypes.byref(f)))
+        return n.value, f.value
 
     # -- single problems ----------------------------------------------------
     def hegv(self, H, S):

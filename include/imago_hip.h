@@ -217,7 +217,7 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * kernels of the last imago_elecstat, 12 = those of the last
  * imago_residual_q (both without the copies to the host), 13 = the
  * reciprocal-space sum kernel of the last imago_residual_q, 14 = the
- * device work of the last imago_xc_overlap. */
+ * device work of the last imago_xc_overlap, 15 = its GEMMs. */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
@@ -283,6 +283,12 @@ int imago_xc_rho_op(imago_ctx *ctx, int site, int gga, int *num_ray_points,
  * LDA operator, upper triangle (row <= column) filled as the reference
  * fills it, lower triangle 0. */
 int imago_xc_overlap(imago_ctx *ctx, double *exch_corr_overlap);
+
+/* Sizes of the exchange-correlation mesh (after imago_xc_mesh): all ray
+ * points of all sites, and the flops of the GEMMs of the last
+ * imago_xc_overlap (0 before one).  Either output may be NULL.  For
+ * measurement. */
+int imago_xc_counts(imago_ctx *ctx, long *ray_points, double *gemm_flops);
 
 #ifdef __cplusplus
 }
