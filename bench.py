@@ -354,7 +354,10 @@ def bench_xcmesh(args, system, ctx, log):
     for i in range(args.warmup):
         ctx.xc_mesh()
         ctx.xc_overlap()
-    walls_mesh, ms_ov, walls_ov, ms_gemm = [], [], [], []
+    walls_mesh, ms_ov, walls_ov, ms_gemm, ms_pot = [], [], [], [], []
+    # charge coefficients for the potential terms: a neutral-atom-like
+    # profile on every potential type
+    rho_fit = np.concatenate([0.05 * np.exp(-0.3 * np.log(p.alphas)) for p in system.pot_types])
     for i in range(args.steps):
         t0 = time.perf_counter()
         mx = ctx.xc_mesh()
@@ -364,6 +367,8 @@ def bench_xcmesh(args, system, ctx, log):
         walls_ov.append(time.perf_counter() - t0)
         ms_ov.append(ctx.timing(14))
         ms_gemm.append(ctx.timing(15))
+        ctx.xc_potential(rho_fit)
+        ms_pot.append(ctx.timing(16))
         log("step %d: mesh %.1f ms (wall), overlap %.3f ms (device), %.1f ms with the copy"
             % (i, walls_mesh[-1] * 1e3, ms_ov[-1], walls_ov[-1] * 1e3))
     mesh_ms = float(np.mean(walls_mesh)) * 1e3
@@ -403,6 +408,7 @@ def bench_xcmesh(args, system, ctx, log):
                    "parallelism": "single GPU"},
         "stages_ms": {"xc_mesh_wall": round(mesh_ms, 3), "xc_overlap": round(ov_ms, 3),
                       "xc_overlap_gemm": round(gemm_ms, 3),
+                      "xc_potential": round(float(np.mean(ms_pot)), 3),
                       "xc_overlap_wall_with_copy": round(float(np.mean(walls_ov)) * 1e3, 3)},
         "ray_points": npoints,
         "roofline": roofline,

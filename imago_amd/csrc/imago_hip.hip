@@ -193,7 +193,7 @@ struct imago_ctx {
     double *ival = nullptr;       // [items] trace value of each item
     double *foldX = nullptr;      // [images][64 * ne] density folded per image
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
-    double timing[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double timing[17] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // second stream and handle: the two halves of the batched tridiagonal
     // divide and conquer run side by side (created on first use)
@@ -2544,6 +2544,110 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
     return IMAGO_OK;
 }
 
+extern "C" int imago_xc_potential(imago_ctx *c, const double *rho_total, const double *rho_core,
+                                  double *exch_corr_pot)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
+    if (!rho_total || !exch_corr_pot) return fail(IMAGO_E_ARG, "null array");
+    HIPCHK(hipSetDevice(c->device));
+    const int P = c->h.pot_dim, ns = c->h.nsites;
+    const int mp = std::max(c->xc_maxpts, 1);
+    const imago::XcArgs X = xc_args(c);
+    // the same batches as imago_xc_overlap: rows bounded from the geometry,
+    // the unscaled operator of the batch's points in B_b (points x rows)
+    const int batch = std::max(1, std::min(ns, 64));
+    std::vector<double> hr(2 * (size_t)P, 0.0);
+    std::copy(rho_total, rho_total + P, hr.begin());
+    if (rho_core) std::copy(rho_core, rho_core + P, hr.begin() + P);
+    double *rho = nullptr, *pot = nullptr, *v = nullptr;
+    unsigned char *nz = nullptr;
+    int *rows = nullptr, *cnt = nullptr, *slot = nullptr, *first = nullptr, *npt = nullptr;
+    int *rank = nullptr, *err = nullptr;
+    RCHK(dev_alloc(c, &rho, 2 * (size_t)P));
+    RCHK(dev_alloc(c, &pot, 3 * (size_t)P));
+    RCHK(dev_alloc(c, &v, 3 * (size_t)batch * mp));
+    RCHK(dev_alloc(c, &nz, (size_t)batch * P));
+    RCHK(dev_alloc(c, &rows, (size_t)batch * P));
+    RCHK(dev_alloc(c, &rank, (size_t)batch * P));
+    RCHK(dev_alloc(c, &cnt, (size_t)batch));
+    RCHK(dev_alloc(c, &slot, (size_t)batch * mp));
+    RCHK(dev_alloc(c, &first, (size_t)batch));
+    RCHK(dev_alloc(c, &npt, (size_t)batch));
+    RCHK(dev_alloc(c, &err, 1));
+    double *B = nullptr, *part = nullptr;
+    size_t b_cap = 0, p_cap = 0;
+    HIPCHK(hipMemcpyAsync(rho, hr.data(), 2 * (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(pot, 0, 3 * (size_t)P * 8, c->stream));
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    std::vector<int> hslot((size_t)batch * mp), hcnt(batch), hfirst(batch), hnpt(batch);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int s0 = 0; s0 < ns; s0 += batch) {
+        const int nb = std::min(batch, ns - s0);
+        const int p0 = c->xc_off[s0], p1 = c->xc_off[s0 + nb];
+        const int npb = p1 - p0;
+        if (npb == 0) continue;
+        for (int b = 0; b < nb; ++b) {
+            hfirst[b] = c->xc_off[s0 + b] - p0;
+            hnpt[b] = c->xc_npts[s0 + b];
+            for (int q = c->xc_off[s0 + b]; q < c->xc_off[s0 + b + 1]; ++q) hslot[q - p0] = b;
+        }
+        HIPCHK(hipMemcpyAsync(slot, hslot.data(), (size_t)npb * 4, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(first, hfirst.data(), (size_t)nb * 4, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(npt, hnpt.data(), (size_t)nb * 4, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(imago::launch_xc_rows(X, s0, nb, c->xc_dmax, nz, c->stream));
+        HIPCHK(imago::launch_xc_compact(nz, nb, P, rows, cnt, c->stream));
+        HIPCHK(hipMemcpyAsync(hcnt.data(), cnt, (size_t)nb * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int mm = 0, nn = 0;
+        for (int b = 0; b < nb; ++b) {
+            mm = std::max(mm, hcnt[b]);
+            nn = std::max(nn, hnpt[b]);
+        }
+        if (mm == 0 || nn == 0) continue;
+        const size_t b_need = (size_t)nb * mm * nn, p_need = (size_t)nb * mm * 3;
+        if (b_need > b_cap) {
+            if (B) dev_release(c, B);
+            RCHK(dev_alloc(c, &B, b_need));
+            b_cap = b_need;
+        }
+        if (p_need > p_cap) {
+            if (part) dev_release(c, part);
+            RCHK(dev_alloc(c, &part, p_need));
+            p_cap = p_need;
+        }
+        HIPCHK(hipMemsetAsync(rank, 0xff, (size_t)nb * P * 4, c->stream));
+        HIPCHK(imago::launch_xc_rank(rows, cnt, nb, P, rank, c->stream));
+        HIPCHK(hipMemsetAsync(B, 0, b_need * 8, c->stream));
+        HIPCHK(imago::launch_xc_op_rows(X, c->xc_pts + 3 * (size_t)p0, nullptr, npb, slot, first,
+                                        rank, mm, nn, B, err, c->stream));
+        HIPCHK(imago::launch_xc_potential_batch(B, mm, nn, slot, first, npt, rows, cnt, rank, nb,
+                                                P, c->xc_w + p0, rho, rho + P, npb, v, part, pot,
+                                                c->stream));
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(exch_corr_pot, pot, 3 * (size_t)P * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timing[16] = ms;
+    for (void *q : {(void *)rho, (void *)pot, (void *)v, (void *)nz, (void *)rows, (void *)rank,
+                    (void *)cnt, (void *)slot, (void *)first, (void *)npt, (void *)err,
+                    (void *)B, (void *)part})
+        if (q) dev_release(c, q);
+    if (herr)
+        return fail(IMAGO_E_HIP, "exchange-correlation potential: a point touched a term outside "
+                                 "its site's row bound (internal error)");
+    return IMAGO_OK;
+}
+
 extern "C" int imago_xc_counts(imago_ctx *c, long *ray_points, double *gemm_flops)
 {
     if (!c) return fail(IMAGO_E_ARG, "null context");
@@ -2555,7 +2659,7 @@ extern "C" int imago_xc_counts(imago_ctx *c, long *ray_points, double *gemm_flop
 
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 15) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 16) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

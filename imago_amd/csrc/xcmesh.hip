@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) void k_xc_op_rows(XcArgs X, const double *__re
         }
     }
     const int b = pt_slot[p];
-    const double sw = sqrt(w[p]);
+    const double sw = w ? sqrt(w[p]) : 1.0;
     const int *rk = rank + (size_t)b * X.pot_dim + X.type_term0[t];
     double *Bb = B + (size_t)b * lda * ldn + (p - slot_first[b]);
     bool bad = false;
@@ -248,6 +248,98 @@ __global__ __launch_bounds__(256) void k_xc_op_rows(XcArgs X, const double *__re
                 bad = true;
         }
     if (bad) *err = 1;
+}
+
+// wignerXC / wignerXCEnergy (potentialUpdate.F90:2835-2894), Hartree:
+// the potential and energy density of the Wigner interpolation.
+__device__ __forceinline__ void xc_wigner(double rho, double *pot, double *en)
+{
+    const double r13 = pow(rho, 0.3333333333333333333);
+    const double f = 1.0 + 12.57 * r13;
+    *pot = -r13 * (0.984 + (0.943656 + 8.8963 * r13) / (f * f));
+    *en = -0.738 * r13 * (1.0 + 0.959 / f);
+}
+
+// One thread per ray point of a batch: the total and core densities
+// op . rho over the slot's rows in term order (negative values set to 0),
+// then v[k][p] = w_p v_k for the Wigner potential and energy density of
+// the total and the energy density of the core (potentialUpdate.F90:
+// 855-1007).  B_b holds the unscaled operator (points x rows).
+__global__ __launch_bounds__(256) void k_xc_point_pot(const double *__restrict__ B, int lda,
+                                                      int ldn, const int *__restrict__ pt_slot,
+                                                      const int *__restrict__ slot_first,
+                                                      const int *__restrict__ rows,
+                                                      const int *__restrict__ count, int pot_dim,
+                                                      const double *__restrict__ w,
+                                                      const double *__restrict__ rho_total,
+                                                      const double *__restrict__ rho_core,
+                                                      int npts, double *__restrict__ v)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npts) return;
+    const int b = pt_slot[p], m = count[b];
+    const double *Bb = B + (size_t)b * lda * ldn + (p - slot_first[b]);
+    const int *rb = rows + (size_t)b * pot_dim;
+    double core = 0.0, total = 0.0;
+    for (int c = 0; c < m; ++c) core = core + Bb[(size_t)c * ldn] * rho_core[rb[c]];
+    for (int c = 0; c < m; ++c) total = total + Bb[(size_t)c * ldn] * rho_total[rb[c]];
+    if (core < 0.0) core = 0.0;
+    if (total < 0.0) total = 0.0;
+    double pot, en;
+    xc_wigner(total, &pot, &en);
+    const double r13 = pow(core, 0.3333333333333333333);
+    const double enc = -0.738 * r13 * (1.0 + 0.959 / (1.0 + 12.57 * r13));
+    v[p] = w[p] * pot;
+    v[(size_t)npts + p] = w[p] * en;
+    v[2 * (size_t)npts + p] = w[p] * enc;
+}
+
+// part[b][c][k] = sum over the slot's points, in order, of v[k][p] B_b(p, c)
+__global__ __launch_bounds__(256) void k_xc_pot_partial(const double *__restrict__ B, int lda,
+                                                        int ldn, const int *__restrict__ count,
+                                                        const int *__restrict__ slot_first,
+                                                        const int *__restrict__ slot_npts,
+                                                        const double *__restrict__ v, int npts,
+                                                        double *__restrict__ part)
+{
+    const int b = blockIdx.y;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= count[b]) return;
+    const double *Bb = B + (size_t)b * lda * ldn + (size_t)c * ldn;
+    const int f = slot_first[b], n = slot_npts[b];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int q = 0; q < n; ++q) {
+        const double o = Bb[q];
+        a0 += v[f + q] * o;
+        a1 += v[(size_t)npts + f + q] * o;
+        a2 += v[2 * (size_t)npts + f + q] * o;
+    }
+    double *pc = part + ((size_t)b * lda + c) * 3;
+    pc[0] = a0;
+    pc[1] = a1;
+    pc[2] = a2;
+}
+
+// pot[k][r] += part[b][rank[b][r]][k] for the slots b of the batch in order
+__global__ __launch_bounds__(256) void k_xc_pot_add(const double *__restrict__ part, int lda,
+                                                    const int *__restrict__ rank, int nb,
+                                                    int pot_dim, double *__restrict__ pot)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= pot_dim) return;
+    double a0 = pot[r], a1 = pot[pot_dim + r], a2 = pot[2 * (size_t)pot_dim + r];
+    for (int b = 0; b < nb; ++b) {
+        const int c = rank[(size_t)b * pot_dim + r];
+        if (c < 0) continue;
+        const double *pc = part + ((size_t)b * lda + c) * 3;
+        a0 += pc[0];
+        a1 += pc[1];
+        a2 += pc[2];
+    }
+    pot[r] = a0;
+    pot[pot_dim + r] = a1;
+    pot[2 * (size_t)pot_dim + r] = a2;
 }
 
 // The GGA operator of one site's points: as k_xc_op for the first value,
@@ -416,6 +508,26 @@ hipError_t launch_xc_op_rows(const XcArgs &X, const double *pts, const double *w
     else
         hipLaunchKernelGGL(k_xc_op_rows<64>, dim3(xc_blocks(n)), dim3(256), 0, stream, X, pts, w,
                            npts, pt_slot, slot_first, rank, lda, ldn, B, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_xc_potential_batch(const double *B, int lda, int ldn, const int *pt_slot,
+                                     const int *slot_first, const int *slot_npts,
+                                     const int *rows, const int *count, const int *rank, int nb,
+                                     int pot_dim, const double *w, const double *rho_total,
+                                     const double *rho_core, int npts, double *v, double *part,
+                                     double *pot, hipStream_t stream)
+{
+    if (npts <= 0 || nb <= 0) return hipSuccess;
+    if (nb > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_xc_point_pot, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, stream,
+                       B, lda, ldn, pt_slot, slot_first, rows, count, pot_dim, w, rho_total,
+                       rho_core, npts, v);
+    hipLaunchKernelGGL(k_xc_pot_partial, dim3((unsigned)((lda + 255) / 256), (unsigned)nb),
+                       dim3(256), 0, stream, B, lda, ldn, count, slot_first, slot_npts, v, npts,
+                       part);
+    hipLaunchKernelGGL(k_xc_pot_add, dim3((unsigned)((pot_dim + 255) / 256)), dim3(256), 0,
+                       stream, part, lda, rank, nb, pot_dim, pot);
     return hipGetLastError();
 }
 

@@ -48,12 +48,23 @@ hipError_t launch_xc_rank(const int *rows, const int *count, int nb, int pot_dim
 
 // The LDA operator of npts points of a batch of sites written into the
 // product operands: B_b(p - slot_first[b], rank[b][term]) = op(term, p) *
-// sqrt(w[p]), b = pt_slot[p], B_b = B + b * lda * ldn (column-major, ldn
+// sqrt(w[p]) (w null: unscaled), b = pt_slot[p], B_b = B + b * lda * ldn (column-major, ldn
 // rows = points, lda columns = terms).  *err is set if a term outside a
 // slot's rows is not 0.
 hipError_t launch_xc_op_rows(const XcArgs &X, const double *pts, const double *w, int npts,
                              const int *pt_slot, const int *slot_first, const int *rank, int lda,
                              int ldn, double *B, int *err, hipStream_t stream);
+
+// The exchange-correlation potential terms of one batch (Wigner, spin 1,
+// LDA; potentialUpdate.F90:833-1008) added into pot[3][pot_dim]: B holds
+// the unscaled operator of the batch (k_xc_op_rows with w null), v
+// [3][npts] and part [nb][lda][3] are scratch.
+hipError_t launch_xc_potential_batch(const double *B, int lda, int ldn, const int *pt_slot,
+                                     const int *slot_first, const int *slot_npts,
+                                     const int *rows, const int *count, const int *rank, int nb,
+                                     int pot_dim, const double *w, const double *rho_total,
+                                     const double *rho_core, int npts, double *v, double *part,
+                                     double *pot, hipStream_t stream);
 
 // Row lists of nslots flag rows: rows[s][0 .. count[s]) = the terms whose
 // flag nz[s][term] is set, ascending.

@@ -222,6 +222,15 @@ module O_ImagoHIP
          real(c_double), intent(out) :: exchCorrOverlap(*)
       end function imago_xc_overlap
 
+      ! exchCorrPot(:, 1:3) before the solve (potentialUpdate.F90:833-1008)
+      integer(c_int) function imago_xc_potential(ctx, rhoTotal, rhoCore, exchCorrPot) &
+            & bind(C, name="imago_xc_potential")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: rhoTotal(*), rhoCore(*)
+         real(c_double), intent(out) :: exchCorrPot(*)
+      end function imago_xc_potential
+
       integer(c_int) function imago_xc_counts(ctx, rayPoints, gemmFlops) &
             & bind(C, name="imago_xc_counts")
          import :: c_int, c_ptr, c_double, c_long

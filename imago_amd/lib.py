@@ -40,6 +40,7 @@ EXPORTS = [
     "imago_secular", "imago_get_eigenvectors", "imago_density_traces", "imago_zhegv",
     "imago_dsygv", "imago_last_timing", "imago_ep_counts", "imago_elecstat", "imago_residual_q",
     "imago_xc_mesh", "imago_xc_rho_op", "imago_xc_overlap", "imago_xc_counts",
+    "imago_xc_potential",
 ]
 
 
@@ -88,6 +89,7 @@ def load():
     L.imago_xc_rho_op.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
     L.imago_xc_overlap.argtypes = [ctypes.c_void_p, _dp]
     L.imago_xc_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), _dp]
+    L.imago_xc_potential.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -284,6 +286,15 @@ class Context:
         """exchCorrOverlap, potDim x potDim (upper triangle)."""
         out = np.zeros((self.pot_dim, self.pot_dim), order="F")
         _check(load().imago_xc_overlap(self._h, _p(out)))
+        return out
+
+    def xc_potential(self, rho_total, rho_core=None):
+        """exchCorrPot(:, 1:3) before the solve (Wigner, spin 1, LDA), as
+        [3][potDim]."""
+        r1 = np.ascontiguousarray(rho_total, dtype=np.float64)
+        r3 = None if rho_core is None else np.ascontiguousarray(rho_core, dtype=np.float64)
+        out = np.zeros((3, self.pot_dim))
+        _check(load().imago_xc_potential(self._h, _p(r1), None if r3 is None else _p(r3), _p(out)))
         return out
 
     def xc_counts(self):

@@ -217,7 +217,9 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * kernels of the last imago_elecstat, 12 = those of the last
  * imago_residual_q (both without the copies to the host), 13 = the
  * reciprocal-space sum kernel of the last imago_residual_q, 14 = the
- * device work of the last imago_xc_overlap, 15 = its GEMMs. */
+ * device work of the last imago_xc_overlap, 15 = its GEMMs, 16 = the
+ * device work of the last imago_xc_potential (with the batches' host
+ * synchronisations). */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
@@ -283,6 +285,17 @@ int imago_xc_rho_op(imago_ctx *ctx, int site, int gga, int *num_ray_points,
  * LDA operator, upper triangle (row <= column) filled as the reference
  * fills it, lower triangle 0. */
 int imago_xc_overlap(imago_ctx *ctx, double *exch_corr_overlap);
+
+/* The exchange-correlation terms of makeSCFPot on the mesh, before its
+ * dposvx solve (potentialUpdate.F90:833-1008), for the Wigner functional
+ * (xcCode 100 / 300), spin 1, LDA: at every ray point the total and core
+ * densities op . rho (negative values set to 0); then exch_corr_pot
+ * [3][potDim] = sum over the points of radialWeight * (potential, energy
+ * density of the total, energy density of the core) * op.  rho_total and
+ * rho_core (may be NULL: 0) are the fitted charge coefficients
+ * generalRho(:, 1) and (:, 3) [potDim]. */
+int imago_xc_potential(imago_ctx *ctx, const double *rho_total, const double *rho_core,
+                       double *exch_corr_pot);
 
 /* Sizes of the exchange-correlation mesh (after imago_xc_mesh): all ray
  * points of all sites, and the flops of the GEMMs of the last

@@ -25,6 +25,7 @@
  *                error_fn mathSubs.f90:22-161
  *   XC mesh      exchCorr.f90:63-342 (getECMeshParameters), 345-875
  *                (makeECMeshAndOverlap), 920-980 (makeSampleVectors)
+ *   XC potential potentialUpdate.F90:833-1008, 2835-2894 (Wigner, LDA)
  */
 #include <math.h>
 #include <stdlib.h>
@@ -1598,6 +1599,62 @@ int orc_xc_overlap_sites(const imago_system_desc *s, const imago_xc_mesh_params 
                 if (o[k] == 0.0) continue;   /* adds only zeros */
                 double *col = ovl + (size_t)k * P;
                 for (int r = 0; r <= k; ++r) col[r] = col[r] + o[k] * o[r] * rw[j];
+            }
+        }
+    }
+    free(rw);
+    free(op);
+    return 0;
+}
+
+/* wignerXC and wignerXCEnergy, potentialUpdate.F90:2835-2894 (Hartree). */
+static void xc_wigner(double rho, double *pot, double *en)
+{
+    const double r13 = pow(rho, 0.3333333333333333333);
+    const double f = 1.0 + 12.57 * r13;
+    *pot = -r13 * (0.984 + (0.943656 + 8.8963 * r13) / (f * f));
+    *en = -0.738 * r13 * (1.0 + 0.959 / f);
+}
+
+void orc_xc_wigner(double rho, double *pot, double *en) { xc_wigner(rho, pot, en); }
+
+static double xc_wigner_energy(double rho)
+{
+    const double r13 = pow(rho, 0.3333333333333333333);
+    return -0.738 * r13 * (1.0 + 0.959 / (1.0 + 12.57 * r13));
+}
+
+/* The exchange-correlation potential terms of makeSCFPot for the Wigner
+ * functional (xcCode 100/300), spin 1, LDA (potentialUpdate.F90:833-1008):
+ * at every ray point the total and core densities op . rho (negative
+ * values set to 0), then exchCorrPot(:, k) += radialWeight * v_k * op for
+ * v = (potential, energy density) of the total and the energy density of
+ * the core.  out[3][potDim] is exchCorrPot(:, 1:3) before the solve. */
+int orc_xc_potential(const imago_system_desc *s, const imago_xc_mesh_params *p, double th,
+                     const double *cov, const double *vec, const double *pls,
+                     const double *maxrad, int maxpts, const double *rho_total,
+                     const double *rho_core, double *out)
+{
+    const int P = es_pot_dim(s);
+    memset(out, 0, sizeof(double) * 3 * (size_t)P);
+    double *rw = malloc(sizeof(double) * maxpts);
+    double *op = malloc(sizeof(double) * (size_t)maxpts * P);
+    for (int i = 0; i < s->num_pot_sites; ++i) {
+        const int np = orc_xc_site(s, p, th, cov, vec, pls, maxrad, maxpts, i, 0, rw, op);
+        for (int j = 0; j < np; ++j) {
+            const double *o = op + (size_t)j * P;
+            double core = 0.0, total = 0.0;
+            for (int k = 0; k < P; ++k) core = core + o[k] * rho_core[k];
+            for (int k = 0; k < P; ++k) total = total + o[k] * rho_total[k];
+            if (core < 0.0) core = 0.0;
+            if (total < 0.0) total = 0.0;
+            double v[3];
+            xc_wigner(total, &v[0], &v[1]);
+            v[2] = xc_wigner_energy(core);
+            for (int k = 0; k < 3; ++k) {
+                const double f = rw[j] * v[k];
+                double *col = out + (size_t)k * P;
+                for (int t = 0; t < P; ++t) col[t] = col[t] + f * o[t];
             }
         }
     }

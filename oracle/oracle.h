@@ -125,6 +125,17 @@ int orc_xc_overlap_sites(const imago_system_desc *sys,
                          const double *pls, const double *maxrad, int maxpts,
                          int i_begin, int i_end, double *ovl);
 
+/* wignerXC (potentialUpdate.F90:2835-2867): potential and energy density. */
+void orc_xc_wigner(double rho, double *pot, double *en);
+
+/* exchCorrPot(:, 1:3) of makeSCFPot before its solve, Wigner functional,
+ * spin 1, LDA (potentialUpdate.F90:833-1008); out[3][potDim]. */
+int orc_xc_potential(const imago_system_desc *sys, const imago_xc_mesh_params *p,
+                     double th, const double *cov, const double *vec,
+                     const double *pls, const double *maxrad, int maxpts,
+                     const double *rho_total, const double *rho_core,
+                     double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,9 @@ def lib():
         L.orc_xc_site.restype = ctypes.c_int
         L.orc_xc_overlap.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp, _dp,
                                      _dp, _dp, ctypes.c_int, _dp]
+        L.orc_xc_wigner.argtypes = [ctypes.c_double, _dp, _dp]
+        L.orc_xc_potential.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp,
+                                       _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp]
         L.orc_xc_overlap_sites.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, _dp,
                                            _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, _dp]
@@ -217,6 +220,13 @@ def residual_q(system, log_elec_thresh=None, recip_cells=None, sites=None):
     return out
 
 
+def xc_wigner(rho):
+    """wignerXC (potentialUpdate.F90:2835-2867): (potential, energy density)."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    lib().orc_xc_wigner(float(rho), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
 class XcOracleMesh:
     """getECMeshParameters of the oracle for one system and mesh input."""
 
@@ -254,6 +264,19 @@ class XcOracleMesh:
                               _p(self.pls), _p(self.maxrad), max(self.maxpts, 1), int(site),
                               1 if gga else 0, _p(rw), _p(op))
         return n, rw, op
+
+    def potential(self, rho_total, rho_core=None):
+        """exchCorrPot(:, 1:3) before the solve (Wigner, spin 1, LDA), as
+        [3][potDim]."""
+        P = self.system.pot_dim
+        r1 = np.ascontiguousarray(rho_total, dtype=np.float64)
+        r3 = np.zeros(P) if rho_core is None else np.ascontiguousarray(rho_core, dtype=np.float64)
+        out = np.zeros((3, P))
+        lib().orc_xc_potential(ctypes.byref(self.desc), ctypes.byref(self.params),
+                               float(self.system.log_elec_thresh), _p(self.cov), _p(self.vec),
+                               _p(self.pls), _p(self.maxrad), max(self.maxpts, 1), _p(r1),
+                               _p(r3), _p(out))
+        return out
 
     def overlap(self, sites=None):
         """exchCorrOverlap; sites = (begin, end) sums those sites only."""

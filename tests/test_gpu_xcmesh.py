@@ -82,3 +82,25 @@ def test_errors(gpu_ctx):
     gpu_ctx.xc_mesh(XcMesh(num_sample_vectors=4))
     with pytest.raises(ImagoError):
         gpu_ctx.xc_rho_op(99)
+
+
+@pytest.mark.parametrize("name", ["small", "si64"])
+def test_xc_potential(gpu_ctx, oracle_mod, name):
+    if name == "small":
+        s, mesh = configs.small_test_system(), XcMesh()
+    else:
+        s, mesh = configs.si64_supercell(), XcMesh(num_sample_vectors=8)
+    print("[gpu] xc potential", name, flush=True)
+    gpu_ctx.set_system(s)
+    gpu_ctx.xc_mesh(mesh)
+    om = oracle_mod.XcOracleMesh(s, mesh)
+    rng = np.random.default_rng(9)
+    rho = rng.normal(size=s.pot_dim) * 0.02 + 0.01
+    core = np.abs(rng.normal(size=s.pot_dim)) * 0.005
+    got = gpu_ctx.xc_potential(rho, core)
+    ref = om.potential(rho, core)
+    for k in range(3):
+        assert rel_err(got[k], ref[k]) < TOL, (k, rel_err(got[k], ref[k]))
+    got0 = gpu_ctx.xc_potential(rho)          # no core density: its row is 0
+    assert np.all(got0[2] == 0.0)
+    assert rel_err(got0[:2], got[:2]) < TOL

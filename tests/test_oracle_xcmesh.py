@@ -9,7 +9,9 @@ exchCorr.f90 of the reference):
   radial rule against the exact Gaussian integral for an isolated site;
 - the operator of every point against a brute-force image sum (all lattice
   points within the cut-off sphere, no nearest-lattice-vector search);
-- the overlap against the sum of w op op^T over the sites' outputs.
+- the overlap against the sum of w op op^T over the sites' outputs;
+- the Wigner functional's potential as the derivative of its energy, and
+  the exchange-correlation terms against a numpy sum over the sites.
 """
 import math
 
@@ -212,3 +214,45 @@ def test_isolated_site_radial_rule(oracle_mod):
             ex, _ = integrate.quad(lambda r: r * r * math.exp(-c * r * r), 0, cap, epsabs=0,
                                    epsrel=1e-12, limit=200)
             assert ov[i, j] == pytest.approx(0.5 * 10 * ex, rel=0.01), (i, j)
+
+
+def test_wigner_potential_is_the_derivative(oracle_mod):
+    """The Wigner potential is d(rho e_xc)/d rho of its energy density (to
+    the rounding of the reference's constant 8.8963 = 8.89628..., 2e-6
+    relative), and the energy density is -0.738 rho^(1/3) (1 + 0.959 /
+    (1 + 12.57 rho^(1/3)))."""
+    for rho in (1e-6, 1e-3, 0.03, 0.5, 2.0, 40.0):
+        pot, en = oracle_mod.xc_wigner(rho)
+        r = rho ** (1.0 / 3.0)
+        assert en == pytest.approx(-0.738 * r * (1 + 0.959 / (1 + 12.57 * r)), rel=1e-14)
+        h = rho * 1e-5
+        d = ((rho + h) * oracle_mod.xc_wigner(rho + h)[1] -
+             (rho - h) * oracle_mod.xc_wigner(rho - h)[1]) / (2 * h)
+        assert pot == pytest.approx(d, rel=1e-5), rho
+    assert oracle_mod.xc_wigner(0.0) == (0.0, 0.0)
+
+
+def test_potential_accumulation(oracle_mod):
+    s = configs.small_test_system()
+    m = oracle_mod.XcOracleMesh(s, XcMesh(num_sample_vectors=10))
+    rng = np.random.default_rng(5)
+    rho = rng.normal(size=s.pot_dim) * 0.05 + 0.02
+    core = np.abs(rng.normal(size=s.pot_dim)) * 0.01
+    got = m.potential(rho, core)
+    want = np.zeros((3, s.pot_dim))
+    clamped = 0
+    for site in range(len(s.pot_site_pos)):
+        n, rw, op = m.site(site)
+        o = op[:, :n, 0]
+        tot = o.T @ rho
+        cor = o.T @ core
+        clamped += int(np.sum(tot < 0))
+        tot = np.maximum(tot, 0.0)
+        cor = np.maximum(cor, 0.0)
+        for j in range(n):
+            pot, en = oracle_mod.xc_wigner(tot[j])
+            enc = oracle_mod.xc_wigner(cor[j])[1]
+            for k, val in enumerate((pot, en, enc)):
+                want[k] += rw[j] * val * o[:, j]
+    assert clamped > 0          # the clamp of negative densities is exercised
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
