@@ -213,6 +213,15 @@ struct imago_ctx {
     double *xc_sizes = nullptr;                  // real-space cell sizes
     double *xc_dmax = nullptr;                   // [site] longest ray
     double xc_gemm_flops = 0.0;                  // of the last imago_xc_overlap
+    // unscaled operator blocks of every batch of sites, built by the first
+    // imago_xc_potential after imago_xc_mesh and kept for the next ones
+    struct XcBatch {
+        int s0, nb, p0, npb, mm, nn;
+        double *B;
+        int *rows, *cnt, *rank, *slot, *first;
+    };
+    std::vector<XcBatch> xc_cache;
+    bool xc_cached = false;
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -264,6 +273,8 @@ static void ctx_free_system(imago_ctx *c)
     c->xc_total = 0;
     c->xc_pts = c->xc_w = c->xc_pls = c->xc_vec = c->xc_sitecov = c->xc_sizes = nullptr;
     c->xc_dmax = nullptr;
+    c->xc_cache.clear();
+    c->xc_cached = false;
     c->has_system = false;
 }
 
@@ -2288,6 +2299,12 @@ extern "C" int imago_xc_mesh(imago_ctx *c, const imago_xc_mesh_params *params,
         if (q) dev_release(c, q);
     c->xc_pts = c->xc_w = c->xc_pls = c->xc_vec = c->xc_sitecov = c->xc_sizes = nullptr;
     c->xc_dmax = nullptr;
+    for (const auto &q : c->xc_cache)
+        for (void *x : {(void *)q.B, (void *)q.rows, (void *)q.cnt, (void *)q.rank,
+                        (void *)q.slot, (void *)q.first})
+            if (x) dev_release(c, x);
+    c->xc_cache.clear();
+    c->xc_cached = false;
     c->xc_ready = false;
     c->xcp = p;
     c->xc_th = log_elec_thresh;
@@ -2544,6 +2561,76 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
     return IMAGO_OK;
 }
 
+// The operator blocks of every batch of sites (rows bounded from the
+// geometry, the unscaled operator of the batch's points in B_b, points x
+// rows), built once per mesh and kept: about potDim/6 x points doubles
+// (4.4 GB for a-SiO2).
+static int xc_build_cache(imago_ctx *c)
+{
+    const int P = c->h.pot_dim, ns = c->h.nsites;
+    const int mp = std::max(c->xc_maxpts, 1);
+    const imago::XcArgs X = xc_args(c);
+    const int batch = std::max(1, std::min(ns, 64));
+    unsigned char *nz = nullptr;
+    int *err = nullptr;
+    RCHK(dev_alloc(c, &nz, (size_t)batch * P));
+    RCHK(dev_alloc(c, &err, 1));
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    std::vector<int> hslot((size_t)batch * mp), hcnt(batch), hfirst(batch);
+    for (int s0 = 0; s0 < ns; s0 += batch) {
+        imago_ctx::XcBatch q{};
+        q.s0 = s0;
+        q.nb = std::min(batch, ns - s0);
+        q.p0 = c->xc_off[s0];
+        q.npb = c->xc_off[s0 + q.nb] - q.p0;
+        if (q.npb == 0) continue;
+        for (int b = 0; b < q.nb; ++b) {
+            hfirst[b] = c->xc_off[s0 + b] - q.p0;
+            for (int k = c->xc_off[s0 + b]; k < c->xc_off[s0 + b + 1]; ++k) hslot[k - q.p0] = b;
+        }
+        RCHK(dev_alloc(c, &q.rows, (size_t)q.nb * P));
+        RCHK(dev_alloc(c, &q.rank, (size_t)q.nb * P));
+        RCHK(dev_alloc(c, &q.cnt, (size_t)q.nb));
+        RCHK(dev_alloc(c, &q.slot, (size_t)q.npb));
+        RCHK(dev_alloc(c, &q.first, (size_t)q.nb));
+        c->xc_cache.push_back(q);   // owned by the cache from here on
+        HIPCHK(hipMemcpyAsync(q.slot, hslot.data(), (size_t)q.npb * 4, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(q.first, hfirst.data(), (size_t)q.nb * 4, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(imago::launch_xc_rows(X, s0, q.nb, c->xc_dmax, nz, c->stream));
+        HIPCHK(imago::launch_xc_compact(nz, q.nb, P, q.rows, q.cnt, c->stream));
+        HIPCHK(hipMemcpyAsync(hcnt.data(), q.cnt, (size_t)q.nb * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int mm = 0, nn = 0;
+        for (int b = 0; b < q.nb; ++b) {
+            mm = std::max(mm, hcnt[b]);
+            nn = std::max(nn, c->xc_npts[s0 + b]);
+        }
+        imago_ctx::XcBatch &qq = c->xc_cache.back();
+        qq.mm = mm;
+        qq.nn = nn;
+        if (mm == 0 || nn == 0) continue;
+        RCHK(dev_alloc(c, &qq.B, (size_t)q.nb * mm * nn));
+        HIPCHK(hipMemsetAsync(qq.rank, 0xff, (size_t)q.nb * P * 4, c->stream));
+        HIPCHK(imago::launch_xc_rank(qq.rows, qq.cnt, q.nb, P, qq.rank, c->stream));
+        HIPCHK(hipMemsetAsync(qq.B, 0, (size_t)q.nb * mm * nn * 8, c->stream));
+        HIPCHK(imago::launch_xc_op_rows(X, c->xc_pts + 3 * (size_t)q.p0, nullptr, q.npb, qq.slot,
+                                        qq.first, qq.rank, mm, nn, qq.B, err, c->stream));
+    }
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, nz);
+    dev_release(c, err);
+    if (herr)
+        return fail(IMAGO_E_HIP, "exchange-correlation potential: a point touched a term outside "
+                                 "its site's row bound (internal error)");
+    c->xc_cached = true;
+    return IMAGO_OK;
+}
+
 extern "C" int imago_xc_potential(imago_ctx *c, const double *rho_total, const double *rho_core,
                                   double *exch_corr_pot)
 {
@@ -2551,100 +2638,45 @@ extern "C" int imago_xc_potential(imago_ctx *c, const double *rho_total, const d
     if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
     if (!rho_total || !exch_corr_pot) return fail(IMAGO_E_ARG, "null array");
     HIPCHK(hipSetDevice(c->device));
-    const int P = c->h.pot_dim, ns = c->h.nsites;
-    const int mp = std::max(c->xc_maxpts, 1);
-    const imago::XcArgs X = xc_args(c);
-    // the same batches as imago_xc_overlap: rows bounded from the geometry,
-    // the unscaled operator of the batch's points in B_b (points x rows)
-    const int batch = std::max(1, std::min(ns, 64));
+    const int P = c->h.pot_dim;
+    if (!c->xc_cached) RCHK(xc_build_cache(c));
     std::vector<double> hr(2 * (size_t)P, 0.0);
     std::copy(rho_total, rho_total + P, hr.begin());
     if (rho_core) std::copy(rho_core, rho_core + P, hr.begin() + P);
-    double *rho = nullptr, *pot = nullptr, *v = nullptr;
-    unsigned char *nz = nullptr;
-    int *rows = nullptr, *cnt = nullptr, *slot = nullptr, *first = nullptr, *npt = nullptr;
-    int *rank = nullptr, *err = nullptr;
+    size_t v_need = 1, p_need = 1;
+    for (const auto &q : c->xc_cache) {
+        v_need = std::max(v_need, (size_t)q.nb * 3 * q.nn);
+        p_need = std::max(p_need, (size_t)q.nb * 3 * q.mm);
+    }
+    double *rho = nullptr, *pot = nullptr, *v = nullptr, *part = nullptr;
     RCHK(dev_alloc(c, &rho, 2 * (size_t)P));
     RCHK(dev_alloc(c, &pot, 3 * (size_t)P));
-    RCHK(dev_alloc(c, &v, 3 * (size_t)batch * mp));
-    RCHK(dev_alloc(c, &nz, (size_t)batch * P));
-    RCHK(dev_alloc(c, &rows, (size_t)batch * P));
-    RCHK(dev_alloc(c, &rank, (size_t)batch * P));
-    RCHK(dev_alloc(c, &cnt, (size_t)batch));
-    RCHK(dev_alloc(c, &slot, (size_t)batch * mp));
-    RCHK(dev_alloc(c, &first, (size_t)batch));
-    RCHK(dev_alloc(c, &npt, (size_t)batch));
-    RCHK(dev_alloc(c, &err, 1));
-    double *B = nullptr, *part = nullptr;
-    size_t b_cap = 0, p_cap = 0;
+    RCHK(dev_alloc(c, &v, v_need));
+    RCHK(dev_alloc(c, &part, p_need));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(hipMemcpyAsync(rho, hr.data(), 2 * (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemsetAsync(pot, 0, 3 * (size_t)P * 8, c->stream));
-    HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
-    std::vector<int> hslot((size_t)batch * mp), hcnt(batch), hfirst(batch), hnpt(batch);
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
-    for (int s0 = 0; s0 < ns; s0 += batch) {
-        const int nb = std::min(batch, ns - s0);
-        const int p0 = c->xc_off[s0], p1 = c->xc_off[s0 + nb];
-        const int npb = p1 - p0;
-        if (npb == 0) continue;
-        for (int b = 0; b < nb; ++b) {
-            hfirst[b] = c->xc_off[s0 + b] - p0;
-            hnpt[b] = c->xc_npts[s0 + b];
-            for (int q = c->xc_off[s0 + b]; q < c->xc_off[s0 + b + 1]; ++q) hslot[q - p0] = b;
-        }
-        HIPCHK(hipMemcpyAsync(slot, hslot.data(), (size_t)npb * 4, hipMemcpyHostToDevice,
-                              c->stream));
-        HIPCHK(hipMemcpyAsync(first, hfirst.data(), (size_t)nb * 4, hipMemcpyHostToDevice,
-                              c->stream));
-        HIPCHK(hipMemcpyAsync(npt, hnpt.data(), (size_t)nb * 4, hipMemcpyHostToDevice,
-                              c->stream));
-        HIPCHK(imago::launch_xc_rows(X, s0, nb, c->xc_dmax, nz, c->stream));
-        HIPCHK(imago::launch_xc_compact(nz, nb, P, rows, cnt, c->stream));
-        HIPCHK(hipMemcpyAsync(hcnt.data(), cnt, (size_t)nb * 4, hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        int mm = 0, nn = 0;
-        for (int b = 0; b < nb; ++b) {
-            mm = std::max(mm, hcnt[b]);
-            nn = std::max(nn, hnpt[b]);
-        }
-        if (mm == 0 || nn == 0) continue;
-        const size_t b_need = (size_t)nb * mm * nn, p_need = (size_t)nb * mm * 3;
-        if (b_need > b_cap) {
-            if (B) dev_release(c, B);
-            RCHK(dev_alloc(c, &B, b_need));
-            b_cap = b_need;
-        }
-        if (p_need > p_cap) {
-            if (part) dev_release(c, part);
-            RCHK(dev_alloc(c, &part, p_need));
-            p_cap = p_need;
-        }
-        HIPCHK(hipMemsetAsync(rank, 0xff, (size_t)nb * P * 4, c->stream));
-        HIPCHK(imago::launch_xc_rank(rows, cnt, nb, P, rank, c->stream));
-        HIPCHK(hipMemsetAsync(B, 0, b_need * 8, c->stream));
-        HIPCHK(imago::launch_xc_op_rows(X, c->xc_pts + 3 * (size_t)p0, nullptr, npb, slot, first,
-                                        rank, mm, nn, B, err, c->stream));
-        HIPCHK(imago::launch_xc_potential_batch(B, mm, nn, slot, first, npt, rows, cnt, rank, nb,
-                                                P, c->xc_w + p0, rho, rho + P, npb, v, part, pot,
-                                                c->stream));
+    for (const auto &q : c->xc_cache) {
+        if (!q.B) continue;
+        HIPCHK(hipMemsetAsync(v, 0, (size_t)q.nb * 3 * q.nn * 8, c->stream));
+        HIPCHK(imago::launch_xc_point_pot(q.B, q.mm, q.nn, q.slot, q.first, q.rows, q.cnt, P,
+                                          c->xc_w + q.p0, rho, rho + P, q.npb, v, c->stream));
+        // part_b = B_b^T V_b (rows x 3)
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(rocblas_dgemm_strided_batched(
+            c->blas, rocblas_operation_transpose, rocblas_operation_none, q.mm, 3, q.nn, &one, q.B,
+            q.nn, (rocblas_stride)q.mm * q.nn, v, q.nn, (rocblas_stride)3 * q.nn, &zero, part, q.mm,
+            (rocblas_stride)3 * q.mm, q.nb));
+        HIPCHK(imago::launch_xc_pot_add(part, q.mm, q.rank, q.nb, P, pot, c->stream));
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
-    int herr = 0;
-    HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(exch_corr_pot, pot, 3 * (size_t)P * 8, hipMemcpyDeviceToHost,
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[16] = ms;
-    for (void *q : {(void *)rho, (void *)pot, (void *)v, (void *)nz, (void *)rows, (void *)rank,
-                    (void *)cnt, (void *)slot, (void *)first, (void *)npt, (void *)err,
-                    (void *)B, (void *)part})
-        if (q) dev_release(c, q);
-    if (herr)
-        return fail(IMAGO_E_HIP, "exchange-correlation potential: a point touched a term outside "
-                                 "its site's row bound (internal error)");
+    for (void *x : {(void *)rho, (void *)pot, (void *)v, (void *)part}) dev_release(c, x);
     return IMAGO_OK;
 }
 

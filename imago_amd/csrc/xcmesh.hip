@@ -262,9 +262,10 @@ __device__ __forceinline__ void xc_wigner(double rho, double *pot, double *en)
 
 // One thread per ray point of a batch: the total and core densities
 // op . rho over the slot's rows in term order (negative values set to 0),
-// then v[k][p] = w_p v_k for the Wigner potential and energy density of
+// then V_b(p, k) = w_p v_k for the Wigner potential and energy density of
 // the total and the energy density of the core (potentialUpdate.F90:
-// 855-1007).  B_b holds the unscaled operator (points x rows).
+// 855-1007).  B_b holds the unscaled operator (points x rows); V is
+// [nb][3][ldn], zero beyond each slot's points.
 __global__ __launch_bounds__(256) void k_xc_point_pot(const double *__restrict__ B, int lda,
                                                       int ldn, const int *__restrict__ pt_slot,
                                                       const int *__restrict__ slot_first,
@@ -273,13 +274,13 @@ __global__ __launch_bounds__(256) void k_xc_point_pot(const double *__restrict__
                                                       const double *__restrict__ w,
                                                       const double *__restrict__ rho_total,
                                                       const double *__restrict__ rho_core,
-                                                      int npts, double *__restrict__ v)
+                                                      int npts, double *__restrict__ V)
 {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= npts) return;
-    const int b = pt_slot[p], m = count[b];
-    const double *Bb = B + (size_t)b * lda * ldn + (p - slot_first[b]);
+    const int b = pt_slot[p], m = count[b], q = p - slot_first[b];
+    const double *Bb = B + (size_t)b * lda * ldn + q;
     const int *rb = rows + (size_t)b * pot_dim;
     double core = 0.0, total = 0.0;
     for (int c = 0; c < m; ++c) core = core + Bb[(size_t)c * ldn] * rho_core[rb[c]];
@@ -290,38 +291,14 @@ __global__ __launch_bounds__(256) void k_xc_point_pot(const double *__restrict__
     xc_wigner(total, &pot, &en);
     const double r13 = pow(core, 0.3333333333333333333);
     const double enc = -0.738 * r13 * (1.0 + 0.959 / (1.0 + 12.57 * r13));
-    v[p] = w[p] * pot;
-    v[(size_t)npts + p] = w[p] * en;
-    v[2 * (size_t)npts + p] = w[p] * enc;
+    double *Vb = V + (size_t)b * 3 * ldn + q;
+    Vb[0] = w[p] * pot;
+    Vb[ldn] = w[p] * en;
+    Vb[2 * (size_t)ldn] = w[p] * enc;
 }
 
-// part[b][c][k] = sum over the slot's points, in order, of v[k][p] B_b(p, c)
-__global__ __launch_bounds__(256) void k_xc_pot_partial(const double *__restrict__ B, int lda,
-                                                        int ldn, const int *__restrict__ count,
-                                                        const int *__restrict__ slot_first,
-                                                        const int *__restrict__ slot_npts,
-                                                        const double *__restrict__ v, int npts,
-                                                        double *__restrict__ part)
-{
-    const int b = blockIdx.y;
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= count[b]) return;
-    const double *Bb = B + (size_t)b * lda * ldn + (size_t)c * ldn;
-    const int f = slot_first[b], n = slot_npts[b];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int q = 0; q < n; ++q) {
-        const double o = Bb[q];
-        a0 += v[f + q] * o;
-        a1 += v[(size_t)npts + f + q] * o;
-        a2 += v[2 * (size_t)npts + f + q] * o;
-    }
-    double *pc = part + ((size_t)b * lda + c) * 3;
-    pc[0] = a0;
-    pc[1] = a1;
-    pc[2] = a2;
-}
-
-// pot[k][r] += part[b][rank[b][r]][k] for the slots b of the batch in order
+// pot[k][r] += part_b(rank[b][r], k) for the slots b of the batch in order
+// (part_b lda x 3, column-major, one block per slot)
 __global__ __launch_bounds__(256) void k_xc_pot_add(const double *__restrict__ part, int lda,
                                                     const int *__restrict__ rank, int nb,
                                                     int pot_dim, double *__restrict__ pot)
@@ -332,10 +309,10 @@ __global__ __launch_bounds__(256) void k_xc_pot_add(const double *__restrict__ p
     for (int b = 0; b < nb; ++b) {
         const int c = rank[(size_t)b * pot_dim + r];
         if (c < 0) continue;
-        const double *pc = part + ((size_t)b * lda + c) * 3;
+        const double *pc = part + (size_t)b * 3 * lda + c;
         a0 += pc[0];
-        a1 += pc[1];
-        a2 += pc[2];
+        a1 += pc[lda];
+        a2 += pc[2 * (size_t)lda];
     }
     pot[r] = a0;
     pot[pot_dim + r] = a1;
@@ -511,21 +488,22 @@ hipError_t launch_xc_op_rows(const XcArgs &X, const double *pts, const double *w
     return hipGetLastError();
 }
 
-hipError_t launch_xc_potential_batch(const double *B, int lda, int ldn, const int *pt_slot,
-                                     const int *slot_first, const int *slot_npts,
-                                     const int *rows, const int *count, const int *rank, int nb,
-                                     int pot_dim, const double *w, const double *rho_total,
-                                     const double *rho_core, int npts, double *v, double *part,
-                                     double *pot, hipStream_t stream)
+hipError_t launch_xc_point_pot(const double *B, int lda, int ldn, const int *pt_slot,
+                               const int *slot_first, const int *rows, const int *count,
+                               int pot_dim, const double *w, const double *rho_total,
+                               const double *rho_core, int npts, double *V, hipStream_t stream)
 {
-    if (npts <= 0 || nb <= 0) return hipSuccess;
-    if (nb > 65535) return hipErrorInvalidValue;
+    if (npts <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_xc_point_pot, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, stream,
                        B, lda, ldn, pt_slot, slot_first, rows, count, pot_dim, w, rho_total,
-                       rho_core, npts, v);
-    hipLaunchKernelGGL(k_xc_pot_partial, dim3((unsigned)((lda + 255) / 256), (unsigned)nb),
-                       dim3(256), 0, stream, B, lda, ldn, count, slot_first, slot_npts, v, npts,
-                       part);
+                       rho_core, npts, V);
+    return hipGetLastError();
+}
+
+hipError_t launch_xc_pot_add(const double *part, int lda, const int *rank, int nb, int pot_dim,
+                             double *pot, hipStream_t stream)
+{
+    if (nb <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_xc_pot_add, dim3((unsigned)((pot_dim + 255) / 256)), dim3(256), 0,
                        stream, part, lda, rank, nb, pot_dim, pot);
     return hipGetLastError();

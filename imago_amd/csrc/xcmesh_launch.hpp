@@ -55,16 +55,18 @@ hipError_t launch_xc_op_rows(const XcArgs &X, const double *pts, const double *w
                              const int *pt_slot, const int *slot_first, const int *rank, int lda,
                              int ldn, double *B, int *err, hipStream_t stream);
 
-// The exchange-correlation potential terms of one batch (Wigner, spin 1,
-// LDA; potentialUpdate.F90:833-1008) added into pot[3][pot_dim]: B holds
-// the unscaled operator of the batch (k_xc_op_rows with w null), v
-// [3][npts] and part [nb][lda][3] are scratch.
-hipError_t launch_xc_potential_batch(const double *B, int lda, int ldn, const int *pt_slot,
-                                     const int *slot_first, const int *slot_npts,
-                                     const int *rows, const int *count, const int *rank, int nb,
-                                     int pot_dim, const double *w, const double *rho_total,
-                                     const double *rho_core, int npts, double *v, double *part,
-                                     double *pot, hipStream_t stream);
+// The exchange-correlation terms of one batch (Wigner, spin 1, LDA;
+// potentialUpdate.F90:833-1008), in two steps around a batched GEMM part_b
+// = B_b^T V_b: V [nb][3][ldn] = w_p times (potential, energy density of
+// the total, energy density of the core) at every point, zero beyond a
+// slot's points (zeroed by the caller); then pot[k][r] += part_b(rank, k)
+// slot by slot.  B holds the unscaled operator (k_xc_op_rows, w null).
+hipError_t launch_xc_point_pot(const double *B, int lda, int ldn, const int *pt_slot,
+                               const int *slot_first, const int *rows, const int *count,
+                               int pot_dim, const double *w, const double *rho_total,
+                               const double *rho_core, int npts, double *V, hipStream_t stream);
+hipError_t launch_xc_pot_add(const double *part, int lda, const int *rank, int nb, int pot_dim,
+                             double *pot, hipStream_t stream);
 
 // Row lists of nslots flag rows: rows[s][0 .. count[s]) = the terms whose
 // flag nz[s][term] is set, ascending.

@@ -218,8 +218,8 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * imago_residual_q (both without the copies to the host), 13 = the
  * reciprocal-space sum kernel of the last imago_residual_q, 14 = the
  * device work of the last imago_xc_overlap, 15 = its GEMMs, 16 = the
- * device work of the last imago_xc_potential (with the batches' host
- * synchronisations). */
+ * device work of the last imago_xc_potential (without the first call's
+ * operator evaluation). */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
@@ -293,7 +293,10 @@ int imago_xc_overlap(imago_ctx *ctx, double *exch_corr_overlap);
  * [3][potDim] = sum over the points of radialWeight * (potential, energy
  * density of the total, energy density of the core) * op.  rho_total and
  * rho_core (may be NULL: 0) are the fitted charge coefficients
- * generalRho(:, 1) and (:, 3) [potDim]. */
+ * generalRho(:, 1) and (:, 3) [potDim].  The first call after
+ * imago_xc_mesh evaluates the operator at every ray point and keeps it on
+ * the device (about potDim/6 x ray points doubles); later calls (one per
+ * SCF iteration) reuse it. */
 int imago_xc_potential(imago_ctx *ctx, const double *rho_total, const double *rho_core,
                        double *exch_corr_pot);
 
