@@ -222,6 +222,19 @@ struct imago_ctx {
     };
     std::vector<XcBatch> xc_cache;
     bool xc_cached = false;
+    // potential-side overlaps kept for imago_pot_solve: [0] potAlphaOverlap
+    // (imago_elecstat), [1] exchCorrOverlap (imago_xc_overlap); the
+    // equilibrated upper triangle, its Cholesky factor and the scales,
+    // made on the first solve
+    struct SpdMat {
+        double *A = nullptr;      // the matrix as computed (upper triangle used)
+        double *As = nullptr;     // equilibrated copy
+        double *U = nullptr;      // its Cholesky factor (upper)
+        double *s = nullptr;      // scale factors (1 when not equilibrated)
+        bool factored = false;
+        int info = 0;
+    };
+    SpdMat spd[2];
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -275,6 +288,7 @@ static void ctx_free_system(imago_ctx *c)
     c->xc_dmax = nullptr;
     c->xc_cache.clear();
     c->xc_cached = false;
+    for (auto &m : c->spd) m = imago_ctx::SpdMat{};
     c->has_system = false;
 }
 
@@ -2056,6 +2070,14 @@ extern "C" int imago_elecstat(imago_ctx *c, double log_elec_thresh, double *pot_
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[11] = ms;
+    // potAlphaOverlap stays on the device for imago_pot_solve
+    imago_ctx::SpdMat &m = c->spd[0];
+    for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.s})
+        if (q) dev_release(c, q);
+    m = imago_ctx::SpdMat{};
+    RCHK(dev_alloc(c, &m.A, PP));
+    HIPCHK(hipMemcpyAsync(m.A, buf, PP * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     dev_release(c, buf);
     dev_release(c, (void *)dsizes);
     dev_release(c, A.msep);
@@ -2305,6 +2327,13 @@ extern "C" int imago_xc_mesh(imago_ctx *c, const imago_xc_mesh_params *params,
             if (x) dev_release(c, x);
     c->xc_cache.clear();
     c->xc_cached = false;
+    {
+        // an overlap of the previous mesh no longer applies
+        imago_ctx::SpdMat &m = c->spd[1];
+        for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.s})
+            if (q) dev_release(c, q);
+        m = imago_ctx::SpdMat{};
+    }
     c->xc_ready = false;
     c->xcp = p;
     c->xc_th = log_elec_thresh;
@@ -2552,8 +2581,16 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
     for (hipEvent_t e : gev) (void)hipEventDestroy(e);
     c->timing[15] = gms;
     c->xc_gemm_flops = gflop;
-    for (void *q : {(void *)C, (void *)A, (void *)Cs, (void *)nz, (void *)rows, (void *)rank,
-                    (void *)cnt, (void *)slot, (void *)first, (void *)err})
+    {
+        // exchCorrOverlap stays on the device for imago_pot_solve
+        imago_ctx::SpdMat &m = c->spd[1];
+        for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.s})
+            if (q) dev_release(c, q);
+        m = imago_ctx::SpdMat{};
+        m.A = C;
+    }
+    for (void *q : {(void *)A, (void *)Cs, (void *)nz, (void *)rows, (void *)rank, (void *)cnt,
+                    (void *)slot, (void *)first, (void *)err})
         if (q) dev_release(c, q);
     if (herr)
         return fail(IMAGO_E_HIP, "exchange-correlation overlap: a point touched a term outside "
@@ -2677,6 +2714,134 @@ extern "C" int imago_xc_potential(imago_ctx *c, const double *rho_total, const d
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[16] = ms;
     for (void *x : {(void *)rho, (void *)pot, (void *)v, (void *)part}) dev_release(c, x);
+    return IMAGO_OK;
+}
+
+// ---- potential-side SPD solves (solveDPOSVX, interfaces.F90:236-293) ----
+
+// As(i, j) = s_i A(i, j) s_j on the upper triangle (j >= i), 0 below
+__global__ __launch_bounds__(256) void k_spd_scale(const double *__restrict__ A,
+                                                   const double *__restrict__ s, int n,
+                                                   double *__restrict__ As)
+{
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (size_t)n * n) return;
+    const int i = (int)(g % n), j = (int)(g / n);
+    As[g] = i <= j ? s[i] * A[g] * s[j] : 0.0;
+}
+
+// B(i, r) *= s_i for an n x nrhs block
+__global__ __launch_bounds__(256) void k_spd_rows(double *__restrict__ B,
+                                                  const double *__restrict__ s, int n, int nrhs)
+{
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (size_t)n * nrhs) return;
+    B[g] *= s[g % n];
+}
+
+__global__ __launch_bounds__(256) void k_axpy(double *__restrict__ y,
+                                              const double *__restrict__ x, size_t n)
+{
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g < n) y[g] += x[g];
+}
+
+// The equilibration of dposvx('E') (dpoequ / dlaqsy): s_i = 1/sqrt(a_ii),
+// applied when min(s)/max(s) < 0.1 or the largest diagonal is outside
+// [tiny/eps, eps/tiny]; then the Cholesky factor of the (scaled) upper
+// triangle.  info as dpotrf (i > 0: leading minor i not positive), or
+// i for a_ii <= 0 as dpoequ.
+static int spd_factor(imago_ctx *c, imago_ctx::SpdMat &m, int n)
+{
+    std::vector<double> diag(n), sc(n, 1.0);
+    HIPCHK(hipMemcpy2DAsync(diag.data(), sizeof(double), m.A, (size_t)(n + 1) * sizeof(double),
+                            sizeof(double), n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double smin = diag[0], amax = diag[0];
+    for (int i = 0; i < n; ++i) {
+        smin = std::min(smin, diag[i]);
+        amax = std::max(amax, diag[i]);
+    }
+    m.info = 0;
+    for (int i = 0; i < n; ++i)
+        if (!(diag[i] > 0.0)) {
+            m.info = i + 1;
+            m.factored = true;
+            return IMAGO_OK;
+        }
+    const double eps = 1.1102230246251565e-16, tiny = 2.2250738585072014e-308;
+    const double small = tiny / eps, large = 1.0 / small;
+    const double scond = std::sqrt(smin) / std::sqrt(amax);
+    const bool equed = scond < 0.1 || amax < small || amax > large;
+    if (equed)
+        for (int i = 0; i < n; ++i) sc[i] = 1.0 / std::sqrt(diag[i]);
+    const size_t nn = (size_t)n * n;
+    RCHK(dev_alloc(c, &m.As, nn));
+    RCHK(dev_alloc(c, &m.U, nn));
+    const double *ds = nullptr;
+    RCHK(dev_upload(c, &ds, sc));
+    m.s = const_cast<double *>(ds);
+    hipLaunchKernelGGL(k_spd_scale, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream,
+                       m.A, m.s, n, m.As);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m.U, m.As, nn * 8, hipMemcpyDeviceToDevice, c->stream));
+    int *dinfo = nullptr;
+    RCHK(dev_alloc(c, &dinfo, 1));
+    BLASCHK(rocsolver_dpotrf(c->blas, rocblas_fill_upper, n, m.U, n, dinfo));
+    int hinfo = 0;
+    HIPCHK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, dinfo);
+    m.info = hinfo;
+    m.factored = true;
+    return IMAGO_OK;
+}
+
+extern "C" int imago_pot_solve(imago_ctx *c, int matrix, int nrhs, double *b, int *info)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (matrix != IMAGO_SOLVE_POT_OVERLAP && matrix != IMAGO_SOLVE_XC_OVERLAP)
+        return fail(IMAGO_E_ARG, "unknown matrix %d", matrix);
+    if (nrhs < 1 || !b) return fail(IMAGO_E_ARG, "no right-hand sides");
+    imago_ctx::SpdMat &m = c->spd[matrix == IMAGO_SOLVE_POT_OVERLAP ? 0 : 1];
+    if (!m.A)
+        return fail(IMAGO_E_STATE, "%s first", matrix == IMAGO_SOLVE_POT_OVERLAP
+                                                  ? "imago_elecstat" : "imago_xc_overlap");
+    HIPCHK(hipSetDevice(c->device));
+    const int n = c->h.pot_dim;
+    if (!m.factored) RCHK(spd_factor(c, m, n));
+    if (info) *info = m.info;
+    if (m.info != 0)
+        return fail(IMAGO_E_NOTPD, "the overlap is not positive definite (minor %d)", m.info);
+    const size_t nb = (size_t)n * nrhs;
+    double *x = nullptr, *r = nullptr;
+    RCHK(dev_alloc(c, &x, nb));
+    RCHK(dev_alloc(c, &r, nb));
+    const unsigned g = (unsigned)((nb + 255) / 256);
+    // x = S (As^-1 (S b)), then two steps of refinement on the scaled system
+    HIPCHK(hipMemcpyAsync(x, b, nb * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, x, m.s, n, nrhs);
+    HIPCHK(hipGetLastError());
+    double *bs = nullptr;
+    RCHK(dev_alloc(c, &bs, nb));
+    HIPCHK(hipMemcpyAsync(bs, x, nb * 8, hipMemcpyDeviceToDevice, c->stream));
+    BLASCHK(rocsolver_dpotrs(c->blas, rocblas_fill_upper, n, nrhs, m.U, n, x, n));
+    for (int it = 0; it < 2; ++it) {
+        const double one = 1.0, mone = -1.0;
+        HIPCHK(hipMemcpyAsync(r, bs, nb * 8, hipMemcpyDeviceToDevice, c->stream));
+        BLASCHK(rocblas_dsymm(c->blas, rocblas_side_left, rocblas_fill_upper, n, nrhs, &mone,
+                              m.As, n, x, n, &one, r, n));
+        BLASCHK(rocsolver_dpotrs(c->blas, rocblas_fill_upper, n, nrhs, m.U, n, r, n));
+        hipLaunchKernelGGL(k_axpy, dim3(g), dim3(256), 0, c->stream, x, r, nb);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, x, m.s, n, nrhs);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b, x, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, x);
+    dev_release(c, r);
+    dev_release(c, bs);
     return IMAGO_OK;
 }
 

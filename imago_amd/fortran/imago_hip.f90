@@ -13,6 +13,7 @@ module O_ImagoHIP
    implicit none
 
    integer(c_int), parameter :: IMAGO_OK = 0
+   integer(c_int), parameter :: IMAGO_SOLVE_POT_OVERLAP = 1, IMAGO_SOLVE_XC_OVERLAP = 2
    integer(c_int), parameter :: IMAGO_OL = 1, IMAGO_KE = 2, IMAGO_NP = 3, &
          & IMAGO_EP = 4, IMAGO_EP_DIRECT = 5
 
@@ -230,6 +231,16 @@ module O_ImagoHIP
          real(c_double), intent(in) :: rhoTotal(*), rhoCore(*)
          real(c_double), intent(out) :: exchCorrPot(*)
       end function imago_xc_potential
+
+      ! solveDPOSVX with potAlphaOverlap (1) or exchCorrOverlap (2) on the device
+      integer(c_int) function imago_pot_solve(ctx, matrix, nrhs, b, info) &
+            & bind(C, name="imago_pot_solve")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: matrix, nrhs
+         real(c_double), intent(inout) :: b(*)
+         integer(c_int), intent(out) :: info
+      end function imago_pot_solve
 
       integer(c_int) function imago_xc_counts(ctx, rayPoints, gemmFlops) &
             & bind(C, name="imago_xc_counts")

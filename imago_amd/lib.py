@@ -40,7 +40,7 @@ EXPORTS = [
     "imago_secular", "imago_get_eigenvectors", "imago_density_traces", "imago_zhegv",
     "imago_dsygv", "imago_last_timing", "imago_ep_counts", "imago_elecstat", "imago_residual_q",
     "imago_xc_mesh", "imago_xc_rho_op", "imago_xc_overlap", "imago_xc_counts",
-    "imago_xc_potential",
+    "imago_xc_potential", "imago_pot_solve",
 ]
 
 
@@ -90,6 +90,7 @@ def load():
     L.imago_xc_overlap.argtypes = [ctypes.c_void_p, _dp]
     L.imago_xc_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), _dp]
     L.imago_xc_potential.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    L.imago_pot_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _ip]
     _lib = L
     return L
 
@@ -296,6 +297,16 @@ class Context:
         out = np.zeros((3, self.pot_dim))
         _check(load().imago_xc_potential(self._h, _p(r1), None if r3 is None else _p(r3), _p(out)))
         return out
+
+    def pot_solve(self, matrix, b):
+        """solveDPOSVX with potAlphaOverlap (matrix='pot') or exchCorrOverlap
+        ('xc') on the device; b [potDim] or [potDim, nrhs]."""
+        which = {"pot": 1, "xc": 2}[matrix]
+        x = np.asfortranarray(np.array(b, dtype=np.float64, copy=True))
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        info = ctypes.c_int()
+        _check(load().imago_pot_solve(self._h, which, nrhs, _p(x), ctypes.byref(info)))
+        return x
 
     def xc_counts(self):
         """(all ray points, flops of the last overlap's GEMMs)."""

@@ -300,6 +300,22 @@ int imago_xc_overlap(imago_ctx *ctx, double *exch_corr_overlap);
 int imago_xc_potential(imago_ctx *ctx, const double *rho_total, const double *rho_core,
                        double *exch_corr_pot);
 
+/* ---- potential-side solves ---------------------------------------------- *
+ * solveDPOSVX (interfaces.F90:236-293) with one of the potential-side
+ * overlaps, which stay on the device after they are built:
+ * IMAGO_SOLVE_POT_OVERLAP, potAlphaOverlap of the last imago_elecstat (the
+ * charge fit and the electrostatic potential of makeSCFPot,
+ * potentialUpdate.F90:236, 786), or IMAGO_SOLVE_XC_OVERLAP, exchCorrOverlap of
+ * the last imago_xc_overlap (:1010).  Upper triangle, equilibrated as
+ * dposvx('E') does, Cholesky factor made on the first solve and kept (the
+ * matrices do not change between SCF iterations), two steps of iterative
+ * refinement.  b is potDim x nrhs column-major, overwritten by the
+ * solution; *info as dposvx (i > 0: not positive definite, with
+ * IMAGO_E_NOTPD). */
+#define IMAGO_SOLVE_POT_OVERLAP 1
+#define IMAGO_SOLVE_XC_OVERLAP 2
+int imago_pot_solve(imago_ctx *ctx, int matrix, int nrhs, double *b, int *info);
+
 /* Sizes of the exchange-correlation mesh (after imago_xc_mesh): all ray
  * points of all sites, and the flops of the GEMMs of the last
  * imago_xc_overlap (0 before one).  Either output may be NULL.  For

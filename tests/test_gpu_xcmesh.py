@@ -104,3 +104,35 @@ def test_xc_potential(gpu_ctx, oracle_mod, name):
     got0 = gpu_ctx.xc_potential(rho)          # no core density: its row is 0
     assert np.all(got0[2] == 0.0)
     assert rel_err(got0[:2], got[:2]) < TOL
+
+
+@pytest.mark.parametrize("name", ["small", "si64"])
+def test_pot_solve(gpu_ctx, oracle_mod, name):
+    """solveDPOSVX on the device against LAPACK dposvx('E', 'U'): the
+    backward error at rounding level, and the solutions within the
+    conditioning (the overlaps reach 1e10 on Si64)."""
+    from scipy.linalg import lapack
+    if name == "small":
+        s, mesh = configs.small_test_system(), XcMesh()
+    else:
+        s, mesh = configs.si64_supercell(), XcMesh(num_sample_vectors=8)
+    print("[gpu] pot solve", name, flush=True)
+    gpu_ctx.set_system(s)
+    es = gpu_ctx.elecstat()
+    gpu_ctx.xc_mesh(mesh)
+    xo = gpu_ctx.xc_overlap()
+    rng = np.random.default_rng(3)
+    for which, A in (("pot", es["potAlphaOverlap"]), ("xc", xo)):
+        U = np.triu(A)
+        Af = U + np.triu(A, 1).T
+        b = rng.normal(size=(s.pot_dim, 2))
+        x = gpu_ctx.pot_solve(which, b)
+        res = np.abs(Af @ x - b).max() / (np.abs(Af).max() * np.abs(x).max())
+        assert res < 1e-13, (which, res)
+        out = lapack.dposvx(U, b, fact="E", lower=0)
+        xr, rcond = out[5], out[6]
+        cond = 1.0 / max(rcond, 1e-300)
+        err = np.abs(x - xr).max() / np.abs(xr).max()
+        assert err < max(1e-12, 1e-15 * cond), (which, err, cond)
+        x1 = gpu_ctx.pot_solve(which, b[:, 0])   # the kept factor, one right-hand side
+        assert np.allclose(x1, x[:, 0], rtol=1e-12, atol=1e-12 * np.abs(x).max())
