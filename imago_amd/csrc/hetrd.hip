@@ -25,8 +25,17 @@ hipError_t trd_one(const TrdLaunch &L)
     const size_t nn = (size_t)L.n * L.n;
     for (int m0 = 0; m0 < L.nb; m0 += fit) {
         const int nb = std::min(fit, L.nb - m0);
-        st = hipMemsetAsync(L.bar, 0, sizeof(unsigned) * ((size_t)nb * P * kTrdFlagStride + 1),
-                            L.stream);
+        // every exchange slot empty, the abort word clear
+        const size_t words = sizeof(T) / 4;
+        st = hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(L.rbuf), (int)kTrdEmptyWord,
+                               nb * nn * words, L.stream);
+        if (st == hipSuccess)
+            st = hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(L.pbuf), (int)kTrdEmptyWord,
+                                   nb * nn * words, L.stream);
+        if (st == hipSuccess)
+            st = hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(L.dbuf), (int)kTrdEmptyWord,
+                                   (size_t)nb * L.n * kTrdMaxGroups * words, L.stream);
+        if (st == hipSuccess) st = hipMemsetAsync(L.abortw, 0, sizeof(unsigned), L.stream);
         if (st != hipSuccess) return st;
         TrdArgs<T> A;
         A.n = L.n;
@@ -37,10 +46,10 @@ hipError_t trd_one(const TrdLaunch &L)
         A.d = L.d + (size_t)m0 * L.n;
         A.e = L.e + (size_t)m0 * L.n;
         A.tau = static_cast<T *>(L.tau) + (size_t)m0 * L.n;
-        A.vbuf = static_cast<T *>(L.vbuf);
+        A.rbuf = static_cast<T *>(L.rbuf);
         A.pbuf = static_cast<T *>(L.pbuf);
         A.dbuf = static_cast<T *>(L.dbuf);
-        A.bar = L.bar;
+        A.abortw = L.abortw;
         A.err = L.err;
         A.spin_limit = 2000000;
         void *args[] = {&A};

@@ -12,12 +12,12 @@
 //   bit-identically;
 //   forms p = tau A v on its own rows and the partial p^H v;
 //   publishes p, the partial and (the owner of row j+1) row j+1;
-//   after the barrier reads p, the partials and row j+1, forms
+//   waits for p, the partials and row j+1 of the others, forms
 //   w = p - tau/2 (p^H v) v, applies A -= v w^H + w v^H to its rows and
 //   the same update to row j+1, the next pivot row.
-// Only p, the dot partials and one row per step go through memory, double
-// buffered by step parity.  The tridiagonal (d, e), tau and the reflectors
-// (LAPACK's lower layout) are written by work-group 0.
+// Only p, the dot partials and one row per step go through memory.  The
+// tridiagonal (d, e) and tau are written by work-group 0, the reflectors
+// (LAPACK's lower layout) by the work-group owning their rows.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -76,6 +76,40 @@ __device__ __forceinline__ double2 sc_scale(double s, double2 a)
 {
     return make_double2(s * a.x, s * a.y);
 }
+// acc += a * b as fused multiply-adds
+__device__ __forceinline__ void sc_fma_acc(double &acc, double a, double b) { acc = fma(a, b, acc); }
+__device__ __forceinline__ void sc_fma_acc(double2 &acc, double2 a, double2 b)
+{
+    acc.x = fma(-a.y, b.y, fma(a.x, b.x, acc.x));
+    acc.y = fma(a.y, b.x, fma(a.x, b.y, acc.y));
+}
+// x - u conj(w) - y conj(v) as fused multiply-adds (the rank-2 update)
+__device__ __forceinline__ double sc_rank2(double x, double u, double w, double y, double v)
+{
+    return fma(-y, v, fma(-u, w, x));
+}
+__device__ __forceinline__ double2 sc_rank2(double2 x, double2 u, double2 w, double2 y, double2 v)
+{
+    double re = fma(-u.x, w.x, x.x);
+    re = fma(-u.y, w.y, re);
+    re = fma(-y.x, v.x, re);
+    re = fma(-y.y, v.y, re);
+    double im = fma(-u.y, w.x, x.y);
+    im = fma(u.x, w.y, im);
+    im = fma(-y.y, v.x, im);
+    im = fma(y.x, v.y, im);
+    return make_double2(re, im);
+}
+// c ? x : y, component by component (keeps register arrays in registers)
+__device__ __forceinline__ double sc_sel(bool c, double x, double y) { return c ? x : y; }
+__device__ __forceinline__ double2 sc_sel(bool c, double2 x, double2 y)
+{
+    return make_double2(c ? x.x : y.x, c ? x.y : y.y);
+}
+// An empty asm that takes x as read and written: the load producing x is
+// issued where it stands and not moved into a later conditional block.
+__device__ __forceinline__ void sc_pin(double &x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void sc_pin(double2 &x) { asm volatile("" : "+v"(x.x), "+v"(x.y)); }
 __device__ __forceinline__ double sc_shfl_xor(double v, int m) { return __shfl_xor(v, m); }
 __device__ __forceinline__ double2 sc_shfl_xor(double2 v, int m)
 {
@@ -110,6 +144,24 @@ __device__ __forceinline__ double2 pub_load(const double2 *p)
     return make_double2(pub_word_load(q), pub_word_load(q + 1));
 }
 
+// Every exchanged value has a slot of its own per column step, filled with
+// kTrdEmpty before the launch: a reader waits on the words it needs until
+// none holds the fill, so a value and its arrival travel together and no
+// flag or barrier is needed.  kTrdEmpty is a signalling NaN; arithmetic
+// only produces quiet NaNs, so no computed value equals it (an input entry
+// with exactly that bit pattern would make the wait time out).
+constexpr unsigned kTrdEmptyWord = 0xFFF5A5A5u;
+constexpr unsigned long long kTrdEmpty = 0xFFF5A5A5FFF5A5A5ull;
+__device__ __forceinline__ bool trd_word_ready(double v)
+{
+    return (unsigned long long)__double_as_longlong(v) != kTrdEmpty;
+}
+__device__ __forceinline__ bool trd_ready(double v) { return trd_word_ready(v); }
+__device__ __forceinline__ bool trd_ready(double2 v)
+{
+    return trd_word_ready(v.x) && trd_word_ready(v.y);
+}
+
 template <class T>
 struct TrdArgs {
     int n, P;             // order; work-groups per matrix (ceil(n / 16))
@@ -118,61 +170,13 @@ struct TrdArgs {
     T *V;                 // [nb][n * n] reflectors below the diagonal, zero above
     double *d, *e;        // [nb][n] diagonal, subdiagonal (e[n-1] = 0)
     T *tau;               // [nb][n] (tau[n-1] = 0)
-    T *vbuf;              // [nb][2][n] published pivot rows
-    T *pbuf;              // [nb][2][n] published p
-    T *dbuf;              // [nb][2][kTrdMaxGroups] published dot partials
-    unsigned *bar;        // [nb][P][kTrdFlagStride] barrier flags, then one abort
-                          // word; zero at launch
-    int *err;             // set to 1 when a barrier wait times out
-    int spin_limit;       // polls before a barrier wait gives up
+    T *rbuf;              // [nb][n][n] step j: pivot row j+1, columns j+1 ..
+    T *pbuf;              // [nb][n][n] step j: p, rows j+1 ..
+    T *dbuf;              // [nb][n][kTrdMaxGroups] step j: dot partials
+    unsigned *abortw;     // zero at launch; set when a wait gives up
+    int *err;             // set to 1 when a wait times out
+    int spin_limit;       // polls before a wait gives up
 };
-
-// Barrier of the P work-groups of one matrix without a shared counter:
-// work-group w stores the number of the step it has published into its own
-// flag (one 64-byte line each) after every thread's published stores have
-// completed; the lanes of wave 0 poll the P flags of the matrix in
-// parallel (two per lane above 64) until all have reached the step.  Every
-// 64 polls the abort word is read; a wait longer than limit polls sets it
-// (every later wait of the launch then passes within 64 polls) and *err,
-// so the kernel always ends.
-constexpr int kTrdFlagStride = 16;   // unsigned per flag: one 64-byte line
-__device__ __forceinline__ void trd_barrier(unsigned *flags, int w, int P, unsigned step,
-                                            unsigned *abortw, int *err, int limit)
-{
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    const int t = threadIdx.x;
-    if (t == 0)
-        __hip_atomic_store(flags + (size_t)w * kTrdFlagStride, step, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    if (t < 64) {
-        unsigned *f0 = flags + (size_t)(t < P ? t : 0) * kTrdFlagStride;
-        unsigned *f1 = flags + (size_t)(t + 64 < P ? t + 64 : 0) * kTrdFlagStride;
-        const bool two = P > 64;
-        int polls = 0;
-        while (true) {
-            const unsigned v0 = __hip_atomic_load(f0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned v1 =
-                two ? __hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : step;
-            if (__all(v0 >= step && v1 >= step)) break;
-            if ((++polls & 63) == 0) {
-                if (__hip_atomic_load(abortw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
-                    break;
-                if (polls > limit) {
-                    if (t == 0) {
-                        __hip_atomic_store(abortw, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                    break;
-                }
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
 
 // wave sum of a double; every lane gets the total
 __device__ __forceinline__ double trd_wave_sum(double v)
@@ -183,16 +187,27 @@ __device__ __forceinline__ double trd_wave_sum(double v)
 }
 
 // grid = nb * P work-groups of kTrdThreads, all co-resident (cooperative
-// launch); n <= 32 * S and P <= kTrdMaxGroups.
+// launch); n <= 32 * S and P <= kTrdMaxGroups.  One column step j:
+//   the owner of row j+1 publishes it (staged in LDS at the end of step
+//   j-1, so the stores leave before the reflector is formed);
+//   every work-group forms the reflector from the pivot row j in LDS and the
+//   norm partials of step j-1, p = tau A v on its rows and its part of
+//   p^H v, and publishes both;
+//   every thread waits for its columns of p and of row j+1, wave 0 for the
+//   dot partials (summed in a fixed order, the same in every work-group);
+//   the rank-2 update of the work-group's rows and of row j+1, whose norm
+//   partials and (its owner) row j+2 are staged for the next step.
 template <class T, int S>
 __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
 {
     constexpr int NW = kTrdThreads / 64;
+    constexpr int MC = (kTrdGroups * S + kTrdThreads - 1) / kTrdThreads;   // columns a thread waits on
+    static_assert(2 * MC <= 30, "wait mask");
     __shared__ T vl[kTrdGroups * S];   // v of this step
     __shared__ T pl[kTrdGroups * S];   // p of this step
     __shared__ T rl[kTrdGroups * S];   // the pivot row
+    __shared__ T stg[kTrdGroups * S];  // the next row to publish (its owner)
     __shared__ T red[NW][kTrdRows];
-    __shared__ T prow[kTrdRows];
     __shared__ double nrm[NW];         // partial |x(2:)|^2 per wave
     __shared__ T dot;                  // p^H v
     const int n = A_.n, P = A_.P;
@@ -200,14 +215,14 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     const int t = threadIdx.x, r = t & (kTrdRows - 1), g = t >> 4;
     const int lane = t & 63, wave = t >> 6;
     const int i = kTrdRows * w + r;
+    // the largest column group of this wave's lanes (wave-uniform)
+    const int gtop = 4 * __builtin_amdgcn_readfirstlane(wave) + 3;
     const size_t nn = (size_t)n * n;
     const T *A = A_.A + m * nn;
     T *V = A_.V + m * nn;
-    T *rbuf = A_.vbuf + (size_t)m * 2 * n, *pbuf = A_.pbuf + (size_t)m * 2 * n;
-    T *dbuf = A_.dbuf + (size_t)m * 2 * kTrdMaxGroups;
-    unsigned *flags = A_.bar + (size_t)m * P * kTrdFlagStride;
-    unsigned *abortw = A_.bar + (size_t)A_.nb_launch * P * kTrdFlagStride;
-    const bool out = w == 0;   // work-group 0 writes d, e, tau and V
+    T *rbuf = A_.rbuf + m * nn, *pbuf = A_.pbuf + m * nn;
+    T *dbuf = A_.dbuf + (size_t)m * n * kTrdMaxGroups;
+    const bool out = w == 0;   // work-group 0 writes d, e and tau
 
     T a[S];
 #pragma unroll
@@ -215,20 +230,36 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
         const int c = g + kTrdGroups * s;
         a[s] = (i < n && c < n) ? A[(size_t)c * n + i] : sc_zero<T>();
     }
-    // pivot row 0: A(0, c) for every column
-    for (int c = t; c < n; c += kTrdThreads) rl[c] = A[(size_t)c * n];
+    // pivot row 0: A(0, c) for every column; its norm partials; row 1 staged
+    double x2 = 0.0;
+    for (int c = t; c < n; c += kTrdThreads) {
+        const T x = A[(size_t)c * n];
+        rl[c] = x;
+        if (c >= 2) x2 += sc_abs2(x);
+    }
+    x2 = trd_wave_sum(x2);
+    if (lane == 0) nrm[wave] = x2;
+    if (w == 0 && r == 1) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int c = g + kTrdGroups * s;
+            if (c >= 1 && c < n) stg[c] = a[s];
+        }
+    }
+    // The loads of a[] complete here.  Left pending on some paths, they make
+    // the compiler wait for every outstanding memory access at each a[s]'s
+    // first use inside the column loop, on every step; vmcnt also counts
+    // stores, so the owner of the pivot row would stall there until its
+    // device-scope row stores were acknowledged.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    unsigned nstep = 0;
     for (int j = 0; j + 1 < n; ++j) {
-        const int slot = j & 1;
-        // ---- reflector from conj(row j): |x(2:)|^2 over the work-group ----
-        double x2 = 0.0;
-        for (int c = j + 2 + t; c < n; c += kTrdThreads) x2 += sc_abs2(rl[c]);
-        x2 = trd_wave_sum(x2);
-        if (lane == 0) nrm[wave] = x2;
-        __syncthreads();
-        // every thread forms the reflector from the same partials, so all
-        // work-groups and threads hold bit-identical beta, tau and scale
+        T *ps = pbuf + (size_t)j * n, *rs = rbuf + (size_t)j * n;
+        T *ds = dbuf + (size_t)j * kTrdMaxGroups;
+        if (w == ((j + 1) >> 4))
+            for (int c = j + 1 + t; c < n; c += kTrdThreads) pub_store(rs + c, stg[c]);
+        // ---- the reflector; every thread forms it from the same partials,
+        // so all work-groups and threads hold bit-identical beta, tau, scale
         T tau, scale;
         double beta;
         {
@@ -249,33 +280,47 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
                 scale = sc_make<T>(zr / den, -zi / den);
             }
         }
-        // v into LDS for the dot, the update and the next pivot row (read
-        // after the next work-group barrier); the hemv forms v from rl
-        for (int c = j + 1 + t; c < n; c += kTrdThreads) {
-            const T v = c == j + 1 ? sc_one<T>() : sc_mul(scale, sc_conj(rl[c]));
-            vl[c] = v;
-            if (out) V[(size_t)j * n + c] = v;
-        }
+        // v into LDS for the hemv, the dot and the update
+        for (int c = j + 1 + t; c < n; c += kTrdThreads)
+            vl[c] = c == j + 1 ? sc_one<T>() : sc_mul(scale, sc_conj(rl[c]));
         if (out && t == 0) {
             A_.tau[(size_t)m * n + j] = tau;
             A_.e[(size_t)m * n + j] = beta;
             A_.d[(size_t)m * n + j] = sc_re(rl[j]);
         }
+        __syncthreads();
 
         // ---- p = tau A v on this work-group's rows, partial p^H v ----
+        // Columns go in batches of four without branches (masked by
+        // selects, so their LDS loads are in flight together); a wave skips
+        // a batch whose columns all lie at or before j, a work-group whose
+        // rows all do skips the product.
         const bool row_on = i > j && i < n;
-        T sp = sc_zero<T>();
-        if (row_on) {
+        const bool wg_on = kTrdRows * w + kTrdRows - 1 > j;
+        T sp0 = sc_zero<T>(), sp1 = sc_zero<T>();
+        if (wg_on) {
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const int c = g + kTrdGroups * s;
-                if (c > j && c < n) {
-                    const T vc = c == j + 1 ? sc_one<T>() : sc_mul(scale, sc_conj(rl[c]));
-                    sp = sc_add(sp, sc_mul(a[s], vc));
+            for (int s0 = 0; s0 < S; s0 += 4) {
+                if (gtop + kTrdGroups * (s0 + 3) > j) {
+                    T vc[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) vc[q] = vl[g + kTrdGroups * (s0 + q)];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) sc_pin(vc[q]);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int c = g + kTrdGroups * (s0 + q);
+                        vc[q] = sc_sel(c > j && c < n, vc[q], sc_zero<T>());
+                    }
+                    sc_fma_acc(sp0, a[s0], vc[0]);
+                    sc_fma_acc(sp1, a[s0 + 1], vc[1]);
+                    sc_fma_acc(sp0, a[s0 + 2], vc[2]);
+                    sc_fma_acc(sp1, a[s0 + 3], vc[3]);
                 }
-                if ((s & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
+        T sp = row_on ? sc_add(sp0, sp1) : sc_zero<T>();
         sp = sc_add(sp, sc_shfl_xor(sp, 16));
         sp = sc_add(sp, sc_shfl_xor(sp, 32));
         if (lane < kTrdRows) red[wave][lane] = sp;
@@ -284,65 +329,134 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
             T p = sc_zero<T>();
             for (int q = 0; q < NW; ++q) p = sc_add(p, red[q][t]);
             p = row_on ? sc_mul(tau, p) : sc_zero<T>();
-            prow[t] = p;
-            if (row_on) pub_store(pbuf + slot * n + i, p);
+            if (row_on) pub_store(ps + i, p);
             // conj(p_i) v_i over the 16 rows, the same tree in every work-group
             T dp = row_on ? sc_mulc(vl[i], p) : sc_zero<T>();
             for (int o = 8; o >= 1; o >>= 1) dp = sc_add(dp, sc_shfl_xor(dp, o));
-            if (t == 0) pub_store(dbuf + slot * kTrdMaxGroups + w, dp);
+            if (t == 0) pub_store(ds + w, dp);
         }
-        // the owner of row j+1 publishes it (columns j+1 ..), staged in LDS
-        // so that every thread issues a few of the stores
-        if (w == ((j + 1) >> 4)) {
-            if (r == ((j + 1) & (kTrdRows - 1))) {
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int c = g + kTrdGroups * s;
-                    if (c > j && c < n) pl[c] = a[s];
-                }
-            }
-            __syncthreads();
-            for (int c = j + 1 + t; c < n; c += kTrdThreads) pub_store(rbuf + slot * n + c, pl[c]);
-        }
-        trd_barrier(flags, w, P, ++nstep, abortw, A_.err, A_.spin_limit);
 
-        // ---- w = p - tau/2 (p^H v) v; A -= v w^H + w v^H; next pivot row ----
-        for (int c = j + 1 + t; c < n; c += kTrdThreads) {
-            pl[c] = pub_load(pbuf + slot * n + c);
-            rl[c] = pub_load(rbuf + slot * n + c);
-        }
-        if (wave == 0) {
-            // partials of work-groups lane and lane + 64, then a fixed tree
-            T x = lane < P ? pub_load(dbuf + slot * kTrdMaxGroups + lane) : sc_zero<T>();
-            if (lane + 64 < P) x = sc_add(x, pub_load(dbuf + slot * kTrdMaxGroups + lane + 64));
-            for (int o = 32; o >= 1; o >>= 1) x = sc_add(x, sc_shfl_xor(x, o));
-            if (lane == 0) dot = x;
+        // ---- wait for p and row j+1 (every thread its columns) and the
+        // dot partials (wave 0): all loads of a round in flight together
+        {
+            unsigned pend = 0;
+#pragma unroll
+            for (int k = 0; k < MC; ++k)
+                if (j + 1 + t + kTrdThreads * k < n) pend |= 3u << (2 * k);
+            if (wave == 0) {
+                if (lane < P) pend |= 1u << 30;
+                if (lane + 64 < P) pend |= 1u << 31;
+            }
+            T d0 = sc_zero<T>(), d1 = sc_zero<T>();
+            int polls = 0;
+            while (pend) {
+                T gp[MC], gr[MC], g0 = sc_zero<T>(), g1 = sc_zero<T>();
+#pragma unroll
+                for (int k = 0; k < MC; ++k) {
+                    const int c = j + 1 + t + kTrdThreads * k;
+                    if (pend & (1u << (2 * k))) gp[k] = pub_load(ps + c);
+                    if (pend & (2u << (2 * k))) gr[k] = pub_load(rs + c);
+                }
+                if (pend & (1u << 30)) g0 = pub_load(ds + lane);
+                if (pend & (1u << 31)) g1 = pub_load(ds + lane + 64);
+#pragma unroll
+                for (int k = 0; k < MC; ++k) {
+                    const int c = j + 1 + t + kTrdThreads * k;
+                    if ((pend & (1u << (2 * k))) && trd_ready(gp[k])) {
+                        pl[c] = gp[k];
+                        pend &= ~(1u << (2 * k));
+                    }
+                    if ((pend & (2u << (2 * k))) && trd_ready(gr[k])) {
+                        rl[c] = gr[k];
+                        pend &= ~(2u << (2 * k));
+                    }
+                }
+                if ((pend & (1u << 30)) && trd_ready(g0)) {
+                    d0 = g0;
+                    pend &= ~(1u << 30);
+                }
+                if ((pend & (1u << 31)) && trd_ready(g1)) {
+                    d1 = g1;
+                    pend &= ~(1u << 31);
+                }
+                if (!pend) break;
+                if ((++polls & 63) == 0) {
+                    if (__hip_atomic_load(A_.abortw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                        break;
+                    if (polls > A_.spin_limit) {
+                        __hip_atomic_store(A_.abortw, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_or(A_.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            // Every path out of the loop has waited for its loads; saying so
+            // keeps the compiler from waiting again (on this step's stores
+            // too) before each reuse of their registers in the next step.
+            __builtin_amdgcn_s_waitcnt(0);
+            if (wave == 0) {
+                // partials of work-groups lane and lane + 64, then a fixed tree
+                T x = sc_add(d0, d1);
+                for (int o = 32; o >= 1; o >>= 1) x = sc_add(x, sc_shfl_xor(x, o));
+                if (lane == 0) dot = x;
+            }
         }
         __syncthreads();
+
+        // ---- w = p - tau/2 (p^H v) v in place of p; A -= v w^H + w v^H;
+        // next pivot row ----
         const T a2 = sc_mul(sc_scale(-0.5, tau), dot);
-        if (row_on) {
-            const T vi = vl[i];
-            const T wi = sc_add(prow[r], sc_mul(a2, vi));
+        for (int c = j + 1 + t; c < n; c += kTrdThreads) pl[c] = sc_add(pl[c], sc_mul(a2, vl[c]));
+        // Each work-group writes the part of V on its own rows here, a step
+        // before the next wait on its memory accesses.
+        if (t < kTrdRows && row_on) V[(size_t)j * n + i] = vl[i];
+        __syncthreads();
+        if (wg_on && row_on) {
+            const T vi = vl[i], wi = pl[i];
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const int c = g + kTrdGroups * s;
-                if (c > j && c < n) {
-                    const T vc = vl[c];
-                    const T wc = sc_add(pl[c], sc_mul(a2, vc));
-                    a[s] = sc_sub(a[s], sc_add(sc_mulc(vi, wc), sc_mulc(wi, vc)));
+            for (int s0 = 0; s0 < S; s0 += 4) {
+                if (gtop + kTrdGroups * (s0 + 3) > j) {
+                    T vc[4], wc[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int c = g + kTrdGroups * (s0 + q);
+                        vc[q] = vl[c];
+                        wc[q] = pl[c];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        sc_pin(vc[q]);
+                        sc_pin(wc[q]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int c = g + kTrdGroups * (s0 + q);
+                        const T x = sc_rank2(a[s0 + q], vi, wc[q], wi, vc[q]);
+                        a[s0 + q] = sc_sel(c > j && c < n, x, a[s0 + q]);
+                    }
                 }
-                if ((s & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
-        // row j+1 of the updated matrix (v_{j+1} = 1): the next pivot row
+        // the owner of row j+2 stages it for the next step's publication
+        // (columns up to j+1 are staged too and never published)
+        if (w == ((j + 2) >> 4) && r == ((j + 2) & (kTrdRows - 1)) && j + 2 < n) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) stg[g + kTrdGroups * s] = a[s];
+        }
+        // row j+1 of the updated matrix (v_{j+1} = 1): the next pivot row,
+        // and the partials of its norm beyond the next subdiagonal
         {
-            const T wj1 = sc_add(pl[j + 1], a2);
-            __syncthreads();
+            const T wj1 = pl[j + 1];
+            double y2 = 0.0;
             for (int c = j + 1 + t; c < n; c += kTrdThreads) {
-                const T vc = vl[c];
-                const T wc = sc_add(pl[c], sc_mul(a2, vc));
-                rl[c] = sc_sub(rl[c], sc_add(sc_conj(wc), sc_mulc(wj1, vc)));
+                const T x = sc_rank2(rl[c], sc_one<T>(), pl[c], wj1, vl[c]);
+                rl[c] = x;
+                if (c >= j + 3) y2 += sc_abs2(x);
             }
+            y2 = trd_wave_sum(y2);
+            if (lane == 0) nrm[wave] = y2;
         }
         __syncthreads();
     }

@@ -21,16 +21,14 @@ struct TrdLaunch {
     void *V;                // [nb][n * n] reflectors (LAPACK 'L' layout), zeroed by the caller
     double *d, *e;          // [nb][n]
     void *tau;              // [nb][n]
-    void *vbuf, *pbuf;      // [nb][2][n] exchanged pivot rows and p
-    void *dbuf;             // [nb][2][128] exchanged dot partials
-    unsigned *bar;          // trd_flag_words(n, nb) barrier flags
+    void *rbuf, *pbuf;      // [nb][n][n] exchanged pivot rows and p, a slot per step
+    void *dbuf;             // [nb][n][128] exchanged dot partials, a slot per step
+    unsigned *abortw;       // [1] set when a wait gives up
     int *err;               // [1], zeroed by the caller
     hipStream_t stream;
 };
 
 bool hetrd_coop_supported(int n, bool real);
-// barrier words the launch needs: one 64-byte flag per work-group, an abort word
-inline size_t trd_flag_words(int n, int nb) { return (size_t)nb * ((n + 15) / 16) * 16 + 1; }
 // A = Q T Q^H for every matrix of the batch; matrices are processed in
 // groups whose work-groups are co-resident on the device.
 hipError_t launch_hetrd_coop(const TrdLaunch &L);

@@ -1405,13 +1405,13 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     RCHK(dev_alloc(c, &owner, (size_t)N));
     RCHK(dev_alloc(c, &rank, (size_t)N));
     RCHK(dev_alloc(c, &tau, (size_t)n * nb * es));
-    RCHK(dev_alloc(c, &vbuf, (size_t)2 * n * nb * es));
-    RCHK(dev_alloc(c, &pbuf, (size_t)2 * n * nb * es));
-    RCHK(dev_alloc(c, &dbuf, (size_t)2 * 128 * nb * es));
+    RCHK(dev_alloc(c, &vbuf, nn * nb * es));
+    RCHK(dev_alloc(c, &pbuf, nn * nb * es));
+    RCHK(dev_alloc(c, &dbuf, (size_t)n * 128 * nb * es));
     RCHK(dev_alloc(c, &Tf, (size_t)nb * std::max(nblk, 1) * K * K * es));
     RCHK(dev_alloc(c, &W, (size_t)nb * K * n * es));
     RCHK(dev_alloc(c, &W2, (size_t)nb * K * n * es));
-    RCHK(dev_alloc(c, &bar, trd_flag_words(n, nb)));
+    RCHK(dev_alloc(c, &bar, 1));
     RCHK(dev_alloc(c, &err, 1));
     HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int) * nb, c->stream));
