@@ -163,15 +163,16 @@ hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, 
 
 // zlarft / dlarft (direct = 'F', storev = 'C') for block b of matrix m,
 // one work-group each: G = V_b^H V_b over the rows below the block's first
-// reflector (64-row tiles of V_b staged in LDS, four entries of G per
+// reflector (16-row tiles of V_b staged in LDS, K*K/256 entries of G per
 // thread), then T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i.
+// With K = 64 the complex tile, G and T take 150 KB of LDS.
 // Rows of a reflector above its unit entry are zero in V.
 template <class T>
 __global__ __launch_bounds__(256) void k_trd_larft(const T *__restrict__ V,
                                                    const T *__restrict__ tau, int n, int nblocks,
                                                    T *__restrict__ Tout)
 {
-    constexpr int K = kTrdBlock, R = 64;
+    constexpr int K = kTrdBlock, R = 16;
     __shared__ T tile[R][K + 1];
     __shared__ T G[K][K + 1];
     __shared__ T Ts[K][K + 1];

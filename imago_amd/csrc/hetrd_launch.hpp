@@ -10,7 +10,7 @@ namespace imago {
 // largest orders: 32 column groups x 32 complex / 64 real registers a lane
 constexpr int kTrdMaxOrderComplex = 1024;
 constexpr int kTrdMaxOrderReal = 2048;
-constexpr int kTrdBlock = 32;   // reflectors per compact-WY block of the back-transform
+constexpr int kTrdBlock = 64;   // reflectors per compact-WY block of the back-transform
 
 // Buffers of one batch; T = double2 (complex Hermitian) or double (real
 // symmetric) for every pointer typed void here.
