@@ -242,6 +242,72 @@ hipError_t launch_trd_larft(const void *V, const void *tau, int n, int nb, bool 
     return hipGetLastError();
 }
 
+// zpotf2 / dpotf2 (uplo = 'U') of the diagonal block A(j:j+jb, j:j+jb) of
+// every matrix, one work-group each, the block in LDS: for each column k
+// the pivot d = sqrt(a(k,k)), row k of U scaled by 1/d, then the trailing
+// block updated, a(r,c) -= conj(u(k,r)) u(k,c) for k < r <= c.  The first
+// pivot that is not positive (or NaN) sets info[m] = j + k + 1 unless an
+// earlier block already set it; the launch with j = 0 clears info first.
+template <class T>
+__global__ __launch_bounds__(256) void k_potf2_diag(T *__restrict__ A, int n, size_t stride, int j,
+                                                    int jb, int *__restrict__ info)
+{
+    constexpr int B = kPotrfBlock;
+    static_assert(B == 64, "one column per lane of a wave, four row phases");
+    __shared__ T a[B][B + 1];   // a[c][r] = A(j + r, j + c)
+    __shared__ int first;
+    const int t = threadIdx.x, m = blockIdx.x;
+    T *Am = A + m * stride;
+    for (int e = t; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        a[c][r] = r <= c ? Am[(size_t)(j + c) * n + j + r] : sc_zero<T>();
+    }
+    if (t == 0) first = j == 0 ? 0 : info[m];
+    __syncthreads();
+    // thread t updates column c = t % 64 at the rows q + 4i, q = t / 64
+    const int c = t & (B - 1), q = t >> 6;
+    for (int k = 0; k < jb; ++k) {
+        const double akk = sc_re(a[k][k]);
+        const double d = sqrt(akk);
+        const double rd = 1.0 / d;
+        __syncthreads();
+        if (t == 0) {
+            if (!(akk > 0.0) && first == 0) first = j + k + 1;
+            a[k][k] = sc_make<T>(d, 0.0);
+        }
+        if (t > k && t < jb) a[t][k] = sc_scale(rd, a[t][k]);
+        __syncthreads();
+        if (c > k && c < jb) {
+            const T ukc = a[c][k];
+#pragma unroll
+            for (int i = 0; i < B / 4; ++i) {
+                const int r = q + 4 * i;
+                if (r > k && r <= c) a[c][r] = sc_sub(a[c][r], sc_mul(sc_conj(a[r][k]), ukc));
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        if (r <= c) Am[(size_t)(j + c) * n + j + r] = a[c][r];
+    }
+    if (t == 0) info[m] = first;
+}
+
+hipError_t launch_potf2_diag(void *A, int n, int nb, int j, int jb, bool real, int *info,
+                             hipStream_t stream)
+{
+    if (jb < 1 || jb > kPotrfBlock || j < 0 || j + jb > n) return hipErrorInvalidValue;
+    const size_t stride = (size_t)n * n;
+    if (real)
+        hipLaunchKernelGGL(k_potf2_diag<double>, dim3((unsigned)nb), dim3(256), 0, stream,
+                           static_cast<double *>(A), n, stride, j, jb, info);
+    else
+        hipLaunchKernelGGL(k_potf2_diag<double2>, dim3((unsigned)nb), dim3(256), 0, stream,
+                           static_cast<double2 *>(A), n, stride, j, jb, info);
+    return hipGetLastError();
+}
+
 bool hetrd_coop_supported(int n, bool real)
 {
     return n >= 2 && n <= (real ? kTrdMaxOrderReal : kTrdMaxOrderComplex);

@@ -11,6 +11,7 @@ namespace imago {
 constexpr int kTrdMaxOrderComplex = 1024;
 constexpr int kTrdMaxOrderReal = 2048;
 constexpr int kTrdBlock = 64;   // reflectors per compact-WY block of the back-transform
+constexpr int kPotrfBlock = 64;  // columns per diagonal block of the blocked Cholesky
 
 // Buffers of one batch; T = double2 (complex Hermitian) or double (real
 // symmetric) for every pointer typed void here.
@@ -29,6 +30,12 @@ struct TrdLaunch {
 };
 
 bool hetrd_coop_supported(int n, bool real);
+// The unblocked Cholesky (upper, A = U^H U) of the diagonal block
+// A(j:j+jb, j:j+jb) of each of nb matrices (n x n, column-major, stride
+// n*n), jb <= kPotrfBlock; info[m] as zpotrf (the first leading minor that
+// is not positive definite, 1-based), cleared by the launch with j = 0.
+hipError_t launch_potf2_diag(void *A, int n, int nb, int j, int jb, bool real, int *info,
+                             hipStream_t stream);
 // A = Q T Q^H for every matrix of the batch; matrices are processed in
 // groups whose work-groups are co-resident on the device.
 hipError_t launch_hetrd_coop(const TrdLaunch &L);

@@ -1319,17 +1319,89 @@ static int ensure_secular_buffers(imago_ctx *c)
 // Matrices are n x n column-major with stride n*n; info[k] follows zhegv:
 // n + i when the leading minor i of S(k) is not positive definite, i when
 // heevd fails.
+// C = alpha op(A) op(B) + beta C over a strided batch, complex or real
+static rocblas_status gemm_sb(rocblas_handle hb, bool cx, rocblas_operation ta,
+                              rocblas_operation tb, int m, int n, int k, double alpha,
+                              const void *A, int lda, rocblas_stride sa, const void *B, int ldb,
+                              rocblas_stride sb, double beta, void *C, int ldc, rocblas_stride sc,
+                              int batch)
+{
+    if (cx) {
+        const rocblas_double_complex al(alpha, 0.0), be(beta, 0.0);
+        return rocblas_zgemm_strided_batched(hb, ta, tb, m, n, k, &al,
+                                             (const rocblas_double_complex *)A, lda, sa,
+                                             (const rocblas_double_complex *)B, ldb, sb, &be,
+                                             (rocblas_double_complex *)C, ldc, sc, batch);
+    }
+    const rocblas_operation ra =
+        ta == rocblas_operation_conjugate_transpose ? rocblas_operation_transpose : ta;
+    const rocblas_operation rb =
+        tb == rocblas_operation_conjugate_transpose ? rocblas_operation_transpose : tb;
+    return rocblas_dgemm_strided_batched(hb, ra, rb, m, n, k, &alpha, (const double *)A, lda, sa,
+                                         (const double *)B, ldb, sb, &beta, (double *)C, ldc, sc,
+                                         batch);
+}
+
+// The Cholesky factor S = U^H U of a batch, blocked right-looking as
+// LAPACK zpotrf: for each block of kPotrfBlock columns the diagonal block
+// by k_potf2_diag, the block row beside it by a triangular solve
+// (U_jj^-H A_j,rest) and the trailing matrix by a rank-k update, each one
+// batched call.  The update is a GEMM over the whole trailing square
+// (rocBLAS's herk runs at a fraction of its speed at these shapes), so the
+// strictly lower triangle is overwritten as well; every reader of the
+// factor takes the upper triangle only.  IMAGO_POTRF=rocsolver selects
+// rocSOLVER's batched potrf instead.
+static int potrf_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int *dinfo)
+{
+    const rocblas_stride nn = (rocblas_stride)n * n;
+    const char *env = getenv("IMAGO_POTRF");
+    if (env && strcmp(env, "rocsolver") == 0) {
+        if (cx)
+            BLASCHK(rocsolver_zpotrf_strided_batched(c->blas, rocblas_fill_upper, n,
+                                                     (rocblas_double_complex *)B, n, nn, dinfo, nb));
+        else
+            BLASCHK(rocsolver_dpotrf_strided_batched(c->blas, rocblas_fill_upper, n, (double *)B, n,
+                                                     nn, dinfo, nb));
+        return IMAGO_OK;
+    }
+    const size_t es = cx ? sizeof(double2) : sizeof(double);
+    char *base = static_cast<char *>(B);
+    for (int j = 0; j < n; j += kPotrfBlock) {
+        const int jb = std::min(kPotrfBlock, n - j), rest = n - j - jb;
+        HIPCHK(launch_potf2_diag(B, n, nb, j, jb, !cx, dinfo, c->stream));
+        if (rest == 0) break;
+        void *Ajj = base + ((size_t)j * n + j) * es;
+        void *Ajr = base + ((size_t)(j + jb) * n + j) * es;
+        void *Arr = base + ((size_t)(j + jb) * n + j + jb) * es;
+        if (cx) {
+            const rocblas_double_complex one(1.0, 0.0);
+            const double m1 = -1.0, p1 = 1.0;
+            BLASCHK(rocblas_ztrsm_strided_batched(
+                c->blas, rocblas_side_left, rocblas_fill_upper,
+                rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, jb, rest, &one,
+                (const rocblas_double_complex *)Ajj, n, nn, (rocblas_double_complex *)Ajr, n, nn,
+                nb));
+            BLASCHK(gemm_sb(c->blas, true, rocblas_operation_conjugate_transpose,
+                            rocblas_operation_none, rest, rest, jb, m1, Ajr, n, nn, Ajr, n, nn, p1,
+                            Arr, n, nn, nb));
+        } else {
+            const double one = 1.0, m1 = -1.0;
+            BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
+                                                  rocblas_operation_transpose,
+                                                  rocblas_diagonal_non_unit, jb, rest, &one,
+                                                  (const double *)Ajj, n, nn, (double *)Ajr, n, nn,
+                                                  nb));
+            BLASCHK(gemm_sb(c->blas, false, rocblas_operation_transpose, rocblas_operation_none,
+                            rest, rest, jb, m1, Ajr, n, nn, Ajr, n, nn, one, Arr, n, nn, nb));
+        }
+    }
+    return IMAGO_OK;
+}
+
 static int factor_gv_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int *dinfo, int *hinfo,
                              bool *bad)
 {
-    const rocblas_stride nn = (rocblas_stride)n * n;
-    if (cx) {
-        BLASCHK(rocsolver_zpotrf_strided_batched(c->blas, rocblas_fill_upper, n,
-                                                 (rocblas_double_complex *)B, n, nn, dinfo, nb));
-    } else {
-        BLASCHK(rocsolver_dpotrf_strided_batched(c->blas, rocblas_fill_upper, n, (double *)B, n, nn,
-                                                 dinfo, nb));
-    }
+    RCHK(potrf_batched(c, cx, n, nb, B, dinfo));
     HIPCHK(hipMemcpyAsync(hinfo, dinfo, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     *bad = false;
@@ -1363,28 +1435,8 @@ static bool use_coop_heevd(int n, bool cx)
     return hetrd_coop_supported(n, !cx) && !(env && strcmp(env, "rocsolver") == 0);
 }
 
-// C = alpha op(A) op(B) + beta C over a strided batch, complex or real
-static rocblas_status gemm_sb(rocblas_handle hb, bool cx, rocblas_operation ta,
-                              rocblas_operation tb, int m, int n, int k, double alpha,
-                              const void *A, int lda, rocblas_stride sa, const void *B, int ldb,
-                              rocblas_stride sb, double beta, void *C, int ldc, rocblas_stride sc,
-                              int batch)
-{
-    if (cx) {
-        const rocblas_double_complex al(alpha, 0.0), be(beta, 0.0);
-        return rocblas_zgemm_strided_batched(hb, ta, tb, m, n, k, &al,
-                                             (const rocblas_double_complex *)A, lda, sa,
-                                             (const rocblas_double_complex *)B, ldb, sb, &be,
-                                             (rocblas_double_complex *)C, ldc, sc, batch);
-    }
-    const rocblas_operation ra =
-        ta == rocblas_operation_conjugate_transpose ? rocblas_operation_transpose : ta;
-    const rocblas_operation rb =
-        tb == rocblas_operation_conjugate_transpose ? rocblas_operation_transpose : tb;
-    return rocblas_dgemm_strided_batched(hb, ra, rb, m, n, k, &alpha, (const double *)A, lda, sa,
-                                         (const double *)B, ldb, sb, &beta, (double *)C, ldc, sc,
-                                         batch);
-}
+// heevd_coop's return when the cooperative launch is refused (A untouched)
+constexpr int kCoopRefused = 1000;
 
 static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *evals,
                       double *ework, int *dinfo)
@@ -1423,7 +1475,19 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         HIPCHK(hipEventCreate(&c->evh1));
     }
     HIPCHK(hipEventRecord(c->evh0, c->stream));
-    HIPCHK(launch_hetrd_coop(L));
+    const hipError_t le = launch_hetrd_coop(L);
+    if (le == hipErrorIllegalState || le == hipErrorCooperativeLaunchTooLarge) {
+        // the runtime refuses the cooperative launch (another process holds
+        // the device's cooperative queue, or its work-groups would not all
+        // be resident); A is untouched, the caller takes rocSOLVER's heevd
+        (void)hipGetLastError();
+        for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank,
+                        (void *)tau, (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf,
+                        (void *)W, (void *)W2, (void *)bar, (void *)err})
+            dev_release(c, p);
+        return kCoopRefused;
+    }
+    HIPCHK(le);
     HIPCHK(hipEventRecord(c->evh1, c->stream));
     // d and e of the batch are back to back, e[n-1] = 0 between problems:
     // each half of the batch is one block-diagonal problem, the two halves
@@ -1497,8 +1561,10 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));
-        if (use_coop_heevd(n, true)) {
-            RCHK(heevd_coop(c, true, n, nb, A, evals, ework, dinfo));
+        const int rc = use_coop_heevd(n, true) ? heevd_coop(c, true, n, nb, A, evals, ework, dinfo)
+                                               : kCoopRefused;
+        if (rc != kCoopRefused) {
+            RCHK(rc);
         } else {
             BLASCHK(rocsolver_zheevd_strided_batched(c->blas, rocblas_evect_original,
                                                      rocblas_fill_upper, n, a, n, nn, evals, n,
@@ -1517,8 +1583,11 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));
-        if (use_coop_heevd(n, false)) {
-            RCHK(heevd_coop(c, false, n, nb, A, evals, ework, dinfo));
+        const int rc = use_coop_heevd(n, false)
+                           ? heevd_coop(c, false, n, nb, A, evals, ework, dinfo)
+                           : kCoopRefused;
+        if (rc != kCoopRefused) {
+            RCHK(rc);
         } else {
             BLASCHK(rocsolver_dsyevd_strided_batched(c->blas, rocblas_evect_original,
                                                      rocblas_fill_upper, n, a, n, nn, evals, n,
@@ -1955,9 +2024,19 @@ static int single_gv(imago_ctx *c, bool cx, int n, double *A, double *B, double 
     HIPCHK(hipMemcpy(dB, B, nn * sizeof(double), hipMemcpyHostToDevice));
     int inf = 0;
     int rc = solve_gv_batched(c, cx, n, 1, dA, dB, dw, de, di, &inf);
+    // B returns the factor in its upper triangle and, as in LAPACK, its
+    // strictly lower triangle as it came in (the device copy's is overwritten)
+    std::vector<double> hB(nn);
     if (rc == IMAGO_OK && hipMemcpy(A, dA, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
-        hipMemcpy(B, dB, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+        hipMemcpy(hB.data(), dB, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
         hipMemcpy(w, dw, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
+        const int es = cx ? 2 : 1;
+        for (int col = 0; col < n; ++col)
+            for (int row = 0; row <= col; ++row)
+                for (int x = 0; x < es; ++x) {
+                    const size_t at = ((size_t)col * n + row) * es + x;
+                    B[at] = hB[at];
+                }
     } else if (rc == IMAGO_OK) {
         rc = fail(IMAGO_E_HIP, "copy of eigenproblem results failed");
     }

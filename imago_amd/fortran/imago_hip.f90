@@ -287,12 +287,14 @@ contains
       integer(c_int) :: info, status
       call initImagoHIP(0)
       allocate (w(N))
+      info = 0
       status = imago_zhegv(imagoCtx, int(N,c_int), A, B, w, info)
+      ! a failure of the library itself leaves info unset: report it first
+      if (status /= IMAGO_OK .and. info == 0) call hip_check(status, 'imago_zhegv')
       if (info /= 0) then
          write (20,*) 'ZHEGV error code=', info
          stop
       endif
-      call hip_check(status, 'imago_zhegv')
       eigenValues(:resultDim) = w(:resultDim)
       deallocate (w)
    end subroutine solveZHEGV_hip
@@ -309,13 +311,15 @@ contains
       integer(c_int) :: info, status
       call initImagoHIP(0)
       allocate (w(valeDim))
+      info = 0
       status = imago_dsygv(imagoCtx, int(valeDim,c_int), valeValeGamma, &
             & valeValeOLGamma, w, info)
+      ! a failure of the library itself leaves info unset: report it first
+      if (status /= IMAGO_OK .and. info == 0) call hip_check(status, 'imago_dsygv')
       if (info /= 0) then
          write (20,*) 'DSYGV error code=', info
          stop
       endif
-      call hip_check(status, 'imago_dsygv')
       eigenValues(:numStates) = w(:numStates)
       deallocate (w)
    end subroutine solveDSYGV_hip

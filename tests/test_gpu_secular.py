@@ -99,6 +99,27 @@ def test_not_positive_definite_overlap_is_reported(gpu_ctx):
     assert e.value.name == "IMAGO_E_NOTPD"
 
 
+@pytest.mark.parametrize("cx", [True, False])
+def test_not_positive_definite_reports_the_leading_minor(gpu_ctx, cx):
+    """zpotrf's info through the blocked Cholesky: the first pivot that is
+    not positive sits at order 71 (the second diagonal block), so the error
+    carries ZHEGV's code n + 71; a later negative pivot does not replace it."""
+    n = 100
+    rng = np.random.default_rng(6)
+    S = np.eye(n) * 4.0
+    E = rng.normal(size=(n, n)) * 0.01
+    if cx:
+        E = E + 1j * rng.normal(size=(n, n)) * 0.01
+    S = S + E + E.conj().T
+    S[70, 70] = -1.0
+    S[90, 90] = -3.0
+    H = np.diag(np.arange(n, dtype=float)).astype(S.dtype)
+    with pytest.raises(ImagoError) as e:
+        gpu_ctx.hegv(H, S)
+    assert e.value.name == "IMAGO_E_NOTPD"
+    assert "code=%d" % (n + 71) in str(e.value), str(e.value)
+
+
 def test_secular_before_assembly_is_a_state_error(gpu_ctx):
     s = configs.small_test_system(seed=5)
     gpu_ctx.set_system(s)
