@@ -162,50 +162,24 @@ hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, 
 }
 
 // zlarft / dlarft (direct = 'F', storev = 'C') for block b of matrix m,
-// one work-group each: G = V_b^H V_b over the rows below the block's first
-// reflector (16-row tiles of V_b staged in LDS, K*K/256 entries of G per
-// thread), then T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i.
-// With K = 64 the complex tile, G and T take 150 KB of LDS.
-// Rows of a reflector above its unit entry are zero in V.
+// one work-group each.  On entry T[m][b] holds G = V_b^H V_b (the caller's
+// batched GEMM over all n rows: V is zero above each reflector's unit
+// entry); the strict upper triangle of G is read into LDS and T built in
+// place of it: T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i.
 template <class T>
-__global__ __launch_bounds__(256) void k_trd_larft(const T *__restrict__ V,
-                                                   const T *__restrict__ tau, int n, int nblocks,
-                                                   T *__restrict__ Tout)
+__global__ __launch_bounds__(256) void k_trd_larft(const T *__restrict__ tau, int n, int nblocks,
+                                                   T *__restrict__ TG)
 {
-    constexpr int K = kTrdBlock, R = 16;
-    __shared__ T tile[R][K + 1];
+    constexpr int K = kTrdBlock;
     __shared__ T G[K][K + 1];
     __shared__ T Ts[K][K + 1];
     const int m = blockIdx.x / nblocks, b = blockIdx.x % nblocks;
     const int j0 = b * K, nbk = min(K, n - 1 - j0);
-    const size_t nn = (size_t)n * n;
-    const T *Vm = V + m * nn;
     const T *tm = tau + (size_t)m * n;
-    T acc[K * K / 256];
-#pragma unroll
-    for (int q = 0; q < K * K / 256; ++q) acc[q] = sc_zero<T>();
-    for (int r0 = j0 + 1; r0 < n; r0 += R) {
-        for (int e = threadIdx.x; e < R * K; e += 256) {
-            const int rr = e % R, cc = e / R, r = r0 + rr;
-            tile[rr][cc] = (r < n && cc < nbk) ? Vm[(size_t)(j0 + cc) * n + r] : sc_zero<T>();
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < K * K / 256; ++q) {
-            const int e = threadIdx.x + 256 * q, kr = e % K, ic = e / K;
-            if (kr < ic) {
-                T s = acc[q];
-                for (int rr = 0; rr < R; ++rr)
-                    s = sc_add(s, sc_mulc(tile[rr][ic], tile[rr][kr]));   // conj(vk) vi
-                acc[q] = s;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < K * K / 256; ++q) {
-        const int e = threadIdx.x + 256 * q, kr = e % K, ic = e / K;
-        G[kr][ic] = acc[q];
+    T *Tg = TG + ((size_t)m * nblocks + b) * K * K;
+    for (int e = threadIdx.x; e < K * K; e += 256) {
+        const int kr = e % K, ic = e / K;
+        G[kr][ic] = kr < ic ? Tg[e] : sc_zero<T>();
         Ts[kr][ic] = sc_zero<T>();
     }
     __syncthreads();
@@ -221,23 +195,20 @@ __global__ __launch_bounds__(256) void k_trd_larft(const T *__restrict__ V,
         if (threadIdx.x == 0) Ts[i][i] = ti;
         __syncthreads();
     }
-    T *To = Tout + ((size_t)m * nblocks + b) * K * K;
-    for (int e = threadIdx.x; e < K * K; e += 256) To[e] = Ts[e % K][e / K];
+    for (int e = threadIdx.x; e < K * K; e += 256) Tg[e] = Ts[e % K][e / K];
 }
 
-hipError_t launch_trd_larft(const void *V, const void *tau, int n, int nb, bool real, void *T,
-                            hipStream_t stream)
+hipError_t launch_trd_larft(const void *tau, int n, int nb, bool real, void *T, hipStream_t stream)
 {
     const int nblocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;
     if (nblocks <= 0) return hipSuccess;
     if (real)
         hipLaunchKernelGGL(k_trd_larft<double>, dim3((unsigned)(nb * nblocks)), dim3(256), 0,
-                           stream, static_cast<const double *>(V),
-                           static_cast<const double *>(tau), n, nblocks, static_cast<double *>(T));
+                           stream, static_cast<const double *>(tau), n, nblocks,
+                           static_cast<double *>(T));
     else
         hipLaunchKernelGGL(k_trd_larft<double2>, dim3((unsigned)(nb * nblocks)), dim3(256), 0,
-                           stream, static_cast<const double2 *>(V),
-                           static_cast<const double2 *>(tau), n, nblocks,
+                           stream, static_cast<const double2 *>(tau), n, nblocks,
                            static_cast<double2 *>(T));
     return hipGetLastError();
 }

@@ -51,8 +51,8 @@ hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, 
 // Compact-WY factors of the reflectors of every matrix: T[m][b] (kTrdBlock
 // x kTrdBlock, upper, column-major) with H(j0) ... H(j0 + nbk - 1) =
 // I - V_b T V_b^H for block b (j0 = kTrdBlock b), zlarft / dlarft forward,
-// columnwise.
-hipError_t launch_trd_larft(const void *V, const void *tau, int n, int nb, bool real, void *T,
-                            hipStream_t stream);
+// columnwise; T holds G = V_b^H V_b on entry (the strict upper triangle is
+// read) and T on exit.
+hipError_t launch_trd_larft(const void *tau, int n, int nb, bool real, void *T, hipStream_t stream);
 
 }  // namespace imago

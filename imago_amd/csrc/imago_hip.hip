@@ -1451,7 +1451,9 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     int *err = nullptr;
     const int h0 = (nb + 1) / 2, h1 = nb - h0;
     const int N0 = h0 * n, N1 = h1 * n;
-    RCHK(dev_alloc(c, &V, nn * nb * es));
+    // V carries K*n zero elements past its last matrix: the GEMM forming the
+    // compact-WY Gram blocks reads whole K-column blocks
+    RCHK(dev_alloc(c, &V, (nn * nb + (size_t)K * n) * es));
     RCHK(dev_alloc(c, &Z, (size_t)N0 * N0 + (size_t)N1 * N1));
     RCHK(dev_alloc(c, &wall, (size_t)N));
     RCHK(dev_alloc(c, &owner, (size_t)N));
@@ -1467,7 +1469,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     RCHK(dev_alloc(c, &err, 1));
     HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int) * nb, c->stream));
-    HIPCHK(hipMemsetAsync(V, 0, es * nn * nb, c->stream));
+    HIPCHK(hipMemsetAsync(V, 0, es * (nn * nb + (size_t)K * n), c->stream));
     TrdLaunch L{n,     nb,   !cx,  A,    V,   evals, ework, tau,
                 vbuf,  pbuf, dbuf, bar,  err, c->stream};
     if (!c->evh0) {
@@ -1516,9 +1518,19 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     if (h1 > 0)
         HIPCHK(launch_trd_unmix(Z1, wall + N0, n, h1, h0, owner + N0, rank + N0, evals, A, !cx,
                                 c->stream));
-    HIPCHK(launch_trd_larft(V, tau, n, nb, !cx, Tf, c->stream));
     const rocblas_stride snn = (rocblas_stride)nn, sw = (rocblas_stride)K * n,
                          st = (rocblas_stride)std::max(nblk, 1) * K * K;
+    // G_b = V_b^H V_b for every block of each matrix (one strided-batched
+    // GEMM per matrix over all n rows), then the factors T_b in place of G_b;
+    // the columns a last block reads beyond n only reach G entries the
+    // factor ignores
+    for (int m = 0; m < nb && nblk > 0; ++m) {
+        const char *Vm = V + (size_t)m * nn * es;
+        BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_conjugate_transpose, rocblas_operation_none,
+                        K, K, n, 1.0, Vm, n, sw, Vm, n, sw, 0.0, Tf + (size_t)m * st * es, K,
+                        (rocblas_stride)K * K, nblk));
+    }
+    HIPCHK(launch_trd_larft(tau, n, nb, !cx, Tf, c->stream));
     char *Ac = static_cast<char *>(A);
     for (int b = nblk - 1; b >= 0; --b) {
         const int j0 = b * K, kb = std::min(K, n - 1 - j0), r0 = j0 + 1, mb = n - r0;
