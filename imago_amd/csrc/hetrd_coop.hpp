@@ -5,7 +5,8 @@
 //
 // Each matrix lives in the registers of P = ceil(n / 16) work-groups for the
 // whole reduction: work-group w holds rows 16w .. 16w+15, thread t of the
-// work-group row 16w + (t & 15) at the columns (t >> 4) + 32 s, s < S.  One
+// work-group rows 16w + (t & 7) and 16w + (t & 7) + 8 at the columns
+// (t >> 3) + 64 s, s < S / 2.  One
 // column step is one exchange between the work-groups of a matrix:
 //   every work-group holds the pivot row j of the current matrix in LDS and
 //   builds the reflector from conj(row j) (zlarfg / dlarfg) itself,
@@ -23,7 +24,7 @@
 
 namespace imago {
 
-constexpr int kTrdThreads = 512;   // 16 rows x 32 column groups
+constexpr int kTrdThreads = 512;   // 8 row pairs x 64 column groups
 constexpr int kTrdRows = 16;
 constexpr int kTrdGroups = 32;
 constexpr int kTrdMaxGroups = 128;   // work-groups per matrix (order <= 2048)
@@ -202,6 +203,10 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
 {
     constexpr int NW = kTrdThreads / 64;
     constexpr int MC = (kTrdGroups * S + kTrdThreads - 1) / kTrdThreads;   // columns a thread waits on
+    // a lane holds two rows (r and r + 8) of the columns g + 64 s, s < S2:
+    // each v (and w) read from LDS serves two matrix entries
+    constexpr int GC = 2 * kTrdGroups, S2 = S / 2, QB = S2 < 4 ? S2 : 4;
+    static_assert(S % 2 == 0 && S2 % QB == 0, "column batches");
     static_assert(2 * MC <= 30, "wait mask");
     __shared__ T vl[kTrdGroups * S];   // v of this step
     __shared__ T pl[kTrdGroups * S];   // p of this step
@@ -212,11 +217,12 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     __shared__ T dot;                  // p^H v
     const int n = A_.n, P = A_.P;
     const int m = blockIdx.x / P, w = blockIdx.x % P;
-    const int t = threadIdx.x, r = t & (kTrdRows - 1), g = t >> 4;
+    const int t = threadIdx.x, r = t & 7, g = t >> 3;
     const int lane = t & 63, wave = t >> 6;
-    const int i = kTrdRows * w + r;
+    const int i0 = kTrdRows * w + r, i1 = i0 + 8;   // this lane's rows
+    const int it = kTrdRows * w + (t & (kTrdRows - 1)); // row of thread t < 16
     // the largest column group of this wave's lanes (wave-uniform)
-    const int gtop = 4 * __builtin_amdgcn_readfirstlane(wave) + 3;
+    const int gtop = 8 * __builtin_amdgcn_readfirstlane(wave) + 7;
     const size_t nn = (size_t)n * n;
     const T *A = A_.A + m * nn;
     T *V = A_.V + m * nn;
@@ -224,11 +230,12 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     T *dbuf = A_.dbuf + (size_t)m * n * kTrdMaxGroups;
     const bool out = w == 0;   // work-group 0 writes d, e and tau
 
-    T a[S];
+    T a[S2][2];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int c = g + kTrdGroups * s;
-        a[s] = (i < n && c < n) ? A[(size_t)c * n + i] : sc_zero<T>();
+    for (int s = 0; s < S2; ++s) {
+        const int c = g + GC * s;
+        a[s][0] = (i0 < n && c < n) ? A[(size_t)c * n + i0] : sc_zero<T>();
+        a[s][1] = (i1 < n && c < n) ? A[(size_t)c * n + i1] : sc_zero<T>();
     }
     // pivot row 0: A(0, c) for every column; its norm partials; row 1 staged
     double x2 = 0.0;
@@ -241,10 +248,7 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     if (lane == 0) nrm[wave] = x2;
     if (w == 0 && r == 1) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const int c = g + kTrdGroups * s;
-            if (c >= 1 && c < n) stg[c] = a[s];
-        }
+        for (int s = 0; s < S2; ++s) stg[g + GC * s] = a[s][0];
     }
     // The loads of a[] complete here.  Left pending on some paths, they make
     // the compiler wait for every outstanding memory access at each a[s]'s
@@ -295,43 +299,50 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
         // selects, so their LDS loads are in flight together); a wave skips
         // a batch whose columns all lie at or before j, a work-group whose
         // rows all do skips the product.
-        const bool row_on = i > j && i < n;
+        const bool on0 = i0 > j && i0 < n, on1 = i1 > j && i1 < n;
+        const bool row_on = it > j && it < n;   // thread t < 16's row
         const bool wg_on = kTrdRows * w + kTrdRows - 1 > j;
         T sp0 = sc_zero<T>(), sp1 = sc_zero<T>();
         if (wg_on) {
 #pragma unroll
-            for (int s0 = 0; s0 < S; s0 += 4) {
-                if (gtop + kTrdGroups * (s0 + 3) > j) {
-                    T vc[4];
+            for (int s0 = 0; s0 < S2; s0 += QB) {
+                if (gtop + GC * (s0 + QB - 1) > j) {
+                    T vc[QB];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) vc[q] = vl[g + kTrdGroups * (s0 + q)];
+                    for (int q = 0; q < QB; ++q) vc[q] = vl[g + GC * (s0 + q)];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) sc_pin(vc[q]);
+                    for (int q = 0; q < QB; ++q) sc_pin(vc[q]);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int c = g + kTrdGroups * (s0 + q);
+                    for (int q = 0; q < QB; ++q) {
+                        const int c = g + GC * (s0 + q);
                         vc[q] = sc_sel(c > j && c < n, vc[q], sc_zero<T>());
+                        sc_fma_acc(sp0, a[s0 + q][0], vc[q]);
+                        sc_fma_acc(sp1, a[s0 + q][1], vc[q]);
                     }
-                    sc_fma_acc(sp0, a[s0], vc[0]);
-                    sc_fma_acc(sp1, a[s0 + 1], vc[1]);
-                    sc_fma_acc(sp0, a[s0 + 2], vc[2]);
-                    sc_fma_acc(sp1, a[s0 + 3], vc[3]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        T sp = row_on ? sc_add(sp0, sp1) : sc_zero<T>();
-        sp = sc_add(sp, sc_shfl_xor(sp, 16));
-        sp = sc_add(sp, sc_shfl_xor(sp, 32));
-        if (lane < kTrdRows) red[wave][lane] = sp;
+        // sums over the eight column groups of the wave, rows r and r + 8
+        sp0 = on0 ? sp0 : sc_zero<T>();
+        sp1 = on1 ? sp1 : sc_zero<T>();
+#pragma unroll
+        for (int o = 8; o < 64; o <<= 1) {
+            sp0 = sc_add(sp0, sc_shfl_xor(sp0, o));
+            sp1 = sc_add(sp1, sc_shfl_xor(sp1, o));
+        }
+        if (lane < 8) {
+            red[wave][lane] = sp0;
+            red[wave][lane + 8] = sp1;
+        }
         __syncthreads();
         if (t < kTrdRows) {
             T p = sc_zero<T>();
             for (int q = 0; q < NW; ++q) p = sc_add(p, red[q][t]);
             p = row_on ? sc_mul(tau, p) : sc_zero<T>();
-            if (row_on) pub_store(ps + i, p);
+            if (row_on) pub_store(ps + it, p);
             // conj(p_i) v_i over the 16 rows, the same tree in every work-group
-            T dp = row_on ? sc_mulc(vl[i], p) : sc_zero<T>();
+            T dp = row_on ? sc_mulc(vl[it], p) : sc_zero<T>();
             for (int o = 8; o >= 1; o >>= 1) dp = sc_add(dp, sc_shfl_xor(dp, o));
             if (t == 0) pub_store(ds + w, dp);
         }
@@ -410,30 +421,36 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
         for (int c = j + 1 + t; c < n; c += kTrdThreads) pl[c] = sc_add(pl[c], sc_mul(a2, vl[c]));
         // Each work-group writes the part of V on its own rows here, a step
         // before the next wait on its memory accesses.
-        if (t < kTrdRows && row_on) V[(size_t)j * n + i] = vl[i];
+        if (t < kTrdRows && row_on) V[(size_t)j * n + it] = vl[it];
         __syncthreads();
-        if (wg_on && row_on) {
-            const T vi = vl[i], wi = pl[i];
+        if (wg_on && (on0 || on1)) {
+            // rows at or before j keep their values (the selects); their v
+            // and w entries may be stale or unset
+            const T v0 = vl[i0 < n ? i0 : 0], w0 = pl[i0 < n ? i0 : 0];
+            const T v1 = vl[i1 < n ? i1 : 0], w1 = pl[i1 < n ? i1 : 0];
 #pragma unroll
-            for (int s0 = 0; s0 < S; s0 += 4) {
-                if (gtop + kTrdGroups * (s0 + 3) > j) {
-                    T vc[4], wc[4];
+            for (int s0 = 0; s0 < S2; s0 += QB) {
+                if (gtop + GC * (s0 + QB - 1) > j) {
+                    T vc[QB], wc[QB];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int c = g + kTrdGroups * (s0 + q);
+                    for (int q = 0; q < QB; ++q) {
+                        const int c = g + GC * (s0 + q);
                         vc[q] = vl[c];
                         wc[q] = pl[c];
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
+                    for (int q = 0; q < QB; ++q) {
                         sc_pin(vc[q]);
                         sc_pin(wc[q]);
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int c = g + kTrdGroups * (s0 + q);
-                        const T x = sc_rank2(a[s0 + q], vi, wc[q], wi, vc[q]);
-                        a[s0 + q] = sc_sel(c > j && c < n, x, a[s0 + q]);
+                    for (int q = 0; q < QB; ++q) {
+                        const int c = g + GC * (s0 + q);
+                        const bool col = c > j && c < n;
+                        const T x0 = sc_rank2(a[s0 + q][0], v0, wc[q], w0, vc[q]);
+                        const T x1 = sc_rank2(a[s0 + q][1], v1, wc[q], w1, vc[q]);
+                        a[s0 + q][0] = sc_sel(col && on0, x0, a[s0 + q][0]);
+                        a[s0 + q][1] = sc_sel(col && on1, x1, a[s0 + q][1]);
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -441,9 +458,13 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
         }
         // the owner of row j+2 stages it for the next step's publication
         // (columns up to j+1 are staged too and never published)
-        if (w == ((j + 2) >> 4) && r == ((j + 2) & (kTrdRows - 1)) && j + 2 < n) {
+        {
+            const int rj = (j + 2) & (kTrdRows - 1);
+            if (w == ((j + 2) >> 4) && r == (rj & 7) && j + 2 < n) {
+                const bool hi = rj >= 8;
 #pragma unroll
-            for (int s = 0; s < S; ++s) stg[g + kTrdGroups * s] = a[s];
+                for (int s = 0; s < S2; ++s) stg[g + GC * s] = sc_sel(hi, a[s][1], a[s][0]);
+            }
         }
         // row j+1 of the updated matrix (v_{j+1} = 1): the next pivot row,
         // and the partials of its norm beyond the next subdiagonal
