@@ -5,8 +5,8 @@
 //
 // Each matrix lives in the registers of P = ceil(n / 16) work-groups for the
 // whole reduction: work-group w holds rows 16w .. 16w+15, thread t of the
-// work-group rows 16w + (t & 7) and 16w + (t & 7) + 8 at the columns
-// (t >> 3) + 64 s, s < S / 2.  One
+// work-group rows 16w + (t % 8) and 16w + (t % 8) + 8 at the columns
+// (t / 8) + 64 s, s < S / 2.  One
 // column step is one exchange between the work-groups of a matrix:
 //   every work-group holds the pivot row j of the current matrix in LDS and
 //   builds the reflector from conj(row j) (zlarfg / dlarfg) itself,
@@ -24,7 +24,7 @@
 
 namespace imago {
 
-constexpr int kTrdThreads = 512;   // 8 row pairs x 64 column groups
+constexpr int kTrdThreads = 512;
 constexpr int kTrdRows = 16;
 constexpr int kTrdGroups = 32;
 constexpr int kTrdMaxGroups = 128;   // work-groups per matrix (order <= 2048)
@@ -203,10 +203,13 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
 {
     constexpr int NW = kTrdThreads / 64;
     constexpr int MC = (kTrdGroups * S + kTrdThreads - 1) / kTrdThreads;   // columns a thread waits on
-    // a lane holds two rows (r and r + 8) of the columns g + 64 s, s < S2:
-    // each v (and w) read from LDS serves two matrix entries
-    constexpr int GC = 2 * kTrdGroups, S2 = S / 2, QB = S2 < 4 ? S2 : 4;
-    static_assert(S % 2 == 0 && S2 % QB == 0, "column batches");
+    // a lane holds RPL rows (r, r + LR, ...) of the columns g + GC s,
+    // s < S2: each v (and w) read from LDS serves RPL matrix entries.  Two
+    // rows measured best: four take 6.6 ms instead of 6.2 at 4 x 896
+    // complex (the update's registers), the same at order 1944 real.
+    constexpr int RPL = 2, LR = kTrdRows / RPL, GC = kTrdThreads / LR;
+    constexpr int S2 = S / RPL, QB = S2 < 4 ? S2 : 4;
+    static_assert(S % RPL == 0 && S2 % QB == 0 && GC * S2 == kTrdGroups * S, "column batches");
     static_assert(2 * MC <= 30, "wait mask");
     __shared__ T vl[kTrdGroups * S];   // v of this step
     __shared__ T pl[kTrdGroups * S];   // p of this step
@@ -217,12 +220,12 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     __shared__ T dot;                  // p^H v
     const int n = A_.n, P = A_.P;
     const int m = blockIdx.x / P, w = blockIdx.x % P;
-    const int t = threadIdx.x, r = t & 7, g = t >> 3;
+    const int t = threadIdx.x, r = t % LR, g = t / LR;
     const int lane = t & 63, wave = t >> 6;
-    const int i0 = kTrdRows * w + r, i1 = i0 + 8;   // this lane's rows
+    const int i0 = kTrdRows * w + r;                  // this lane's rows: i0 + LR h
     const int it = kTrdRows * w + (t & (kTrdRows - 1)); // row of thread t < 16
     // the largest column group of this wave's lanes (wave-uniform)
-    const int gtop = 8 * __builtin_amdgcn_readfirstlane(wave) + 7;
+    const int gtop = (64 / LR) * __builtin_amdgcn_readfirstlane(wave) + 64 / LR - 1;
     const size_t nn = (size_t)n * n;
     const T *A = A_.A + m * nn;
     T *V = A_.V + m * nn;
@@ -230,12 +233,15 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     T *dbuf = A_.dbuf + (size_t)m * n * kTrdMaxGroups;
     const bool out = w == 0;   // work-group 0 writes d, e and tau
 
-    T a[S2][2];
+    T a[S2][RPL];
 #pragma unroll
     for (int s = 0; s < S2; ++s) {
         const int c = g + GC * s;
-        a[s][0] = (i0 < n && c < n) ? A[(size_t)c * n + i0] : sc_zero<T>();
-        a[s][1] = (i1 < n && c < n) ? A[(size_t)c * n + i1] : sc_zero<T>();
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) {
+            const int ih = i0 + LR * h;
+            a[s][h] = (ih < n && c < n) ? A[(size_t)c * n + ih] : sc_zero<T>();
+        }
     }
     // pivot row 0: A(0, c) for every column; its norm partials; row 1 staged
     double x2 = 0.0;
@@ -246,9 +252,9 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     }
     x2 = trd_wave_sum(x2);
     if (lane == 0) nrm[wave] = x2;
-    if (w == 0 && r == 1) {
+    if (w == 0 && r == 1 % LR) {
 #pragma unroll
-        for (int s = 0; s < S2; ++s) stg[g + GC * s] = a[s][0];
+        for (int s = 0; s < S2; ++s) stg[g + GC * s] = a[s][1 / LR];
     }
     // The loads of a[] complete here.  Left pending on some paths, they make
     // the compiler wait for every outstanding memory access at each a[s]'s
@@ -299,10 +305,17 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
         // selects, so their LDS loads are in flight together); a wave skips
         // a batch whose columns all lie at or before j, a work-group whose
         // rows all do skips the product.
-        const bool on0 = i0 > j && i0 < n, on1 = i1 > j && i1 < n;
+        bool on[RPL], any_on = false;
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) {
+            on[h] = i0 + LR * h > j && i0 + LR * h < n;
+            any_on = any_on || on[h];
+        }
         const bool row_on = it > j && it < n;   // thread t < 16's row
         const bool wg_on = kTrdRows * w + kTrdRows - 1 > j;
-        T sp0 = sc_zero<T>(), sp1 = sc_zero<T>();
+        T sp[RPL];
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) sp[h] = sc_zero<T>();
         if (wg_on) {
 #pragma unroll
             for (int s0 = 0; s0 < S2; s0 += QB) {
@@ -316,24 +329,23 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
                     for (int q = 0; q < QB; ++q) {
                         const int c = g + GC * (s0 + q);
                         vc[q] = sc_sel(c > j && c < n, vc[q], sc_zero<T>());
-                        sc_fma_acc(sp0, a[s0 + q][0], vc[q]);
-                        sc_fma_acc(sp1, a[s0 + q][1], vc[q]);
+#pragma unroll
+                        for (int h = 0; h < RPL; ++h) sc_fma_acc(sp[h], a[s0 + q][h], vc[q]);
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        // sums over the eight column groups of the wave, rows r and r + 8
-        sp0 = on0 ? sp0 : sc_zero<T>();
-        sp1 = on1 ? sp1 : sc_zero<T>();
+        // sums over the column groups of the wave, each of the lane's rows
 #pragma unroll
-        for (int o = 8; o < 64; o <<= 1) {
-            sp0 = sc_add(sp0, sc_shfl_xor(sp0, o));
-            sp1 = sc_add(sp1, sc_shfl_xor(sp1, o));
+        for (int h = 0; h < RPL; ++h) {
+            sp[h] = on[h] ? sp[h] : sc_zero<T>();
+#pragma unroll
+            for (int o = LR; o < 64; o <<= 1) sp[h] = sc_add(sp[h], sc_shfl_xor(sp[h], o));
         }
-        if (lane < 8) {
-            red[wave][lane] = sp0;
-            red[wave][lane + 8] = sp1;
+        if (lane < LR) {
+#pragma unroll
+            for (int h = 0; h < RPL; ++h) red[wave][lane + LR * h] = sp[h];
         }
         __syncthreads();
         if (t < kTrdRows) {
@@ -423,11 +435,16 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
         // before the next wait on its memory accesses.
         if (t < kTrdRows && row_on) V[(size_t)j * n + it] = vl[it];
         __syncthreads();
-        if (wg_on && (on0 || on1)) {
+        if (wg_on && any_on) {
             // rows at or before j keep their values (the selects); their v
             // and w entries may be stale or unset
-            const T v0 = vl[i0 < n ? i0 : 0], w0 = pl[i0 < n ? i0 : 0];
-            const T v1 = vl[i1 < n ? i1 : 0], w1 = pl[i1 < n ? i1 : 0];
+            T vr[RPL], wr[RPL];
+#pragma unroll
+            for (int h = 0; h < RPL; ++h) {
+                const int ih = i0 + LR * h < n ? i0 + LR * h : 0;
+                vr[h] = vl[ih];
+                wr[h] = pl[ih];
+            }
 #pragma unroll
             for (int s0 = 0; s0 < S2; s0 += QB) {
                 if (gtop + GC * (s0 + QB - 1) > j) {
@@ -447,10 +464,11 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
                     for (int q = 0; q < QB; ++q) {
                         const int c = g + GC * (s0 + q);
                         const bool col = c > j && c < n;
-                        const T x0 = sc_rank2(a[s0 + q][0], v0, wc[q], w0, vc[q]);
-                        const T x1 = sc_rank2(a[s0 + q][1], v1, wc[q], w1, vc[q]);
-                        a[s0 + q][0] = sc_sel(col && on0, x0, a[s0 + q][0]);
-                        a[s0 + q][1] = sc_sel(col && on1, x1, a[s0 + q][1]);
+#pragma unroll
+                        for (int h = 0; h < RPL; ++h) {
+                            const T x = sc_rank2(a[s0 + q][h], vr[h], wc[q], wr[h], vc[q]);
+                            a[s0 + q][h] = sc_sel(col && on[h], x, a[s0 + q][h]);
+                        }
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -460,10 +478,15 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
         // (columns up to j+1 are staged too and never published)
         {
             const int rj = (j + 2) & (kTrdRows - 1);
-            if (w == ((j + 2) >> 4) && r == (rj & 7) && j + 2 < n) {
-                const bool hi = rj >= 8;
+            if (w == ((j + 2) >> 4) && r == rj % LR && j + 2 < n) {
+                const int hr = rj / LR;
 #pragma unroll
-                for (int s = 0; s < S2; ++s) stg[g + GC * s] = sc_sel(hi, a[s][1], a[s][0]);
+                for (int s = 0; s < S2; ++s) {
+                    T x = a[s][0];
+#pragma unroll
+                    for (int h = 1; h < RPL; ++h) x = sc_sel(hr == h, a[s][h], x);
+                    stg[g + GC * s] = x;
+                }
             }
         }
         // row j+1 of the updated matrix (v_{j+1} = 1): the next pivot row,
