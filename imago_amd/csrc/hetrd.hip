@@ -224,7 +224,8 @@ __global__ __launch_bounds__(256) void k_potf2_diag(T *__restrict__ A, int n, si
                                                     int jb, int *__restrict__ info)
 {
     constexpr int B = kPotrfBlock;
-    static_assert(B == 64, "one column per lane of a wave, four row phases");
+    constexpr int NP = 256 / B;   // row phases: a thread updates rows q + NP i of its column
+    static_assert(256 % B == 0 && B <= 64, "columns of the block over the work-group");
     __shared__ T a[B][B + 1];   // a[c][r] = A(j + r, j + c)
     __shared__ int first;
     const int t = threadIdx.x, m = blockIdx.x;
@@ -235,8 +236,8 @@ __global__ __launch_bounds__(256) void k_potf2_diag(T *__restrict__ A, int n, si
     }
     if (t == 0) first = j == 0 ? 0 : info[m];
     __syncthreads();
-    // thread t updates column c = t % 64 at the rows q + 4i, q = t / 64
-    const int c = t & (B - 1), q = t >> 6;
+    // thread t updates column c = t % B at the rows q + NP i, q = t / B
+    const int c = t % B, q = t / B;
     for (int k = 0; k < jb; ++k) {
         const double akk = sc_re(a[k][k]);
         const double d = sqrt(akk);
@@ -251,8 +252,8 @@ __global__ __launch_bounds__(256) void k_potf2_diag(T *__restrict__ A, int n, si
         if (c > k && c < jb) {
             const T ukc = a[c][k];
 #pragma unroll
-            for (int i = 0; i < B / 4; ++i) {
-                const int r = q + 4 * i;
+            for (int i = 0; i < B / NP; ++i) {
+                const int r = q + NP * i;
                 if (r > k && r <= c) a[c][r] = sc_sub(a[c][r], sc_mul(sc_conj(a[r][k]), ukc));
             }
         }

@@ -11,7 +11,7 @@ namespace imago {
 constexpr int kTrdMaxOrderComplex = 1024;
 constexpr int kTrdMaxOrderReal = 2048;
 constexpr int kTrdBlock = 64;   // reflectors per compact-WY block of the back-transform
-constexpr int kPotrfBlock = 64;  // columns per diagonal block of the blocked Cholesky
+constexpr int kPotrfBlock = 32;  // columns per diagonal block of the blocked Cholesky
 
 // Buffers of one batch; T = double2 (complex Hermitian) or double (real
 // symmetric) for every pointer typed void here.

@@ -102,7 +102,7 @@ def test_not_positive_definite_overlap_is_reported(gpu_ctx):
 @pytest.mark.parametrize("cx", [True, False])
 def test_not_positive_definite_reports_the_leading_minor(gpu_ctx, cx):
     """zpotrf's info through the blocked Cholesky: the first pivot that is
-    not positive sits at order 71 (the second diagonal block), so the error
+    not positive sits at order 71 (past the first diagonal blocks), so the error
     carries ZHEGV's code n + 71; a later negative pivot does not replace it."""
     n = 100
     rng = np.random.default_rng(6)
