@@ -315,8 +315,10 @@ class Context:
         return n.value, f.value
 
     # -- single problems ----------------------------------------------------
-    def hegv(self, H, S):
-        """zhegv/dsygv itype=1 'V' 'U' on the GPU: returns (w, vectors)."""
+    def hegv(self, H, S, return_factor=False):
+        """zhegv/dsygv itype=1 'V' 'U' on the GPU: returns (w, vectors), and
+        with return_factor B as zhegv leaves it (U of S = U^H U in the upper
+        triangle, the strictly lower triangle of S unchanged)."""
         n = H.shape[0]
         info = ctypes.c_int()
         w = np.zeros(n)
@@ -330,4 +332,4 @@ class Context:
             B = np.asfortranarray(S, dtype=np.float64).copy(order="F")
             _check(load().imago_dsygv(self._h, n, A.ctypes.data_as(_dp), B.ctypes.data_as(_dp),
                                       _p(w), ctypes.byref(info)))
-        return w, A
+        return (w, A, B) if return_factor else (w, A)

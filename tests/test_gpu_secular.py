@@ -100,6 +100,30 @@ def test_not_positive_definite_overlap_is_reported(gpu_ctx):
 
 
 @pytest.mark.parametrize("cx", [True, False])
+@pytest.mark.parametrize("n", [2, 33, 97, 300])
+def test_cholesky_factor_matches_lapack(gpu_ctx, cx, n):
+    """The blocked Cholesky (32-column diagonal blocks, ragged last block)
+    against LAPACK's zpotrf / dpotrf through scipy: B comes back from hegv
+    with U (S = U^H U) in its upper triangle at 1e-12 relative, and its
+    strictly lower triangle as it went in (zhegv's contract)."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(11 + n)
+    R = rng.normal(size=(n, n))
+    M = rng.normal(size=(n, n))
+    if cx:
+        R = R + 1j * rng.normal(size=(n, n))
+        M = M + 1j * rng.normal(size=(n, n))
+    S = R @ R.conj().T + n * np.eye(n)
+    H = M + M.conj().T
+    print("[gpu] cholesky n=%d cx=%s" % (n, cx), flush=True)
+    _, _, got = gpu_ctx.hegv(H, S, return_factor=True)
+    ref = sl.cholesky(S, lower=False)
+    assert np.max(np.abs(np.triu(got) - ref)) < 1e-12 * np.max(np.abs(ref))
+    lower = np.tril(np.ones((n, n), dtype=bool), -1)
+    assert np.array_equal(got[lower], S[lower])
+
+
+@pytest.mark.parametrize("cx", [True, False])
 def test_not_positive_definite_reports_the_leading_minor(gpu_ctx, cx):
     """zpotrf's info through the blocked Cholesky: the first pivot that is
     not positive sits at order 71 (past the first diagonal blocks), so the error
