@@ -61,11 +61,14 @@ def parse():
     p.add_argument("--ep-mode", choices=("direct", "panel"), default="direct",
                    help="direct: EP terms from the stored primitive blocks each iteration "
                         "(IMAGO_EP_DIRECT); panel: all potDim EP panels stored (IMAGO_EP)")
-    p.add_argument("--stage", choices=("step", "elecstat", "xcmesh"), default="step",
-                   help="step: the hot-path step (the benchmark line); elecstat: the "
+    p.add_argument("--stage", choices=("scf", "step", "elecstat", "xcmesh"), default="scf",
+                   help="scf: SCF to convergence (the benchmark line); step: one hot-path "
+                        "pass (set-up + one secular/density iteration); elecstat: the "
                         "electrostatics set-up (imago_elecstat + imago_residual_q); xcmesh: "
                         "the exchange-correlation mesh and overlap (imago_xc_mesh + "
                         "imago_xc_overlap)")
+    p.add_argument("--max-iter", type=int, default=50,
+                   help="LAST_ITERATION of the SCF (makeinputrc's 50)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain 4 GiB device-to-device copy kernel (scale factor "
                         "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -417,6 +420,75 @@ def bench_xcmesh(args, system, ctx, log):
     print(json.dumps(line), flush=True)
 
 
+def bench_scf(args, system, coeffs, ctx, log):
+    """SCF wall time to convergence: imago_set_system, the four integral
+    kinds, makeSCFPot's set-up, then secular / populate / traces /
+    makeSCFPot iterations until testableDelta < 1e-4 and |dE| < 1e-6 Ha."""
+    from imago_amd import scf
+    ep_kind = "ep_direct" if args.ep_mode == "direct" else "ep"
+    runs = []
+    for i in range(args.warmup + args.steps):
+        res = scf.run(ctx, system, coeffs, max_iter=args.max_iter, ep_kind=ep_kind)
+        tag = "warmup" if i < args.warmup else "step"
+        log("%s %d: %d iterations, converged %s, E = %.10f Ha, set-up %.1f ms, loop %.1f ms, "
+            "total %.1f ms" % (tag, i, res.iterations, res.converged, res.total_energy,
+                               res.setup_s * 1e3, res.loop_s * 1e3, res.total_s * 1e3))
+        if i >= args.warmup:
+            runs.append(res)
+    last = runs[-1]
+    for r in last.records:
+        log("  iter %2d  E %.10f  dE %s  delta %.3e  %.2f ms" % (
+            r["iteration"], r["total_energy"],
+            "-" if r["energy_change"] is None else "%.2e" % r["energy_change"],
+            r["testable_delta"], r["wall_s"] * 1e3))
+    wall = float(np.mean([r.total_s for r in runs]))
+    n, nk = system.vale_dim, system.num_kpoints
+    # per-iteration stages of the last iteration (HIP events)
+    st = {"hbuild": ctx.timing(1), "eigensolve": ctx.timing(2), "k_hetrd_coop": ctx.timing(10),
+          "traces": ctx.timing(3), "xc_potential": ctx.timing(16)}
+    kernels = {}
+    if st["k_hetrd_coop"] > 0.0:
+        kernels["k_hetrd_coop"] = {
+            "bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "ms": st["k_hetrd_coop"],
+            "flop": (4.0 / 3.0 if system.gamma else 16.0 / 3.0) * n ** 3 * nk}
+    kernels["eigensolve"] = {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS,
+                             "ms": st["eigensolve"],
+                             "flop": (8.0 if system.gamma else 32.0) * n ** 3 * nk}
+    for v in kernels.values():
+        v["achieved"] = v["flop"] / (v["ms"] * 1e-3) / 1e12
+    dom = "k_hetrd_coop" if "k_hetrd_coop" in kernels else "eigensolve"
+    d = kernels[dom]
+    roofline = {"kernel": dom, "bound": d["bound"], "achieved": round(d["achieved"], 3),
+                "peak": d["peak"], "unit": d["unit"], "frac": round(d["achieved"] / d["peak"], 5),
+                "traffic": profiled_traffic(dom, args.workload)}
+    line = {
+        "metric": METRIC,
+        "value": round(wall, 6),
+        "unit": "s per SCF to 1e-6 Ha (set-up + %d iterations, plain GPU method, seeded "
+                "initial potential)" % last.iterations,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded %s, random-normalised contractions)" % system.name,
+        "config": {"workload": "%s: %d atoms, n=%d, %d k-point(s)%s, potDim=%d, SCF to "
+                               "testableDelta < 1e-4 and |dE| < 1e-6 Ha"
+                               % (system.name, system.num_atoms, n, nk,
+                                  " (Gamma)" if system.gamma else "", system.pot_dim),
+                   "ep_mode": args.ep_mode, "parallelism": "single GPU"},
+        "scf": {"iterations": last.iterations, "converged": last.converged,
+                "total_energy": last.total_energy,
+                "setup_ms": round(last.setup_s * 1e3, 3),
+                "loop_ms": round(last.loop_s * 1e3, 3),
+                "ms_per_iteration": round(last.loop_s * 1e3 / max(1, last.iterations), 3)},
+        "stages_ms": {k: round(v, 3) for k, v in st.items()},
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                        for kk, vv in v.items()} for k, v in kernels.items()},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -438,7 +510,13 @@ def main():
     coeffs = configs.pot_coefficients(system, seed=7)
     num_states = system.vale_dim
     ep_kind = "ep_direct" if args.ep_mode == "direct" else "ep"
-    ctx = Context(0)
+    # the context is closed before the interpreter exits, so its streams,
+    # library handles and device buffers never go through exit handlers
+    with Context(0) as ctx:
+        run(args, system, coeffs, num_states, ep_kind, ctx)
+
+
+def run(args, system, coeffs, num_states, ep_kind, ctx):
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     log("system %s n=%d nk=%d potDim=%d" % (system.name, system.vale_dim, system.num_kpoints,
                                           system.pot_dim))
@@ -449,6 +527,9 @@ def main():
     if args.stage == "xcmesh":
         ctx.set_system(system)
         bench_xcmesh(args, system, ctx, log)
+        return
+    if args.stage == "scf":
+        bench_scf(args, system, coeffs, ctx, log)
         return
     for i in range(args.warmup):
         dt, st, _ = gpu_step(ctx, system, coeffs, num_states, populate, ep_kind)

@@ -97,10 +97,13 @@ __global__ __launch_bounds__(256) void k_trd_owner(const double *__restrict__ Z,
 }
 
 // rank[q] = number of q' < q with owner[q'] == owner[q]; one work-group of
-// 1024 threads, an exclusive scan of the flags of each block in turn
+// 1024 threads, an exclusive scan of the flags of each block in turn.  A
+// block that does not own exactly n vectors (N = nb * n) sets bit 2 of err:
+// its matrix would keep stale columns.
 __global__ __launch_bounds__(1024) void k_trd_rank(const int *__restrict__ owner, int N, int nb,
-                                                   int *__restrict__ rank)
+                                                   int *__restrict__ rank, int *__restrict__ err)
 {
+    const int n = N / nb;
     __shared__ int sc[1024];
     const int t = threadIdx.x;
     const int per = (N + 1023) / 1024;
@@ -120,6 +123,8 @@ __global__ __launch_bounds__(1024) void k_trd_rank(const int *__restrict__ owner
         int base = sc[t] - cnt;
         for (int q = q0; q < q1; ++q)
             if (owner[q] == b) rank[q] = base++;
+        if (t == 1023 && sc[1023] != n)
+            __hip_atomic_fetch_or(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
     }
 }
@@ -144,12 +149,12 @@ __global__ __launch_bounds__(256) void k_trd_gather(const double *__restrict__ Z
 }
 
 hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, int b0,
-                            int *owner, int *rank, double *w, void *X, bool real,
+                            int *owner, int *rank, double *w, void *X, bool real, int *err,
                             hipStream_t stream)
 {
     const int N = n * nb;
     hipLaunchKernelGGL(k_trd_owner, dim3(N), dim3(256), 0, stream, Z, N, n, owner);
-    hipLaunchKernelGGL(k_trd_rank, dim3(1), dim3(1024), 0, stream, owner, N, nb, rank);
+    hipLaunchKernelGGL(k_trd_rank, dim3(1), dim3(1024), 0, stream, owner, N, nb, rank, err);
     if (real)
         hipLaunchKernelGGL(k_trd_gather<double>, dim3(N), dim3(256), 0, stream, Z, wall, N, n,
                            owner, rank, w + (size_t)b0 * n,

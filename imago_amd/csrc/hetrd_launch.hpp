@@ -44,9 +44,10 @@ hipError_t launch_hetrd_coop(const TrdLaunch &L);
 // order N): owner[q] = the block holding eigenvector q (its largest
 // entry), then for every block its eigenvalues in order into w[b0 + b][0..n)
 // and its vectors into X[b0 + b] (n x n, column-major; complex with zero
-// imaginary parts, or real when real is set).
+// imaginary parts, or real when real is set).  Bit 2 of *err is set when a
+// block does not own exactly n eigenvectors.
 hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, int b0,
-                            int *owner, int *rank, double *w, void *X, bool real,
+                            int *owner, int *rank, double *w, void *X, bool real, int *err,
                             hipStream_t stream);
 // Compact-WY factors of the reflectors of every matrix: T[m][b] (kTrdBlock
 // x kTrdBlock, upper, column-major) with H(j0) ... H(j0 + nbk - 1) =
@@ -54,5 +55,26 @@ hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, 
 // columnwise; T holds G = V_b^H V_b on entry (the strict upper triangle is
 // read) and T on exit.
 hipError_t launch_trd_larft(const void *tau, int n, int nb, bool real, void *T, hipStream_t stream);
+
+// Per-matrix eigensolver codes from the divide and conquer of one half of
+// a batch.  The half's nh matrices of order n were solved as one
+// block-diagonal problem of order N = nh * n; a nonzero dstedc info names
+// the failing submatrix as rows info / (N + 1) .. mod(info, N + 1) (1-based,
+// LAPACK dstedc).  Every matrix of the half that overlaps those rows gets n
+// (zheevd's range 1..n for "did not converge"), the others keep 0; an info
+// that names no valid row range marks the whole half.  kinfo points at the
+// half's first matrix.
+inline void map_stedc_info(int n, int nh, int info, int *kinfo)
+{
+    for (int m = 0; m < nh; ++m) kinfo[m] = 0;
+    if (info == 0 || nh <= 0 || n <= 0) return;
+    const long N = (long)n * nh;
+    long r0 = (long)info / (N + 1), r1 = (long)info % (N + 1);
+    if (info < 0 || r0 < 1 || r1 < r0 || r1 > N) {
+        r0 = 1;
+        r1 = N;
+    }
+    for (long m = (r0 - 1) / n; m <= (r1 - 1) / n; ++m) kinfo[m] = n;
+}
 
 }  // namespace imago

@@ -32,6 +32,7 @@
 #include "hetrd_launch.hpp"
 #include "elecstat_launch.hpp"
 #include "xcmesh_launch.hpp"
+#include "populate.hpp"
 
 using namespace imago;
 
@@ -235,6 +236,20 @@ struct imago_ctx {
         int info = 0;
     };
     SpdMat spd[2];
+    // makeSCFPot state (imago_scf_setup / imago_scf_update): the
+    // electrostatic operators on the device, the per-iteration vectors on
+    // the host, the mixing history the reference keeps in module variables
+    // (xl1, xl2, yl1, yl2; potentialUpdate.F90:38-45)
+    struct ScfState {
+        bool ready = false;
+        imago_scf_params p{};
+        std::vector<double> intg, nuc_nl, nuc_l;
+        double *neut_nl = nullptr, *neut_l = nullptr, *resq = nullptr;   // device
+        double *vec = nullptr;      // device scratch, 8 potDim
+        int iteration = 1;
+        bool have_hist = false;
+        std::vector<double> xl1, xl2, yl1, yl2;
+    } scf;
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -289,6 +304,7 @@ static void ctx_free_system(imago_ctx *c)
     c->xc_cache.clear();
     c->xc_cached = false;
     for (auto &m : c->spd) m = imago_ctx::SpdMat{};
+    c->scf = imago_ctx::ScfState{};
     c->has_system = false;
 }
 
@@ -850,7 +866,13 @@ extern "C" int imago_ctx_create(int device, imago_ctx **out)
 extern "C" int imago_ctx_destroy(imago_ctx *c)
 {
     if (!c) return IMAGO_OK;
-    (void)hipSetDevice(c->device);
+    // a runtime that is already being torn down (a destroy from an exit
+    // handler) refuses the device: the device memory goes with the process,
+    // only the host side is freed
+    if (hipSetDevice(c->device) != hipSuccess) {
+        delete c;
+        return IMAGO_OK;
+    }
     (void)hipStreamSynchronize(c->stream);
     ctx_free_system(c);
     c->sysbuf.purge_cache();
@@ -1448,7 +1470,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     double *Z = nullptr, *wall = nullptr;
     int *owner = nullptr, *rank = nullptr;
     unsigned *bar = nullptr;
-    int *err = nullptr;
+    int *err = nullptr, *sinfo = nullptr;
     const int h0 = (nb + 1) / 2, h1 = nb - h0;
     const int N0 = h0 * n, N1 = h1 * n;
     // V carries K*n zero elements past its last matrix: the GEMM forming the
@@ -1467,7 +1489,9 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     RCHK(dev_alloc(c, &W2, (size_t)nb * K * n * es));
     RCHK(dev_alloc(c, &bar, 1));
     RCHK(dev_alloc(c, &err, 1));
+    RCHK(dev_alloc(c, &sinfo, 2));
     HIPCHK(hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(sinfo, 0, 2 * sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync(dinfo, 0, sizeof(int) * nb, c->stream));
     HIPCHK(hipMemsetAsync(V, 0, es * (nn * nb + (size_t)K * n), c->stream));
     TrdLaunch L{n,     nb,   !cx,  A,    V,   evals, ework, tau,
@@ -1485,7 +1509,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         (void)hipGetLastError();
         for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank,
                         (void *)tau, (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf,
-                        (void *)W, (void *)W2, (void *)bar, (void *)err})
+                        (void *)W, (void *)W2, (void *)bar, (void *)err, (void *)sinfo})
             dev_release(c, p);
         return kCoopRefused;
     }
@@ -1509,15 +1533,15 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         HIPCHK(hipEventRecord(c->evf, c->stream));
         HIPCHK(hipStreamWaitEvent(c->stream2, c->evf, 0));
         BLASCHK(rocsolver_dstedc(c->blas2, rocblas_evect_tridiagonal, N1, wall + N0, ework + N0,
-                                 Z1, N1, dinfo + 1));
+                                 Z1, N1, sinfo + 1));
         HIPCHK(hipEventRecord(c->evj, c->stream2));
     }
-    BLASCHK(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, N0, wall, ework, Z, N0, dinfo));
+    BLASCHK(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, N0, wall, ework, Z, N0, sinfo));
     if (h1 > 0) HIPCHK(hipStreamWaitEvent(c->stream, c->evj, 0));
-    HIPCHK(launch_trd_unmix(Z, wall, n, h0, 0, owner, rank, evals, A, !cx, c->stream));
+    HIPCHK(launch_trd_unmix(Z, wall, n, h0, 0, owner, rank, evals, A, !cx, err, c->stream));
     if (h1 > 0)
         HIPCHK(launch_trd_unmix(Z1, wall + N0, n, h1, h0, owner + N0, rank + N0, evals, A, !cx,
-                                c->stream));
+                                err, c->stream));
     const rocblas_stride snn = (rocblas_stride)nn, sw = (rocblas_stride)K * n,
                          st = (rocblas_stride)std::max(nblk, 1) * K * K;
     // G_b = V_b^H V_b for every block of each matrix (one strided-batched
@@ -1543,17 +1567,28 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_none, rocblas_operation_none, mb, n, kb,
                         -1.0, Vb, n, snn, W2, K, sw, 1.0, Yb, n, snn, nb));
     }
-    int herr = 0;
+    int herr = 0, hs[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(hs, sinfo, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float tms = 0.0f;
     HIPCHK(hipEventElapsedTime(&tms, c->evh0, c->evh1));
     c->timing[10] = tms;
     for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank, (void *)tau,
                     (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf, (void *)W, (void *)W2,
-                    (void *)bar, (void *)err})
+                    (void *)bar, (void *)err, (void *)sinfo})
         dev_release(c, p);
-    if (herr) return fail(IMAGO_E_HIP, "tridiagonal reduction: a barrier wait timed out");
+    if (herr & 1) return fail(IMAGO_E_HIP, "tridiagonal reduction: a barrier wait timed out");
+    if (herr & 2)
+        return fail(IMAGO_E_HIP, "eigenvector sort: a matrix did not receive exactly n vectors");
+    if (hs[0] != 0 || hs[1] != 0) {
+        // divide and conquer failed in a half: its matrices report zheevd's
+        // "did not converge" (1..n) through dinfo, read by the caller
+        std::vector<int> kinfo(nb, 0);
+        map_stedc_info(n, h0, hs[0], kinfo.data());
+        if (h1 > 0) map_stedc_info(n, h1, hs[1], kinfo.data() + h0);
+        HIPCHK(hipMemcpy(dinfo, kinfo.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    }
     return IMAGO_OK;
 }
 
@@ -2019,6 +2054,30 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
 // ---------------------------------------------------------------------------
 // single generalized eigenproblems (solveZHEGV / solveDSYGV)
 
+// A(i, j) = conj(A(j, i)) for i > j and Im A(i, i) = 0: the matrix as zhegv
+// 'U' / dsygv 'U' define it from their upper triangle.  The reference's
+// caller fills only the upper triangle (secularEqn.F90:140 zeroes valeVale,
+// unpackMatrix with fullFlag 0 writes rows <= column, matrixSubs.F90:
+// 426-433), and the solve below reads both triangles.
+template <bool COMPLEX>
+__global__ __launch_bounds__(256) void k_herm_from_upper(double *__restrict__ A, int n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * n) return;
+    const int r = (int)(t % n), col = (int)(t / n);
+    if (COMPLEX) {
+        double2 *a = reinterpret_cast<double2 *>(A);
+        if (r == col) {
+            a[t].y = 0.0;
+        } else if (r > col) {
+            const double2 u = a[(size_t)r * n + col];
+            a[t] = make_double2(u.x, -u.y);
+        }
+    } else if (r > col) {
+        A[t] = A[(size_t)r * n + col];
+    }
+}
+
 static int single_gv(imago_ctx *c, bool cx, int n, double *A, double *B, double *w, int *info)
 {
     if (!c) return fail(IMAGO_E_ARG, "null context");
@@ -2034,6 +2093,14 @@ static int single_gv(imago_ctx *c, bool cx, int n, double *A, double *B, double 
     HIPCHK(hipMalloc(&di, sizeof(int)));
     HIPCHK(hipMemcpy(dA, A, nn * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dB, B, nn * sizeof(double), hipMemcpyHostToDevice));
+    {
+        const unsigned g = (unsigned)(((long)n * n + 255) / 256);
+        if (cx)
+            hipLaunchKernelGGL(k_herm_from_upper<true>, dim3(g), dim3(256), 0, c->stream, dA, n);
+        else
+            hipLaunchKernelGGL(k_herm_from_upper<false>, dim3(g), dim3(256), 0, c->stream, dA, n);
+        HIPCHK(hipGetLastError());
+    }
     int inf = 0;
     int rc = solve_gv_batched(c, cx, n, 1, dA, dB, dw, de, di, &inf);
     // B returns the factor in its upper triangle and, as in LAPACK, its
@@ -2130,13 +2197,12 @@ static int es_args(imago_ctx *c, double th, imago::EsArgs &A, const double **dsi
     return IMAGO_OK;
 }
 
-extern "C" int imago_elecstat(imago_ctx *c, double log_elec_thresh, double *pot_alpha_overlap,
-                              double *non_local_neut_q_pot, double *local_neut_q_pot,
-                              double *non_local_nuc_q_pot, double *local_nuc_q_pot)
+// neutralAndNuclearQPot into the device buffer *out: potAlphaOverlap,
+// nonLocalNeutQPot, localNeutQPot (potDim x potDim each), nonLocalNucQPot,
+// localNucQPot (potDim each); potAlphaOverlap is also kept for
+// imago_pot_solve.  The caller releases *out.
+static int elecstat_run(imago_ctx *c, double log_elec_thresh, double **out)
 {
-    if (!pot_alpha_overlap || !non_local_neut_q_pot || !local_neut_q_pot ||
-        !non_local_nuc_q_pot || !local_nuc_q_pot)
-        return fail(IMAGO_E_ARG, "null output array");
     imago::EsArgs A;
     const double *dsizes = nullptr;
     if (c) HIPCHK(hipSetDevice(c->device));
@@ -2148,6 +2214,33 @@ extern "C" int imago_elecstat(imago_ctx *c, double log_elec_thresh, double *pot_
     HIPCHK(imago::launch_es_neutral_nuclear(A, buf, buf + PP, buf + 2 * PP, buf + 3 * PP,
                                             buf + 3 * PP + P, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
+    // potAlphaOverlap stays on the device for imago_pot_solve
+    imago_ctx::SpdMat &m = c->spd[0];
+    for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.s})
+        if (q) dev_release(c, q);
+    m = imago_ctx::SpdMat{};
+    RCHK(dev_alloc(c, &m.A, PP));
+    HIPCHK(hipMemcpyAsync(m.A, buf, PP * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timing[11] = ms;
+    dev_release(c, (void *)dsizes);
+    dev_release(c, A.msep);
+    *out = buf;
+    return IMAGO_OK;
+}
+
+extern "C" int imago_elecstat(imago_ctx *c, double log_elec_thresh, double *pot_alpha_overlap,
+                              double *non_local_neut_q_pot, double *local_neut_q_pot,
+                              double *non_local_nuc_q_pot, double *local_nuc_q_pot)
+{
+    if (!pot_alpha_overlap || !non_local_neut_q_pot || !local_neut_q_pot ||
+        !non_local_nuc_q_pot || !local_nuc_q_pot)
+        return fail(IMAGO_E_ARG, "null output array");
+    double *buf = nullptr;
+    RCHK(elecstat_run(c, log_elec_thresh, &buf));
+    const size_t P = (size_t)c->h.pot_dim, PP = P * P;
     HIPCHK(hipMemcpyAsync(pot_alpha_overlap, buf, PP * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(non_local_neut_q_pot, buf + PP, PP * 8, hipMemcpyDeviceToHost,
                           c->stream));
@@ -2158,29 +2251,17 @@ extern "C" int imago_elecstat(imago_ctx *c, double log_elec_thresh, double *pot_
     HIPCHK(hipMemcpyAsync(local_nuc_q_pot, buf + 3 * PP + P, P * 8, hipMemcpyDeviceToHost,
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->timing[11] = ms;
-    // potAlphaOverlap stays on the device for imago_pot_solve
-    imago_ctx::SpdMat &m = c->spd[0];
-    for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.s})
-        if (q) dev_release(c, q);
-    m = imago_ctx::SpdMat{};
-    RCHK(dev_alloc(c, &m.A, PP));
-    HIPCHK(hipMemcpyAsync(m.A, buf, PP * 8, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
     dev_release(c, buf);
-    dev_release(c, (void *)dsizes);
-    dev_release(c, A.msep);
     return IMAGO_OK;
 }
 
-extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_cells_recip,
-                                const double *cell_dims_recip, const double *cell_sizes_recip,
-                                double *non_local_residual_q)
+// residualQ into the device buffer *outp (numPotTypes x potDim, released by
+// the caller)
+static int residual_q_run(imago_ctx *c, double log_elec_thresh, int num_cells_recip,
+                          const double *cell_dims_recip, const double *cell_sizes_recip,
+                          double **outp)
 {
-    if (!cell_dims_recip || !cell_sizes_recip || !non_local_residual_q)
-        return fail(IMAGO_E_ARG, "null array");
+    if (!cell_dims_recip || !cell_sizes_recip) return fail(IMAGO_E_ARG, "null array");
     if (num_cells_recip < 1) return fail(IMAGO_E_ARG, "no reciprocal cells");
     for (int q = 1; q < num_cells_recip; ++q)
         if (!(cell_sizes_recip[q] > 0.0))
@@ -2312,7 +2393,6 @@ extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_ce
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(imago::launch_es_residual_q(A, R, out, c->stream, c->ev2, c->ev3));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipMemcpyAsync(non_local_residual_q, out, nout * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms = 0.f, ms_r = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -2323,8 +2403,23 @@ extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_ce
                           (const void *)dsrc,
                           (const void *)dsite_src, (const void *)dord, (const void *)dblk,
                           (const void *)E, (const void *)CS, (const void *)S,
-                          (const void *)out, (const void *)dsizes, (const void *)A.msep})
+                          (const void *)dsizes, (const void *)A.msep})
         dev_release(c, const_cast<void *>(p));
+    *outp = out;
+    return IMAGO_OK;
+}
+
+extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_cells_recip,
+                                const double *cell_dims_recip, const double *cell_sizes_recip,
+                                double *non_local_residual_q)
+{
+    if (!non_local_residual_q) return fail(IMAGO_E_ARG, "null array");
+    double *out = nullptr;
+    RCHK(residual_q_run(c, log_elec_thresh, num_cells_recip, cell_dims_recip, cell_sizes_recip,
+                        &out));
+    const size_t nout = (size_t)c->h.npt * c->h.pot_dim;
+    HIPCHK(hipMemcpy(non_local_residual_q, out, nout * 8, hipMemcpyDeviceToHost));
+    dev_release(c, out);
     return IMAGO_OK;
 }
 
@@ -2558,11 +2653,12 @@ extern "C" int imago_xc_rho_op(imago_ctx *c, int site, int gga, int *num_ray_poi
     return IMAGO_OK;
 }
 
-extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
+// exchCorrOverlap, kept on the device for imago_pot_solve; copied to
+// exch_corr_overlap unless it is null
+static int xc_overlap_run(imago_ctx *c, double *exch_corr_overlap)
 {
     if (!c) return fail(IMAGO_E_ARG, "null context");
     if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
-    if (!exch_corr_overlap) return fail(IMAGO_E_ARG, "null output array");
     HIPCHK(hipSetDevice(c->device));
     const int P = c->h.pot_dim, ns = c->h.nsites;
     const int mp = std::max(c->xc_maxpts, 1);
@@ -2657,8 +2753,9 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     int herr = 0;
     HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(exch_corr_overlap, C, (size_t)P * P * 8, hipMemcpyDeviceToHost,
-                          c->stream));
+    if (exch_corr_overlap)
+        HIPCHK(hipMemcpyAsync(exch_corr_overlap, C, (size_t)P * P * 8, hipMemcpyDeviceToHost,
+                              c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -2687,6 +2784,12 @@ extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
         return fail(IMAGO_E_HIP, "exchange-correlation overlap: a point touched a term outside "
                                  "its site's row bound (internal error)");
     return IMAGO_OK;
+}
+
+extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
+{
+    if (!exch_corr_overlap) return fail(IMAGO_E_ARG, "null output array");
+    return xc_overlap_run(c, exch_corr_overlap);
 }
 
 // The operator blocks of every batch of sites (rows bounded from the
@@ -2942,6 +3045,327 @@ extern "C" int imago_xc_counts(imago_ctx *c, long *ray_points, double *gemm_flop
     if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
     if (ray_points) *ray_points = c->xc_total;
     if (gemm_flops) *gemm_flops = c->xc_gemm_flops;
+    return IMAGO_OK;
+}
+
+// ---- the SCF iteration: populateStandard and makeSCFPot -----------------
+
+extern "C" int imago_populate(int num_kpoints, int num_states, const double *eigenvalues,
+                              const double *kpoint_weights, double num_electrons,
+                              double *populations, double *occupied_energy, double *band_gap,
+                              int *gap_kind)
+{
+    if (num_kpoints < 1 || num_states < 1 || !eigenvalues || !kpoint_weights || !populations)
+        return fail(IMAGO_E_ARG, "bad populate arguments");
+    imago::PopulateResult r;
+    if (imago::populate_standard(num_kpoints, 1, num_states, eigenvalues, kpoint_weights,
+                                 num_electrons, populations, &r) != 0)
+        return fail(IMAGO_E_ARG, "%d states at %d k-points cannot hold %g electrons", num_states,
+                    num_kpoints, num_electrons);
+    if (occupied_energy) *occupied_energy = r.occupied_energy;
+    if (band_gap) *band_gap = r.band_gap;
+    if (gap_kind) *gap_kind = r.gap_kind;
+    return IMAGO_OK;
+}
+
+// d_b = B_b u_b for the operator blocks of a batch of sites needs u_b, the
+// potential change at the batch's compact rows (0 past a site's count)
+__global__ __launch_bounds__(256) void k_scf_gather_rows(const double *__restrict__ v,
+                                                         const int *__restrict__ rows,
+                                                         const int *__restrict__ cnt, int P,
+                                                         int mm, int nb, double *__restrict__ u)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= nb * mm) return;
+    const int b = g / mm, r = g % mm;
+    u[g] = r < cnt[b] ? v[rows[(size_t)b * P + r]] : 0.0;
+}
+
+// per site b of a batch: sum_k w_k d_k^2 and sum_k w_k over its points
+// (d_b laid out [b][nn], the point order of the operator blocks)
+__global__ __launch_bounds__(256) void k_scf_site_rms(const double *__restrict__ d, int nn,
+                                                      const double *__restrict__ w,
+                                                      const int *__restrict__ first, int nb,
+                                                      int npb, double *__restrict__ out)
+{
+    __shared__ double s0[256], s1[256];
+    const int b = blockIdx.x;
+    const int k0 = first[b], k1 = b + 1 < nb ? first[b + 1] : npb;
+    double a = 0.0, ww = 0.0;
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += 256) {
+        const double x = d[(size_t)b * nn + (k - k0)];
+        a += w[k] * x * x;
+        ww += w[k];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = ww;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s0[threadIdx.x] += s0[threadIdx.x + h];
+            s1[threadIdx.x] += s1[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * b] = s0[0];
+        out[2 * b + 1] = s1[0];
+    }
+}
+
+extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
+                               const imago_xc_mesh_params *mesh, const double *covalent_radius,
+                               int num_cells_recip, const double *cell_dims_recip,
+                               const double *cell_sizes_recip)
+{
+    if (!c || !c->has_system) return fail(IMAGO_E_STATE, "imago_scf_setup before imago_set_system");
+    if (!params || !mesh || !covalent_radius) return fail(IMAGO_E_ARG, "null argument");
+    if (params->xc_code != 100)
+        return fail(IMAGO_E_ARG, "xc_code %d: only 100 (Wigner, spin 1, LDA) is built",
+                    params->xc_code);
+    if (params->feedback_level < 0 || params->feedback_level > 2)
+        return fail(IMAGO_E_ARG, "feedback level %d", params->feedback_level);
+    HIPCHK(hipSetDevice(c->device));
+    const HostSys &h = c->h;
+    const int P = h.pot_dim;
+    imago_ctx::ScfState &S = c->scf;
+    for (void *q : {(void *)S.neut_nl, (void *)S.resq, (void *)S.vec})
+        if (q) dev_release(c, q);
+    S = imago_ctx::ScfState{};
+    S.p = *params;
+    // intgConsts (potential.f90:150-160)
+    const double pi32 = std::pow(3.141592653589793, 1.5);
+    S.intg.assign(P, 0.0);
+    for (int t = 0; t < h.npt; ++t)
+        for (int a = 0; a < h.pot_nalpha[t]; ++a)
+            S.intg[h.type_term0[t] + a] = (double)h.pot_mult[t] * pi32 /
+                                          std::pow(h.pot_alphas[(size_t)t * h.maxpa + a], 1.5);
+    // electrostatic operators: potAlphaOverlap (kept in spd[0]),
+    // nonLocalNeutQPot and localNeutQPot on the device, the nuclear vectors
+    // on the host
+    double *es = nullptr;
+    RCHK(elecstat_run(c, params->log_elec_thresh, &es));
+    const size_t PP = (size_t)P * P;
+    S.neut_nl = es;
+    S.neut_l = es + 2 * PP;
+    S.nuc_nl.resize(P);
+    S.nuc_l.resize(P);
+    HIPCHK(hipMemcpy(S.nuc_nl.data(), es + 3 * PP, (size_t)P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(S.nuc_l.data(), es + 3 * PP + P, (size_t)P * 8, hipMemcpyDeviceToHost));
+    RCHK(residual_q_run(c, params->log_elec_thresh, num_cells_recip, cell_dims_recip,
+                        cell_sizes_recip, &S.resq));
+    // exchange-correlation mesh, overlap and operator blocks
+    int maxpts = 0;
+    RCHK(imago_xc_mesh(c, mesh, params->log_elec_thresh, covalent_radius, &maxpts));
+    RCHK(xc_overlap_run(c, nullptr));
+    if (!c->xc_cached) RCHK(xc_build_cache(c));
+    RCHK(dev_alloc(c, &S.vec, 8 * (size_t)P));
+    // factor both overlaps now (the first solve would)
+    for (int m = 0; m < 2; ++m)
+        if (!c->spd[m].factored) RCHK(spd_factor(c, c->spd[m], P));
+    S.iteration = 1;
+    S.ready = true;
+    return IMAGO_OK;
+}
+
+// y = op(M) x on the device for an m x n column-major M (trans: M^T x);
+// x and y are host vectors
+static int scf_gemv(imago_ctx *c, bool trans, int m, int n, const double *M, const double *x,
+                    double *y)
+{
+    double *dx = c->scf.vec, *dy = c->scf.vec + 4 * (size_t)c->h.pot_dim;
+    const int lx = trans ? m : n, ly = trans ? n : m;
+    const double one = 1.0, zero = 0.0;
+    HIPCHK(hipMemcpyAsync(dx, x, (size_t)lx * 8, hipMemcpyHostToDevice, c->stream));
+    BLASCHK(rocblas_dgemv(c->blas, trans ? rocblas_operation_transpose : rocblas_operation_none,
+                          m, n, &one, M, m, dx, 1, &zero, dy, 1));
+    HIPCHK(hipMemcpyAsync(y, dy, (size_t)ly * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IMAGO_OK;
+}
+
+// the largest site RMS of the real-space potential change over the ray
+// points (potentialUpdate.F90:1350-1431): per site sqrt(sum w d^2 / sum w)
+// with d = exchRhoOp(:, k) . outputPot at each of its points
+static int scf_testable_delta(imago_ctx *c, const std::vector<double> &outpot, double *delta)
+{
+    const int P = c->h.pot_dim;
+    size_t u_need = 1, d_need = 1;
+    int nbmax = 1;
+    for (const auto &q : c->xc_cache) {
+        u_need = std::max(u_need, (size_t)q.nb * q.mm);
+        d_need = std::max(d_need, (size_t)q.nb * q.nn);
+        nbmax = std::max(nbmax, q.nb);
+    }
+    double *v = nullptr, *u = nullptr, *d = nullptr, *sums = nullptr;
+    RCHK(dev_alloc(c, &v, (size_t)P));
+    RCHK(dev_alloc(c, &u, u_need));
+    RCHK(dev_alloc(c, &d, d_need));
+    RCHK(dev_alloc(c, &sums, 2 * (size_t)c->h.nsites));
+    HIPCHK(hipMemcpyAsync(v, outpot.data(), (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(sums, 0, 2 * (size_t)c->h.nsites * 8, c->stream));
+    for (const auto &q : c->xc_cache) {
+        if (!q.B) continue;
+        const int ng = (q.nb * q.mm + 255) / 256;
+        hipLaunchKernelGGL(k_scf_gather_rows, dim3(ng), dim3(256), 0, c->stream, v, q.rows, q.cnt,
+                           P, q.mm, q.nb, u);
+        HIPCHK(hipGetLastError());
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(rocblas_dgemv_strided_batched(
+            c->blas, rocblas_operation_none, q.nn, q.mm, &one, q.B, q.nn,
+            (rocblas_stride)q.mm * q.nn, u, 1, (rocblas_stride)q.mm, &zero, d, 1,
+            (rocblas_stride)q.nn, q.nb));
+        hipLaunchKernelGGL(k_scf_site_rms, dim3(q.nb), dim3(256), 0, c->stream, d, q.nn,
+                           c->xc_w + q.p0, q.first, q.nb, q.npb, sums + 2 * (size_t)q.s0);
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<double> hs(2 * (size_t)c->h.nsites);
+    HIPCHK(hipMemcpyAsync(hs.data(), sums, hs.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (void *x : {(void *)v, (void *)u, (void *)d, (void *)sums}) dev_release(c, x);
+    double best = 0.0;
+    for (int i = 0; i < c->h.nsites; ++i)
+        if (c->xc_npts[i] > 0) best = std::max(std::sqrt(hs[2 * i] / hs[2 * i + 1]), best);
+    *delta = best;
+    return IMAGO_OK;
+}
+
+extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const double *pot_rho,
+                                double *pot_coeffs, double *rho_coeffs,
+                                imago_scf_record *record)
+{
+    if (!c || !c->scf.ready) return fail(IMAGO_E_STATE, "imago_scf_update before imago_scf_setup");
+    if (!traces3 || !pot_rho || !pot_coeffs || !record) return fail(IMAGO_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const HostSys &h = c->h;
+    imago_ctx::ScfState &S = c->scf;
+    const int P = h.pot_dim;
+    const std::vector<double> &ic = S.intg;
+    auto dot = [P](const double *a, const double *b) {
+        double t = 0.0;
+        for (int i = 0; i < P; ++i) t += a[i] * b[i];
+        return t;
+    };
+    // charge fit: potAlphaOverlap [g1 g2] = [potRho intgConsts] (:191-239)
+    std::vector<double> g(2 * (size_t)P);
+    std::copy(pot_rho, pot_rho + P, g.begin());
+    std::copy(ic.begin(), ic.end(), g.begin() + P);
+    int info = 0;
+    if (imago_pot_solve(c, IMAGO_SOLVE_POT_OVERLAP, 2, g.data(), &info) != IMAGO_OK)
+        return fail(IMAGO_E_SOLVE, "Fitting dposvx failed. INFO= %d", info);
+    std::vector<double> g1(g.begin(), g.begin() + P), g2(g.begin() + P, g.end());
+    const double fitted = dot(ic.data(), g1.data());
+    const double electron_diff = S.p.num_electrons - fitted;
+    {
+        const double scale = electron_diff / dot(ic.data(), g2.data());   // (:300-301)
+        for (int i = 0; i < P; ++i) g2[i] = g1[i] + g2[i] * scale;
+    }
+    // no core charge: total = valence (:329-342)
+    g1 = g2;
+    std::vector<double> g4 = g1, g5(P, 0.0), g6(std::max(P, h.npt), 0.0), g7(P, 0.0);
+    for (int t = 0; t < h.npt; ++t) {   // neutral / residual / nuclear split (:416-484)
+        const int i0 = h.type_term0[t], i1 = i0 + h.pot_nalpha[t] - 1;
+        const double nuc = h.pot_z[t] * (double)h.pot_mult[t];
+        double sum = 0.0;
+        for (int j = i0; j <= i1; ++j) sum += ic[j] * g1[j];
+        g5[i0] = (sum - nuc) / ic[i0];
+        g4[i0] = g1[i0] - g5[i0];
+        g6[t] = g5[i0];
+        g7[i1] = -nuc / ic[i1];
+    }
+    // electrostatic potential components (:536-606); the core terms vanish
+    std::vector<double> e1(P), e2(P), e6(P), gv(P);
+    RCHK(scf_gemv(c, true, P, P, S.neut_nl, g4.data(), e1.data()));
+    for (int i = 0; i < P; ++i) gv[i] = g2[i] - g5[i];
+    RCHK(scf_gemv(c, true, P, P, S.neut_l, gv.data(), e2.data()));
+    RCHK(scf_gemv(c, true, h.npt, P, S.resq, g6.data(), e6.data()));
+    const std::vector<double> &e4 = S.nuc_nl, &e5 = S.nuc_l;
+    double es_energy = 0.0;
+    for (int i = 0; i < P; ++i)
+        es_energy += 0.5 * (g1[i] + g7[i]) * (e1[i] + e6[i]) + 0.5 * g2[i] * (e2[i] + e5[i]) +
+                     0.5 * (g2[i] - g5[i]) * e5[i];
+    es_energy = es_energy - dot(g2.data(), e4.data()) + traces3[1];
+    std::vector<double> pes(2 * (size_t)P);
+    for (int i = 0; i < P; ++i) {
+        pes[i] = e1[i] + e2[i] - e4[i] + e5[i];
+        pes[P + i] = e6[i];
+    }
+    if (imago_pot_solve(c, IMAGO_SOLVE_POT_OVERLAP, 2, pes.data(), &info) != IMAGO_OK)
+        return fail(IMAGO_E_SOLVE, "Electrostatics dposvx failed. INFO= %d", info);
+    // exchange-correlation potential on the mesh (:720-1015)
+    std::vector<double> xc(3 * (size_t)P);
+    RCHK(imago_xc_potential(c, g1.data(), nullptr, xc.data()));
+    if (imago_pot_solve(c, IMAGO_SOLVE_XC_OVERLAP, 3, xc.data(), &info) != IMAGO_OK)
+        return fail(IMAGO_E_SOLVE, "Exchange-correlation dposvx failed. INFO= %d", info);
+    // exchCorrEnergy = sum_i (potAlphaOverlap(:, i) . xc_energy) rho_total(i)
+    // (:1033-1039; the core term is zero)
+    std::vector<double> sx(P);
+    RCHK(scf_gemv(c, true, P, P, c->spd[0].A, xc.data() + P, sx.data()));
+    const double xc_energy = dot(sx.data(), g1.data());
+    const double kinetic = traces3[2];
+    const double total = kinetic + es_energy + xc_energy;
+    record->iteration = S.iteration;
+    record->electron_diff = electron_diff;
+    record->kinetic_energy = kinetic;
+    record->elec_stat_energy = es_energy;
+    record->exch_corr_energy = xc_energy;
+    record->total_energy = total;
+    record->testable_delta = 0.0;
+    record->converged = 0;
+    if (rho_coeffs) {
+        std::copy(g1.begin(), g1.end(), rho_coeffs);
+        std::copy(g2.begin(), g2.end(), rho_coeffs + P);
+    }
+    if (total > 0.0)
+        return fail(IMAGO_E_SOLVE, "Total energy is positive (%.8f Ha): the structure might have a "
+                                   "serious problem", total);
+    // mixing history (:1301-1340)
+    std::vector<double> yl0(P), xl0(pot_coeffs, pot_coeffs + P), outpot(P);
+    for (int i = 0; i < P; ++i) yl0[i] = pes[i] + pes[P + i] + xc[i];
+    if (!S.have_hist) {
+        S.xl1 = S.xl2 = xl0;
+        S.yl1 = S.yl2 = yl0;
+        S.have_hist = true;
+    }
+    for (int i = 0; i < P; ++i) outpot[i] = yl0[i] - xl0[i];
+    double delta = 0.0;
+    RCHK(scf_testable_delta(c, outpot, &delta));
+    record->testable_delta = delta;
+    // the two-step feedback (:1604-1666)
+    double th1 = 0.0, th2 = 0.0;
+    if (S.p.feedback_level != 0 && S.iteration > 1) {
+        std::vector<double> rl0(P), d1(P), d2(P), u1(P), u2(P);
+        for (int i = 0; i < P; ++i) {
+            rl0[i] = yl0[i] - xl0[i];
+            d1[i] = rl0[i] - (S.yl1[i] - S.xl1[i]);
+            d2[i] = rl0[i] - (S.yl2[i] - S.xl2[i]);
+        }
+        RCHK(scf_gemv(c, false, P, P, c->spd[0].A, d1.data(), u1.data()));
+        RCHK(scf_gemv(c, false, P, P, c->spd[0].A, d2.data(), u2.data()));
+        const double s11 = dot(u1.data(), d1.data()), t1 = dot(u1.data(), rl0.data());
+        const double s12 = dot(u2.data(), d1.data()), s22 = dot(u2.data(), d2.data());
+        const double t2 = dot(u2.data(), rl0.data());
+        th1 = t1 / s11;
+        if (S.p.feedback_level != 1 && S.iteration > 2) {
+            const double det = s11 * s22 - s12 * s12;
+            if (det / (s11 * s22) >= 0.01) {
+                th1 = (s22 * t1 - s12 * t2) / det;
+                th2 = (-s12 * t1 + s11 * t2) / det;
+            }
+        }
+    }
+    const double r = S.p.relax_factor;
+    for (int i = 0; i < P; ++i)
+        pot_coeffs[i] = (1.0 - r) * ((1.0 - th1 - th2) * xl0[i] + th1 * S.xl1[i] + th2 * S.xl2[i]) +
+                        r * ((1.0 - th1 - th2) * yl0[i] + th1 * S.yl1[i] + th2 * S.yl2[i]);
+    S.xl2 = S.xl1;
+    S.xl1 = xl0;
+    S.yl2 = S.yl1;
+    S.yl1 = yl0;
+    if (delta < S.p.convg_test)
+        record->converged = 1;
+    else
+        S.iteration += 1;
     return IMAGO_OK;
 }
 

@@ -40,8 +40,27 @@ EXPORTS = [
     "imago_secular", "imago_get_eigenvectors", "imago_density_traces", "imago_zhegv",
     "imago_dsygv", "imago_last_timing", "imago_ep_counts", "imago_elecstat", "imago_residual_q",
     "imago_xc_mesh", "imago_xc_rho_op", "imago_xc_overlap", "imago_xc_counts",
-    "imago_xc_potential", "imago_pot_solve",
+    "imago_xc_potential", "imago_pot_solve", "imago_populate", "imago_scf_setup",
+    "imago_scf_update",
 ]
+
+
+class ScfParams(ctypes.Structure):
+    """imago_scf_params (the MAIN_INPUT_DATA block of imago.dat)."""
+    _fields_ = [("num_electrons", ctypes.c_double), ("feedback_level", ctypes.c_int),
+                ("relax_factor", ctypes.c_double), ("convg_test", ctypes.c_double),
+                ("xc_code", ctypes.c_int), ("log_elec_thresh", ctypes.c_double)]
+
+
+class ScfRecord(ctypes.Structure):
+    """imago_scf_record (one fort.7 / fort.14 row)."""
+    _fields_ = [("iteration", ctypes.c_int), ("converged", ctypes.c_int),
+                ("electron_diff", ctypes.c_double), ("testable_delta", ctypes.c_double),
+                ("kinetic_energy", ctypes.c_double), ("elec_stat_energy", ctypes.c_double),
+                ("exch_corr_energy", ctypes.c_double), ("total_energy", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class ImagoError(RuntimeError):
@@ -91,6 +110,12 @@ def load():
     L.imago_xc_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), _dp]
     L.imago_xc_potential.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     L.imago_pot_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _ip]
+    L.imago_populate.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, _dp,
+                                 _dp, _dp, _ip]
+    L.imago_scf_setup.argtypes = [ctypes.c_void_p, ctypes.POINTER(ScfParams), ctypes.c_void_p,
+                                  _dp, ctypes.c_int, _dp, _dp]
+    L.imago_scf_update.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp,
+                                   ctypes.POINTER(ScfRecord)]
     _lib = L
     return L
 
@@ -308,6 +333,39 @@ class Context:
         _check(load().imago_pot_solve(self._h, which, nrhs, _p(x), ctypes.byref(info)))
         return x
 
+    # -- SCF (makeSCFPot) -----------------------------------------------------
+    def scf_setup(self, num_electrons, mesh=None, feedback_level=2, relax_factor=0.2,
+                  convg_test=1.0e-4, xc_code=100, log_elec_thresh=None, covalent_radius=None,
+                  recip_cells=None):
+        """imago_scf_setup for the current system: the electrostatic and
+        exchange-correlation operators of makeSCFPot on the device."""
+        from imago_amd._desc import make_xc_params
+        from imago_amd.system import XcMesh
+        th = self.system.log_elec_thresh if log_elec_thresh is None else log_elec_thresh
+        params = ScfParams(float(num_electrons), int(feedback_level), float(relax_factor),
+                           float(convg_test), int(xc_code), float(th))
+        xcp = make_xc_params(mesh or XcMesh())
+        cov = np.ascontiguousarray(
+            [p.covalent_radius for p in self.system.pot_types] if covalent_radius is None
+            else covalent_radius, dtype=np.float64)
+        dims, sizes = (self.system.recip_cells(th) if recip_cells is None else recip_cells)
+        dims = np.ascontiguousarray(dims, dtype=np.float64)
+        sizes = np.ascontiguousarray(sizes, dtype=np.float64)
+        _check(load().imago_scf_setup(self._h, ctypes.byref(params), ctypes.byref(xcp), _p(cov),
+                                      len(sizes), _p(dims), _p(sizes)))
+
+    def scf_update(self, traces3, pot_rho, pot_coeffs):
+        """imago_scf_update (makeSCFPot): returns (next coefficients,
+        record dict, fitted charge [2, potDim])."""
+        tr = np.ascontiguousarray(traces3, dtype=np.float64)
+        pr = np.ascontiguousarray(pot_rho, dtype=np.float64)
+        c = np.array(pot_coeffs, dtype=np.float64, copy=True)
+        rho = np.zeros((2, self.pot_dim))
+        rec = ScfRecord()
+        _check(load().imago_scf_update(self._h, _p(tr), _p(pr), _p(c), _p(rho),
+                                       ctypes.byref(rec)))
+        return c, rec.as_dict(), rho
+
     def xc_counts(self):
         """(all ray points, flops of the last overlap's GEMMs)."""
         n, f = ctypes.c_long(), ctypes.c_double()
@@ -333,3 +391,17 @@ class Context:
             _check(load().imago_dsygv(self._h, n, A.ctypes.data_as(_dp), B.ctypes.data_as(_dp),
                                       _p(w), ctypes.byref(info)))
         return (w, A, B) if return_factor else (w, A)
+
+
+def populate(eigenvalues, kpoint_weights, num_electrons):
+    """imago_populate (populateStandard): occupations [nk, ns] and
+    (occupied energy, band gap, gap kind)."""
+    ev = np.ascontiguousarray(eigenvalues, dtype=np.float64)
+    nk, ns = ev.shape
+    kw = np.ascontiguousarray(kpoint_weights, dtype=np.float64)
+    pop = np.zeros((nk, ns))
+    occ, gap = ctypes.c_double(), ctypes.c_double()
+    kind = ctypes.c_int()
+    _check(load().imago_populate(nk, ns, _p(ev), _p(kw), float(num_electrons), _p(pop),
+                                 ctypes.byref(occ), ctypes.byref(gap), ctypes.byref(kind)))
+    return pop, (occ.value, gap.value, kind.value)

@@ -322,6 +322,79 @@ int imago_pot_solve(imago_ctx *ctx, int matrix, int nrhs, double *b, int *info);
  * measurement. */
 int imago_xc_counts(imago_ctx *ctx, long *ray_points, double *gemm_flops);
 
+/* ---- the SCF iteration (mainSCF, src/imago/imago.F90:488-523) ----------- *
+ * One iteration is imago_secular -> imago_populate -> imago_density_traces
+ * -> imago_scf_update, as the reference's secularEqnSCF -> populateStates
+ * -> makeValenceRho -> makeSCFPot. */
+
+/* populateStandard (src/imago/populate.F90:190-485) with its merge-sort
+ * order (sortSubs.f90:18-177): spin 1, no thermal smearing, no excited core
+ * state.  eigenvalues and populations are [num_kpoints][num_states]
+ * (k-major, as imago_secular returns them); kpoint_weights sum to 2.
+ * Returns IMAGO_E_ARG when the states cannot hold num_electrons.  Any of
+ * the three scalar outputs may be NULL.  Host code; no context. */
+int imago_populate(int num_kpoints, int num_states, const double *eigenvalues,
+                   const double *kpoint_weights, double num_electrons, double *populations,
+                   double *occupied_energy, double *band_gap, int *gap_kind);
+
+/* Input of makeSCFPot: the MAIN_INPUT_DATA block of imago.dat as makeinput
+ * writes it (makeinput.py:5340-5352) and NUM_ELECTRONS. */
+typedef struct imago_scf_params {
+    double num_electrons;     /* NUM_ELECTRONS                              */
+    int feedback_level;       /* FEEDBACK_LEVEL (makeinput writes 2)        */
+    double relax_factor;      /* RELAXATION_FACTOR (makeinput writes 0.2)   */
+    double convg_test;        /* CONVERGENCE_TEST (makeinputrc: 1e-4)       */
+    int xc_code;              /* XC_CODE: 100 = Wigner, spin 1, LDA, no rel. */
+    double log_elec_thresh;   /* -ln(electroConvg) for the electrostatics   */
+} imago_scf_params;
+
+/* One row of the reference's per-iteration records: fort.7 (iteration,
+ * electron fit error, convergence, total energy; potentialUpdate.F90:
+ * 1806-1812) and fort.14 (kinetic, electrostatic, exchange-correlation,
+ * total; :1825-1827). */
+typedef struct imago_scf_record {
+    int iteration;            /* currIteration of this update (1-based)     */
+    int converged;            /* testableDelta < convgTest (:1839)          */
+    double electron_diff;     /* numElectrons - fitted valence charge       */
+    double testable_delta;    /* largest site RMS of the potential change   */
+    double kinetic_energy;
+    double elec_stat_energy;
+    double exch_corr_energy;
+    double total_energy;      /* Hartree                                    */
+} imago_scf_record;
+
+/* Set-up of makeSCFPot for the system of the last imago_set_system: the
+ * electrostatic matrices of neutralAndNuclearQPot and residualQ
+ * (imago_elecstat, imago_residual_q; reciprocal cells as for
+ * imago_residual_q), the exchange-correlation mesh, its overlap and
+ * operator (imago_xc_mesh with mesh and covalent_radius[numPotTypes],
+ * imago_xc_overlap), all kept on the device; intgConsts (potential.f90:
+ * 150-160).  Resets the iteration count to 1 and the mixing history.
+ * IMAGO_E_ARG for an xc_code other than 100. */
+int imago_scf_setup(imago_ctx *ctx, const imago_scf_params *params,
+                    const imago_xc_mesh_params *mesh, const double *covalent_radius,
+                    int num_cells_recip, const double *cell_dims_recip,
+                    const double *cell_sizes_recip);
+
+/* makeSCFPot (src/imago/potentialUpdate.F90:53-1866) for spin 1, no core
+ * states, non-relativistic LDA (Wigner): the charge fit of pot_rho
+ * (potRho of imago_density_traces) against potAlphaOverlap, the
+ * electron-count correction, the neutral / residual / nuclear charge split,
+ * the electrostatic potential and energy, the exchange-correlation
+ * potential and energy on the ray mesh, the total energy, the RMS
+ * real-space potential change per site, and the mixed potential
+ * (relaxation with the two-step feedback of :1604-1666).  traces3 =
+ * (Tr rho S, Tr rho NP, Tr rho KE) of imago_density_traces.  pot_coeffs
+ * [potDim] holds the coefficients used in this iteration on entry and the
+ * next iteration's on return.  rho_coeffs [2][potDim] (may be NULL) gets
+ * the fitted total and valence charge coefficients (generalRho(:, 1:2),
+ * the scfV.dat columns rhoT and rhoV).  The iteration count advances
+ * unless the record says converged, as currIteration does.  IMAGO_E_SOLVE
+ * when a dposvx solve fails and when the total energy is positive (the
+ * reference stops in both cases). */
+int imago_scf_update(imago_ctx *ctx, const double *traces3, const double *pot_rho,
+                     double *pot_coeffs, double *rho_coeffs, imago_scf_record *record);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,36 @@ def test_single_hegv_matches_lapack(gpu_ctx, oracle_mod):
     assert np.allclose(V.conj().T @ S @ V, np.eye(n), atol=1e-10)
 
 
+@pytest.mark.parametrize("cx", [True, False])
+@pytest.mark.parametrize("n", [7, 64, 150])
+def test_single_hegv_reads_only_the_upper_triangle(gpu_ctx, oracle_mod, cx, n):
+    """solveZHEGV / solveDSYGV's caller fills only the upper triangle of A
+    (secularEqn.F90:140 zeroes valeVale, unpackMatrix fullFlag 0 writes rows
+    <= column, matrixSubs.F90:426-433): A with a zero strictly lower
+    triangle, and a nonzero imaginary part on the diagonal that zhegv 'U'
+    ignores, must give LAPACK's eigenvalues of the Hermitian matrix the
+    upper triangle defines."""
+    rng = np.random.default_rng(40 + n)
+    M = rng.normal(size=(n, n))
+    R = rng.normal(size=(n, n))
+    if cx:
+        M = M + 1j * rng.normal(size=(n, n))
+        R = R + 1j * rng.normal(size=(n, n))
+    H = M + M.conj().T
+    S = R @ R.conj().T + n * np.eye(n)
+    Hu = np.triu(H)
+    if cx:
+        Hu[np.diag_indices(n)] += 1j * rng.normal(size=n)
+    print("[gpu] hegv upper only n=%d cx=%s" % (n, cx), flush=True)
+    w, V = gpu_ctx.hegv(Hu, S)
+    w2, V2, info = oracle_mod.hegv(H, S)
+    assert info == 0
+    assert np.max(np.abs(w - w2)) < 1e-10 * max(1.0, np.max(np.abs(w2)))
+    # the eigenvectors solve the full problem
+    assert np.max(np.abs(H @ V - S @ V * w)) < 1e-9 * np.max(np.abs(H))
+    assert np.allclose(V.conj().T @ S @ V, np.eye(n), atol=1e-10)
+
+
 def test_not_positive_definite_overlap_is_reported(gpu_ctx):
     rng = np.random.default_rng(4)
     n = 16
