@@ -244,6 +244,7 @@ struct imago_ctx {
         bool ready = false;
         imago_scf_params p{};
         std::vector<double> intg, nuc_nl, nuc_l;
+        double *es_buf = nullptr;   // device: the elecstat arrays (one allocation)
         double *neut_nl = nullptr, *neut_l = nullptr, *resq = nullptr;   // device
         double *vec = nullptr;      // device scratch, 8 potDim
         int iteration = 1;
@@ -3129,7 +3130,7 @@ extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
     const HostSys &h = c->h;
     const int P = h.pot_dim;
     imago_ctx::ScfState &S = c->scf;
-    for (void *q : {(void *)S.neut_nl, (void *)S.resq, (void *)S.vec})
+    for (void *q : {(void *)S.es_buf, (void *)S.resq, (void *)S.vec})
         if (q) dev_release(c, q);
     S = imago_ctx::ScfState{};
     S.p = *params;
@@ -3146,7 +3147,8 @@ extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
     double *es = nullptr;
     RCHK(elecstat_run(c, params->log_elec_thresh, &es));
     const size_t PP = (size_t)P * P;
-    S.neut_nl = es;
+    S.es_buf = es;
+    S.neut_nl = es + PP;
     S.neut_l = es + 2 * PP;
     S.nuc_nl.resize(P);
     S.nuc_l.resize(P);
