@@ -402,6 +402,12 @@ void orc_use_external_primitives(int on) { g_use_ext = on && g_ext_ol; }
 static long g_tuples = 0;
 long orc_last_tuple_count(void) { return g_tuples; }
 
+/* Threads of the assembly loops (the outer atom loop of each integral
+ * kind runs in parallel; every (i, j) pair writes its own output block and
+ * its own interaction-mask words).  1 = the reference's serial order. */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
 /* ------------------------------------------------------------------ */
 /* System helpers */
 
@@ -622,6 +628,7 @@ typedef struct {
 static void prim2(int kind, double a1, double a2, const double *A,
                   const double *B, int sw, double *sh)
 {
+#pragma omp atomic
     ++g_tuples;
     if (kind == IMAGO_OL)
         (g_use_ext ? g_ext_ol : orc_overlap2c)(a1, a2, A, B, sw, sh);
@@ -632,6 +639,7 @@ static void prim2(int kind, double a1, double a2, const double *A,
 static void prim3(int kind, double a1, double a2, double a3, const double *A,
                   const double *B, const double *C, int sw, double *sh)
 {
+#pragma omp atomic
     ++g_tuples;
     if (kind == IMAGO_EP)
         (g_use_ext ? g_ext_ep : orc_electron3c)(a1, a2, a3, A, B, C, sw, sh);
@@ -747,7 +755,7 @@ static int electronic_pe(const ctx_t *c, term_t *tm, int a1, int a2, int t1,
 
 /* The atom-pair / lattice / alpha loop nest shared by gaussOverlapOL/KE/NP
  * and gaussOverlapEP.  out: packed [nk][2*npack] or [npack] (gamma). */
-static int assemble_one(const ctx_t *c, term_t *tm, unsigned long *mask,
+static int assemble_one(const ctx_t *c, const term_t *tm0, unsigned long *mask,
                         int nmaskw, double *out)
 {
     const imago_system_desc *s = c->s;
@@ -755,10 +763,6 @@ static int assemble_one(const ctx_t *c, term_t *tm, unsigned long *mask,
     int n = c->n;
     long npack = (long)n * (n + 1) / 2;
     int maxa = c->maxa, maxs = s->max_states;
-    double *P2 = malloc(sizeof(double) * 16 * maxa * maxs);
-    double *P12 = malloc(sizeof(double) * maxs * maxs);
-    double *cpr = malloc(sizeof(double) * 2 * maxs * maxs * nk); /* currentPair */
-    double sh[256];
     /* computePhaseFactors, kpoints.f90:578-609 */
     double *phase = malloc(sizeof(double) * 2 * nk * s->num_cells);
     if (!s->gamma)
@@ -770,7 +774,20 @@ static int assemble_one(const ctx_t *c, term_t *tm, unsigned long *mask,
                 phase[2 * (q + nk * k) + 1] = sin(dot);
             }
     int rc = 0;
-    for (int i = 0; i < s->num_atoms && rc == 0; ++i) {
+#pragma omp parallel num_threads(g_threads)
+    {
+    term_t tmv = *tm0;
+    term_t *tm = &tmv;
+    double *P2 = malloc(sizeof(double) * 16 * maxa * maxs);
+    double *P12 = malloc(sizeof(double) * maxs * maxs);
+    double *cpr = malloc(sizeof(double) * 2 * maxs * maxs * nk); /* currentPair */
+    double sh[256];
+#pragma omp for schedule(dynamic, 1)
+    for (int i = 0; i < s->num_atoms; ++i) {
+        int stop;
+#pragma omp atomic read
+        stop = rc;
+        if (stop) continue;
         int t1 = s->atom_type[i], e1 = s->type_element[t1];
         int n1 = s->type_num_orb_alphas[4 * t1], ns1 = s->type_num_states[t1];
         const double *A = s->atom_pos + 3 * i;
@@ -788,6 +805,7 @@ static int assemble_one(const ctx_t *c, term_t *tm, unsigned long *mask,
             if (sepsq > negl) continue;
             double maxLR = sepsq + negl + 2.0 * sqrt(sepsq * negl);
             if (maxLR > s->cell_sizes[s->num_cells - 1]) {
+#pragma omp atomic write
                 rc = IMAGO_E_LATTICE;
                 break;
             }
@@ -931,6 +949,7 @@ static int assemble_one(const ctx_t *c, term_t *tm, unsigned long *mask,
     free(P2);
     free(P12);
     free(cpr);
+    }
     free(phase);
     return rc;
 }

@@ -73,6 +73,9 @@ void orc_build_hamiltonian(long len, const double *np, const double *ke,
 double orc_matrix_element_mult(int dim1, int n, const double *m1,
                                const double *m2);
 
+/* threads of the assembly's outer atom loop (default 1: serial) */
+void orc_set_threads(int n);
+
 /* number of primitive tuples evaluated by the last orc_assemble call */
 long orc_last_tuple_count(void);
 
