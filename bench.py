@@ -424,6 +424,117 @@ def bench_xcmesh(args, system, ctx, log):
     print(json.dumps(line), flush=True)
 
 
+def host_info():
+    """CPU model, core count, compiler and LAPACK of the host (for the CPU
+    baseline's record)."""
+    info = {"machine": platform.machine()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        info["cores_available"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["cores_available"] = os.cpu_count()
+    try:
+        import subprocess
+        info["oracle_cc"] = subprocess.run(["gcc", "--version"], capture_output=True,
+                                           text=True).stdout.splitlines()[0]
+    except Exception:
+        pass
+    info["oracle_cflags"] = "-O2 -fPIC -std=c11 -ffp-contract=off -fopenmp"
+    try:
+        from threadpoolctl import threadpool_info
+        libs = [(d.get("internal_api"), d.get("version"), d.get("num_threads"))
+                for d in threadpool_info() if d.get("user_api") == "blas"]
+        info["lapack"] = "scipy %s, %s" % (__import__("scipy").__version__,
+                                           ", ".join("%s %s" % (a, b) for a, b, _ in libs))
+    except Exception:
+        pass
+    return info
+
+
+def scf_cpu_baseline(system, iterations, first_inputs, threads, log, ep_chunk=64):
+    """The oracle's SCF on this workload, timed on this host with the given
+    thread count (assembly loops over atoms with OpenMP, LAPACK threads):
+    set-up in full (OL, KE, NP, every EP term in chunks of ep_chunk terms,
+    the electrostatics and the exchange-correlation mesh of makeSCFPot),
+    one iteration's secular equation in full (zhegv at every k-point), its
+    H build and traces over one chunk of EP terms scaled to all terms, and
+    one makeSCFPot call on the GPU run's first-iteration traces; the SCF
+    time is set-up + iterations x iteration (the GPU run's iteration count)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import scf as oscf
+    from threadpoolctl import threadpool_limits
+    L = oracle.lib()
+    L.orc_set_threads(threads)
+    t = {}
+    with threadpool_limits(threads):
+        n, nk, P = system.vale_dim, system.num_kpoints, system.pot_dim
+        t0 = time.perf_counter()
+        S = oracle.assemble(system, "ol")
+        K = oracle.assemble(system, "ke")
+        N = oracle.assemble(system, "np")
+        t["ol_ke_np"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        V = None
+        for j0 in range(0, P, ep_chunk):
+            V = oracle.assemble(system, "ep", j0, min(P, j0 + ep_chunk))
+            if j0 == 0:
+                V0 = V
+        t["ep_all_terms"] = time.perf_counter() - t0
+        del V
+        t0 = time.perf_counter()
+        pot = oscf.ScfPotential(system)
+        t["scfpot_setup"] = time.perf_counter() - t0
+        nc = V0.shape[0]
+        coeffs = first_inputs[2]
+        t0 = time.perf_counter()
+        H = [oracle.build_hamiltonian(N[k], K[k], V0[:, k], coeffs[:nc]) for k in range(nk)]
+        t["hbuild_chunk"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        vecs, evals = [], []
+        for k in range(nk):
+            w, v, info = oracle.hegv(oracle.unpack(H[k], n), oracle.unpack(S[k], n))
+            evals.append(w)
+            vecs.append(v)
+        t["zhegv_all_k"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        occ = oracle.populate(np.array(evals), system.kpoint_weights, oscf.num_electrons(system))
+        t["populate"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cols = np.concatenate([np.full(c + 1, c) for c in range(n)])
+        rows = np.concatenate([np.arange(c + 1) for c in range(n)])
+        for k in range(nk):
+            X = vecs[k]
+            rho = (X * occ[k]) @ X.conj().T
+            rp = rho[rows, cols]
+            if system.gamma:
+                rp = rp.real
+            for M in (S[k], N[k], K[k]):
+                oracle.matrix_element_mult(M, rp, n)
+            for j in range(nc):
+                oracle.matrix_element_mult(V0[j, k], rp, n)
+        t["traces_chunk"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pot.update(first_inputs[0], first_inputs[1], first_inputs[2])
+        t["make_scf_pot"] = time.perf_counter() - t0
+    L.orc_set_threads(1)
+    scale = P / nc
+    setup = t["ol_ke_np"] + t["ep_all_terms"] + t["scfpot_setup"]
+    per_iter = (t["hbuild_chunk"] * scale + t["zhegv_all_k"] + t["populate"]
+                + t["traces_chunk"] * scale + t["make_scf_pot"])
+    total = setup + iterations * per_iter
+    log("cpu baseline (%d thread(s)): set-up %.2f s, %.3f s per iteration, SCF %.1f s"
+        % (threads, setup, per_iter, total))
+    return total, setup, per_iter, t
+
+
 def bench_scf(args, system, coeffs, ctx, log):
     """SCF wall time to convergence: imago_set_system, the four integral
     kinds, makeSCFPot's set-up, then secular / populate / traces /
@@ -500,6 +611,32 @@ def bench_scf(args, system, coeffs, ctx, log):
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        log("cpu baseline leg: the oracle's SCF on %s" % system.name)
+        info = host_info()
+        cores = int(info.get("cores_available") or 1)
+        c1, s1, p1, t1 = scf_cpu_baseline(system, last.iterations, last.first_inputs, 1, log)
+        call, sall, pall, tall = scf_cpu_baseline(system, last.iterations, last.first_inputs,
+                                                  cores, log)
+        line["cpu_baseline"] = {
+            "value": round(c1, 3), "unit": "s per SCF to 1e-6 Ha (%d iterations)"
+                                           % last.iterations,
+            "cores": 1, "kind": "port",
+            "sample": ("oracle (C restatement + SciPy LAPACK) on %s: set-up in full (OL/KE/NP, "
+                       "all %d EP terms, electrostatics, XC mesh), one iteration's zhegv at all "
+                       "%d k-points, H build and traces over %d of %d EP terms scaled, one "
+                       "makeSCFPot; SCF = set-up + %d x iteration"
+                       % (system.name, system.pot_dim, system.num_kpoints, 64,
+                          system.pot_dim, last.iterations)),
+            "setup_s": round(s1, 3), "per_iteration_s": round(p1, 4),
+            "parts_s": {k: round(v, 4) for k, v in t1.items()},
+            "all_cores": {"value": round(call, 3), "cores": cores, "setup_s": round(sall, 3),
+                          "per_iteration_s": round(pall, 4),
+                          "parts_s": {k: round(v, 4) for k, v in tall.items()}},
+            "host": info,
+        }
+        line["vs_cpu"] = {"speedup_1core": round(c1 / wall, 1),
+                          "speedup_all_cores": round(call / wall, 1)}
     print(json.dumps(line), flush=True)
 
 

@@ -41,7 +41,9 @@ EXPORTS = [
     "imago_dsygv", "imago_last_timing", "imago_ep_counts", "imago_elecstat", "imago_residual_q",
     "imago_xc_mesh", "imago_xc_rho_op", "imago_xc_overlap", "imago_xc_counts",
     "imago_xc_potential", "imago_pot_solve", "imago_populate", "imago_scf_setup",
-    "imago_scf_update",
+    "imago_scf_update", "imago_site_descriptors", "imago_mlp_create", "imago_mlp_destroy",
+    "imago_mlp_num_params", "imago_mlp_params", "imago_mlp_forward", "imago_mlp_gradient",
+    "imago_mlp_adam_step", "imago_mlp_train",
 ]
 
 
@@ -116,6 +118,22 @@ def load():
                                   _dp, ctypes.c_int, _dp, _dp]
     L.imago_scf_update.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp,
                                    ctypes.POINTER(ScfRecord)]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.imago_site_descriptors.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int,
+                                         _dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _dp]
+    L.imago_mlp_create.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_uint,
+                                   ctypes.POINTER(ctypes.c_void_p)]
+    L.imago_mlp_destroy.argtypes = [ctypes.c_void_p]
+    L.imago_mlp_num_params.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+    L.imago_mlp_params.argtypes = [ctypes.c_void_p, _fp, _fp]
+    L.imago_mlp_forward.argtypes = [ctypes.c_void_p, ctypes.c_int, _fp, _fp]
+    L.imago_mlp_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int, _fp, _fp, _dp, _fp]
+    L.imago_mlp_adam_step.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float]
+    L.imago_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_int, _fp, _fp, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_uint, _dp]
     _lib = L
     return L
 
@@ -405,3 +423,83 @@ def populate(eigenvalues, kpoint_weights, num_electrons):
     _check(load().imago_populate(nk, ns, _p(ev), _p(kw), float(num_electrons), _p(pop),
                                  ctypes.byref(occ), ctypes.byref(gap), ctypes.byref(kind)))
     return pop, (occ.value, gap.value, kind.value)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """imago_mlp: the per-site MLP on a context's device (FP32)."""
+
+    def __init__(self, ctx, widths, seed=0):
+        self.widths = [int(w) for w in widths]
+        h = ctypes.c_void_p()
+        w = np.ascontiguousarray(self.widths, dtype=np.int32)
+        _check(load().imago_mlp_create(ctx._h, len(w) - 1, w.ctypes.data_as(_ip), int(seed),
+                                       ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_long()
+        _check(load().imago_mlp_num_params(self._h, ctypes.byref(n)))
+        self.num_params = n.value
+
+    def close(self):
+        if self._h:
+            load().imago_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.num_params, dtype=np.float32)
+        _check(load().imago_mlp_params(self._h, _fptr(p), None))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(load().imago_mlp_params(self._h, None, _fptr(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.widths[-1]), dtype=np.float32)
+        _check(load().imago_mlp_forward(self._h, x.shape[0], _fptr(x), _fptr(y)))
+        return y
+
+    def gradient(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(load().imago_mlp_gradient(self._h, x.shape[0], _fptr(x), _fptr(t),
+                                         ctypes.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(load().imago_mlp_adam_step(self._h, lr, beta1, beta2, eps))
+
+    def train(self, x, target, epochs, minibatch=256, lr=1e-3, beta1=0.9, beta2=0.999,
+              eps=1e-8, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        losses = np.zeros(epochs)
+        _check(load().imago_mlp_train(self._h, x.shape[0], _fptr(x), _fptr(t), int(epochs),
+                                      int(minibatch), lr, beta1, beta2, eps, int(seed),
+                                      _p(losses)))
+        return losses
+
+
+def site_descriptors(ctx, pos, weight, cells, cutoff, r_min, eta):
+    """imago_site_descriptors: [num_sites][96]."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    w = np.ascontiguousarray(weight, dtype=np.float64)
+    cells = np.ascontiguousarray(cells, dtype=np.float64)
+    out = np.zeros((len(pos), 96))
+    _check(load().imago_site_descriptors(ctx._h, len(pos), _p(pos), _p(w), len(cells),
+                                         _p(cells), float(cutoff), float(r_min), float(eta),
+                                         _p(out)))
+    return out

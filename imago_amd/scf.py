@@ -48,6 +48,7 @@ class ScfResult:
     setup_s: float = 0.0
     loop_s: float = 0.0
     total_s: float = 0.0
+    first_inputs: tuple = None    # (traces3, potRho, coefficients) of iteration 1
 
     @property
     def total_energy(self) -> float:
@@ -91,6 +92,8 @@ def run(ctx, system, coeffs0, max_iter=50, energy_tol=1.0e-6, mesh=None, num_sta
         ev = ctx.secular(coeffs, ns)
         occ, (e_occ, gap, gap_kind) = lib.populate(ev, system.kpoint_weights, electrons)
         tr, prho = ctx.density_traces(occ)
+        if res.first_inputs is None:
+            res.first_inputs = (tr.copy(), prho.copy(), coeffs.copy())
         coeffs, rec, rho = ctx.scf_update(tr, prho, coeffs)
         rec.update(occupied_energy=e_occ, band_gap=gap, gap_kind=gap_kind,
                    charge_trace=float(tr[0]), wall_s=time.perf_counter() - t0)

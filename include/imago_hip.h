@@ -395,6 +395,52 @@ int imago_scf_setup(imago_ctx *ctx, const imago_scf_params *params,
 int imago_scf_update(imago_ctx *ctx, const double *traces3, const double *pot_rho,
                      double *pot_coeffs, double *rho_coeffs, imago_scf_record *record);
 
+/* ---- the learned initial potential --------------------------------------- *
+ * A per-site MLP predicts, from the site's environment, the correction of
+ * the atomic-superposition coefficients that scfV.dat carries (the input of
+ * initPotCoeffs, src/imago/potential.f90:433-564; makeinput.py:5805-5941
+ * writes the superposition).  The reference has no learned guess; these
+ * entry points produce the coefficient vector it reads. */
+
+/* Environment descriptors of num_sites sites: pos [num_sites][3] (bohr),
+ * weight [num_sites] (species weight of each site as a neighbour), cells
+ * [num_cells][3] the lattice translations to sum over (the origin
+ * included), 32 Gaussian radial channels exp(-eta (r - mu_k)^2) with mu_k
+ * from r_min to cutoff, the cosine cut-off at cutoff; per channel the l = 0,
+ * 1, 2 rotation invariants.  desc [num_sites][96]: channel k at k (l = 0),
+ * 32 + k (l = 1), 64 + k (l = 2). */
+int imago_site_descriptors(imago_ctx *ctx, int num_sites, const double *pos,
+                           const double *weight, int num_cells, const double *cells,
+                           double cutoff, double r_min, double eta, double *desc);
+
+typedef struct imago_mlp imago_mlp;
+/* An MLP of num_layers layers, widths[0..num_layers] (tanh on the hidden
+ * layers, linear output), Glorot-uniform weights from seed, zero biases; on
+ * ctx's device and stream.  Parameters are packed per layer as W_l
+ * [w_{l+1}][w_l] row-major, then b_l. */
+int imago_mlp_create(imago_ctx *ctx, int num_layers, const int *widths, unsigned seed,
+                     imago_mlp **mlp);
+int imago_mlp_destroy(imago_mlp *mlp);
+int imago_mlp_num_params(imago_mlp *mlp, long *num_params);
+/* Copy the packed parameters out (get may be NULL) and/or in (set may be
+ * NULL; setting clears Adam's moments). */
+int imago_mlp_params(imago_mlp *mlp, float *get, const float *set);
+/* y [batch][w_L] = MLP(x [batch][w_0]), FP32. */
+int imago_mlp_forward(imago_mlp *mlp, int batch, const float *x, float *y);
+/* Mean squared error against target [batch][w_L] and its gradient with
+ * respect to the packed parameters (grad may be NULL; it stays on the
+ * device for imago_mlp_adam_step). */
+int imago_mlp_gradient(imago_mlp *mlp, int batch, const float *x, const float *target,
+                       double *loss, float *grad);
+/* One Adam step (Kingma and Ba) with the last gradient. */
+int imago_mlp_adam_step(imago_mlp *mlp, float lr, float beta1, float beta2, float eps);
+/* epochs of Adam over shuffled minibatches of the num_samples pairs (x,
+ * target), resident on the device; epoch_loss[epochs] (may be NULL) gets
+ * each epoch's mean loss. */
+int imago_mlp_train(imago_mlp *mlp, int num_samples, const float *x, const float *target,
+                    int epochs, int minibatch, float lr, float beta1, float beta2, float eps,
+                    unsigned seed, double *epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif

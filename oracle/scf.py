@@ -68,6 +68,16 @@ def dposvx(A, B):
     return np.asarray(x), int(info)
 
 
+def wigner(rho):
+    """wignerXC / wignerXCEnergy (potentialUpdate.F90:2835-2894): the Wigner
+    interpolation's potential and energy density (Hartree) at densities rho."""
+    r13 = np.cbrt(rho)
+    f = 1.0 + 12.57 * r13
+    pot = -r13 * (0.984 + (0.943656 + 8.8963 * r13) / (f * f))
+    en = -0.738 * r13 * (1.0 + 0.959 / f)
+    return pot, en
+
+
 class ScfPotential:
     """makeSCFPot for one system: the set-up matrices (elecStat.f90, the
     exchange-correlation mesh of exchCorr.f90) and the mixing history that
@@ -92,16 +102,34 @@ class ScfPotential:
         self.residual_q = oracle.residual_q(system)
         self.xc = oracle.XcOracleMesh(system, mesh)
         self.xc_overlap = self.xc.overlap()
+        # exchRhoOp and radialWeight of every site with a mesh, kept as the
+        # reference keeps them in its HDF5 file and reads them back in every
+        # makeSCFPot (potentialUpdate.F90:810-830)
         self.sites = []
         for i in range(len(system.pot_site_pos)):
             t = next(ti for ti, p in enumerate(system.pot_types) if i in p.sites)
             if system.pot_types[t].covalent_radius < SMALL_THRESH:
                 continue
             n, rw, op = self.xc.site(i)
-            self.sites.append((n, rw[:n].copy(), np.asarray(op[:, :n, 0])))
+            self.sites.append((n, rw[:n].copy(), np.ascontiguousarray(op[:, :n, 0])))
         self.iteration = 1
         self.hist = None
         self.converged = False
+
+    def xc_potential(self, rho_total, rho_core):
+        """exchCorrPot(:, 1:3) before its solve (potentialUpdate.F90:833-1008,
+        Wigner, spin 1, LDA) from the stored site operators: at every ray
+        point the total and core densities op . rho (negative values set to
+        0), then sum over points of radialWeight * (potential, energy
+        density of the total, energy density of the core) * op."""
+        out = np.zeros((self.P, 3))
+        for n, rw, op in self.sites:
+            tot = np.maximum(rho_total @ op, 0.0)
+            core = np.maximum(rho_core @ op, 0.0)
+            pot, en = wigner(tot)
+            _, enc = wigner(core)
+            out += op @ np.stack([rw * pot, rw * en, rw * enc], 1)
+        return out
 
     def update(self, traces, pot_rho, pot_coeffs):
         """One makeSCFPot call.  traces = (chargeDensityTrace, nucPotTrace,
@@ -153,7 +181,7 @@ class ScfPotential:
         if info != 0:
             raise RuntimeError("Electrostatics dposvx failed. INFO= %d" % info)
         # exchange-correlation potential on the mesh (:720-1015)
-        xc = self.xc.potential(g1, g3).T.copy()           # [P, 3]
+        xc = self.xc_potential(g1, g3)                    # [P, 3]
         xc, info = dposvx(self.xc_overlap, xc)
         if info != 0:
             raise RuntimeError("Exchange-correlation dposvx failed. INFO= %d" % info)

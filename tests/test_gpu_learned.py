@@ -1,0 +1,75 @@
+"""The learned initial guess's kernels on the GPU against the NumPy oracle
+(oracle/mlp.py): site descriptors (FP64) at 1e-12, the MLP forward pass
+and its backpropagated gradient (FP32 kernels against the FP64 oracle),
+the Adam update, and a training run that lowers the loss."""
+import numpy as np
+import pytest
+
+from imago_amd import learned, lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(widths, rng, scale=0.3):
+    n = sum(widths[l] * widths[l + 1] + widths[l + 1] for l in range(len(widths) - 1))
+    return (rng.normal(scale=scale, size=n)).astype(np.float32)
+
+
+def test_descriptors_match_oracle(gpu_ctx):
+    import mlp as omlp
+    rng = np.random.default_rng(4)
+    for cell in (learned.diamond_cell(2, (1, 1, 1), rng), learned.layered_cell(0, (2, 1, 1), rng)):
+        print("[gpu] descriptor %s" % cell.name, flush=True)
+        cells = learned.descriptor_cells(cell)
+        w = np.array([learned.species_weight(cell.pot_types[0].element)] * cell.num_atoms)
+        w[::3] = 0.3          # mixed weights
+        got = lib.site_descriptors(gpu_ctx, cell.pot_site_pos, w, cells, learned.CUTOFF,
+                                   learned.R_MIN, learned.ETA)
+        ref = omlp.descriptor(cell.pot_site_pos, w, cells, learned.CUTOFF, learned.R_MIN,
+                              learned.ETA)
+        assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_mlp_forward_and_gradient_match_oracle(gpu_ctx):
+    import mlp as omlp
+    rng = np.random.default_rng(5)
+    widths = list(learned.WIDTHS)
+    m = lib.Mlp(gpu_ctx, widths, seed=3)
+    p = _params(widths, rng, 0.15)
+    m.set_params(p)
+    assert np.array_equal(m.get_params(), p)
+    for batch in (1, 37, 300):
+        x = rng.normal(size=(batch, widths[0])).astype(np.float32)
+        t = rng.normal(size=(batch, widths[-1])).astype(np.float32)
+        print("[gpu] mlp batch %d" % batch, flush=True)
+        y = m.forward(x)
+        yr = omlp.forward(p.astype(np.float64), widths, x.astype(np.float64))
+        assert np.max(np.abs(y - yr)) < 1e-4 * np.max(np.abs(yr))
+        loss, g = m.gradient(x, t)
+        lr_, gr = omlp.loss_grad(p.astype(np.float64), widths, x.astype(np.float64),
+                                 t.astype(np.float64))
+        assert loss == pytest.approx(lr_, rel=1e-5)
+        assert np.linalg.norm(g - gr) < 1e-4 * np.linalg.norm(gr)
+    # one Adam step on the last gradient, against the oracle's update of the
+    # same gradient
+    m.adam_step(lr=1e-2)
+    p1, _, _ = omlp.adam(p.astype(np.float64), g.astype(np.float64), np.zeros(p.size),
+                         np.zeros(p.size), 1, lr=1e-2)
+    assert np.max(np.abs(m.get_params() - p1)) < 1e-6
+    m.close()
+
+
+def test_mlp_training_lowers_the_loss(gpu_ctx):
+    rng = np.random.default_rng(6)
+    widths = [8, 32, 32, 4]
+    x = rng.normal(size=(2048, 8)).astype(np.float32)
+    W = rng.normal(size=(8, 4))
+    t = np.tanh(x @ W).astype(np.float32) + 0.1 * x[:, :4] ** 2
+    m = lib.Mlp(gpu_ctx, widths, seed=1)
+    print("[gpu] mlp training", flush=True)
+    losses = m.train(x, t, epochs=60, minibatch=128, lr=3e-3, seed=2)
+    assert losses[-1] < 0.1 * losses[0], losses[[0, -1]]
+    # the trained parameters reproduce the reported loss on the whole set
+    loss, _ = m.gradient(x, t)
+    assert loss < 2.0 * losses[-1]
+    m.close()

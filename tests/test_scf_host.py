@@ -118,3 +118,26 @@ def test_oracle_scf_small_cell_converges_with_exact_charge(oracle_mod):
         # the constrained fit holds exactly NUM_ELECTRONS (potentialUpdate.F90:300-305)
         assert float(intg @ r["rho_vale"]) == pytest.approx(ne, rel=1e-12)
         assert r["total_energy"] < 0.0
+
+
+def test_oracle_scf_xc_from_stored_operators_matches_c_routine(oracle_mod):
+    """The oracle's makeSCFPot takes the exchange-correlation terms from the
+    stored site operators (as the reference reads exchRhoOp back from its
+    HDF5 file); they equal the C restatement's accumulation (orc_xc_potential,
+    which re-evaluates the operator), and the numpy Wigner functional equals
+    orc_xc_wigner."""
+    import scf as oscf
+    s = configs.small_test_system(seed=5)
+    pot = oscf.ScfPotential(s)
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        rt = rng.normal(size=s.pot_dim) * 0.05 + 0.02
+        rc = np.abs(rng.normal(size=s.pot_dim)) * 0.01
+        a = pot.xc_potential(rt, rc)
+        b = pot.xc.potential(rt, rc).T
+        assert np.allclose(a, b, rtol=1e-11, atol=1e-13 * np.max(np.abs(b)))
+    for rho in (0.0, 1e-6, 0.3, 2.7, 40.0):
+        p, e = oscf.wigner(np.array([rho]))
+        pc, ec = oracle_mod.xc_wigner(rho)
+        assert p[0] == pytest.approx(pc, rel=1e-14, abs=1e-300)
+        assert e[0] == pytest.approx(ec, rel=1e-14, abs=1e-300)
