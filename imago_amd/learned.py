@@ -266,8 +266,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--cells", type=int, default=512)
     ap.add_argument("--eval", type=int, default=64, help="held-out 64-atom cells")
-    ap.add_argument("--epochs", type=int, default=400)
-    ap.add_argument("--max-iter", type=int, default=150)
+    ap.add_argument("--epochs", type=int, default=1000)
+    ap.add_argument("--max-iter", type=int, default=100,
+                    help="iterations allowed to a training cell (cells that do not converge "
+                         "are left out)")
+    ap.add_argument("--eval-max-iter", type=int, default=200)
     ap.add_argument("--out", default=MODEL_PATH)
     ap.add_argument("--report", default=None)
     ap.add_argument("--seed", type=int, default=20)
@@ -318,26 +321,38 @@ def main(argv=None):
         for q, cell in enumerate(rng_cells):
             atomic = guess.superposition_guess(ctx, cell, db)
             learned = model.predict(ctx, cell, atomic)
-            r0 = scf.run(ctx, cell, atomic, max_iter=args.max_iter)
-            r1 = scf.run(ctx, cell, learned, max_iter=args.max_iter)
+            r0 = scf.run(ctx, cell, atomic, max_iter=args.eval_max_iter)
+            r1 = scf.run(ctx, cell, learned, max_iter=args.eval_max_iter)
+            # how far each guess starts from the converged potential
+            err_sup = float(np.linalg.norm(atomic - r0.coeffs))
+            err_learned = float(np.linalg.norm(learned - r0.coeffs))
             rows.append(dict(name=cell.name, it_sup=r0.iterations, it_learned=r1.iterations,
                              conv_sup=r0.converged, conv_learned=r1.converged,
                              e_sup=r0.total_energy, e_learned=r1.total_energy,
-                             t_sup=r0.total_s, t_learned=r1.total_s))
-            log("eval %d %s: superposition %d it (%.2f s), learned %d it (%.2f s), dE %.2e" % (
-                q, cell.name, r0.iterations, r0.total_s, r1.iterations, r1.total_s,
-                r1.total_energy - r0.total_energy))
-        it0 = np.array([r["it_sup"] for r in rows])
-        it1 = np.array([r["it_learned"] for r in rows])
+                             t_sup=r0.total_s, t_learned=r1.total_s,
+                             start_err_sup=err_sup, start_err_learned=err_learned))
+            log("eval %d %s: superposition %d it (%.2f s), learned %d it (%.2f s), dE %.2e, "
+                "start error %.3g -> %.3g" % (
+                    q, cell.name, r0.iterations, r0.total_s, r1.iterations, r1.total_s,
+                    r1.total_energy - r0.total_energy, err_sup, err_learned))
         report["eval"] = rows
-        report["mean_iterations_superposition"] = float(it0.mean())
-        report["mean_iterations_learned"] = float(it1.mean())
-        report["iteration_reduction"] = float(1.0 - it1.sum() / it0.sum())
-        report["max_abs_energy_difference"] = float(max(abs(r["e_learned"] - r["e_sup"])
-                                                        for r in rows))
-        log("held-out: %.1f -> %.1f iterations (%.0f %% fewer), max |dE| %.2e Ha" % (
-            it0.mean(), it1.mean(), 100 * report["iteration_reduction"],
-            report["max_abs_energy_difference"]))
+        both = [r for r in rows if r["conv_sup"] and r["conv_learned"]]
+        report["eval_converged_by_both"] = len(both)
+        if both:
+            it0 = np.array([r["it_sup"] for r in both])
+            it1 = np.array([r["it_learned"] for r in both])
+            t0s = np.array([r["t_sup"] for r in both])
+            t1s = np.array([r["t_learned"] for r in both])
+            report["mean_iterations_superposition"] = float(it0.mean())
+            report["mean_iterations_learned"] = float(it1.mean())
+            report["iteration_reduction"] = float(1.0 - it1.sum() / it0.sum())
+            report["time_reduction"] = float(1.0 - t1s.sum() / t0s.sum())
+            report["max_abs_energy_difference"] = float(max(abs(r["e_learned"] - r["e_sup"])
+                                                            for r in both))
+            log("held-out (%d of %d converged by both): %.1f -> %.1f iterations (%.0f %% "
+                "fewer), max |dE| %.2e Ha" % (
+                    len(both), len(rows), it0.mean(), it1.mean(),
+                    100 * report["iteration_reduction"], report["max_abs_energy_difference"]))
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1, default=float)
