@@ -477,49 +477,55 @@ def scf_cpu_baseline(system, iterations, first_inputs, threads, log, ep_chunk=64
     with threadpool_limits(threads):
         n, nk, P = system.vale_dim, system.num_kpoints, system.pot_dim
         t0 = time.perf_counter()
-        S = oracle.assemble(system, "ol")
-        K = oracle.assemble(system, "ke")
-        N = oracle.assemble(system, "np")
+        S = oracle.assemble(system, "ol", raw=True)       # [nk][packed, (re, im)]
+        K = oracle.assemble(system, "ke", raw=True)
+        N = oracle.assemble(system, "np", raw=True)
         t["ol_ke_np"] = time.perf_counter() - t0
         t0 = time.perf_counter()
-        V = None
         for j0 in range(0, P, ep_chunk):
-            V = oracle.assemble(system, "ep", j0, min(P, j0 + ep_chunk))
+            V = oracle.assemble(system, "ep", j0, min(P, j0 + ep_chunk), raw=True)
             if j0 == 0:
                 V0 = V
+            del V
         t["ep_all_terms"] = time.perf_counter() - t0
-        del V
         t0 = time.perf_counter()
         pot = oscf.ScfPotential(system)
         t["scfpot_setup"] = time.perf_counter() - t0
         nc = V0.shape[0]
         coeffs = first_inputs[2]
         t0 = time.perf_counter()
-        H = [oracle.build_hamiltonian(N[k], K[k], V0[:, k], coeffs[:nc]) for k in range(nk)]
+        H = [oracle.build_hamiltonian_raw(N[k], K[k], [V0[j, k] for j in range(nc)], coeffs[:nc])
+             for k in range(nk)]
         t["hbuild_chunk"] = time.perf_counter() - t0
+
+        def full(a):
+            return oracle.unpack(a if system.gamma else a[0::2] + 1j * a[1::2], n)
+        Hf = [full(H[k]) for k in range(nk)]
+        Sf = [full(S[k]) for k in range(nk)]
         t0 = time.perf_counter()
         vecs, evals = [], []
         for k in range(nk):
-            w, v, info = oracle.hegv(oracle.unpack(H[k], n), oracle.unpack(S[k], n))
+            w, v, info = oracle.hegv(Hf[k], Sf[k])
             evals.append(w)
             vecs.append(v)
         t["zhegv_all_k"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         occ = oracle.populate(np.array(evals), system.kpoint_weights, oscf.num_electrons(system))
         t["populate"] = time.perf_counter() - t0
-        t0 = time.perf_counter()
         cols = np.concatenate([np.full(c + 1, c) for c in range(n)])
         rows = np.concatenate([np.arange(c + 1) for c in range(n)])
+        t0 = time.perf_counter()
         for k in range(nk):
             X = vecs[k]
             rho = (X * occ[k]) @ X.conj().T
             rp = rho[rows, cols]
-            if system.gamma:
-                rp = rp.real
+            rpr = np.ascontiguousarray(rp.real if system.gamma
+                                       else np.stack([rp.real, rp.imag], -1).ravel())
+            dim1 = 1 if system.gamma else 2
             for M in (S[k], N[k], K[k]):
-                oracle.matrix_element_mult(M, rp, n)
+                oracle.matrix_element_mult_raw(M, rpr, n, dim1)
             for j in range(nc):
-                oracle.matrix_element_mult(V0[j, k], rp, n)
+                oracle.matrix_element_mult_raw(V0[j, k], rpr, n, dim1)
         t["traces_chunk"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         pot.update(first_inputs[0], first_inputs[1], first_inputs[2])
@@ -551,15 +557,33 @@ def bench_scf(args, system, coeffs, ctx, log):
                           atom_iterations=list(db.iterations.values()))
         log("superposition guess: %d isolated-atom SCF(s), %s iterations, %.2f s"
             % (len(db.entries), list(db.iterations.values()), db.seconds))
-    runs = []
-    for i in range(args.warmup + args.steps):
-        res = scf.run(ctx, system, coeffs, max_iter=args.max_iter, ep_kind=ep_kind)
-        tag = "warmup" if i < args.warmup else "step"
-        log("%s %d: %d iterations, converged %s, E = %.10f Ha, set-up %.1f ms, loop %.1f ms, "
-            "total %.1f ms" % (tag, i, res.iterations, res.converged, res.total_energy,
-                               res.setup_s * 1e3, res.loop_s * 1e3, res.total_s * 1e3))
-        if i >= args.warmup:
-            runs.append(res)
+    def leg(name, start):
+        """warmup + steps SCFs from start() (called inside the timed region)."""
+        out = []
+        for i in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            c0 = start()
+            t_guess = time.perf_counter() - t0
+            res = scf.run(ctx, system, c0, max_iter=args.max_iter, ep_kind=ep_kind)
+            res.total_s += t_guess
+            res.guess_s = t_guess
+            tag = "warmup" if i < args.warmup else "step"
+            log("%s %s %d: %d iterations, converged %s, E = %.10f Ha, guess %.1f ms, set-up "
+                "%.1f ms, loop %.1f ms, total %.1f ms"
+                % (name, tag, i, res.iterations, res.converged, res.total_energy, t_guess * 1e3,
+                   res.setup_s * 1e3, res.loop_s * 1e3, res.total_s * 1e3))
+            if i >= args.warmup:
+                out.append(res)
+        return out
+
+    plain = leg("plain", lambda: coeffs)
+    runs = plain
+    learned_runs = None
+    from imago_amd import learned
+    if args.guess == "superposition" and os.path.exists(learned.MODEL_PATH):
+        model = learned.LearnedGuess.load()
+        learned_runs = leg("learned", lambda: model.predict(ctx, system, coeffs))
+        runs = learned_runs
     last = runs[-1]
     for r in last.records:
         log("  iter %2d  E %.10f  dE %s  delta %.3e  %.2f ms" % (
@@ -567,6 +591,7 @@ def bench_scf(args, system, coeffs, ctx, log):
             "-" if r["energy_change"] is None else "%.2e" % r["energy_change"],
             r["testable_delta"], r["wall_s"] * 1e3))
     wall = float(np.mean([r.total_s for r in runs]))
+    wall_plain = float(np.mean([r.total_s for r in plain]))
     n, nk = system.vale_dim, system.num_kpoints
     # per-iteration stages of the last iteration (HIP events)
     st = {"hbuild": ctx.timing(1), "eigensolve": ctx.timing(2), "k_hetrd_coop": ctx.timing(10),
@@ -589,8 +614,9 @@ def bench_scf(args, system, coeffs, ctx, log):
     line = {
         "metric": METRIC,
         "value": round(wall, 6),
-        "unit": "s per SCF to 1e-6 Ha (set-up + %d iterations, plain GPU method, %s "
-                "initial potential)" % (last.iterations, args.guess),
+        "unit": "s per SCF to 1e-6 Ha (%s; set-up + %d iterations)" % (
+            "learned initial potential, the guess's inference included" if learned_runs
+            else "plain GPU method, %s initial potential" % args.guess, last.iterations),
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall * 1e3, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -600,6 +626,15 @@ def bench_scf(args, system, coeffs, ctx, log):
                                % (system.name, system.num_atoms, n, nk,
                                   " (Gamma)" if system.gamma else "", system.pot_dim),
                    "ep_mode": args.ep_mode, "parallelism": "single GPU"},
+        "scf_plain_gpu": {"seconds": round(wall_plain, 6), "iterations": plain[-1].iterations,
+                          "converged": plain[-1].converged,
+                          "total_energy": plain[-1].total_energy, "guess": guess_info},
+        "scf_learned_gpu": None if not learned_runs else {
+            "seconds": round(wall, 6), "iterations": last.iterations,
+            "converged": last.converged, "total_energy": last.total_energy,
+            "guess_ms": round(float(np.mean([r.guess_s for r in learned_runs])) * 1e3, 3),
+            "iteration_reduction": round(1.0 - last.iterations / plain[-1].iterations, 4),
+            "energy_difference": last.total_energy - plain[-1].total_energy},
         "scf": {"iterations": last.iterations, "converged": last.converged, "guess": guess_info,
                 "total_energy": last.total_energy,
                 "setup_ms": round(last.setup_s * 1e3, 3),
@@ -615,19 +650,20 @@ def bench_scf(args, system, coeffs, ctx, log):
         log("cpu baseline leg: the oracle's SCF on %s" % system.name)
         info = host_info()
         cores = int(info.get("cores_available") or 1)
-        c1, s1, p1, t1 = scf_cpu_baseline(system, last.iterations, last.first_inputs, 1, log)
-        call, sall, pall, tall = scf_cpu_baseline(system, last.iterations, last.first_inputs,
+        base = plain[-1]     # the reference's CPU path starts from the superposition
+        c1, s1, p1, t1 = scf_cpu_baseline(system, base.iterations, base.first_inputs, 1, log)
+        call, sall, pall, tall = scf_cpu_baseline(system, base.iterations, base.first_inputs,
                                                   cores, log)
         line["cpu_baseline"] = {
             "value": round(c1, 3), "unit": "s per SCF to 1e-6 Ha (%d iterations)"
-                                           % last.iterations,
+                                           % base.iterations,
             "cores": 1, "kind": "port",
             "sample": ("oracle (C restatement + SciPy LAPACK) on %s: set-up in full (OL/KE/NP, "
                        "all %d EP terms, electrostatics, XC mesh), one iteration's zhegv at all "
                        "%d k-points, H build and traces over %d of %d EP terms scaled, one "
                        "makeSCFPot; SCF = set-up + %d x iteration"
                        % (system.name, system.pot_dim, system.num_kpoints, 64,
-                          system.pot_dim, last.iterations)),
+                          system.pot_dim, base.iterations)),
             "setup_s": round(s1, 3), "per_iteration_s": round(p1, 4),
             "parts_s": {k: round(v, 4) for k, v in t1.items()},
             "all_cores": {"value": round(call, 3), "cores": cores, "setup_s": round(sall, 3),
@@ -636,7 +672,9 @@ def bench_scf(args, system, coeffs, ctx, log):
             "host": info,
         }
         line["vs_cpu"] = {"speedup_1core": round(c1 / wall, 1),
-                          "speedup_all_cores": round(call / wall, 1)}
+                          "speedup_all_cores": round(call / wall, 1),
+                          "plain_gpu_speedup_1core": round(c1 / wall_plain, 1),
+                          "plain_gpu_speedup_all_cores": round(call / wall_plain, 1)}
     print(json.dumps(line), flush=True)
 
 

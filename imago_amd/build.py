@@ -50,15 +50,17 @@ FLANG = "/opt/rocm/bin/amdflang"
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """Compile the iso_c_binding module and the small Fortran host that
-    links against libimago_hip.so (imago_amd/fortran/_bin/hip_driver)."""
+    """Compile the iso_c_binding module and the Fortran hosts that link
+    against libimago_hip.so (imago_amd/fortran/_bin/hip_driver: a solve;
+    scf_driver: the integrals and the SCF loop)."""
     os.makedirs(FBIN, exist_ok=True)
     mod = [FLANG, "-c", "-J" + FBIN, os.path.join(FORTRAN, "imago_hip.f90"), "-o",
            os.path.join(FBIN, "imago_hip.o")]
-    drv = [FLANG, "-J" + FBIN, os.path.join(FORTRAN, "hip_driver.f90"),
-           os.path.join(FBIN, "imago_hip.o"), "-L" + HERE, "-limago_hip",
-           "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(FBIN, "hip_driver")]
-    for cmd in (mod, drv):
+    drvs = [[FLANG, "-J" + FBIN, os.path.join(FORTRAN, name + ".f90"),
+             os.path.join(FBIN, "imago_hip.o"), "-L" + HERE, "-limago_hip",
+             "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(FBIN, name)]
+            for name in ("hip_driver", "scf_driver")]
+    for cmd in [mod] + drvs:
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -14,6 +14,7 @@
 #include <array>
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <atomic>
 #include <chrono>
 #include <thread>
@@ -236,6 +237,14 @@ struct imago_ctx {
         int info = 0;
     };
     SpdMat spd[2];
+    // run-time choices of the batched eigensolve (imago_tune): columns per
+    // diagonal block of the blocked Cholesky (<= kPotrfBlock); the
+    // eigensolver (0: this library's, 1: rocSOLVER's heevd / syevd); the
+    // Cholesky (0: this library's blocked one, 1: rocSOLVER's potrf).
+    // Initialised from IMAGO_HEEVD / IMAGO_POTRF = rocsolver.
+    int tune_potrf_block = kPotrfBlock;
+    int tune_heevd = 0;
+    int tune_potrf = 0;
     // makeSCFPot state (imago_scf_setup / imago_scf_update): the
     // electrostatic operators on the device, the per-iteration vectors on
     // the host, the mixing history the reference keeps in module variables
@@ -860,7 +869,36 @@ extern "C" int imago_ctx_create(int device, imago_ctx **out)
         delete c;
         return fail(IMAGO_E_HIP, "rocblas handle creation failed");
     }
+    {
+        const char *e1 = getenv("IMAGO_HEEVD"), *e2 = getenv("IMAGO_POTRF");
+        c->tune_heevd = (e1 && strcmp(e1, "rocsolver") == 0) ? 1 : 0;
+        c->tune_potrf = (e2 && strcmp(e2, "rocsolver") == 0) ? 1 : 0;
+    }
     *out = c;
+    return IMAGO_OK;
+}
+
+extern "C" int imago_tune(imago_ctx *c, int param, int value)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    switch (param) {
+    case IMAGO_TUNE_POTRF_BLOCK:
+        if (value < 1 || value > kPotrfBlock)
+            return fail(IMAGO_E_ARG, "Cholesky block %d outside 1..%d", value, kPotrfBlock);
+        c->tune_potrf_block = value;
+        break;
+    case IMAGO_TUNE_EIGENSOLVER:
+        if (value != 0 && value != 1) return fail(IMAGO_E_ARG, "eigensolver %d", value);
+        c->tune_heevd = value;
+        break;
+    case IMAGO_TUNE_CHOLESKY:
+        if (value != 0 && value != 1) return fail(IMAGO_E_ARG, "Cholesky %d", value);
+        c->tune_potrf = value;
+        break;
+    default:
+        return fail(IMAGO_E_ARG, "unknown tuning parameter %d", param);
+    }
+    c->chol_valid = false;   // the next secular call factors the overlap again
     return IMAGO_OK;
 }
 
@@ -1377,8 +1415,7 @@ static rocblas_status gemm_sb(rocblas_handle hb, bool cx, rocblas_operation ta,
 static int potrf_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int *dinfo)
 {
     const rocblas_stride nn = (rocblas_stride)n * n;
-    const char *env = getenv("IMAGO_POTRF");
-    if (env && strcmp(env, "rocsolver") == 0) {
+    if (c->tune_potrf == 1) {
         if (cx)
             BLASCHK(rocsolver_zpotrf_strided_batched(c->blas, rocblas_fill_upper, n,
                                                      (rocblas_double_complex *)B, n, nn, dinfo, nb));
@@ -1389,8 +1426,9 @@ static int potrf_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int *din
     }
     const size_t es = cx ? sizeof(double2) : sizeof(double);
     char *base = static_cast<char *>(B);
-    for (int j = 0; j < n; j += kPotrfBlock) {
-        const int jb = std::min(kPotrfBlock, n - j), rest = n - j - jb;
+    const int blk = std::max(1, std::min(c->tune_potrf_block, kPotrfBlock));
+    for (int j = 0; j < n; j += blk) {
+        const int jb = std::min(blk, n - j), rest = n - j - jb;
         HIPCHK(launch_potf2_diag(B, n, nb, j, jb, !cx, dinfo, c->stream));
         if (rest == 0) break;
         void *Ajj = base + ((size_t)j * n + j) * es;
@@ -1452,18 +1490,23 @@ static int factor_gv_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int 
 // The default up to order 1024 (complex) / 2048 (real): at 4 x 896 complex
 // it takes 17 ms where rocSOLVER's batched zheevd takes 30
 // (IMAGO_HEEVD=rocsolver selects the latter).
-static bool use_coop_heevd(int n, bool cx)
+static bool use_coop_heevd(const imago_ctx *c, int n, bool cx)
 {
-    const char *env = getenv("IMAGO_HEEVD");
-    return hetrd_coop_supported(n, !cx) && !(env && strcmp(env, "rocsolver") == 0);
+    return hetrd_coop_supported(n, !cx) && c->tune_heevd == 0;
 }
 
 // heevd_coop's return when the cooperative launch is refused (A untouched)
 constexpr int kCoopRefused = 1000;
 
+// One cooperative reduction in flight per process: contexts on other host
+// threads (the batched stream runs several) wait here rather than launch a
+// second grid that needs to be co-resident with the first.
+static std::mutex g_coop_mu;
+
 static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *evals,
                       double *ework, int *dinfo)
 {
+    std::lock_guard<std::mutex> coop_lock(g_coop_mu);
     const size_t nn = (size_t)n * n, es = cx ? sizeof(double2) : sizeof(double);
     const int N = n * nb, K = kTrdBlock, nblk = (n - 1 + K - 1) / K;
     char *V = nullptr, *tau = nullptr, *vbuf = nullptr, *pbuf = nullptr, *dbuf = nullptr,
@@ -1609,7 +1652,7 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));
-        const int rc = use_coop_heevd(n, true) ? heevd_coop(c, true, n, nb, A, evals, ework, dinfo)
+        const int rc = use_coop_heevd(c, n, true) ? heevd_coop(c, true, n, nb, A, evals, ework, dinfo)
                                                : kCoopRefused;
         if (rc != kCoopRefused) {
             RCHK(rc);
@@ -1631,7 +1674,7 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));
-        const int rc = use_coop_heevd(n, false)
+        const int rc = use_coop_heevd(c, n, false)
                            ? heevd_coop(c, false, n, nb, A, evals, ework, dinfo)
                            : kCoopRefused;
         if (rc != kCoopRefused) {

@@ -540,3 +540,4 @@ extern "C" int imago_mlp_train(imago_mlp *m, int num_samples, const float *x, co
     HIPCHK(hipStreamSynchronize(m->stream));
     return IMAGO_OK;
 }
+

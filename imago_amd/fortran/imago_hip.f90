@@ -46,6 +46,22 @@ module O_ImagoHIP
       real(c_double) :: r_sample_in, r_sample_out, r_sample_space
    end type imago_xc_mesh_params
 
+   ! imago_scf_params: the MAIN_INPUT_DATA block of imago.dat
+   type, bind(C) :: imago_scf_params
+      real(c_double) :: num_electrons
+      integer(c_int) :: feedback_level
+      real(c_double) :: relax_factor, convg_test
+      integer(c_int) :: xc_code
+      real(c_double) :: log_elec_thresh
+   end type imago_scf_params
+
+   ! imago_scf_record: one fort.7 / fort.14 row
+   type, bind(C) :: imago_scf_record
+      integer(c_int) :: iteration, converged
+      real(c_double) :: electron_diff, testable_delta
+      real(c_double) :: kinetic_energy, elec_stat_energy, exch_corr_energy, total_energy
+   end type imago_scf_record
+
    type(c_ptr), save :: imagoCtx = c_null_ptr
 
    interface
@@ -241,6 +257,41 @@ module O_ImagoHIP
          real(c_double), intent(inout) :: b(*)
          integer(c_int), intent(out) :: info
       end function imago_pot_solve
+
+      ! populateStandard (populate.F90:190-485)
+      integer(c_int) function imago_populate(numKPoints, numStates, eigenValues, &
+            & kPointWeight, numElectrons, populations, occupiedEnergy, bandGap, gapKind) &
+            & bind(C, name="imago_populate")
+         import :: c_int, c_double
+         integer(c_int), value :: numKPoints, numStates
+         real(c_double), intent(in) :: eigenValues(*), kPointWeight(*)
+         real(c_double), value :: numElectrons
+         real(c_double), intent(out) :: populations(*), occupiedEnergy, bandGap
+         integer(c_int), intent(out) :: gapKind
+      end function imago_populate
+
+      ! makeSCFPot's set-up (electrostatics, XC mesh, overlaps on the device)
+      integer(c_int) function imago_scf_setup(ctx, params, mesh, covalentRadius, &
+            & numCellsRecip, cellDimsRecip, cellSizesRecip) bind(C, name="imago_scf_setup")
+         import :: c_int, c_ptr, c_double, imago_scf_params, imago_xc_mesh_params
+         type(c_ptr), value :: ctx
+         type(imago_scf_params), intent(in) :: params
+         type(imago_xc_mesh_params), intent(in) :: mesh
+         real(c_double), intent(in) :: covalentRadius(*)
+         integer(c_int), value :: numCellsRecip
+         real(c_double), intent(in) :: cellDimsRecip(3,*), cellSizesRecip(*)
+      end function imago_scf_setup
+
+      ! makeSCFPot (potentialUpdate.F90:53-1866): potCoeffs in and out
+      integer(c_int) function imago_scf_update(ctx, traces3, potRho, potCoeffs, &
+            & rhoCoeffs, record) bind(C, name="imago_scf_update")
+         import :: c_int, c_ptr, c_double, imago_scf_record
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: traces3(3), potRho(*)
+         real(c_double), intent(inout) :: potCoeffs(*)
+         real(c_double), intent(out) :: rhoCoeffs(*)
+         type(imago_scf_record), intent(out) :: record
+      end function imago_scf_update
 
       integer(c_int) function imago_xc_counts(ctx, rayPoints, gemmFlops) &
             & bind(C, name="imago_xc_counts")

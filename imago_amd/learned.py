@@ -215,9 +215,20 @@ class LearnedGuess:
             if own:
                 m.close()
             dy = y * net["y_std"] + net["y_mean"]
+            # how the correction is applied, chosen on validation cells
+            # (select_application): a scale and the number of most diffuse
+            # terms corrected (the sharp terms keep their atomic values)
+            scale = float(net["scale"]) if "scale" in net else 1.0
+            nterm = int(net["terms"]) if "terms" in net else NPOT
             for t, d in zip(types, dy):
-                out[term0[t]:term0[t] + NPOT] += d
+                out[term0[t]:term0[t] + nterm] += scale * d[:nterm]
         return out
+
+    def with_application(self, element, scale, terms):
+        nets = {el: dict(net) for el, net in self.nets.items()}
+        nets[element]["scale"] = np.float64(scale)
+        nets[element]["terms"] = np.int64(terms)
+        return LearnedGuess(nets)
 
 
 # -- generation, training, evaluation ------------------------------------------
@@ -262,6 +273,47 @@ def train_models(ctx, samples, epochs=400, minibatch=256, lr=1e-3, seed=0, log=N
     return LearnedGuess(nets)
 
 
+APPLICATIONS = ((1.0, 16), (0.5, 16), (1.0, 8), (0.5, 8), (1.0, 12))
+
+
+def select_application(ctx, model, cells, db, max_iter, log=None):
+    """Per element, the (scale, diffuse terms) of APPLICATIONS with the
+    fewest SCF iterations summed over validation cells of that element
+    (a cell that does not converge counts max_iter); the superposition
+    alone is kept when no application beats it."""
+    chosen = {}
+    for el in list(model.nets):
+        mine = [c for c in cells if c.pot_types[0].element == el]
+        if not mine:
+            continue
+        base = []
+        atomics = []
+        for c in mine:
+            a = guess.superposition_guess(ctx, c, db)
+            atomics.append(a)
+            r = scf.run(ctx, c, a, max_iter=max_iter)
+            base.append(r.iterations if r.converged else max_iter)
+        best = (sum(base), None)
+        scores = {"superposition": sum(base)}
+        for scale, terms in APPLICATIONS:
+            trial = model.with_application(el, scale, terms)
+            tot = 0
+            for c, a in zip(mine, atomics):
+                r = scf.run(ctx, c, trial.predict(ctx, c, a), max_iter=max_iter)
+                tot += r.iterations if r.converged else max_iter
+            scores["scale %.1f, %d terms" % (scale, terms)] = tot
+            if tot < best[0]:
+                best = (tot, (scale, terms))
+        if log:
+            log("element %d validation iterations: %s" % (el, scores))
+        chosen[el] = dict(scores=scores, choice=best[1])
+        if best[1] is None:
+            model = model.with_application(el, 0.0, 0)
+        else:
+            model = model.with_application(el, *best[1])
+    return model, chosen
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--cells", type=int, default=512)
@@ -274,89 +326,124 @@ def main(argv=None):
     ap.add_argument("--out", default=MODEL_PATH)
     ap.add_argument("--report", default=None)
     ap.add_argument("--seed", type=int, default=20)
+    ap.add_argument("--validation", type=int, default=16,
+                    help="32-atom validation cells for choosing how the correction is applied")
+    ap.add_argument("--model", default=None,
+                    help="skip generation and training: select and evaluate this model")
     args = ap.parse_args(argv)
     log = lambda *a: print("[learned]", *a, flush=True)
     report = {"cells": args.cells, "eval_cells": args.eval, "epochs": args.epochs}
     with lib.Context(0) as ctx:
         db = guess.AtomicPotentials()
-        feats, targs, alphas = {}, {}, {}
-        t_gen = time.perf_counter()
-        skipped = 0
-        for q, cell in enumerate(training_cells(args.cells, args.seed)):
-            t0 = time.perf_counter()
-            try:
-                atomic, res = converge(ctx, cell, db, args.max_iter)
-            except lib.ImagoError as e:
-                skipped += 1
-                log("cell %d %s: %s, skipped" % (q, cell.name, e))
-                continue
-            if not res.converged:
-                skipped += 1
-                log("cell %d %s: not converged in %d iterations, skipped" % (q, cell.name,
-                                                                             res.iterations))
-                continue
-            f = features(site_descriptors(ctx, cell))
-            y = site_targets(cell, atomic, res.coeffs)
-            el = cell.pot_types[0].element
-            feats.setdefault(el, []).append(f)
-            targs.setdefault(el, []).append(y)
-            alphas[el] = cell.pot_types[0].alphas
-            log("cell %d %s: %d atoms, %d iterations, E = %.8f, %.2f s" % (
-                q, cell.name, cell.num_atoms, res.iterations, res.total_energy,
-                time.perf_counter() - t0))
-        report["generation_s"] = time.perf_counter() - t_gen
-        report["skipped"] = skipped
-        samples = {el: (np.concatenate(feats[el]), np.concatenate(targs[el]), alphas[el])
-                   for el in feats}
+        if args.model:
+            model = LearnedGuess.load(args.model)
+        else:
+            model = generate_and_train(ctx, db, args, report, log)
         t0 = time.perf_counter()
-        model = train_models(ctx, samples, epochs=args.epochs, log=log)
-        report["training_s"] = time.perf_counter() - t0
-        report["training_sites"] = {str(el): int(len(samples[el][0])) for el in samples}
-        report["final_loss"] = {str(el): float(model.nets[el]["losses"][-1]) for el in model.nets}
+        vcells = training_cells(args.validation, args.seed + 500, sizes=(32,))
+        model, chosen = select_application(ctx, model, vcells, db, args.eval_max_iter, log)
+        report["selection"] = {str(k): v for k, v in chosen.items()}
+        report["selection_s"] = time.perf_counter() - t0
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
         model.save(args.out)
-        # held-out 64-atom cells: superposition vs learned guess
-        rows = []
-        rng_cells = training_cells(args.eval, args.seed + 1000, sizes=(64,))
-        for q, cell in enumerate(rng_cells):
-            atomic = guess.superposition_guess(ctx, cell, db)
-            learned = model.predict(ctx, cell, atomic)
-            r0 = scf.run(ctx, cell, atomic, max_iter=args.eval_max_iter)
-            r1 = scf.run(ctx, cell, learned, max_iter=args.eval_max_iter)
-            # how far each guess starts from the converged potential
-            err_sup = float(np.linalg.norm(atomic - r0.coeffs))
-            err_learned = float(np.linalg.norm(learned - r0.coeffs))
-            rows.append(dict(name=cell.name, it_sup=r0.iterations, it_learned=r1.iterations,
-                             conv_sup=r0.converged, conv_learned=r1.converged,
-                             e_sup=r0.total_energy, e_learned=r1.total_energy,
-                             t_sup=r0.total_s, t_learned=r1.total_s,
-                             start_err_sup=err_sup, start_err_learned=err_learned))
-            log("eval %d %s: superposition %d it (%.2f s), learned %d it (%.2f s), dE %.2e, "
-                "start error %.3g -> %.3g" % (
-                    q, cell.name, r0.iterations, r0.total_s, r1.iterations, r1.total_s,
-                    r1.total_energy - r0.total_energy, err_sup, err_learned))
-        report["eval"] = rows
-        both = [r for r in rows if r["conv_sup"] and r["conv_learned"]]
-        report["eval_converged_by_both"] = len(both)
-        if both:
-            it0 = np.array([r["it_sup"] for r in both])
-            it1 = np.array([r["it_learned"] for r in both])
-            t0s = np.array([r["t_sup"] for r in both])
-            t1s = np.array([r["t_learned"] for r in both])
-            report["mean_iterations_superposition"] = float(it0.mean())
-            report["mean_iterations_learned"] = float(it1.mean())
-            report["iteration_reduction"] = float(1.0 - it1.sum() / it0.sum())
-            report["time_reduction"] = float(1.0 - t1s.sum() / t0s.sum())
-            report["max_abs_energy_difference"] = float(max(abs(r["e_learned"] - r["e_sup"])
-                                                            for r in both))
-            log("held-out (%d of %d converged by both): %.1f -> %.1f iterations (%.0f %% "
-                "fewer), max |dE| %.2e Ha" % (
-                    len(both), len(rows), it0.mean(), it1.mean(),
-                    100 * report["iteration_reduction"], report["max_abs_energy_difference"]))
+        evaluate(ctx, model, db, args, report, log)
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1, default=float)
     return report
+
+
+def generate_and_train(ctx, db, args, report, log):
+    """Converge the training cells from the superposition guess, collect
+    (descriptor, converged - atomic) per site, train one network per
+    element."""
+    feats, targs, alphas = {}, {}, {}
+    t_gen = time.perf_counter()
+    skipped = 0
+    for q, cell in enumerate(training_cells(args.cells, args.seed)):
+        t0 = time.perf_counter()
+        try:
+            atomic, res = converge(ctx, cell, db, args.max_iter)
+        except lib.ImagoError as e:
+            skipped += 1
+            log("cell %d %s: %s, skipped" % (q, cell.name, e))
+            continue
+        if not res.converged:
+            skipped += 1
+            log("cell %d %s: not converged in %d iterations, skipped" % (q, cell.name,
+                                                                         res.iterations))
+            continue
+        f = features(site_descriptors(ctx, cell))
+        y = site_targets(cell, atomic, res.coeffs)
+        el = cell.pot_types[0].element
+        feats.setdefault(el, []).append(f)
+        targs.setdefault(el, []).append(y)
+        alphas[el] = cell.pot_types[0].alphas
+        log("cell %d %s: %d atoms, %d iterations, E = %.8f, %.2f s" % (
+            q, cell.name, cell.num_atoms, res.iterations, res.total_energy,
+            time.perf_counter() - t0))
+    report["generation_s"] = time.perf_counter() - t_gen
+    report["skipped"] = skipped
+    samples = {el: (np.concatenate(feats[el]), np.concatenate(targs[el]), alphas[el])
+               for el in feats}
+    t0 = time.perf_counter()
+    model = train_models(ctx, samples, epochs=args.epochs, log=log)
+    report["training_s"] = time.perf_counter() - t0
+    report["training_sites"] = {str(el): int(len(samples[el][0])) for el in samples}
+    report["final_loss"] = {str(el): float(model.nets[el]["losses"][-1]) for el in model.nets}
+    return model
+
+
+def evaluate(ctx, model, db, args, report, log):
+    """Held-out 64-atom cells: the SCF from the superposition and from the
+    learned guess, iterations, time and final energies side by side."""
+    # held-out 64-atom cells: superposition vs learned guess
+    rows = []
+    rng_cells = training_cells(args.eval, args.seed + 1000, sizes=(64,))
+    for q, cell in enumerate(rng_cells):
+        atomic = guess.superposition_guess(ctx, cell, db)
+        learned = model.predict(ctx, cell, atomic)
+        r0 = scf.run(ctx, cell, atomic, max_iter=args.eval_max_iter)
+        r1 = scf.run(ctx, cell, learned, max_iter=args.eval_max_iter)
+        # how far each guess starts from the converged potential
+        err_sup = float(np.linalg.norm(atomic - r0.coeffs))
+        err_learned = float(np.linalg.norm(learned - r0.coeffs))
+        rows.append(dict(name=cell.name, it_sup=r0.iterations, it_learned=r1.iterations,
+                         conv_sup=r0.converged, conv_learned=r1.converged,
+                         e_sup=r0.total_energy, e_learned=r1.total_energy,
+                         t_sup=r0.total_s, t_learned=r1.total_s,
+                         start_err_sup=err_sup, start_err_learned=err_learned))
+        log("eval %d %s: superposition %d it (%.2f s), learned %d it (%.2f s), dE %.2e, "
+            "start error %.3g -> %.3g" % (
+                q, cell.name, r0.iterations, r0.total_s, r1.iterations, r1.total_s,
+                r1.total_energy - r0.total_energy, err_sup, err_learned))
+    report["eval"] = rows
+    both = [r for r in rows if r["conv_sup"] and r["conv_learned"]]
+    report["eval_converged_by_both"] = len(both)
+    if both:
+        it0 = np.array([r["it_sup"] for r in both])
+        it1 = np.array([r["it_learned"] for r in both])
+        t0s = np.array([r["t_sup"] for r in both])
+        t1s = np.array([r["t_learned"] for r in both])
+        report["mean_iterations_superposition"] = float(it0.mean())
+        report["mean_iterations_learned"] = float(it1.mean())
+        report["iteration_reduction"] = float(1.0 - it1.sum() / it0.sum())
+        report["time_reduction"] = float(1.0 - t1s.sum() / t0s.sum())
+        report["max_abs_energy_difference"] = float(max(abs(r["e_learned"] - r["e_sup"])
+                                                        for r in both))
+        log("held-out (%d of %d converged by both): %.1f -> %.1f iterations (%.0f %% "
+            "fewer), max |dE| %.2e Ha" % (
+                len(both), len(rows), it0.mean(), it1.mean(),
+                100 * report["iteration_reduction"], report["max_abs_energy_difference"]))
+    cap = args.eval_max_iter
+    it0 = np.array([r["it_sup"] if r["conv_sup"] else cap for r in rows])
+    it1 = np.array([r["it_learned"] if r["conv_learned"] else cap for r in rows])
+    report["all_cells_iterations"] = dict(superposition=float(it0.mean()),
+                                          learned=float(it1.mean()),
+                                          converged_superposition=int(sum(
+                                              r["conv_sup"] for r in rows)),
+                                          converged_learned=int(sum(
+                                              r["conv_learned"] for r in rows)))
 
 
 if __name__ == "__main__":

@@ -43,7 +43,7 @@ EXPORTS = [
     "imago_xc_potential", "imago_pot_solve", "imago_populate", "imago_scf_setup",
     "imago_scf_update", "imago_site_descriptors", "imago_mlp_create", "imago_mlp_destroy",
     "imago_mlp_num_params", "imago_mlp_params", "imago_mlp_forward", "imago_mlp_gradient",
-    "imago_mlp_adam_step", "imago_mlp_train",
+    "imago_mlp_adam_step", "imago_mlp_train", "imago_tune",
 ]
 
 
@@ -134,6 +134,7 @@ def load():
     L.imago_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_int, _fp, _fp, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                   ctypes.c_float, ctypes.c_uint, _dp]
+    L.imago_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     _lib = L
     return L
 
@@ -383,6 +384,13 @@ class Context:
         _check(load().imago_scf_update(self._h, _p(tr), _p(pr), _p(c), _p(rho),
                                        ctypes.byref(rec)))
         return c, rec.as_dict(), rho
+
+    TUNE = {"potrf_block": 1, "eigensolver": 2, "cholesky": 3}
+
+    def tune(self, **kw):
+        """imago_tune: potrf_block=1..32, eigensolver=0|1, cholesky=0|1."""
+        for k, v in kw.items():
+            _check(load().imago_tune(self._h, self.TUNE[k], int(v)))
 
     def xc_counts(self):
         """(all ray points, flops of the last overlap's GEMMs)."""

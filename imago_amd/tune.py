@@ -1,0 +1,83 @@
+"""The autotuner of the batched eigensolve.
+
+imago_tune exposes the run-time choices of the per-iteration stage: the
+columns per diagonal block of the blocked Cholesky (8, 16, 24, 32), and the
+eigensolver (this library's cooperative tridiagonal reduction with divide
+and conquer and the compact-WY back-transform, or rocSOLVER's heevd /
+syevd).  The best choice depends on the matrix order, the batch (k-points)
+and the scalar type; round-one measurements put the optimum at 32 columns
+for 4 x 896 complex and at 16 for 1 x 1944 real.  ``Autotuner`` times each
+candidate on the system the context holds, two imago_secular calls per
+candidate (the first also factors the overlap), scores a candidate by the
+cost of an SCF of ``iterations`` iterations, keeps the fastest per
+(order rounded up to 128, k-points, complex / real) in a cache (a JSON file when a path is
+given) and applies it.
+"""
+from __future__ import annotations
+
+import json
+import os
+import threading
+import time
+
+BLOCKS = (8, 16, 24, 32)
+
+
+def key(system) -> str:
+    """The class a choice is kept for: the order rounded up to a multiple
+    of 128, the k-point count, complex or real."""
+    n = -(-system.vale_dim // 128) * 128
+    return "n%d_k%d_%s" % (n, system.num_kpoints, "real" if system.gamma else "cx")
+
+
+def coop_supported(system) -> bool:
+    """Orders the library's own tridiagonal reduction takes (1024 complex,
+    2048 real)."""
+    return system.vale_dim <= (2048 if system.gamma else 1024)
+
+
+class Autotuner:
+    def __init__(self, path: str | None = None, iterations: int = 30):
+        self.path = path
+        self.iterations = iterations
+        self.cache = {}
+        self.log = {}
+        self._lock = threading.Lock()
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.cache = json.load(f)
+
+    def candidates(self, system):
+        solvers = (0, 1) if coop_supported(system) else (1,)
+        return [dict(potrf_block=b, eigensolver=e) for e in solvers for b in BLOCKS]
+
+    def choose(self, ctx, system, coeffs):
+        """The cached choice for this system's class, or a timing of every
+        candidate on it (the system must be set and assembled on ctx)."""
+        k = key(system)
+        with self._lock:
+            got = self.cache.get(k)
+        if got is None:
+            n = system.vale_dim
+            scores = []
+            for cand in self.candidates(system):
+                ctx.tune(**cand)
+                t0 = time.perf_counter()
+                ctx.secular(coeffs, n)
+                t1 = time.perf_counter()
+                ctx.secular(coeffs, n)
+                t2 = time.perf_counter()
+                first, steady = t1 - t0, t2 - t1
+                scores.append((first + (self.iterations - 1) * steady, cand, first, steady))
+            best = min(scores, key=lambda x: x[0])
+            got = dict(best[1])
+            with self._lock:
+                self.cache[k] = got
+                self.log[k] = [dict(c, first_ms=round(f * 1e3, 3), steady_ms=round(s * 1e3, 3))
+                               for _, c, f, s in scores]
+                if self.path:
+                    os.makedirs(os.path.dirname(os.path.abspath(self.path)), exist_ok=True)
+                    with open(self.path, "w") as f:
+                        json.dump(self.cache, f, indent=1)
+        ctx.tune(**got)
+        return got

@@ -203,6 +203,20 @@ int imago_zhegv(imago_ctx *ctx, int n, double *A, double *B, double *w,
 int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
                 int *info);
 
+/* ---- run-time choices of the batched eigensolve ------------------------ *
+ * For the autotuner (imago_amd/tune.py): IMAGO_TUNE_POTRF_BLOCK, columns per
+ * diagonal block of the blocked Cholesky, 1..32 (default 32);
+ * IMAGO_TUNE_EIGENSOLVER, 0 = this library's tridiagonal reduction and
+ * back-transform (default where the order allows it), 1 = rocSOLVER's
+ * heevd / syevd; IMAGO_TUNE_CHOLESKY, 0 = this library's blocked Cholesky,
+ * 1 = rocSOLVER's potrf.  Results do not depend on the choice beyond
+ * rounding.  Setting any of them makes the next imago_secular factor the
+ * overlap again. */
+#define IMAGO_TUNE_POTRF_BLOCK 1
+#define IMAGO_TUNE_EIGENSOLVER 2
+#define IMAGO_TUNE_CHOLESKY 3
+int imago_tune(imago_ctx *ctx, int param, int value);
+
 /* ---- timing ----------------------------------------------------------- *
  * Milliseconds of the last launch of the named stage measured with HIP
  * events on the library's stream: 0 = last imago_assemble, 1 = H build of

@@ -326,6 +326,36 @@ def build_hamiltonian(np_mat, ke_mat, v_terms, coeffs):
     return h
 
 
+def hamiltonian(system, coeffs, chunk=32, threads=None):
+    """H(k) = NP + KE + sum_j c_j V_j for every k-point, packed, with the
+    potential terms assembled ``chunk`` at a time and accumulated in term
+    order as secularEqnSCF's readPackedMatrixAccum loop does
+    (secularEqn.F90:148-171, matrixSubs.F90:339-376: a factor of exactly
+    0.0 adds the term unscaled); host memory stays at one chunk of panels.
+    threads sets the assembly's OpenMP threads for the call."""
+    if threads:
+        lib().orc_set_threads(int(threads))
+    try:
+        h = assemble(system, "np", raw=True) + assemble(system, "ke", raw=True)   # [nk][..]
+        P = system.pot_dim
+        for j0 in range(0, P, chunk):
+            V = assemble(system, "ep", j0, min(P, j0 + chunk), raw=True)
+            for q in range(V.shape[0]):
+                c = float(coeffs[j0 + q])
+                if c == 0.0:
+                    h += V[q]
+                else:
+                    h += V[q] * c
+            del V
+    finally:
+        if threads:
+            lib().orc_set_threads(1)
+    if system.gamma:
+        return h.reshape(1, -1)
+    h = h.reshape(h.shape[0], -1, 2)
+    return h[..., 0] + 1j * h[..., 1]
+
+
 def build_hamiltonian_raw(np_mat, ke_mat, v_terms, coeffs):
     """orc_build_hamiltonian on interleaved packed arrays (C loop of
     secularEqn.F90:148-171)."""

@@ -73,3 +73,31 @@ def test_mlp_training_lowers_the_loss(gpu_ctx):
     loss, _ = m.gradient(x, t)
     assert loss < 2.0 * losses[-1]
     m.close()
+
+
+def test_trained_guess_needs_fewer_iterations(gpu_ctx):
+    """What the trained network (imago_amd/data/learned_guess.npz) gains over
+    the method without it: the SCF of the 64-atom Si workload and of two
+    held-out layered C cells from the learned guess needs fewer iterations
+    than from the atomic superposition, and ends at the same total energy
+    within 5e-5 Ha (the stop rule |dE| < 1e-6 leaves the slowly converging
+    tail uncertain at that level)."""
+    from imago_amd import configs, guess, scf
+    model = learned.LearnedGuess.load()
+    rng = np.random.default_rng(77)
+    cells = [configs.si64_supercell(),
+             learned.layered_cell(configs.C, learned.LAYERED_REPS[32], rng),
+             learned.layered_cell(configs.C, learned.LAYERED_REPS[32], rng)]
+    db = guess.AtomicPotentials()
+    tot0 = tot1 = 0
+    for cell in cells:
+        atomic = guess.superposition_guess(gpu_ctx, cell, db)
+        r0 = scf.run(gpu_ctx, cell, atomic, max_iter=150)
+        r1 = scf.run(gpu_ctx, cell, model.predict(gpu_ctx, cell, atomic), max_iter=150)
+        print("[gpu] %s: superposition %d iterations, learned %d, dE %.2e" % (
+            cell.name, r0.iterations, r1.iterations, r1.total_energy - r0.total_energy), flush=True)
+        assert r0.converged and r1.converged
+        assert abs(r1.total_energy - r0.total_energy) < 5e-5
+        tot0 += r0.iterations
+        tot1 += r1.iterations
+    assert tot1 < 0.85 * tot0, (tot0, tot1)
