@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -541,3 +542,70 @@ extern "C" int imago_mlp_train(imago_mlp *m, int num_samples, const float *x, co
     return IMAGO_OK;
 }
 
+// ---- forward pass that allocates nothing -----------------------------------
+//
+// For callers that cannot allocate (code under HIP graph capture, serving
+// loops): the hidden activations live in a file-scope device array of
+// kNoAllocFloats floats (4 MiB on every device the library runs on, held
+// until the process ends).  Its address on each device comes from
+// hipGetSymbolAddress on that device's first call; host code never uses
+// the array's name as a device address.  Every call on a device shares
+// the device's array: calls that overlap on one device race.
+
+namespace {
+
+constexpr size_t kNoAllocFloats = (size_t)1 << 20;
+__device__ __align__(256) float scratch[kNoAllocFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_device(int dev, float **out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(IMAGO_E_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(IMAGO_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return IMAGO_OK;
+}
+
+}  // namespace
+
+extern "C" int imago_mlp_forward_noalloc(imago_mlp *m, int batch, const float *x_dev,
+                                         float *y_dev)
+{
+    if (!m || batch < 1 || !x_dev || !y_dev) return fail(IMAGO_E_ARG, "bad forward arguments");
+    const int L = (int)m->w.size() - 1;
+    // two hidden buffers used alternately: the widest hidden layer, twice
+    size_t widest = 0;
+    for (int l = 1; l < L; ++l) widest = std::max(widest, (size_t)m->w[l]);
+    const size_t need = 2 * widest * (size_t)batch;
+    if (need > kNoAllocFloats)
+        return fail(IMAGO_E_ARG, "batch %d needs %zu scratch floats, the limit is %zu (batch <= "
+                                 "%zu for this network)", batch, need, kNoAllocFloats,
+                    widest ? kNoAllocFloats / (2 * widest) : (size_t)0);
+    HIPCHK(hipSetDevice(m->device));
+    float *base = nullptr;
+    RCHK(scratch_on_device(m->device, &base));
+    float *buf[2] = {base, base + widest * (size_t)batch};
+    const float *in = x_dev;
+    for (int l = 0; l < L; ++l) {
+        const int din = m->w[l], dout = m->w[l + 1];
+        float *out = l + 1 < L ? buf[l % 2] : y_dev;
+        const float *W = m->p + m->off[l], *b = W + (size_t)dout * din;
+        gemm(m->stream, false, true, batch, dout, din, in, W, out, 0);
+        hipLaunchKernelGGL(k_bias_act, dim3(blocks((size_t)batch * dout)), dim3(256), 0,
+                           m->stream, out, b, batch, dout, l + 1 < L ? 1 : 0);
+        in = out;
+    }
+    HIPCHK(hipGetLastError());
+    return IMAGO_OK;
+}

@@ -43,7 +43,7 @@ EXPORTS = [
     "imago_xc_potential", "imago_pot_solve", "imago_populate", "imago_scf_setup",
     "imago_scf_update", "imago_site_descriptors", "imago_mlp_create", "imago_mlp_destroy",
     "imago_mlp_num_params", "imago_mlp_params", "imago_mlp_forward", "imago_mlp_gradient",
-    "imago_mlp_adam_step", "imago_mlp_train", "imago_tune",
+    "imago_mlp_adam_step", "imago_mlp_train", "imago_mlp_forward_noalloc", "imago_tune",
 ]
 
 
@@ -135,6 +135,8 @@ def load():
                                   ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                   ctypes.c_float, ctypes.c_uint, _dp]
     L.imago_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.imago_mlp_forward_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p]
     _lib = L
     return L
 

@@ -455,6 +455,16 @@ int imago_mlp_train(imago_mlp *mlp, int num_samples, const float *x, const float
                     int epochs, int minibatch, float lr, float beta1, float beta2, float eps,
                     unsigned seed, double *epoch_loss);
 
+/* imago_mlp_forward without any allocation: x_dev [batch][w_0] and y_dev
+ * [batch][w_L] are device pointers on the MLP's device; the hidden
+ * activations use a 4 MiB device array of the library (2 x the widest hidden
+ * width x batch floats must fit: batch <= 4096 for 128-wide layers), no
+ * hipMalloc, hipFree or memset.  The work is enqueued on the context's
+ * stream and the call returns without waiting; calls that overlap on one
+ * device share the array and race.  No counterpart in the reference: it is
+ * for callers that cannot allocate (HIP graph capture, serving loops). */
+int imago_mlp_forward_noalloc(imago_mlp *mlp, int batch, const float *x_dev, float *y_dev);
+
 #ifdef __cplusplus
 }
 #endif
