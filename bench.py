@@ -635,6 +635,9 @@ def bench_scf(args, system, coeffs, ctx, log):
             "guess_ms": round(float(np.mean([r.guess_s for r in learned_runs])) * 1e3, 3),
             "iteration_reduction": round(1.0 - last.iterations / plain[-1].iterations, 4),
             "energy_difference": last.total_energy - plain[-1].total_energy},
+        "iteration_wall_ms": {k: round(1e3 * float(np.mean([r["stage_s"][k] for r in
+                                                             last.records[1:]])), 3)
+                              for k in last.records[-1]["stage_s"]},
         "scf": {"iterations": last.iterations, "converged": last.converged, "guess": guess_info,
                 "total_energy": last.total_energy,
                 "setup_ms": round(last.setup_s * 1e3, 3),

@@ -90,13 +90,19 @@ def run(ctx, system, coeffs0, max_iter=50, energy_tol=1.0e-6, mesh=None, num_sta
     for it in range(max_iter):
         t0 = time.perf_counter()
         ev = ctx.secular(coeffs, ns)
+        t1 = time.perf_counter()
         occ, (e_occ, gap, gap_kind) = lib.populate(ev, system.kpoint_weights, electrons)
+        t2 = time.perf_counter()
         tr, prho = ctx.density_traces(occ)
+        t3 = time.perf_counter()
         if res.first_inputs is None:
             res.first_inputs = (tr.copy(), prho.copy(), coeffs.copy())
         coeffs, rec, rho = ctx.scf_update(tr, prho, coeffs)
+        t4 = time.perf_counter()
         rec.update(occupied_energy=e_occ, band_gap=gap, gap_kind=gap_kind,
-                   charge_trace=float(tr[0]), wall_s=time.perf_counter() - t0)
+                   charge_trace=float(tr[0]), wall_s=t4 - t0,
+                   stage_s=dict(secular=t1 - t0, populate=t2 - t1, traces=t3 - t2,
+                                make_scf_pot=t4 - t3))
         de = None if last_e is None else rec["total_energy"] - last_e
         rec["energy_change"] = de
         last_e = rec["total_energy"]

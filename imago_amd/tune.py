@@ -5,8 +5,8 @@ columns per diagonal block of the blocked Cholesky (8, 16, 24, 32), and the
 eigensolver (this library's cooperative tridiagonal reduction with divide
 and conquer and the compact-WY back-transform, or rocSOLVER's heevd /
 syevd).  The best choice depends on the matrix order, the batch (k-points)
-and the scalar type; round-one measurements put the optimum at 32 columns
-for 4 x 896 complex and at 16 for 1 x 1944 real.  ``Autotuner`` times each
+and the scalar type; measured optima are 32 columns for 4 x 896 complex
+and 16 for 1 x 1944 real.  ``Autotuner`` times each
 candidate on the system the context holds, two imago_secular calls per
 candidate (the first also factors the overlap), scores a candidate by the
 cost of an SCF of ``iterations`` iterations, keeps the fastest per
