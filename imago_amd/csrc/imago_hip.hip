@@ -31,6 +31,7 @@
 #include "assemble_launch.hpp"
 #include "geom_kernel.hpp"
 #include "hetrd_launch.hpp"
+#include "tridiag_dc_launch.hpp"
 #include "elecstat_launch.hpp"
 #include "xcmesh_launch.hpp"
 #include "populate.hpp"
@@ -195,7 +196,7 @@ struct imago_ctx {
     double *ival = nullptr;       // [items] trace value of each item
     double *foldX = nullptr;      // [images][64 * ne] density folded per image
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
-    double timing[17] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double timing[18] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // second stream and handle: the two halves of the batched tridiagonal
     // divide and conquer run side by side (created on first use)
@@ -245,6 +246,12 @@ struct imago_ctx {
     int tune_potrf_block = kPotrfBlock;
     int tune_heevd = 0;
     int tune_potrf = 0;
+    // the tridiagonal stage of this library's eigensolver (0: its batched
+    // divide and conquer, 1: rocSOLVER's dstedc; IMAGO_STEDC = rocsolver)
+    // and the divide and conquer's merge plan for the last order, with its
+    // device copy
+    int tune_tridiag = 0;
+    DcPlan dcplan;
     // makeSCFPot state (imago_scf_setup / imago_scf_update): the
     // electrostatic operators on the device, the per-iteration vectors on
     // the host, the mixing history the reference keeps in module variables
@@ -873,6 +880,8 @@ extern "C" int imago_ctx_create(int device, imago_ctx **out)
         const char *e1 = getenv("IMAGO_HEEVD"), *e2 = getenv("IMAGO_POTRF");
         c->tune_heevd = (e1 && strcmp(e1, "rocsolver") == 0) ? 1 : 0;
         c->tune_potrf = (e2 && strcmp(e2, "rocsolver") == 0) ? 1 : 0;
+        const char *e3 = getenv("IMAGO_STEDC");
+        c->tune_tridiag = (e3 && strcmp(e3, "rocsolver") == 0) ? 1 : 0;
     }
     *out = c;
     return IMAGO_OK;
@@ -895,10 +904,52 @@ extern "C" int imago_tune(imago_ctx *c, int param, int value)
         if (value != 0 && value != 1) return fail(IMAGO_E_ARG, "Cholesky %d", value);
         c->tune_potrf = value;
         break;
+    case IMAGO_TUNE_TRIDIAG:
+        if (value != 0 && value != 1) return fail(IMAGO_E_ARG, "tridiagonal solver %d", value);
+        c->tune_tridiag = value;
+        break;
     default:
         return fail(IMAGO_E_ARG, "unknown tuning parameter %d", param);
     }
     c->chol_valid = false;   // the next secular call factors the overlap again
+    return IMAGO_OK;
+}
+
+static int dc_ensure_plan(imago_ctx *c, int n);
+
+extern "C" int imago_tridiag_eig(imago_ctx *c, int n, int nb, const double *d, const double *e,
+                                 double *w, double *Z)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (n < 1 || nb < 1 || !d || !w || !Z || (n > 1 && !e))
+        return fail(IMAGO_E_ARG, "bad tridiagonal arguments");
+    if (n > kDcMaxOrder) return fail(IMAGO_E_ARG, "order %d above %d", n, kDcMaxOrder);
+    HIPCHK(hipSetDevice(c->device));
+    RCHK(dc_ensure_plan(c, n));
+    const size_t N = (size_t)n * nb;
+    std::vector<double> eh(N, 0.0);
+    for (int b = 0; b < nb; ++b)
+        for (int i = 0; i + 1 < n; ++i) eh[(size_t)b * n + i] = e[(size_t)b * (n - 1) + i];
+    double *dd = nullptr, *de = nullptr, *dZ = nullptr;
+    char *wk = nullptr;
+    RCHK(dev_alloc(c, &dd, N));
+    RCHK(dev_alloc(c, &de, N));
+    RCHK(dev_alloc(c, &dZ, N * n));
+    RCHK(dev_alloc(c, &wk, dc_work_bytes(n, nb)));
+    HIPCHK(hipMemcpyAsync(dd, d, N * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(de, eh.data(), N * 8, hipMemcpyHostToDevice, c->stream));
+    DcWork dw;
+    dc_work_carve(wk, n, nb, dw);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(launch_tridiag_dc(c->dcplan, nb, dd, de, dw, dd, dZ, true, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(w, dd, N * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(Z, dZ, N * n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timing[17] = ms;
+    for (void *p : {(void *)dd, (void *)de, (void *)dZ, (void *)wk}) dev_release(c, p);
     return IMAGO_OK;
 }
 
@@ -925,6 +976,7 @@ extern "C" int imago_ctx_destroy(imago_ctx *c)
     if (c->evj) (void)hipEventDestroy(c->evj);
     if (c->evh0) (void)hipEventDestroy(c->evh0);
     if (c->evh1) (void)hipEventDestroy(c->evh1);
+    if (c->dcplan.dev) (void)hipFree(c->dcplan.dev);
     if (c->stream2) {
         (void)hipStreamSynchronize(c->stream2);
         (void)hipStreamDestroy(c->stream2);
@@ -1503,6 +1555,26 @@ constexpr int kCoopRefused = 1000;
 // second grid that needs to be co-resident with the first.
 static std::mutex g_coop_mu;
 
+// the divide and conquer's merge plan for order n, on the device
+static int dc_ensure_plan(imago_ctx *c, int n)
+{
+    if (c->dcplan.n == n && c->dcplan.dev) return IMAGO_OK;
+    if (c->dcplan.dev) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->dcplan.dev));
+        c->dcplan.dev = nullptr;
+        c->dcplan.n = 0;
+    }
+    DcPlan p;
+    dc_make_plan(n, p);
+    const size_t bytes = std::max<size_t>(p.host.size(), 1) * sizeof(int);
+    HIPCHK(hipMalloc(&p.dev, bytes));
+    if (!p.host.empty())
+        HIPCHK(hipMemcpy(p.dev, p.host.data(), p.host.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->dcplan = std::move(p);
+    return IMAGO_OK;
+}
+
 static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *evals,
                       double *ework, int *dinfo)
 {
@@ -1519,11 +1591,18 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     const int N0 = h0 * n, N1 = h1 * n;
     // V carries K*n zero elements past its last matrix: the GEMM forming the
     // compact-WY Gram blocks reads whole K-column blocks
+    const bool own_dc = c->tune_tridiag == 0;
+    char *dcw = nullptr;
     RCHK(dev_alloc(c, &V, (nn * nb + (size_t)K * n) * es));
-    RCHK(dev_alloc(c, &Z, (size_t)N0 * N0 + (size_t)N1 * N1));
-    RCHK(dev_alloc(c, &wall, (size_t)N));
-    RCHK(dev_alloc(c, &owner, (size_t)N));
-    RCHK(dev_alloc(c, &rank, (size_t)N));
+    if (own_dc) {
+        RCHK(dc_ensure_plan(c, n));
+        RCHK(dev_alloc(c, &dcw, dc_work_bytes(n, nb)));
+    } else {
+        RCHK(dev_alloc(c, &Z, (size_t)N0 * N0 + (size_t)N1 * N1));
+        RCHK(dev_alloc(c, &wall, (size_t)N));
+        RCHK(dev_alloc(c, &owner, (size_t)N));
+        RCHK(dev_alloc(c, &rank, (size_t)N));
+    }
     RCHK(dev_alloc(c, &tau, (size_t)n * nb * es));
     RCHK(dev_alloc(c, &vbuf, nn * nb * es));
     RCHK(dev_alloc(c, &pbuf, nn * nb * es));
@@ -1553,12 +1632,20 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         (void)hipGetLastError();
         for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank,
                         (void *)tau, (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf,
-                        (void *)W, (void *)W2, (void *)bar, (void *)err, (void *)sinfo})
+                        (void *)W, (void *)W2, (void *)bar, (void *)err, (void *)sinfo,
+                        (void *)dcw})
             dev_release(c, p);
         return kCoopRefused;
     }
     HIPCHK(le);
     HIPCHK(hipEventRecord(c->evh1, c->stream));
+    if (own_dc) {
+        // the batched divide and conquer: eigenvalues ascending per matrix
+        // into evals, eigenvectors into A
+        DcWork dw;
+        dc_work_carve(dcw, n, nb, dw);
+        HIPCHK(launch_tridiag_dc(c->dcplan, nb, evals, ework, dw, evals, A, !cx, c->stream));
+    } else {
     // d and e of the batch are back to back, e[n-1] = 0 between problems:
     // each half of the batch is one block-diagonal problem, the two halves
     // on two streams (one call on the whole batch merges across the blocks
@@ -1586,6 +1673,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     if (h1 > 0)
         HIPCHK(launch_trd_unmix(Z1, wall + N0, n, h1, h0, owner + N0, rank + N0, evals, A, !cx,
                                 err, c->stream));
+    }
     const rocblas_stride snn = (rocblas_stride)nn, sw = (rocblas_stride)K * n,
                          st = (rocblas_stride)std::max(nblk, 1) * K * K;
     // G_b = V_b^H V_b for every block of each matrix (one strided-batched
@@ -1620,7 +1708,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     c->timing[10] = tms;
     for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank, (void *)tau,
                     (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf, (void *)W, (void *)W2,
-                    (void *)bar, (void *)err, (void *)sinfo})
+                    (void *)bar, (void *)err, (void *)sinfo, (void *)dcw})
         dev_release(c, p);
     if (herr & 1) return fail(IMAGO_E_HIP, "tridiagonal reduction: a barrier wait timed out");
     if (herr & 2)
@@ -3416,7 +3504,7 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
 
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 16) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 17) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

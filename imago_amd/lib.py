@@ -44,6 +44,7 @@ EXPORTS = [
     "imago_scf_update", "imago_site_descriptors", "imago_mlp_create", "imago_mlp_destroy",
     "imago_mlp_num_params", "imago_mlp_params", "imago_mlp_forward", "imago_mlp_gradient",
     "imago_mlp_adam_step", "imago_mlp_train", "imago_mlp_forward_noalloc", "imago_tune",
+    "imago_tridiag_eig",
 ]
 
 
@@ -135,6 +136,8 @@ def load():
                                   ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                   ctypes.c_float, ctypes.c_uint, _dp]
     L.imago_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.imago_tridiag_eig.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                    _dp]
     L.imago_mlp_forward_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_void_p]
     _lib = L
@@ -387,12 +390,26 @@ class Context:
                                        ctypes.byref(rec)))
         return c, rec.as_dict(), rho
 
-    TUNE = {"potrf_block": 1, "eigensolver": 2, "cholesky": 3}
+    TUNE = {"potrf_block": 1, "eigensolver": 2, "cholesky": 3, "tridiag": 4}
 
     def tune(self, **kw):
-        """imago_tune: potrf_block=1..32, eigensolver=0|1, cholesky=0|1."""
+        """imago_tune: potrf_block=1..32, eigensolver=0|1, cholesky=0|1,
+        tridiag=0|1 (the divide and conquer: this library's or rocSOLVER's)."""
         for k, v in kw.items():
             _check(load().imago_tune(self._h, self.TUNE[k], int(v)))
+
+    def tridiag_eig(self, d, e):
+        """imago_tridiag_eig: eigenvalues (ascending) and eigenvectors
+        (columns) of symmetric tridiagonals; d (nb, n) or (n,), e (nb, n-1)
+        or (n-1,).  Returns (w, Z) of shapes (nb, n), (nb, n, n)."""
+        d = np.ascontiguousarray(np.atleast_2d(d), dtype=np.float64)
+        nb, n = d.shape
+        e = np.ascontiguousarray(np.asarray(e, dtype=np.float64).reshape(nb, max(n - 1, 0)))
+        w = np.empty((nb, n))
+        Z = np.empty((nb, n, n))
+        ep = e if e.size else np.zeros(1)
+        _check(load().imago_tridiag_eig(self._h, n, nb, _p(d), _p(ep), _p(w), _p(Z)))
+        return w, np.ascontiguousarray(np.swapaxes(Z, 1, 2))
 
     def xc_counts(self):
         """(all ray points, flops of the last overlap's GEMMs)."""

@@ -209,13 +209,29 @@ int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
  * IMAGO_TUNE_EIGENSOLVER, 0 = this library's tridiagonal reduction and
  * back-transform (default where the order allows it), 1 = rocSOLVER's
  * heevd / syevd; IMAGO_TUNE_CHOLESKY, 0 = this library's blocked Cholesky,
- * 1 = rocSOLVER's potrf.  Results do not depend on the choice beyond
+ * 1 = rocSOLVER's potrf; IMAGO_TUNE_TRIDIAG, the tridiagonal stage of this
+ * library's eigensolver, 0 = its batched divide and conquer, 1 = rocSOLVER's
+ * dstedc.  Results do not depend on the choice beyond
  * rounding.  Setting any of them makes the next imago_secular factor the
  * overlap again. */
 #define IMAGO_TUNE_POTRF_BLOCK 1
 #define IMAGO_TUNE_EIGENSOLVER 2
 #define IMAGO_TUNE_CHOLESKY 3
+#define IMAGO_TUNE_TRIDIAG 4
 int imago_tune(imago_ctx *ctx, int param, int value);
+
+/* The tridiagonal stage of the eigensolver on its own: eigenvalues
+ * (ascending, w[b * n + i]) and eigenvectors (column-major n x n per
+ * problem, Z[b * n * n + ...]) of nb real symmetric tridiagonals of order
+ * n <= 2048, diagonals d[b * n + i], off-diagonals e[b * (n - 1) + i]; host
+ * arrays.  No counterpart in the reference's interface: it is the step
+ * inside the LAPACK zhegv / dsygv that solveZHEGV / solveDSYGV call
+ * (interfaces.F90:105, 185), this library's batched divide and conquer
+ * (the default of imago_secular's eigensolve; IMAGO_TUNE_TRIDIAG = 1 or
+ * IMAGO_STEDC=rocsolver selects rocSOLVER's dstedc there instead).
+ * imago_last_timing stage 17 gives its device time. */
+int imago_tridiag_eig(imago_ctx *ctx, int n, int nb, const double *d,
+                      const double *e, double *w, double *Z);
 
 /* ---- timing ----------------------------------------------------------- *
  * Milliseconds of the last launch of the named stage measured with HIP
@@ -233,7 +249,8 @@ int imago_tune(imago_ctx *ctx, int param, int value);
  * reciprocal-space sum kernel of the last imago_residual_q, 14 = the
  * device work of the last imago_xc_overlap, 15 = its GEMMs, 16 = the
  * device work of the last imago_xc_potential (without the first call's
- * operator evaluation). */
+ * operator evaluation), 17 = the device time of the last
+ * imago_tridiag_eig. */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
