@@ -233,6 +233,8 @@ struct imago_ctx {
         double *A = nullptr;      // the matrix as computed (upper triangle used)
         double *As = nullptr;     // equilibrated copy
         double *U = nullptr;      // its Cholesky factor (upper)
+        double *Af = nullptr;     // the equilibrated matrix, both triangles
+        double *Ainv = nullptr;   // its inverse from the factor, both triangles
         double *s = nullptr;      // scale factors (1 when not equilibrated)
         bool factored = false;
         int info = 0;
@@ -3056,6 +3058,16 @@ __global__ __launch_bounds__(256) void k_spd_scale(const double *__restrict__ A,
     As[g] = i <= j ? s[i] * A[g] * s[j] : 0.0;
 }
 
+// F = the symmetric matrix whose upper triangle U holds (both triangles)
+__global__ __launch_bounds__(256) void k_sym_full(const double *__restrict__ U, int n,
+                                                  double *__restrict__ F)
+{
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (size_t)n * n) return;
+    const int i = (int)(g % n), j = (int)(g / n);
+    F[g] = i <= j ? U[g] : U[(size_t)i * n + j];
+}
+
 // B(i, r) *= s_i for an n x nrhs block
 __global__ __launch_bounds__(256) void k_spd_rows(double *__restrict__ B,
                                                   const double *__restrict__ s, int n, int nrhs)
@@ -3063,13 +3075,6 @@ __global__ __launch_bounds__(256) void k_spd_rows(double *__restrict__ B,
     const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (g >= (size_t)n * nrhs) return;
     B[g] *= s[g % n];
-}
-
-__global__ __launch_bounds__(256) void k_axpy(double *__restrict__ y,
-                                              const double *__restrict__ x, size_t n)
-{
-    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (g < n) y[g] += x[g];
 }
 
 // The equilibration of dposvx('E') (dpoequ / dlaqsy): s_i = 1/sqrt(a_ii),
@@ -3117,9 +3122,27 @@ static int spd_factor(imago_ctx *c, imago_ctx::SpdMat &m, int n)
     int hinfo = 0;
     HIPCHK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    dev_release(c, dinfo);
     m.info = hinfo;
     m.factored = true;
+    if (hinfo == 0) {
+        // the inverse from the factor (dpotri), kept with the equilibrated
+        // matrix in full storage: every later solve is GEMMs against them
+        const unsigned g = (unsigned)((nn + 255) / 256);
+        double *Ui = nullptr;
+        RCHK(dev_alloc(c, &m.Af, nn));
+        RCHK(dev_alloc(c, &m.Ainv, nn));
+        RCHK(dev_alloc(c, &Ui, nn));
+        HIPCHK(hipMemcpyAsync(Ui, m.U, nn * 8, hipMemcpyDeviceToDevice, c->stream));
+        BLASCHK(rocsolver_dpotri(c->blas, rocblas_fill_upper, n, Ui, n, dinfo));
+        hipLaunchKernelGGL(k_sym_full, dim3(g), dim3(256), 0, c->stream, Ui, n, m.Ainv);
+        hipLaunchKernelGGL(k_sym_full, dim3(g), dim3(256), 0, c->stream, m.As, n, m.Af);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dev_release(c, Ui);
+        if (hinfo != 0) return fail(IMAGO_E_SOLVE, "dpotri of the factored overlap: info %d", hinfo);
+    }
+    dev_release(c, dinfo);
     return IMAGO_OK;
 }
 
@@ -3144,22 +3167,22 @@ extern "C" int imago_pot_solve(imago_ctx *c, int matrix, int nrhs, double *b, in
     RCHK(dev_alloc(c, &x, nb));
     RCHK(dev_alloc(c, &r, nb));
     const unsigned g = (unsigned)((nb + 255) / 256);
-    // x = S (As^-1 (S b)), then two steps of refinement on the scaled system
-    HIPCHK(hipMemcpyAsync(x, b, nb * 8, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, x, m.s, n, nrhs);
-    HIPCHK(hipGetLastError());
+    // x = S (As^-1 (S b)) with the inverse from the factor, then two steps
+    // of refinement on the scaled system: r = S b - As x, x += As^-1 r
     double *bs = nullptr;
     RCHK(dev_alloc(c, &bs, nb));
-    HIPCHK(hipMemcpyAsync(bs, x, nb * 8, hipMemcpyDeviceToDevice, c->stream));
-    BLASCHK(rocsolver_dpotrs(c->blas, rocblas_fill_upper, n, nrhs, m.U, n, x, n));
+    HIPCHK(hipMemcpyAsync(bs, b, nb * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, bs, m.s, n, nrhs);
+    HIPCHK(hipGetLastError());
+    const double one = 1.0, mone = -1.0, zero = 0.0;
+    BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, nrhs, n,
+                          &one, m.Ainv, n, bs, n, &zero, x, n));
     for (int it = 0; it < 2; ++it) {
-        const double one = 1.0, mone = -1.0;
         HIPCHK(hipMemcpyAsync(r, bs, nb * 8, hipMemcpyDeviceToDevice, c->stream));
-        BLASCHK(rocblas_dsymm(c->blas, rocblas_side_left, rocblas_fill_upper, n, nrhs, &mone,
-                              m.As, n, x, n, &one, r, n));
-        BLASCHK(rocsolver_dpotrs(c->blas, rocblas_fill_upper, n, nrhs, m.U, n, r, n));
-        hipLaunchKernelGGL(k_axpy, dim3(g), dim3(256), 0, c->stream, x, r, nb);
-        HIPCHK(hipGetLastError());
+        BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, nrhs,
+                              n, &mone, m.Af, n, x, n, &one, r, n));
+        BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, nrhs,
+                              n, &one, m.Ainv, n, r, n, &one, x, n));
     }
     hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, x, m.s, n, nrhs);
     HIPCHK(hipGetLastError());

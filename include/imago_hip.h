@@ -338,9 +338,10 @@ int imago_xc_potential(imago_ctx *ctx, const double *rho_total, const double *rh
  * charge fit and the electrostatic potential of makeSCFPot,
  * potentialUpdate.F90:236, 786), or IMAGO_SOLVE_XC_OVERLAP, exchCorrOverlap of
  * the last imago_xc_overlap (:1010).  Upper triangle, equilibrated as
- * dposvx('E') does, Cholesky factor made on the first solve and kept (the
- * matrices do not change between SCF iterations), two steps of iterative
- * refinement.  b is potDim x nrhs column-major, overwritten by the
+ * dposvx('E') does; on the first solve the Cholesky factor and from it the
+ * inverse (dpotri) are made and kept (the matrices do not change between SCF
+ * iterations), so a solve is a product with the inverse and two steps of
+ * iterative refinement against the equilibrated matrix.  b is potDim x nrhs column-major, overwritten by the
  * solution; *info as dposvx (i > 0: not positive definite, with
  * IMAGO_E_NOTPD). */
 #define IMAGO_SOLVE_POT_OVERLAP 1
