@@ -254,6 +254,21 @@ struct imago_ctx {
     // device copy
     int tune_tridiag = 0;
     DcPlan dcplan;
+    // eigenvector columns of the last imago_secular still to finish: the
+    // tridiagonal eigenvectors sit in the result matrices, the reflectors of
+    // the reduction (V) and their compact-WY factors (Tf) are kept, and
+    // columns [done, n) wait for the back-transform Y = Q Z and the solve
+    // X = U^-1 Y.  The density needs only the occupied columns; the rest
+    // are finished when they are asked for.
+    struct PendingVectors {
+        bool active = false;
+        bool reflectors = false;   // false: the back-transform is done, only U^-1 left
+        bool cx = true;
+        int n = 0, nb = 0, done = 0;
+        char *V = nullptr, *Tf = nullptr;
+        void *A = nullptr;
+        const void *B = nullptr;
+    } pend;
     // makeSCFPot state (imago_scf_setup / imago_scf_update): the
     // electrostatic operators on the device, the per-iteration vectors on
     // the host, the mixing history the reference keeps in module variables
@@ -324,6 +339,7 @@ static void ctx_free_system(imago_ctx *c)
     c->xc_cached = false;
     for (auto &m : c->spd) m = imago_ctx::SpdMat{};
     c->scf = imago_ctx::ScfState{};
+    c->pend = imago_ctx::PendingVectors{};
     c->has_system = false;
 }
 
@@ -1584,7 +1600,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     const size_t nn = (size_t)n * n, es = cx ? sizeof(double2) : sizeof(double);
     const int N = n * nb, K = kTrdBlock, nblk = (n - 1 + K - 1) / K;
     char *V = nullptr, *tau = nullptr, *vbuf = nullptr, *pbuf = nullptr, *dbuf = nullptr,
-         *Tf = nullptr, *W = nullptr, *W2 = nullptr;
+         *Tf = nullptr;
     double *Z = nullptr, *wall = nullptr;
     int *owner = nullptr, *rank = nullptr;
     unsigned *bar = nullptr;
@@ -1610,8 +1626,6 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     RCHK(dev_alloc(c, &pbuf, nn * nb * es));
     RCHK(dev_alloc(c, &dbuf, (size_t)n * 128 * nb * es));
     RCHK(dev_alloc(c, &Tf, (size_t)nb * std::max(nblk, 1) * K * K * es));
-    RCHK(dev_alloc(c, &W, (size_t)nb * K * n * es));
-    RCHK(dev_alloc(c, &W2, (size_t)nb * K * n * es));
     RCHK(dev_alloc(c, &bar, 1));
     RCHK(dev_alloc(c, &err, 1));
     RCHK(dev_alloc(c, &sinfo, 2));
@@ -1634,8 +1648,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         (void)hipGetLastError();
         for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank,
                         (void *)tau, (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf,
-                        (void *)W, (void *)W2, (void *)bar, (void *)err, (void *)sinfo,
-                        (void *)dcw})
+                        (void *)bar, (void *)err, (void *)sinfo, (void *)dcw})
             dev_release(c, p);
         return kCoopRefused;
     }
@@ -1676,7 +1689,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         HIPCHK(launch_trd_unmix(Z1, wall + N0, n, h1, h0, owner + N0, rank + N0, evals, A, !cx,
                                 err, c->stream));
     }
-    const rocblas_stride snn = (rocblas_stride)nn, sw = (rocblas_stride)K * n,
+    const rocblas_stride sw = (rocblas_stride)K * n,
                          st = (rocblas_stride)std::max(nblk, 1) * K * K;
     // G_b = V_b^H V_b for every block of each matrix (one strided-batched
     // GEMM per matrix over all n rows), then the factors T_b in place of G_b;
@@ -1689,18 +1702,12 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
                         (rocblas_stride)K * K, nblk));
     }
     HIPCHK(launch_trd_larft(tau, n, nb, !cx, Tf, c->stream));
-    char *Ac = static_cast<char *>(A);
-    for (int b = nblk - 1; b >= 0; --b) {
-        const int j0 = b * K, kb = std::min(K, n - 1 - j0), r0 = j0 + 1, mb = n - r0;
-        const char *Vb = V + ((size_t)j0 * n + r0) * es;
-        char *Yb = Ac + (size_t)r0 * es;
-        BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_conjugate_transpose, rocblas_operation_none,
-                        kb, n, mb, 1.0, Vb, n, snn, Yb, n, snn, 0.0, W, K, sw, nb));
-        BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_none, rocblas_operation_none, kb, n, kb,
-                        1.0, Tf + (size_t)b * K * K * es, K, st, W, K, sw, 0.0, W2, K, sw, nb));
-        BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_none, rocblas_operation_none, mb, n, kb,
-                        -1.0, Vb, n, snn, W2, K, sw, 1.0, Yb, n, snn, nb));
-    }
+    // the reflectors and their factors stay for the back-transform
+    c->pend.V = V;
+    c->pend.Tf = Tf;
+    c->pend.reflectors = true;
+    V = nullptr;
+    Tf = nullptr;
     int herr = 0, hs[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hs, sinfo, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1709,8 +1716,8 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     HIPCHK(hipEventElapsedTime(&tms, c->evh0, c->evh1));
     c->timing[10] = tms;
     for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank, (void *)tau,
-                    (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf, (void *)W, (void *)W2,
-                    (void *)bar, (void *)err, (void *)sinfo, (void *)dcw})
+                    (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf, (void *)bar, (void *)err,
+                    (void *)sinfo, (void *)dcw})
         dev_release(c, p);
     if (herr & 1) return fail(IMAGO_E_HIP, "tridiagonal reduction: a barrier wait timed out");
     if (herr & 2)
@@ -1726,9 +1733,82 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     return IMAGO_OK;
 }
 
-static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, const void *B,
-                             double *evals, double *ework, int *dinfo, int *hinfo)
+static void pending_clear(imago_ctx *c)
 {
+    dev_release(c, c->pend.V);
+    dev_release(c, c->pend.Tf);
+    c->pend = imago_ctx::PendingVectors{};
+}
+
+// Eigenvector columns [done, upto) of the pending eigensolve: Y = Q Z by
+// the compact-WY blocks from the last to the first (W = V_b^H Y,
+// W = T_b W, Y -= V_b W as batched GEMMs over these columns only), then
+// X = U^-1 Y.  Columns are independent, so later calls continue where this
+// one stops.
+static int finish_columns(imago_ctx *c, int upto)
+{
+    imago_ctx::PendingVectors &p = c->pend;
+    if (!p.active || upto <= p.done) return IMAGO_OK;
+    const int n = p.n, nb = p.nb, c0 = p.done, N = std::min(upto, n) - c0;
+    const bool cx = p.cx;
+    const size_t es = cx ? sizeof(double2) : sizeof(double), nn = (size_t)n * n;
+    char *Ac = static_cast<char *>(p.A) + (size_t)c0 * n * es;
+    if (p.reflectors) {
+        const int K = kTrdBlock, nblk = (n - 1 + K - 1) / K;
+        char *W = nullptr, *W2 = nullptr;
+        RCHK(dev_alloc(c, &W, (size_t)nb * K * N * es));
+        RCHK(dev_alloc(c, &W2, (size_t)nb * K * N * es));
+        const rocblas_stride snn = (rocblas_stride)nn, sw = (rocblas_stride)K * N,
+                             st = (rocblas_stride)std::max(nblk, 1) * K * K;
+        for (int b = nblk - 1; b >= 0; --b) {
+            const int j0 = b * K, kb = std::min(K, n - 1 - j0), r0 = j0 + 1, mb = n - r0;
+            const char *Vb = p.V + ((size_t)j0 * n + r0) * es;
+            char *Yb = Ac + (size_t)r0 * es;
+            BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_conjugate_transpose,
+                            rocblas_operation_none, kb, N, mb, 1.0, Vb, n, snn, Yb, n, snn, 0.0, W,
+                            K, sw, nb));
+            BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_none, rocblas_operation_none, kb, N, kb,
+                            1.0, p.Tf + (size_t)b * K * K * es, K, st, W, K, sw, 0.0, W2, K, sw,
+                            nb));
+            BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_none, rocblas_operation_none, mb, N, kb,
+                            -1.0, Vb, n, snn, W2, K, sw, 1.0, Yb, n, snn, nb));
+        }
+        dev_release(c, W);
+        dev_release(c, W2);
+    }
+    const rocblas_stride nns = (rocblas_stride)nn;
+    if (cx) {
+        const rocblas_double_complex one(1.0, 0.0);
+        BLASCHK(rocblas_ztrsm_strided_batched(
+            c->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+            rocblas_diagonal_non_unit, n, N, &one, (const rocblas_double_complex *)p.B, n, nns,
+            (rocblas_double_complex *)Ac, n, nns, nb));
+    } else {
+        const double one = 1.0;
+        BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
+                                              rocblas_operation_none, rocblas_diagonal_non_unit, n,
+                                              N, &one, (const double *)p.B, n, nns, (double *)Ac,
+                                              n, nns, nb));
+    }
+    p.done += N;
+    if (p.done >= n) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        pending_clear(c);
+    }
+    return IMAGO_OK;
+}
+
+// C = U^-H A U^-1, its eigenpairs, and the eigenvectors X = U^-1 Y of the
+// generalized problem: all columns when defer is false; with defer the
+// columns are left to finish_columns.
+static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, const void *B,
+                             double *evals, double *ework, int *dinfo, int *hinfo,
+                             bool defer = false)
+{
+    if (c->pend.active || c->pend.V || c->pend.Tf) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        pending_clear(c);
+    }
     c->timing[10] = 0.0;
     const rocblas_stride nn = (rocblas_stride)n * n;
     if (cx) {
@@ -1751,9 +1831,6 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
                                                      rocblas_fill_upper, n, a, n, nn, evals, n,
                                                      ework, n, dinfo, nb));
         }
-        BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
-                                              rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
-                                              &one, b, n, nn, a, n, nn, nb));
     } else {
         const double one = 1.0;
         auto *a = (double *)A;
@@ -1774,10 +1851,18 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
                                                      rocblas_fill_upper, n, a, n, nn, evals, n,
                                                      ework, n, dinfo, nb));
         }
-        BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
-                                              rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
-                                              &one, b, n, nn, a, n, nn, nb));
     }
+    // heevd_coop leaves its reflectors in c->pend; rocSOLVER's heevd has
+    // back-transformed already and leaves only X = U^-1 Y
+    imago_ctx::PendingVectors &p = c->pend;
+    p.active = true;
+    p.cx = cx;
+    p.n = n;
+    p.nb = nb;
+    p.done = 0;
+    p.A = A;
+    p.B = B;
+    if (!defer) RCHK(finish_columns(c, n));
     HIPCHK(hipMemcpyAsync(hinfo, dinfo, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return IMAGO_OK;
@@ -1875,8 +1960,11 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
         RCHK(factor_gv_batched(c, cx, h.n, nb, Sm, c->info, inf.data(), &bad));
         c->chol_valid = !bad;
     }
+    // the eigenvector columns are finished by the consumer (the density
+    // needs the occupied ones only)
     if (!bad)
-        RCHK(solve_gv_factored(c, cx, h.n, nb, Hm, Sm, c->evals, c->ework, c->info, inf.data()));
+        RCHK(solve_gv_factored(c, cx, h.n, nb, Hm, Sm, c->evals, c->ework, c->info, inf.data(),
+                               true));
     HIPCHK(hipEventRecord(e3, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms1 = 0, ms2 = 0;
@@ -1923,6 +2011,8 @@ extern "C" int imago_get_eigenvectors(imago_ctx *c, int kpoint, double *out)
         return fail(IMAGO_E_STATE, "no eigenvectors: call imago_secular first");
     if (kpoint < 0 || kpoint >= c->h.nk || !out) return fail(IMAGO_E_ARG, "bad k-point %d", kpoint);
     HIPCHK(hipSetDevice(c->device));
+    RCHK(finish_columns(c, c->num_states));
+    HIPCHK(hipStreamSynchronize(c->stream));
     const long nn = (long)c->h.n * c->h.n;
     const long cols = (long)c->h.n * c->num_states;
     if (c->h.gamma)
@@ -2073,6 +2163,7 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
         if (occupations[q] > 0.0) nocc = std::max(nocc, (int)(q % ns) + 1);
     }
     HIPCHK(hipSetDevice(c->device));
+    RCHK(finish_columns(c, nocc));
     const long nn = (long)h.n * h.n;
     const long per = panel_doubles(c);
     const bool cx = !h.gamma;

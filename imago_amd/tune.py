@@ -7,9 +7,11 @@ and conquer and the compact-WY back-transform, or rocSOLVER's heevd /
 syevd).  The best choice depends on the matrix order, the batch (k-points)
 and the scalar type; measured optima are 32 columns for 4 x 896 complex
 and 16 for 1 x 1944 real.  ``Autotuner`` times each
-candidate on the system the context holds, two imago_secular calls per
-candidate (the first also factors the overlap), scores a candidate by the
-cost of an SCF of ``iterations`` iterations, keeps the fastest per
+candidate on the system the context holds, two passes of imago_secular,
+populate and imago_density_traces per candidate (the first also factors the
+overlap; the traces finish the occupied eigenvector columns the eigensolve
+leaves pending), scores a candidate by the cost of an SCF of
+``iterations`` iterations, keeps the fastest per
 (order rounded up to 128, k-points, complex / real) in a cache (a JSON file when a path is
 given) and applies it.
 """
@@ -58,14 +60,22 @@ class Autotuner:
         with self._lock:
             got = self.cache.get(k)
         if got is None:
+            from . import lib, scf
             n = system.vale_dim
+            electrons = scf.num_electrons(system)
+
+            def one_pass():
+                ev = ctx.secular(coeffs, n)
+                occ, _ = lib.populate(ev, system.kpoint_weights, electrons)
+                ctx.density_traces(occ)
+
             scores = []
             for cand in self.candidates(system):
                 ctx.tune(**cand)
                 t0 = time.perf_counter()
-                ctx.secular(coeffs, n)
+                one_pass()
                 t1 = time.perf_counter()
-                ctx.secular(coeffs, n)
+                one_pass()
                 t2 = time.perf_counter()
                 first, steady = t1 - t0, t2 - t1
                 scores.append((first + (self.iterations - 1) * steady, cand, first, steady))

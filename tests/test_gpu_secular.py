@@ -369,3 +369,26 @@ def test_real_eigensolver_paths_orders(gpu_ctx, oracle_mod, monkeypatch, mode, n
     assert np.max(np.abs(w - w2)) < 1e-10 * scale, np.max(np.abs(w - w2))
     assert np.allclose(V.T @ S @ V, np.eye(n), atol=1e-10)
     assert np.allclose(H @ V, S @ V * w, atol=1e-8 * scale)
+
+
+@pytest.mark.parametrize("gamma", [False, True])
+def test_pending_columns_finish_on_demand(gpu_ctx, gamma):
+    """imago_secular leaves the eigenvector columns to its consumers: the
+    traces finish the occupied columns, imago_get_eigenvectors the rest.
+    The vectors so assembled in two parts equal those of a secular call
+    whose vectors are taken at once."""
+    s = configs.small_test_system(seed=4, gamma=gamma)
+    print("[gpu] pending columns gamma=%s" % gamma, flush=True)
+    _setup(gpu_ctx, s)
+    n = s.vale_dim
+    coeffs = configs.pot_coefficients(s, seed=5)
+    ev = gpu_ctx.secular(coeffs, n)
+    occ = np.zeros((s.num_kpoints, n))
+    occ[:, :3] = 2.0 / s.num_kpoints        # three occupied columns per k-point
+    gpu_ctx.density_traces(occ)
+    parts = [gpu_ctx.eigenvectors(k, n) for k in range(s.num_kpoints)]
+    ev2 = gpu_ctx.secular(coeffs, n)
+    whole = [gpu_ctx.eigenvectors(k, n) for k in range(s.num_kpoints)]
+    assert np.array_equal(ev, ev2)
+    for a, b in zip(parts, whole):
+        assert rel_err(a, b) < 1e-12
