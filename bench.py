@@ -594,7 +594,10 @@ def bench_scf(args, system, coeffs, ctx, log):
     wall_plain = float(np.mean([r.total_s for r in plain]))
     n, nk = system.vale_dim, system.num_kpoints
     # per-iteration stages of the last iteration (HIP events)
-    st = {"hbuild": ctx.timing(1), "eigensolve": ctx.timing(2), "k_hetrd_coop": ctx.timing(10),
+    st = {"hbuild": ctx.timing(1), "eigensolve": ctx.timing(2),
+          "cholesky_transform": ctx.timing(18), "k_hetrd_coop": ctx.timing(10),
+          "tridiagonal_dc": ctx.timing(19), "wy_factors": ctx.timing(22),
+          "back_transform": ctx.timing(20), "vector_solve": ctx.timing(21),
           "traces": ctx.timing(3), "xc_potential": ctx.timing(16)}
     kernels = {}
     if st["k_hetrd_coop"] > 0.0:

@@ -196,7 +196,7 @@ struct imago_ctx {
     double *ival = nullptr;       // [items] trace value of each item
     double *foldX = nullptr;      // [images][64 * ne] density folded per image
     double *tchunk = nullptr;     // [types][kEpdTraceChunks][max_pot_alphas] partial sums
-    double timing[18] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double timing[23] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     // second stream and handle: the two halves of the batched tridiagonal
     // divide and conquer run side by side (created on first use)
@@ -204,6 +204,11 @@ struct imago_ctx {
     rocblas_handle blas2 = nullptr;
     hipEvent_t evf = nullptr, evj = nullptr;
     hipEvent_t evh0 = nullptr, evh1 = nullptr;   // around the tridiagonal reduction
+    // stages of the eigensolve: [0, 1] the Cholesky transform, [2] the end
+    // of the tridiagonal eigensolve, [3] of the compact-WY factors, [4, 5,
+    // 6] around the back-transform and the solve of finished columns
+    hipEvent_t evs[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool evs_pending = false;   // evs[4..6] recorded, not yet read
     // exchange-correlation mesh of the system (imago_xc_mesh)
     bool xc_ready = false;
     imago_xc_mesh_params xcp{};
@@ -994,6 +999,8 @@ extern "C" int imago_ctx_destroy(imago_ctx *c)
     if (c->evj) (void)hipEventDestroy(c->evj);
     if (c->evh0) (void)hipEventDestroy(c->evh0);
     if (c->evh1) (void)hipEventDestroy(c->evh1);
+    for (hipEvent_t ev : c->evs)
+        if (ev) (void)hipEventDestroy(ev);
     if (c->dcplan.dev) (void)hipFree(c->dcplan.dev);
     if (c->stream2) {
         (void)hipStreamSynchronize(c->stream2);
@@ -1548,18 +1555,19 @@ static int factor_gv_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int 
 // a batch, the three stages of LAPACK zheevd / dsyevd:
 //   C = Q T Q^H, T real tridiagonal: k_hetrd_coop, one persistent
 //     cooperative kernel for the whole batch;
-//   T = Z E Z^T: each half of the batch's tridiagonals as one
-//     block-diagonal problem (e[n-1] = 0 splits it) for divide and conquer
-//     (dstedc), the halves on two streams, the eigenvectors sorted back
-//     into their matrices (launch_trd_unmix);
-//   Y = Q Z: the reflectors applied in compact-WY blocks of kTrdBlock
-//     (factors by k_trd_larft, then W = V^H Y, W = T W, Y -= V W as batched
-//     gemm), from the last block to the first.
-// A holds C on entry and Y on exit; evals the eigenvalues in ascending
-// order; dinfo[0], dinfo[1] dstedc's info of each half.
-// The default up to order 1024 (complex) / 2048 (real): at 4 x 896 complex
-// it takes 17 ms where rocSOLVER's batched zheevd takes 30
-// (IMAGO_HEEVD=rocsolver selects the latter).
+//   T = Z E Z^T: this library's batched divide and conquer
+//     (tridiag_dc.hip), or with IMAGO_TUNE_TRIDIAG = 1 rocSOLVER's dstedc
+//     on each half of the batch's tridiagonals as one block-diagonal
+//     problem (e[n-1] = 0 splits it), the halves on two streams, the
+//     eigenvectors sorted back into their matrices (launch_trd_unmix);
+//   Y = Q Z: left to finish_columns, which applies the reflectors kept in
+//     c->pend in compact-WY blocks of kTrdBlock (factors by k_trd_larft) to
+//     the columns its caller needs.
+// A holds C on entry and Z on exit; evals the eigenvalues in ascending
+// order; dinfo the per-matrix codes of rocSOLVER's dstedc (zero with the
+// divide and conquer of this library).
+// The default up to order 1024 (complex) / 2048 (real); IMAGO_HEEVD =
+// rocsolver selects rocSOLVER's batched heevd / syevd instead.
 static bool use_coop_heevd(const imago_ctx *c, int n, bool cx)
 {
     return hetrd_coop_supported(n, !cx) && c->tune_heevd == 0;
@@ -1572,6 +1580,27 @@ constexpr int kCoopRefused = 1000;
 // threads (the batched stream runs several) wait here rather than launch a
 // second grid that needs to be co-resident with the first.
 static std::mutex g_coop_mu;
+
+static int ensure_stage_events(imago_ctx *c)
+{
+    for (hipEvent_t &ev : c->evs)
+        if (!ev) HIPCHK(hipEventCreate(&ev));
+    return IMAGO_OK;
+}
+
+// times of the last finished columns (timing stages 20, 21), read once the
+// stream has passed them
+static int read_finish_events(imago_ctx *c)
+{
+    if (!c->evs_pending) return IMAGO_OK;
+    float a = 0.0f, b = 0.0f;
+    HIPCHK(hipEventElapsedTime(&a, c->evs[4], c->evs[5]));
+    HIPCHK(hipEventElapsedTime(&b, c->evs[5], c->evs[6]));
+    c->timing[20] = a;
+    c->timing[21] = b;
+    c->evs_pending = false;
+    return IMAGO_OK;
+}
 
 // the divide and conquer's merge plan for order n, on the device
 static int dc_ensure_plan(imago_ctx *c, int n)
@@ -1607,10 +1636,10 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     int *err = nullptr, *sinfo = nullptr;
     const int h0 = (nb + 1) / 2, h1 = nb - h0;
     const int N0 = h0 * n, N1 = h1 * n;
-    // V carries K*n zero elements past its last matrix: the GEMM forming the
-    // compact-WY Gram blocks reads whole K-column blocks
     const bool own_dc = c->tune_tridiag == 0;
     char *dcw = nullptr;
+    // V carries K*n zero elements past its last matrix: the GEMM forming the
+    // compact-WY Gram blocks reads whole K-column blocks
     RCHK(dev_alloc(c, &V, (nn * nb + (size_t)K * n) * es));
     if (own_dc) {
         RCHK(dc_ensure_plan(c, n));
@@ -1622,10 +1651,10 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         RCHK(dev_alloc(c, &rank, (size_t)N));
     }
     RCHK(dev_alloc(c, &tau, (size_t)n * nb * es));
+    RCHK(dev_alloc(c, &Tf, (size_t)nb * std::max(nblk, 1) * K * K * es));
     RCHK(dev_alloc(c, &vbuf, nn * nb * es));
     RCHK(dev_alloc(c, &pbuf, nn * nb * es));
     RCHK(dev_alloc(c, &dbuf, (size_t)n * 128 * nb * es));
-    RCHK(dev_alloc(c, &Tf, (size_t)nb * std::max(nblk, 1) * K * K * es));
     RCHK(dev_alloc(c, &bar, 1));
     RCHK(dev_alloc(c, &err, 1));
     RCHK(dev_alloc(c, &sinfo, 2));
@@ -1639,6 +1668,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         HIPCHK(hipEventCreate(&c->evh0));
         HIPCHK(hipEventCreate(&c->evh1));
     }
+    RCHK(ensure_stage_events(c));
     HIPCHK(hipEventRecord(c->evh0, c->stream));
     const hipError_t le = launch_hetrd_coop(L);
     if (le == hipErrorIllegalState || le == hipErrorCooperativeLaunchTooLarge) {
@@ -1689,6 +1719,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         HIPCHK(launch_trd_unmix(Z1, wall + N0, n, h1, h0, owner + N0, rank + N0, evals, A, !cx,
                                 err, c->stream));
     }
+    HIPCHK(hipEventRecord(c->evs[2], c->stream));
     const rocblas_stride sw = (rocblas_stride)K * n,
                          st = (rocblas_stride)std::max(nblk, 1) * K * K;
     // G_b = V_b^H V_b for every block of each matrix (one strided-batched
@@ -1702,6 +1733,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
                         (rocblas_stride)K * K, nblk));
     }
     HIPCHK(launch_trd_larft(tau, n, nb, !cx, Tf, c->stream));
+    HIPCHK(hipEventRecord(c->evs[3], c->stream));
     // the reflectors and their factors stay for the back-transform
     c->pend.V = V;
     c->pend.Tf = Tf;
@@ -1712,9 +1744,13 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hs, sinfo, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    float tms = 0.0f;
+    float tms = 0.0f, tdc = 0.0f, twy = 0.0f;
     HIPCHK(hipEventElapsedTime(&tms, c->evh0, c->evh1));
+    HIPCHK(hipEventElapsedTime(&tdc, c->evh1, c->evs[2]));
+    HIPCHK(hipEventElapsedTime(&twy, c->evs[2], c->evs[3]));
     c->timing[10] = tms;
+    c->timing[19] = tdc;
+    c->timing[22] = twy;
     for (void *p : {(void *)V, (void *)Z, (void *)wall, (void *)owner, (void *)rank, (void *)tau,
                     (void *)vbuf, (void *)pbuf, (void *)dbuf, (void *)Tf, (void *)bar, (void *)err,
                     (void *)sinfo, (void *)dcw})
@@ -1753,6 +1789,8 @@ static int finish_columns(imago_ctx *c, int upto)
     const bool cx = p.cx;
     const size_t es = cx ? sizeof(double2) : sizeof(double), nn = (size_t)n * n;
     char *Ac = static_cast<char *>(p.A) + (size_t)c0 * n * es;
+    RCHK(ensure_stage_events(c));
+    HIPCHK(hipEventRecord(c->evs[4], c->stream));
     if (p.reflectors) {
         const int K = kTrdBlock, nblk = (n - 1 + K - 1) / K;
         char *W = nullptr, *W2 = nullptr;
@@ -1776,6 +1814,7 @@ static int finish_columns(imago_ctx *c, int upto)
         dev_release(c, W);
         dev_release(c, W2);
     }
+    HIPCHK(hipEventRecord(c->evs[5], c->stream));
     const rocblas_stride nns = (rocblas_stride)nn;
     if (cx) {
         const rocblas_double_complex one(1.0, 0.0);
@@ -1790,6 +1829,8 @@ static int finish_columns(imago_ctx *c, int upto)
                                               N, &one, (const double *)p.B, n, nns, (double *)Ac,
                                               n, nns, nb));
     }
+    HIPCHK(hipEventRecord(c->evs[6], c->stream));
+    c->evs_pending = true;
     p.done += N;
     if (p.done >= n) {
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1809,8 +1850,10 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         HIPCHK(hipStreamSynchronize(c->stream));
         pending_clear(c);
     }
-    c->timing[10] = 0.0;
+    c->timing[10] = c->timing[19] = c->timing[22] = 0.0;
+    RCHK(ensure_stage_events(c));
     const rocblas_stride nn = (rocblas_stride)n * n;
+    HIPCHK(hipEventRecord(c->evs[0], c->stream));
     if (cx) {
         const rocblas_double_complex one(1.0, 0.0);
         auto *a = (rocblas_double_complex *)A;
@@ -1822,6 +1865,7 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));
+        HIPCHK(hipEventRecord(c->evs[1], c->stream));
         const int rc = use_coop_heevd(c, n, true) ? heevd_coop(c, true, n, nb, A, evals, ework, dinfo)
                                                : kCoopRefused;
         if (rc != kCoopRefused) {
@@ -1841,6 +1885,7 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
                                               &one, b, n, nn, a, n, nn, nb));
+        HIPCHK(hipEventRecord(c->evs[1], c->stream));
         const int rc = use_coop_heevd(c, n, false)
                            ? heevd_coop(c, false, n, nb, A, evals, ework, dinfo)
                            : kCoopRefused;
@@ -1865,6 +1910,10 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
     if (!defer) RCHK(finish_columns(c, n));
     HIPCHK(hipMemcpyAsync(hinfo, dinfo, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    float tt = 0.0f;
+    HIPCHK(hipEventElapsedTime(&tt, c->evs[0], c->evs[1]));
+    c->timing[18] = tt;
+    RCHK(read_finish_events(c));
     return IMAGO_OK;
 }
 
@@ -2013,6 +2062,7 @@ extern "C" int imago_get_eigenvectors(imago_ctx *c, int kpoint, double *out)
     HIPCHK(hipSetDevice(c->device));
     RCHK(finish_columns(c, c->num_states));
     HIPCHK(hipStreamSynchronize(c->stream));
+    RCHK(read_finish_events(c));
     const long nn = (long)c->h.n * c->h.n;
     const long cols = (long)c->h.n * c->num_states;
     if (c->h.gamma)
@@ -2260,6 +2310,7 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[3] = ms;
+    RCHK(read_finish_events(c));
     if (h.pot_dim > 0 && direct) {
         float ms9 = 0;
         HIPCHK(hipEventElapsedTime(&ms9, c->ev2, c->ev3));
@@ -3618,7 +3669,7 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
 
 extern "C" int imago_last_timing(imago_ctx *c, int stage, double *ms)
 {
-    if (!c || !ms || stage < 0 || stage > 17) return fail(IMAGO_E_ARG, "bad timing query");
+    if (!c || !ms || stage < 0 || stage > 22) return fail(IMAGO_E_ARG, "bad timing query");
     *ms = c->timing[stage];
     return IMAGO_OK;
 }

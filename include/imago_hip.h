@@ -250,7 +250,12 @@ int imago_tridiag_eig(imago_ctx *ctx, int n, int nb, const double *d,
  * device work of the last imago_xc_overlap, 15 = its GEMMs, 16 = the
  * device work of the last imago_xc_potential (without the first call's
  * operator evaluation), 17 = the device time of the last
- * imago_tridiag_eig. */
+ * imago_tridiag_eig; of the last eigensolve: 18 = the Cholesky transform
+ * U^-H H U^-1, 19 = the tridiagonal eigensolve (this library's divide and
+ * conquer or rocSOLVER's dstedc); of the last eigenvector columns finished
+ * (by the traces or imago_get_eigenvectors): 20 = the back-transform,
+ * 21 = the solve X = U^-1 Y; 22 = the compact-WY factors of the last
+ * eigensolve's reflectors. */
 int imago_last_timing(imago_ctx *ctx, int stage, double *ms);
 
 /* Sizes of the EP screening of the current system (after an IMAGO_EP or
