@@ -13,10 +13,12 @@ template <int LMAX, int NE, bool G>
 static hipError_t hsum_one(const EpdLaunch &L)
 {
     if (L.nsubs == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_epd_hsum<LMAX, NE, G>), dim3((unsigned)L.nsubs), dim3(64),
-                       epd_basis_lds(L), L.stream, L.S, L.pairs, L.phase, L.tuples, L.item_tuple,
-                       L.pitem, L.blocks, L.coef_eff, L.type_term0, L.subs, L.nsubs, L.npairs,
-                       L.out);
+    const int nk = G ? 1 : L.S.nk;
+    for (int k0 = 0; k0 < nk; k0 += 4)
+        hipLaunchKernelGGL((k_epd_hsum<LMAX, NE, G>), dim3((unsigned)L.nsubs), dim3(64),
+                           epd_basis_lds(L), L.stream, L.S, L.pairs, L.phase, L.tuples,
+                           L.item_tuple, L.pitem, L.blocks, L.coef_eff, L.type_term0, L.subs,
+                           L.nsubs, L.npairs, L.out, k0);
     return hipGetLastError();
 }
 

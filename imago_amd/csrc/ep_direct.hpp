@@ -26,7 +26,8 @@
 // A wavefront decodes 64 items at once (item -> tuple -> image, exponents,
 // type) and then streams their blocks a few items at a time, so the block
 // loads of several items are in flight together.  The hsum accumulators of
-// at most 4 k-points (or Gamma) stay in registers.
+// 4 k-points (or Gamma) stay in registers; more k-points take one hsum
+// launch per group of four.
 #pragma once
 #include "ep_kernel.hpp"
 
@@ -175,7 +176,8 @@ __global__ void k_epd_pair_runs(const int *__restrict__ skey, long nitems,
 // H sum
 
 // part[sub.out ...] = [k][nent] (complex, or real for Gamma): the run's
-// sum of c'_j V_j(k) over its items, on its pair's block.
+// sum of c'_j V_j(k) over its items, on its pair's block, for the k-points
+// k0 .. k0 + 3 (one launch per group of four).
 template <int LMAX, int NE, bool GAMMA>
 __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__restrict__ pairs,
                                                  const double2 *__restrict__ phase,
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__rest
                                                  const double *__restrict__ coef_eff,
                                                  const int *__restrict__ type_term0,
                                                  const EpdSub *__restrict__ subs, long nsubs,
-                                                 int npairs, double *__restrict__ part)
+                                                 int npairs, double *__restrict__ part, int k0)
 {
     constexpr int SH = (LMAX + 1) * (LMAX + 1);
     constexpr int BS = SH * SH;
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__rest
     int cur_img = -1;
     auto flush = [&]() {
         if (cur_img < 0) return;
-        const double2 *ph = phase + (size_t)(pr.img_off + cur_img) * nk;
+        const double2 *ph = phase + (size_t)(pr.img_off + cur_img) * nk + k0;
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             if (GAMMA) {
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__rest
             } else {
 #pragma unroll
                 for (int k = 0; k < NQ; ++k)
-                    if (k < nk) {
+                    if (k0 + k < nk) {
                         const double2 f = ph[k];
                         Q[e][k].x += f.x * P[e];
                         Q[e][k].y += f.y * P[e];
@@ -296,8 +298,8 @@ __global__ __launch_bounds__(64) void k_epd_hsum(DevSys S, const PairRec *__rest
         } else {
 #pragma unroll
             for (int k = 0; k < NQ; ++k)
-                if (k < nk)
-                    reinterpret_cast<double2 *>(o)[(size_t)k * nent + ee] =
+                if (k0 + k < nk)
+                    reinterpret_cast<double2 *>(o)[(size_t)(k0 + k) * nent + ee] =
                         zero ? make_double2(0.0, 0.0) : Q[e][k];
         }
     }
@@ -337,7 +339,8 @@ __global__ __launch_bounds__(256) void k_epd_hfinal(const DevSys S, const PairRe
 
 // X[(img_off + image) * ent + e] = sum_k Re(conj(phase_k(image)) rw_k,e) for
 // every image of the pair (one work-group per pair); rw is the weighted
-// density in the block layout, [k][nbe] complex, at most 4 k-points.
+// density in the block layout, [k][nbe] complex.  Four k-points at a time
+// are held in registers, the images summed for each group in turn.
 __global__ __launch_bounds__(256) void k_epd_fold(const DevSys S, const PairRec *__restrict__ pairs,
                                                   const double2 *__restrict__ phase,
                                                   const double2 *__restrict__ rw, int ent,
@@ -347,17 +350,20 @@ __global__ __launch_bounds__(256) void k_epd_fold(const DevSys S, const PairRec 
     const int nent = S.type_nstates[S.atom_type[pr.i]] * S.type_nstates[S.atom_type[pr.j]];
     const int nk = S.nk;
     for (int e = threadIdx.x; e < nent; e += blockDim.x) {
-        double2 R[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            R[k] = k < nk ? rw[(size_t)k * S.nbe + pr.blk + e] : make_double2(0.0, 0.0);
-        for (int im = 0; im < pr.img_cnt; ++im) {
-            const double2 *ph = phase + (size_t)(pr.img_off + im) * nk;
-            double x = 0.0;
+        for (int k0 = 0; k0 < nk; k0 += 4) {
+            double2 R[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (k < nk) x += ph[k].x * R[k].x + ph[k].y * R[k].y;
-            X[(size_t)(pr.img_off + im) * ent + e] = x;
+                R[k] = k0 + k < nk ? rw[(size_t)(k0 + k) * S.nbe + pr.blk + e]
+                                   : make_double2(0.0, 0.0);
+            for (int im = 0; im < pr.img_cnt; ++im) {
+                const double2 *ph = phase + (size_t)(pr.img_off + im) * nk + k0;
+                double x = k0 == 0 ? 0.0 : X[(size_t)(pr.img_off + im) * ent + e];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (k0 + k < nk) x += ph[k].x * R[k].x + ph[k].y * R[k].y;
+                X[(size_t)(pr.img_off + im) * ent + e] = x;
+            }
         }
     }
 }

@@ -32,6 +32,7 @@
 #include "geom_kernel.hpp"
 #include "hetrd_launch.hpp"
 #include "tridiag_dc_launch.hpp"
+#include "dos_launch.hpp"
 #include "elecstat_launch.hpp"
 #include "xcmesh_launch.hpp"
 #include "populate.hpp"
@@ -1268,9 +1269,6 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
     if (kind < IMAGO_OL || kind > IMAGO_EP_DIRECT)
         return fail(IMAGO_E_ARG, "unknown integral kind %d", kind);
     if (kind == IMAGO_EP_DIRECT) {
-        if (!c->h.gamma && c->h.nk > 4)
-            return fail(IMAGO_E_ARG, "IMAGO_EP_DIRECT supports Gamma or at most 4 k-points (%d); "
-                                     "assemble IMAGO_EP instead", c->h.nk);
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         RCHK(ep_prepare(c));
@@ -2070,6 +2068,97 @@ extern "C" int imago_get_eigenvectors(imago_ctx *c, int kpoint, double *out)
     else
         HIPCHK(hipMemcpy(out, c->Hfull + (size_t)kpoint * nn, sizeof(double2) * cols,
                          hipMemcpyDeviceToHost));
+    return IMAGO_OK;
+}
+
+extern "C" int imago_dos(imago_ctx *c, const double *energies, int spin, const int *channel,
+                         int num_channels, double emin, double delta, int num_points,
+                         double sigma, double *pdos, double *electron_number,
+                         double *localization)
+{
+    if (!c || !c->has_system || !c->have_vectors)
+        return fail(IMAGO_E_STATE, "imago_dos needs the eigenpairs of imago_secular");
+    if (!c->assembled[0]) return fail(IMAGO_E_STATE, "imago_dos needs the overlap (IMAGO_OL)");
+    if (!energies || !channel || !pdos || num_channels < 1 || num_points < 1 ||
+        !(sigma > 0.0) || !(delta > 0.0) || (spin != 1 && spin != 2))
+        return fail(IMAGO_E_ARG, "bad density-of-states arguments");
+    const HostSys &h = c->h;
+    const int n = h.n, ns = c->num_states, nk = h.nk;
+    for (int v = 0; v < n; ++v)
+        if (channel[v] < 0 || channel[v] >= num_channels)
+            return fail(IMAGO_E_ARG, "basis function %d has channel %d of %d", v, channel[v],
+                        num_channels);
+    HIPCHK(hipSetDevice(c->device));
+    RCHK(finish_columns(c, ns));
+    const bool cx = !h.gamma;
+    const size_t nn = (size_t)n * n, dz = cx ? 2 : 1, rows = (size_t)nk * ns;
+    double *Sf = nullptr, *SC = nullptr, *Q = nullptr, *M = nullptr, *P = nullptr, *G = nullptr,
+           *Ed = nullptr, *wd = nullptr, *out = nullptr, *en = nullptr, *loc = nullptr;
+    RCHK(dev_alloc(c, &Sf, nn * nk * dz));
+    RCHK(dev_alloc(c, &SC, (size_t)n * ns * nk * dz));
+    RCHK(dev_alloc(c, &Q, (size_t)n * ns * nk));
+    RCHK(dev_alloc(c, &M, (size_t)n * num_channels));
+    RCHK(dev_alloc(c, &P, rows * num_channels));
+    RCHK(dev_alloc(c, &G, rows * num_points));
+    RCHK(dev_alloc(c, &Ed, rows));
+    RCHK(dev_alloc(c, &wd, (size_t)nk));
+    RCHK(dev_alloc(c, &out, (size_t)num_channels * num_points));
+    RCHK(dev_alloc(c, &en, (size_t)n));
+    RCHK(dev_alloc(c, &loc, (size_t)ns));
+    std::vector<double> Mh((size_t)n * num_channels, 0.0);
+    for (int v = 0; v < n; ++v) Mh[(size_t)channel[v] * n + v] = 1.0 / spin;
+    HIPCHK(hipMemcpyAsync(M, Mh.data(), Mh.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(Ed, energies, rows * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(wd, h.kw.data(), (size_t)nk * 8, hipMemcpyHostToDevice, c->stream));
+    // the overlap S(k) in full (both triangles), then S C for the states
+    const unsigned ug = (unsigned)((nn * nk + 255) / 256);
+    if (cx)
+        hipLaunchKernelGGL(k_unpack<true>, dim3(ug), dim3(256), 0, c->stream, c->panel[0], c->nbe,
+                           n, nk, c->pairs, c->atom_of_state, c->pair_index, c->d.cumul, h.natoms,
+                           Sf);
+    else
+        hipLaunchKernelGGL(k_unpack<false>, dim3(ug), dim3(256), 0, c->stream, c->panel[0],
+                           c->nbe, n, 1, c->pairs, c->atom_of_state, c->pair_index, c->d.cumul,
+                           h.natoms, Sf);
+    HIPCHK(hipGetLastError());
+    const rocblas_stride snn = (rocblas_stride)nn, sns = (rocblas_stride)n * ns;
+    if (cx) {
+        const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+        BLASCHK(rocblas_zgemm_strided_batched(
+            c->blas, rocblas_operation_none, rocblas_operation_none, n, ns, n, &one,
+            (const rocblas_double_complex *)Sf, n, snn, (const rocblas_double_complex *)c->Hfull,
+            n, snn, &zero, (rocblas_double_complex *)SC, n, sns, nk));
+    } else {
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, ns, n,
+                              &one, Sf, n, c->Hreal, n, &zero, SC, n));
+    }
+    HIPCHK(launch_dos_mulliken(cx ? (const double *)c->Hfull : c->Hreal, (long)nn, SC, n, ns, nk,
+                               cx, Q, c->stream));
+    {
+        // channel sums P[(k, j)][ch] = sum_v Q_kjv M_v,ch, stacked over k
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(rocblas_dgemm_strided_batched(
+            c->blas, rocblas_operation_transpose, rocblas_operation_none, ns, num_channels, n,
+            &one, Q, n, sns, M, n, 0, &zero, P, (rocblas_int)rows, ns, nk));
+        HIPCHK(launch_dos_gauss(Ed, wd, ns, nk, emin, delta, num_points, sigma, G, c->stream));
+        // PDOS = P^T G
+        BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none,
+                              num_channels, num_points, (rocblas_int)rows, &one, P,
+                              (rocblas_int)rows, G, (rocblas_int)rows, &zero, out, num_channels));
+    }
+    HIPCHK(launch_dos_counts(Q, Ed, wd, n, ns, nk, spin, en, loc, c->stream));
+    HIPCHK(hipMemcpyAsync(pdos, out, (size_t)num_channels * num_points * 8,
+                          hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> enh(n), loch(ns);
+    HIPCHK(hipMemcpyAsync(enh.data(), en, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(loch.data(), loc, (size_t)ns * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (electron_number) std::copy(enh.begin(), enh.end(), electron_number);
+    if (localization) std::copy(loch.begin(), loch.end(), localization);
+    for (void *p : {(void *)Sf, (void *)SC, (void *)Q, (void *)M, (void *)P, (void *)G,
+                    (void *)Ed, (void *)wd, (void *)out, (void *)en, (void *)loc})
+        dev_release(c, p);
     return IMAGO_OK;
 }
 

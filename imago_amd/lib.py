@@ -44,7 +44,7 @@ EXPORTS = [
     "imago_scf_update", "imago_site_descriptors", "imago_mlp_create", "imago_mlp_destroy",
     "imago_mlp_num_params", "imago_mlp_params", "imago_mlp_forward", "imago_mlp_gradient",
     "imago_mlp_adam_step", "imago_mlp_train", "imago_mlp_forward_noalloc", "imago_tune",
-    "imago_tridiag_eig",
+    "imago_tridiag_eig", "imago_dos",
 ]
 
 
@@ -138,6 +138,9 @@ def load():
     L.imago_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.imago_tridiag_eig.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                     _dp]
+    L.imago_dos.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int,
+                            ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                            _dp, _dp, _dp]
     L.imago_mlp_forward_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_void_p]
     _lib = L
@@ -410,6 +413,20 @@ class Context:
         ep = e if e.size else np.zeros(1)
         _check(load().imago_tridiag_eig(self._h, n, nb, _p(d), _p(ep), _p(w), _p(Z)))
         return w, np.ascontiguousarray(np.swapaxes(Z, 1, 2))
+
+    def dos(self, energies, spin, channel, num_channels, emin, delta, num_points, sigma):
+        """imago_dos on the eigenpairs of the last secular call: returns
+        (pdos (num_channels, num_points), electron number per basis
+        function, localisation index per state)."""
+        e = np.ascontiguousarray(energies, dtype=np.float64)
+        ch = np.ascontiguousarray(channel, dtype=np.int32)
+        pd = np.zeros((num_points, num_channels))
+        en = np.zeros(self.vale_dim)
+        loc = np.zeros(e.shape[-1])
+        _check(load().imago_dos(self._h, _p(e), int(spin), ch.ctypes.data_as(_ip),
+                                int(num_channels), float(emin), float(delta), int(num_points),
+                                float(sigma), _p(pd), _p(en), _p(loc)))
+        return pd.T.copy(), en, loc
 
     def xc_counts(self):
         """(all ray points, flops of the last overlap's GEMMs)."""

@@ -55,10 +55,20 @@ class ScfResult:
         return self.records[-1]["total_energy"] if self.records else float("nan")
 
 
+PANEL_BUDGET_BYTES = 16 << 30
+
+
 def ep_kind_for(system) -> str:
-    """The EP path the system can use: the direct path (no potDim panels)
-    for Gamma or at most 4 k-points, the stored panels otherwise."""
-    return "ep_direct" if (system.gamma or system.num_kpoints <= 4) else "ep"
+    """The EP path for a system: the direct path (no potDim panels, the H
+    sum and the traces from the stored primitive blocks, four k-points per
+    pass) for Gamma, up to 4 k-points, or when the potDim panels would take
+    more than 16 GiB of HBM; the stored panels (one pass over them per
+    iteration whatever nk) otherwise."""
+    n = system.vale_dim
+    panel = system.pot_dim * system.num_kpoints * n * (n + 1) // 2 * 16
+    if system.gamma or system.num_kpoints <= 4 or panel > PANEL_BUDGET_BYTES:
+        return "ep_direct"
+    return "ep"
 
 
 def setup(ctx, system, mesh=None, ep_kind=None, assemble=True, **params):

@@ -159,8 +159,8 @@ int imago_boys_batch(imago_ctx *ctx, int n, const double *T, double *F7);
  * their primitive blocks once, and imago_secular / imago_density_traces
  * then form sum_j c_j V_j and Tr(rho V_j) from those blocks each call (the
  * fused accumulation of the reference's post-SCF path, integrals.F90:
- * 2860-3605); it needs gamma or at most 4 k-points.  When both are
- * assembled the stored panels are used.
+ * 2860-3605), any number of k-points.  When both are assembled the
+ * stored panels are used.
  * coreDim must be 0, so orthoOL/ortho reduce to packing (intgOrtho.F90). */
 int imago_assemble(imago_ctx *ctx, int kind);
 
@@ -202,6 +202,22 @@ int imago_zhegv(imago_ctx *ctx, int n, double *A, double *B, double *w,
 /* solveDSYGV (interfaces.F90:152-200), real symmetric. */
 int imago_dsygv(imago_ctx *ctx, int n, double *A, double *B, double *w,
                 int *info);
+
+/* ---- density of states (post-SCF) ------------------------------------ *
+ * computeDOS's Gaussian path (dos.F90:160-1231, kPointIntgCode /= 1) on the
+ * eigenpairs of the last imago_secular and the assembled overlap:
+ * energies[k * num_states + j] are the eigenvalues already shifted by the
+ * highest occupied energy (shiftEnergyEigenValues, Hartree); channel[v] the
+ * PDOS channel (0-based) of basis function v (the reference's pdosIndex for
+ * its detailCodePDOS); the energy grid is emin + e * delta, e < num_points
+ * (Hartree); sigma the broadening (Hartree).  Writes pdos[ch + num_channels
+ * * e] (states / eV / cell, before the reference's electron-count factor),
+ * and when not null electron_number[v] (sum over k and states at or below
+ * zero of the Mulliken population times w_k / spin) and localization[j]
+ * (sum over k and v of the squared population times w_k / spin). */
+int imago_dos(imago_ctx *ctx, const double *energies, int spin, const int *channel,
+              int num_channels, double emin, double delta, int num_points, double sigma,
+              double *pdos, double *electron_number, double *localization);
 
 /* ---- run-time choices of the batched eigensolve ------------------------ *
  * For the autotuner (imago_amd/tune.py): IMAGO_TUNE_POTRF_BLOCK, columns per

@@ -236,16 +236,20 @@ def _secular_traces_oracle(oracle_mod, s, coeffs, occ_fn, ns):
     return np.array(evs), (S, N, K, V)
 
 
-@pytest.mark.parametrize("name", ["small_complex", "small_gamma", "si8_cubic_d"])
+@pytest.mark.parametrize("name", ["small_complex", "small_gamma", "si8_cubic_d",
+                                  "si2_primitive"])
 def test_direct_ep_path_matches_oracle(gpu_ctx, oracle_mod, name):
     """IMAGO_EP_DIRECT: H(k) and Tr(rho V_j) from the stored primitive
-    blocks instead of the potDim panels, against the oracle's panels."""
+    blocks instead of the potDim panels, against the oracle's panels; the
+    Si2 primitive cell has 10 k-points (the H sum and the density fold take
+    them four at a time)."""
     s = {"small_complex": lambda: configs.small_test_system(seed=9),
          "small_gamma": lambda: configs.small_test_system(seed=9, gamma=True),
          # cutoff 16 bohr: at 12 the screened overlap of this cell is not
          # positive definite
          "si8_cubic_d": lambda: configs.si64_supercell(reps=(1, 1, 1), mesh=(2, 2, 2),
-                                                       cutoff_bohr=16.0)}[name]()
+                                                       cutoff_bohr=16.0),
+         "si2_primitive": configs.si2_primitive}[name]()
     print("[gpu] direct EP", name, flush=True)
     gpu_ctx.set_system(s)
     for kind in ("ol", "ke", "np", "ep_direct"):
@@ -256,7 +260,7 @@ def test_direct_ep_path_matches_oracle(gpu_ctx, oracle_mod, name):
     ev = gpu_ctx.secular(coeffs, n)
     ref, (S, N, K, V) = _secular_traces_oracle(oracle_mod, s, coeffs, None, n)
     assert np.max(np.abs(ev - ref)) < 1e-8, np.max(np.abs(ev - ref))
-    nel = 6 if name != "si8_cubic_d" else 14 * s.num_atoms
+    nel = 6 if name.startswith("small") else 14 * s.num_atoms
     occ = oracle_mod.populate(ev, s.kpoint_weights, nel)
     tr, prho = gpu_ctx.density_traces(occ)
     t_ref = np.zeros(3)
@@ -276,14 +280,6 @@ def test_direct_ep_path_matches_oracle(gpu_ctx, oracle_mod, name):
     scale = np.max(np.abs(p_ref))
     assert np.max(np.abs(prho - p_ref)) <= 1e-10 * scale, np.max(np.abs(prho - p_ref)) / scale
     assert tr[0] == pytest.approx(nel, rel=1e-9)
-
-
-def test_direct_ep_path_refuses_many_kpoints(gpu_ctx):
-    s = configs.si2_primitive()   # 10 k-points
-    gpu_ctx.set_system(s)
-    with pytest.raises(ImagoError) as e:
-        gpu_ctx.assemble("ep_direct")
-    assert e.value.name == "IMAGO_E_ARG" and "4 k-points" in str(e.value)
 
 
 @pytest.mark.parametrize("n", [2, 3, 33, 64, 97, 300])
