@@ -207,12 +207,15 @@ def write_tdos(path, res: DosResult):
             f.write("%14.4f%14.6f\n" % (e, t))
 
 
-def write_localization(path, res: DosResult):
-    """fort.80: per state the average energy (eV) and localisation index."""
-    ev = res.eigenvalues.mean(axis=0) * HARTREE_EV
+def write_localization(path, res: DosResult, kpoint_weights):
+    """fort.80 (dos.F90:1165-1188): per state the k-weighted energy and the
+    localisation index, both halved (the k-point weights sum to two), the
+    energy in eV."""
+    w = np.asarray(kpoint_weights, dtype=np.float64)
+    avg = (res.eigenvalues * w[:, None]).sum(axis=0) / 2.0 * HARTREE_EV
     with open(path, "w") as f:
-        for e, l in zip(ev, res.localization):
-            f.write("%16.8f%16.8f\n" % (e, l))
+        for e, l in zip(avg, res.localization / 2.0):
+            f.write(" %24.8f  %24.8f\n" % (e, l))
 
 
 @dataclass
