@@ -260,6 +260,20 @@ struct imago_ctx {
     // device copy
     int tune_tridiag = 0;
     DcPlan dcplan;
+    // the divide and conquer's launches captured once in a graph and
+    // replayed (IMAGO_TUNE_GRAPH, default on): its work arrays are kept
+    // for the graph's lifetime, which is keyed by the order, the batch and
+    // the buffers it reads and writes
+    int tune_graph = 1;
+    struct DcGraph {
+        hipGraphExec_t exec = nullptr;
+        int n = 0, nb = 0;
+        bool real = false;
+        const void *d = nullptr, *e = nullptr, *X = nullptr;
+        const int *plan = nullptr;   // the plan's device copy the graph reads
+        char *work = nullptr;
+        size_t work_bytes = 0;
+    } dcg;
     // eigenvector columns of the last imago_secular still to finish: the
     // tridiagonal eigenvectors sit in the result matrices, the reflectors of
     // the reduction (V) and their compact-WY factors (Tf) are kept, and
@@ -906,6 +920,8 @@ extern "C" int imago_ctx_create(int device, imago_ctx **out)
         c->tune_potrf = (e2 && strcmp(e2, "rocsolver") == 0) ? 1 : 0;
         const char *e3 = getenv("IMAGO_STEDC");
         c->tune_tridiag = (e3 && strcmp(e3, "rocsolver") == 0) ? 1 : 0;
+        const char *e4 = getenv("IMAGO_GRAPH");
+        c->tune_graph = (e4 && strcmp(e4, "0") == 0) ? 0 : 1;
     }
     *out = c;
     return IMAGO_OK;
@@ -931,6 +947,10 @@ extern "C" int imago_tune(imago_ctx *c, int param, int value)
     case IMAGO_TUNE_TRIDIAG:
         if (value != 0 && value != 1) return fail(IMAGO_E_ARG, "tridiagonal solver %d", value);
         c->tune_tridiag = value;
+        break;
+    case IMAGO_TUNE_GRAPH:
+        if (value != 0 && value != 1) return fail(IMAGO_E_ARG, "graph replay %d", value);
+        c->tune_graph = value;
         break;
     default:
         return fail(IMAGO_E_ARG, "unknown tuning parameter %d", param);
@@ -1003,6 +1023,8 @@ extern "C" int imago_ctx_destroy(imago_ctx *c)
     for (hipEvent_t ev : c->evs)
         if (ev) (void)hipEventDestroy(ev);
     if (c->dcplan.dev) (void)hipFree(c->dcplan.dev);
+    if (c->dcg.exec) (void)hipGraphExecDestroy(c->dcg.exec);
+    if (c->dcg.work) (void)hipFree(c->dcg.work);
     if (c->stream2) {
         (void)hipStreamSynchronize(c->stream2);
         (void)hipStreamDestroy(c->stream2);
@@ -1620,6 +1642,59 @@ static int dc_ensure_plan(imago_ctx *c, int n)
     return IMAGO_OK;
 }
 
+// The batched divide and conquer on the context's stream as a replay of a
+// graph captured on the first call with these buffers (the plan is on the
+// device already; the capture records kernel launches only, no allocation,
+// copy or memset)
+static int dc_graph_launch(imago_ctx *c, int n, int nb, double *evals, double *ework, void *A,
+                           bool real)
+{
+    imago_ctx::DcGraph &g = c->dcg;
+    const size_t bytes = dc_work_bytes(n, nb);
+    const bool same = g.exec && g.n == n && g.nb == nb && g.real == real && g.d == evals &&
+                      g.e == ework && g.X == A && g.plan == c->dcplan.dev &&
+                      g.work_bytes >= bytes;
+    if (!same) {
+        if (g.exec || g.work) HIPCHK(hipStreamSynchronize(c->stream));
+        if (g.exec) {
+            HIPCHK(hipGraphExecDestroy(g.exec));
+            g.exec = nullptr;
+        }
+        if (g.work_bytes < bytes) {
+            if (g.work) HIPCHK(hipFree(g.work));
+            g.work = nullptr;
+            g.work_bytes = 0;
+            HIPCHK(hipMalloc(&g.work, bytes));
+            g.work_bytes = bytes;
+        }
+        DcWork w;
+        dc_work_carve(g.work, n, nb, w);
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le =
+            launch_tridiag_dc(c->dcplan, nb, evals, ework, w, evals, A, real, c->stream);
+        const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        if (le != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            return fail(IMAGO_E_HIP, "capture of the divide and conquer: %s",
+                        hipGetErrorString(le != hipSuccess ? le : ee));
+        }
+        const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ie);
+        g.n = n;
+        g.nb = nb;
+        g.real = real;
+        g.d = evals;
+        g.e = ework;
+        g.X = A;
+        g.plan = c->dcplan.dev;
+    }
+    HIPCHK(hipGraphLaunch(g.exec, c->stream));
+    return IMAGO_OK;
+}
+
 static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *evals,
                       double *ework, int *dinfo)
 {
@@ -1641,7 +1716,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     RCHK(dev_alloc(c, &V, (nn * nb + (size_t)K * n) * es));
     if (own_dc) {
         RCHK(dc_ensure_plan(c, n));
-        RCHK(dev_alloc(c, &dcw, dc_work_bytes(n, nb)));
+        if (!c->tune_graph) RCHK(dev_alloc(c, &dcw, dc_work_bytes(n, nb)));
     } else {
         RCHK(dev_alloc(c, &Z, (size_t)N0 * N0 + (size_t)N1 * N1));
         RCHK(dev_alloc(c, &wall, (size_t)N));
@@ -1685,9 +1760,13 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
     if (own_dc) {
         // the batched divide and conquer: eigenvalues ascending per matrix
         // into evals, eigenvectors into A
-        DcWork dw;
-        dc_work_carve(dcw, n, nb, dw);
-        HIPCHK(launch_tridiag_dc(c->dcplan, nb, evals, ework, dw, evals, A, !cx, c->stream));
+        if (c->tune_graph) {
+            RCHK(dc_graph_launch(c, n, nb, evals, ework, A, !cx));
+        } else {
+            DcWork dw;
+            dc_work_carve(dcw, n, nb, dw);
+            HIPCHK(launch_tridiag_dc(c->dcplan, nb, evals, ework, dw, evals, A, !cx, c->stream));
+        }
     } else {
     // d and e of the batch are back to back, e[n-1] = 0 between problems:
     // each half of the batch is one block-diagonal problem, the two halves

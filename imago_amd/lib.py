@@ -393,11 +393,12 @@ class Context:
                                        ctypes.byref(rec)))
         return c, rec.as_dict(), rho
 
-    TUNE = {"potrf_block": 1, "eigensolver": 2, "cholesky": 3, "tridiag": 4}
+    TUNE = {"potrf_block": 1, "eigensolver": 2, "cholesky": 3, "tridiag": 4, "graph": 5}
 
     def tune(self, **kw):
         """imago_tune: potrf_block=1..32, eigensolver=0|1, cholesky=0|1,
-        tridiag=0|1 (the divide and conquer: this library's or rocSOLVER's)."""
+        tridiag=0|1 (the divide and conquer: this library's or rocSOLVER's),
+        graph=0|1 (the divide and conquer replayed from a captured graph)."""
         for k, v in kw.items():
             _check(load().imago_tune(self._h, self.TUNE[k], int(v)))
 

@@ -227,13 +227,16 @@ int imago_dos(imago_ctx *ctx, const double *energies, int spin, const int *chann
  * heevd / syevd; IMAGO_TUNE_CHOLESKY, 0 = this library's blocked Cholesky,
  * 1 = rocSOLVER's potrf; IMAGO_TUNE_TRIDIAG, the tridiagonal stage of this
  * library's eigensolver, 0 = its batched divide and conquer, 1 = rocSOLVER's
- * dstedc.  Results do not depend on the choice beyond
+ * dstedc; IMAGO_TUNE_GRAPH, 1 = the divide and conquer's launches captured
+ * once in a HIP graph and replayed (default; IMAGO_GRAPH=0 turns it off),
+ * 0 = launched one by one.  Results do not depend on the choice beyond
  * rounding.  Setting any of them makes the next imago_secular factor the
  * overlap again. */
 #define IMAGO_TUNE_POTRF_BLOCK 1
 #define IMAGO_TUNE_EIGENSOLVER 2
 #define IMAGO_TUNE_CHOLESKY 3
 #define IMAGO_TUNE_TRIDIAG 4
+#define IMAGO_TUNE_GRAPH 5
 int imago_tune(imago_ctx *ctx, int param, int value);
 
 /* The tridiagonal stage of the eigensolver on its own: eigenvalues

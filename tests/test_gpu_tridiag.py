@@ -39,3 +39,33 @@ def test_batches_and_workload_orders(gpu_ctx, n, nb):
     sep = gap > 1e-3 * np.abs(d[0]).max()
     dots = np.abs(np.sum(Z[0][:, sep] * Zr[:, sep], axis=0))
     assert np.all(np.abs(dots - 1.0) < 1e-9)
+
+
+def test_graph_replay_matches_direct_launches(gpu_ctx):
+    """The divide and conquer replayed from its captured graph gives the
+    same bits as its launches issued one by one: the first call captures,
+    later calls replay with other work on the stream between them (the
+    traces), and a system of another order captures again."""
+    from imago_amd import configs
+    for s in (configs.small_test_system(seed=4), configs.small_test_system(seed=5, gamma=True),
+              configs.si64_supercell(reps=(1, 1, 1), mesh=(2, 2, 2), cutoff_bohr=16.0)):
+        print("[gpu] graph replay %s n=%d" % (s.name, s.vale_dim), flush=True)
+        gpu_ctx.set_system(s)
+        for kind in ("ol", "ke", "np", "ep_direct"):
+            gpu_ctx.assemble(kind)
+        n = s.vale_dim
+        coeffs = configs.pot_coefficients(s, seed=2)
+        runs = []
+        for graph in (1, 1, 1, 0):
+            gpu_ctx.tune(graph=graph)
+            ev = gpu_ctx.secular(coeffs, n)
+            vec = [gpu_ctx.eigenvectors(k, n) for k in range(s.num_kpoints)]
+            occ = np.zeros((s.num_kpoints, n))
+            occ[:, :2] = 1.0
+            gpu_ctx.density_traces(occ)
+            runs.append((ev, vec))
+        for ev, vec in runs[1:]:
+            assert np.array_equal(ev, runs[0][0])
+            for a, b in zip(vec, runs[0][1]):
+                assert np.array_equal(a, b)
+    gpu_ctx.tune(graph=1)
