@@ -2,6 +2,8 @@
 
 Each translation unit under csrc/ is compiled to an object in parallel,
 then linked into imago_amd/libimago_hip.so against rocSOLVER/rocBLAS.
+The integral file library (libimago_h5.so: host code over the HDF5 C
+library of the image's conda tree) is built beside it when HDF5 is there.
 
 Usage: python -m imago_amd.build [-v]   (or __graft_entry__.build())
 """
@@ -44,6 +46,25 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
+HDF5_ROOT = os.environ.get("IMAGO_HDF5_ROOT", "/opt/conda")
+H5LIB = os.path.join(HERE, "libimago_h5.so")
+
+
+def build_h5(verbose: bool = False):
+    """libimago_h5.so from csrc/h5store.cpp, or None without HDF5."""
+    if not os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")):
+        return None
+    lib = os.path.join(HDF5_ROOT, "lib")
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HDF5_ROOT, "include"), os.path.join(CSRC, "h5store.cpp"),
+           "-o", H5LIB + ".tmp", "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(H5LIB + ".tmp", H5LIB)
+    return H5LIB
+
+
 FORTRAN = os.path.join(HERE, "fortran")
 FBIN = os.path.join(FORTRAN, "_bin")
 FLANG = "/opt/rocm/bin/amdflang"
@@ -71,7 +92,9 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     jobs = min(len(SOURCES), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(jobs) as ex:
+        h5 = ex.submit(build_h5, verbose)
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
+        h5.result()
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + [
         "-o", LIB + ".tmp", "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas",
         "-Wl,-rpath,/opt/rocm/lib"]

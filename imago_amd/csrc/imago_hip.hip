@@ -156,6 +156,9 @@ struct imago_ctx {
     // panels: 0 OL, 1 KE, 2 NP, 3 EP
     double *panel[4] = {nullptr, nullptr, nullptr, nullptr};
     bool assembled[4] = {false, false, false, false};
+    // (term, k-point) slices of each panel written by imago_set_packed;
+    // a panel loaded from the host counts as assembled once all are
+    std::vector<unsigned char> filled[4];
     bool chol_valid = false;   // Sfull/Sreal hold the Cholesky factor of S(k)
     // secular buffers
     double2 *Hfull = nullptr, *Sfull = nullptr;
@@ -312,6 +315,7 @@ static void ctx_free_system(imago_ctx *c)
     for (int k = 0; k < 4; ++k) {
         c->panel[k] = nullptr;
         c->assembled[k] = false;
+        c->filled[k].clear();
     }
     c->chol_valid = false;
     c->ep_tuples = nullptr;
@@ -1336,6 +1340,7 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->timing[0] = ms;
     c->assembled[slot] = true;
+    c->filled[slot].assign((size_t)nterms * c->h.nk, 1);
     if (kind == IMAGO_OL) c->chol_valid = false;
     return IMAGO_OK;
 }
@@ -1369,6 +1374,64 @@ extern "C" int imago_get_packed(imago_ctx *c, int kind, int term, int kpoint, do
                 for (int x = 0; x < dsz; ++x) out[dsz * pk + x] = blk[dsz * be + x];
             }
     }
+    return IMAGO_OK;
+}
+
+extern "C" int imago_set_packed(imago_ctx *c, int kind, int term, int kpoint, const double *in)
+{
+    if (!c || !c->has_system) return fail(IMAGO_E_STATE, "imago_set_packed before imago_set_system");
+    if (kind < IMAGO_OL || kind > IMAGO_EP) return fail(IMAGO_E_ARG, "unknown integral kind %d", kind);
+    const HostSys &h = c->h;
+    const int nterms = kind == IMAGO_EP ? h.pot_dim : 1;
+    if (term < 0 || term >= nterms || kpoint < 0 || kpoint >= h.nk || !in)
+        return fail(IMAGO_E_ARG, "term %d / k-point %d out of range", term, kpoint);
+    HIPCHK(hipSetDevice(c->device));
+    const int slot = kind - 1;
+    const int dsz = h.gamma ? 1 : 2;
+    const long per = panel_doubles(c);
+    const long len = dsz * c->nbe;
+    const long npk = (long)h.n * (h.n + 1) / 2;
+    // packed upper triangle -> block layout (the inverse of
+    // imago_get_packed); the lower triangles of diagonal blocks stay zero
+    std::vector<double> blk(len, 0.0);
+    std::vector<unsigned char> cover(npk, 0);
+    for (const PairRec &pr : c->hpairs) {
+        const int ns1 = h.nstates[h.atom_type[pr.i]], ns2 = h.nstates[h.atom_type[pr.j]];
+        const long r0 = h.cumul[pr.i], c0 = h.cumul[pr.j];
+        for (int s2 = 0; s2 < ns2; ++s2)
+            for (int s1 = 0; s1 < ns1; ++s1) {
+                const long row = r0 + s1, col = c0 + s2;
+                if (row > col) continue;
+                const long pk = col * (col + 1) / 2 + row;
+                const long be = (long)pr.blk + s1 + (long)ns1 * s2;
+                for (int x = 0; x < dsz; ++x) blk[dsz * be + x] = in[dsz * pk + x];
+                cover[pk] = 1;
+            }
+    }
+    // atom pairs beyond every lattice image's reach have no block: their
+    // entries must be zero, as the assembly leaves them
+    for (long pk = 0; pk < npk; ++pk)
+        if (!cover[pk])
+            for (int x = 0; x < dsz; ++x)
+                if (in[dsz * pk + x] != 0.0)
+                    return fail(IMAGO_E_ARG,
+                                "packed entry %ld is nonzero but its atom pair has no block in "
+                                "this system's pair list",
+                                pk);
+    if (!c->panel[slot]) {
+        RCHK(dev_alloc(c, &c->panel[slot], (size_t)per * nterms));
+        c->filled[slot].assign((size_t)nterms * h.nk, 0);
+    }
+    if (c->filled[slot].size() != (size_t)nterms * h.nk)
+        c->filled[slot].assign((size_t)nterms * h.nk, 0);
+    double *dst = c->panel[slot] + (size_t)term * per + (size_t)kpoint * len;
+    HIPCHK(hipMemcpyAsync(dst, blk.data(), len * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->filled[slot][(size_t)term * h.nk + kpoint] = 1;
+    bool all = true;
+    for (unsigned char f : c->filled[slot]) all = all && f;
+    c->assembled[slot] = all;
+    if (kind == IMAGO_OL) c->chol_valid = false;
     return IMAGO_OK;
 }
 

@@ -124,6 +124,15 @@ module O_ImagoHIP
          real(c_double), intent(out) :: out(*)
       end function imago_get_packed
 
+      ! the matrix a resumed run reads from its integral file (readPackedMatrix)
+      integer(c_int) function imago_set_packed(ctx, kind, term, kpoint, in) &
+            & bind(C, name="imago_set_packed")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: kind, term, kpoint
+         real(c_double), intent(in) :: in(*)
+      end function imago_set_packed
+
       integer(c_int) function imago_secular(ctx, potCoeffs, numStates, &
             & eigenValues, info) bind(C, name="imago_secular")
          import :: c_int, c_ptr, c_double

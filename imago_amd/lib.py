@@ -44,7 +44,7 @@ EXPORTS = [
     "imago_scf_update", "imago_site_descriptors", "imago_mlp_create", "imago_mlp_destroy",
     "imago_mlp_num_params", "imago_mlp_params", "imago_mlp_forward", "imago_mlp_gradient",
     "imago_mlp_adam_step", "imago_mlp_train", "imago_mlp_forward_noalloc", "imago_tune",
-    "imago_tridiag_eig", "imago_dos",
+    "imago_tridiag_eig", "imago_dos", "imago_set_packed",
 ]
 
 
@@ -94,6 +94,8 @@ def load():
     L.imago_boys_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
     L.imago_assemble.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.imago_get_packed.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _dp]
+    L.imago_set_packed.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    _dp]
     L.imago_secular.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _ip]
     L.imago_get_eigenvectors.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
@@ -232,6 +234,34 @@ class Context:
         _check(load().imago_get_packed(self._h, KIND_CODES.get(kind, kind), term, kpoint,
                                        _p(out)))
         return out[0::2] + 1j * out[1::2]
+
+    def get_packed_raw(self, kind, term=0, kpoint=0):
+        """The packed matrix as the C layout holds it: real and imaginary
+        parts interleaved (complex build) or real (gamma)."""
+        n = self.vale_dim
+        out = np.zeros((1 if self.system.gamma else 2) * (n * (n + 1) // 2))
+        _check(load().imago_get_packed(self._h, KIND_CODES.get(kind, kind), term, kpoint,
+                                       _p(out)))
+        return out
+
+    def set_packed(self, kind, packed, term=0, kpoint=0):
+        """Load one packed matrix (complex values, or the interleaved /
+        real layout of get_packed_raw) in place of assembling it."""
+        n = self.vale_dim
+        npack = n * (n + 1) // 2
+        a = np.asarray(packed)
+        if np.iscomplexobj(a):
+            if self.system.gamma:
+                raise ValueError("a gamma build takes real packed matrices")
+            buf = np.empty(2 * npack)
+            buf[0::2], buf[1::2] = a.real, a.imag
+        else:
+            buf = np.ascontiguousarray(a, dtype=np.float64)
+        if buf.size != (1 if self.system.gamma else 2) * npack:
+            raise ValueError("packed matrix of %d doubles for valeDim %d" % (buf.size, n))
+        _check(load().imago_set_packed(self._h, KIND_CODES.get(kind, kind), term, kpoint,
+                                       _p(buf)))
+        return self
 
     # -- secular equation ---------------------------------------------------
     def secular(self, pot_coeffs, num_states):

@@ -170,6 +170,18 @@ int imago_assemble(imago_ctx *ctx, int kind);
 int imago_get_packed(imago_ctx *ctx, int kind, int term, int kpoint,
                      double *out);
 
+/* The inverse of imago_get_packed: load one packed matrix (same layout)
+ * into the device panel of kind, term, k-point, in place of assembling it.
+ * This is the read side of the reference's integral checkpoint: a run
+ * resumed on an integral file whose group status is 1 reads the matrices
+ * instead of computing them (integrals.F90:265-294; readPackedMatrix,
+ * matrixSubs.F90:259-302).  The kind counts as assembled once every term
+ * and k-point is loaded (or imago_assemble built it).  Entries of atom
+ * pairs that have no lattice image within the system's cutoff must be
+ * zero (IMAGO_E_ARG otherwise), as the assembly leaves them. */
+int imago_set_packed(imago_ctx *ctx, int kind, int term, int kpoint,
+                     const double *in);
+
 /* ---- secular equation: secularEqnSCF (secularEqn.F90:37-328) ---------- *
  * H(k) = NP(k) + KE(k) + sum_j c_j V_j(k) for every k-point, with the
  * reference's rule that a coefficient of exactly 0.0 adds V_j unscaled
