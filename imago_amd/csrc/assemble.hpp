@@ -25,6 +25,7 @@
 //     LDS in rounds of 16 primitive blocks; Bloch phases are applied once
 //     per lattice image to the contracted block.
 #pragma once
+#include <type_traits>
 #include "prim.hpp"
 
 namespace imago {
@@ -80,7 +81,39 @@ template <int SH>
 struct ShAcc {
     double v[SH][SH];
     __device__ __forceinline__ void add(int i, int j, double x) { v[i][j] += x; }
+    __device__ __forceinline__ double get(int i, int j) const { return v[i][j]; }
+    __device__ __forceinline__ void zero()
+    {
+#pragma unroll
+        for (int i = 0; i < SH; ++i)
+#pragma unroll
+            for (int j = 0; j < SH; ++j) v[i][j] = 0.0;
+    }
 };
+
+// A lane's primitive block in device memory instead of registers, for the
+// nuclear kind with f shells: its 16 x 16 block summed over every nuclear
+// site and image needs more registers than a lane has (the register
+// version spills 5.8 KB per lane).  Lane-interleaved, so that one entry of
+// the 64 lanes is one coalesced access: entry (i, j) of lane l at
+// o[(i * SH + j) * 64 + l]; each work-group owns SH * SH * 64 doubles.
+template <int SH>
+struct LaneAcc {
+    double *o;   // the work-group's slice + lane
+    __device__ __forceinline__ void add(int i, int j, double x) { o[(i * SH + j) * 64] += x; }
+    __device__ __forceinline__ double get(int i, int j) const { return o[(i * SH + j) * 64]; }
+    __device__ __forceinline__ void zero()
+    {
+        for (int q = 0; q < SH * SH; ++q) o[q * 64] = 0.0;
+    }
+};
+
+// the nuclear kind with f shells accumulates in device memory
+template <int KIND, int LMAX>
+constexpr bool kLaneAccInMemory = KIND == K_NP && LMAX == 3;
+template <int KIND, int LMAX, int SH>
+using AssembleAcc =
+    typename std::conditional<kLaneAccInMemory<KIND, LMAX>, LaneAcc<SH>, ShAcc<SH>>::type;
 
 // LDS bytes needed by k_assemble for given sizes
 __host__ __device__ inline size_t assemble_lds_bytes(int lmax, int ma, int ms, int nk)
@@ -98,7 +131,7 @@ __global__ __launch_bounds__(64) void k_assemble(DevSys S, const PairRec *__rest
                                                  const CandRec *__restrict__ cands,
                                                  const int *__restrict__ cand_off,
                                                  int term0, double *__restrict__ out,
-                                                 long term_stride)
+                                                 long term_stride, double *__restrict__ acc_work)
 {
     constexpr int SH = (LMAX + 1) * (LMAX + 1);
     constexpr int RQ = 16;  // primitive blocks per LDS round
@@ -187,11 +220,10 @@ __global__ __launch_bounds__(64) void k_assemble(DevSys S, const PairRec *__rest
         for (int c0 = 0; c0 < cnt; c0 += 64) {
             const int q = c0 + lane;
             const bool act = q < cnt;
-            ShAcc<SH> sh;
-#pragma unroll
-            for (int i = 0; i < SH; ++i)
-#pragma unroll
-                for (int j = 0; j < SH; ++j) sh.v[i][j] = 0.0;
+            AssembleAcc<KIND, LMAX, SH> sh;
+            if constexpr (kLaneAccInMemory<KIND, LMAX>)
+                sh.o = acc_work + (size_t)blockIdx.x * SH * SH * 64 + lane;
+            sh.zero();
             int code = 0;
             if (act) {
                 code = alist[q];
@@ -258,7 +290,7 @@ __global__ __launch_bounds__(64) void k_assemble(DevSys S, const PairRec *__rest
 #pragma unroll
                     for (int i = 0; i < SH; ++i)
 #pragma unroll
-                        for (int j = 0; j < SH; ++j) dst[i * SH + j] = sh.v[i][j];
+                        for (int j = 0; j < SH; ++j) dst[i * SH + j] = sh.get(i, j);
                     st_a[lane % RQ] = code;
                 }
                 __syncthreads();

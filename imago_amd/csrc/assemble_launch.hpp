@@ -32,7 +32,13 @@ struct AssembleLaunch {
     double *ep_blocks;      // EP: primitive block of every item
     long ep_nitems;
     int nk;
+    // NP with f shells: each work-group's 16 x 16 x 64 accumulator block in
+    // device memory (kAccSlots work-groups per launch)
+    double *acc_work = nullptr;
 };
+
+constexpr int kAccSlots = 1024;
+constexpr size_t kAccSlotDoubles = 16 * 16 * 64;
 
 hipError_t launch_assemble_ol(const AssembleLaunch &L);
 hipError_t launch_assemble_ke(const AssembleLaunch &L);
@@ -103,9 +109,23 @@ hipError_t int_exclusive_scan(const int *in, int *out, long n, void *tmp, size_t
 template <int KIND, int LMAX, int NE, bool G>
 hipError_t launch_one(const AssembleLaunch &L)
 {
+    if constexpr (kLaneAccInMemory<KIND, LMAX>) {
+        // one term; the pairs in launches of at most kAccSlots work-groups,
+        // work-group x of a launch owning slot x of acc_work
+        if (!L.acc_work || L.nterms != 1) return hipErrorInvalidValue;
+        for (int p0 = 0; p0 < L.npairs; p0 += kAccSlots) {
+            const int np = std::min(kAccSlots, L.npairs - p0);
+            hipLaunchKernelGGL((k_assemble<KIND, LMAX, NE, G>), dim3(np, 1), dim3(64), L.lds,
+                               L.stream, L.S, L.pairs + p0, L.imgs, L.phase, L.cands, L.cand_off,
+                               L.term0, L.out, L.term_stride, L.acc_work);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     hipLaunchKernelGGL((k_assemble<KIND, LMAX, NE, G>), dim3(L.npairs, L.nterms), dim3(64), L.lds,
                        L.stream, L.S, L.pairs, L.imgs, L.phase, L.cands, L.cand_off, L.term0,
-                       L.out, L.term_stride);
+                       L.out, L.term_stride, nullptr);
     return hipGetLastError();
 }
 
@@ -192,8 +212,6 @@ hipError_t launch_ep_prims_pass(const AssembleLaunch &L)
 template <int KIND>
 hipError_t launch_kind(const AssembleLaunch &L)
 {
-    // nuclear integrals with f shells are refused by imago_assemble, so the
-    // LMAX = 3 instance is not built for them
 #define IMAGO_CASE(LM)                                                          \
     case LM:                                                                    \
         if (L.ne == 4) return L.gamma ? launch_one<KIND, LM, 4, true>(L)        \
@@ -204,13 +222,7 @@ hipError_t launch_kind(const AssembleLaunch &L)
         IMAGO_CASE(0)
         IMAGO_CASE(1)
         IMAGO_CASE(2)
-    case 3:
-        if constexpr (KIND != K_NP) {
-            if (L.ne == 4) return L.gamma ? launch_one<KIND, 3, 4, true>(L)
-                                          : launch_one<KIND, 3, 4, false>(L);
-            return L.gamma ? launch_one<KIND, 3, 16, true>(L) : launch_one<KIND, 3, 16, false>(L);
-        }
-        return hipErrorInvalidValue;
+        IMAGO_CASE(3)
     default: return hipErrorInvalidValue;
     }
 #undef IMAGO_CASE

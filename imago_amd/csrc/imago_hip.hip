@@ -1270,6 +1270,10 @@ static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride
         return fail(IMAGO_E_ARG, "LDS requirement %zu bytes exceeds 64 KiB (max_states %d, nk %d)",
                     lds, h.maxs, h.nk);
     AssembleLaunch L = make_launch(c, out, term_stride, term0, nterms, lds);
+    if (kind == IMAGO_NP && c->lmax == 3) {
+        // the f-shell nuclear blocks accumulate in device memory
+        RCHK(dev_alloc(c, &L.acc_work, (size_t)kAccSlots * kAccSlotDoubles));
+    }
     hipError_t e;
     switch (kind) {
     case IMAGO_OL: e = launch_assemble_ol(L); break;
@@ -1281,6 +1285,10 @@ static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride
         return fail(IMAGO_E_ARG, "no assembly kernel for kind %d lmax %d", kind, c->lmax);
     HIPCHK(e);
     HIPCHK(hipGetLastError());
+    if (L.acc_work) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dev_release(c, L.acc_work);
+    }
     return IMAGO_OK;
 }
 
@@ -1308,9 +1316,6 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
         c->ep_direct = true;
         return IMAGO_OK;
     }
-    if (kind == IMAGO_NP && c->lmax > 2)
-        return fail(IMAGO_E_ARG, "nuclear-potential integrals with f shells (lmAlphaIndex 16) are "
-                                 "not supported by this build");
     HIPCHK(hipSetDevice(c->device));
     const int slot = kind - 1;
     const long per = panel_doubles(c);

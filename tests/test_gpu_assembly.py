@@ -21,6 +21,8 @@ def _systems():
         "small_gamma": lambda: configs.small_test_system(seed=6, gamma=True),
         "small_f_one_element": lambda: configs.small_test_system(seed=7, two_elements=False,
                                                                   with_f=True, lmax=2),
+        "small_f_gamma": lambda: configs.small_test_system(seed=8, gamma=True, with_f=True,
+                                                           lmax=2),
         "si2_primitive": lambda: configs.si2_primitive(),
         "si8_cubic_d": lambda: configs.si64_supercell(reps=(1, 1, 1), mesh=(2, 2, 2),
                                                       cutoff_bohr=12.0),
@@ -43,13 +45,6 @@ def assembled(request, gpu_ctx, oracle_mod):
 @pytest.mark.parametrize("kind", ["ol", "ke", "np"])
 def test_two_centre_and_nuclear(assembled, gpu_ctx, oracle_mod, kind):
     name, s = assembled
-    if kind == "np" and name == "small_f_one_element":
-        # nuclear integrals with f shells are refused with a clear error
-        from imago_amd.lib import ImagoError
-        with pytest.raises(ImagoError) as e:
-            gpu_ctx.assemble(kind)
-        assert e.value.name == "IMAGO_E_ARG" and "f shells" in str(e.value)
-        return
     gpu_ctx.assemble(kind)
     ref = oracle_mod.assemble(s, kind)
     got = np.array([gpu_ctx.get_packed(kind, 0, k) for k in range(s.num_kpoints)])
