@@ -45,6 +45,7 @@ class SystemDesc(ctypes.Structure):
         ("kpoint_weights", _dp),
         ("gamma", ctypes.c_int),
         ("log_basis_fn_thresh", ctypes.c_double),
+        ("type_num_core_states", _ip),
     ]
 
 
@@ -116,4 +117,6 @@ def make_desc(system):
     s.kpoint_weights = d("kpoint_weights")
     s.gamma = arr["gamma"]
     s.log_basis_fn_thresh = arr["log_basis_fn_thresh"]
+    if np.any(arr["type_num_core_states"]):
+        s.type_num_core_states = i("type_num_core_states")
     return s, keep

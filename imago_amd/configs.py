@@ -73,12 +73,17 @@ def pot_coefficients(system: System, seed: int = 7) -> np.ndarray:
 
 def si2_primitive(seed: int = 1, cutoff_bohr: float = 14.0, mesh=(4, 4, 4),
                   log_thresh: float | None = None, nalpha: int = 12,
-                  npot: int = 16, ls=(0, 0, 1, 0, 1)) -> System:
+                  npot: int = 16, ls=(0, 0, 1, 0, 1), core: bool = False) -> System:
+    """Si2 primitive cell, 1s 2s 2p 3s 3p per atom: all valence (n = 18),
+    or with core=True the first three shells as core states (core 10 +
+    valence 8)."""
     rng = np.random.default_rng(seed)
     a = 5.43 * BOHR_PER_ANGSTROM
     rv = 0.5 * a * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]], dtype=np.float64).T
     pos = np.array([[0, 0, 0], [0.25 * a] * 3])
     at = make_atom_type(rng, SI, ls, nalpha=nalpha)
+    if core:
+        at.core_shells, at.shells = at.shells[:3], at.shells[3:]
     pot = PotType(element=SI, alphas=geometric_alphas(0.1, 1.0e4, npot),
                   nuc_charge=Z_OF[SI], nuc_alpha=NUC_ALPHA[SI], sites=[0, 1],
                   covalent_radius=COV_RADIUS[SI])
@@ -88,7 +93,7 @@ def si2_primitive(seed: int = 1, cutoff_bohr: float = 14.0, mesh=(4, 4, 4),
     return System(real_vectors=rv, atom_types=[at], atom_type_of=[0, 0],
                   atom_pos=pos, pot_types=[pot], pot_site_pos=pos.copy(),
                   kpoints=kp, kpoint_weights=kw, log_basis_fn_thresh=log_thresh,
-                  name="si2_primitive")
+                  name="si2_primitive_core" if core else "si2_primitive")
 
 
 def si64_supercell(seed: int = 2, cutoff_bohr: float = 16.0, mesh=(2, 2, 2),
@@ -183,9 +188,11 @@ def sio2_amorphous(seed: int = 3, n_si: int = 72, n_o: int = 144,
 
 def small_test_system(seed: int = 5, gamma: bool = False, lmax: int = 2,
                       nalpha: int = 5, cutoff_bohr: float = 9.0,
-                      two_elements: bool = True, with_f: bool = False) -> System:
+                      two_elements: bool = True, with_f: bool = False,
+                      core: bool = False) -> System:
     """A few-atom cell with mixed elements and a lopsided basis (different
-    numOrbAlphas per l) that the CPU oracle assembles in seconds."""
+    numOrbAlphas per l) that the CPU oracle assembles in seconds; core=True
+    gives the first type s and p core shells and the second an s one."""
     rng = np.random.default_rng(seed)
     rv = np.array([[7.1, 0.3, 0.0], [0.0, 6.6, 0.4], [0.5, 0.0, 7.4]]).T
     pos = np.array([[0.1, 0.2, 0.0], [3.2, 2.9, 3.1], [1.4, 4.8, 2.2]])
@@ -204,6 +211,12 @@ def small_test_system(seed: int = 5, gamma: bool = False, lmax: int = 2,
                       shells=random_shells(rng, alphas_b, [0, 1]))
         types.append(tb)
         tof = [0, 1, 0]
+    if core:
+        crng = np.random.default_rng(seed + 1000)
+        ta.core_shells = [(l, np.where(np.arange(nalpha) < noa[l], c, 0.0))
+                          for l, c in random_shells(crng, alphas, [0, 1])]
+        if two_elements:
+            types[1].core_shells = random_shells(crng, alphas_b, [0])
     pots = [PotType(element=SI, alphas=geometric_alphas(0.15, 200.0, 4), nuc_charge=4.0,
                     nuc_alpha=5.0, sites=[0, 1] if not two_elements else [0],
                     covalent_radius=COV_RADIUS[SI]),

@@ -139,7 +139,7 @@ hipError_t launch_ep_one(const AssembleLaunch &L)
     const long groups = std::min(per * kNumXcd, 1L << 24);
     hipLaunchKernelGGL((k_ep_store<LMAX, NE, NKR, G>), dim3((unsigned)groups), dim3(64),
                        L.lds, L.stream, L.S, L.pairs, L.phase, L.ep_tuples, L.ep_tseg,
-                       L.ep_blocks, L.order, L.npairs, L.out, L.term_stride);
+                       L.ep_blocks, L.order, L.npairs, L.out, L.term_stride, L.term0, L.nterms);
     return hipGetLastError();
 }
 

@@ -302,14 +302,14 @@ __global__ __launch_bounds__(64) void k_ep_store(
     DevSys S, const PairRec *__restrict__ pairs, const double2 *__restrict__ phase,
     const EpTuple *__restrict__ tuples, const long *__restrict__ tseg,
     const double *__restrict__ blocks, const int *__restrict__ order, int npairs,
-    double *__restrict__ out, long term_stride)
+    double *__restrict__ out, long term_stride, int term0, int nterms)
 {
     constexpr int SH = (LMAX + 1) * (LMAX + 1);
     constexpr int BS = SH * SH;
     constexpr int NQ = GAMMA ? 1 : (NKR > 0 ? NKR : 1);
     extern __shared__ double smem[];
     const int lane = threadIdx.x;
-    const long total = (long)S.pot_dim * npairs;
+    const long total = (long)nterms * npairs;
     const long per = (total + kNumXcd - 1) / kNumXcd;
     // grid-stride over (term, pair) tasks: a dispatch describes at most
     // 2^32 - 1 work-items; the stride is a multiple of kNumXcd, so each
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(64) void k_ep_store(
     for (long g = blockIdx.x; g < per * kNumXcd; g += gridDim.x) {
     const long logical = (g % kNumXcd) * per + g / kNumXcd;
     if (logical >= total) continue;
-    const int term = (int)(logical / npairs);
+    const int term = term0 + (int)(logical / npairs);
     const int pidx = order[logical % npairs];
     const int pt = S.term_type[term], aj = S.term_alpha[term];
     const long task = (long)pt * npairs + pidx;
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(64) void k_ep_store(
     flush();
     // store the block: contiguous entries pr.blk + (row state + ns1 *
     // column state) of every k
-    double *o = out + (size_t)term * term_stride;
+    double *o = out + (size_t)(term - term0) * term_stride;
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
         if (est[e] == 0) continue;

@@ -33,6 +33,7 @@
 #include "hetrd_launch.hpp"
 #include "tridiag_dc_launch.hpp"
 #include "dos_launch.hpp"
+#include "core_ortho_launch.hpp"
 #include "elecstat_launch.hpp"
 #include "xcmesh_launch.hpp"
 #include "populate.hpp"
@@ -78,6 +79,10 @@ struct HostSys {
     std::vector<double> adist, acenter, andist, apdist;
     std::vector<int> term_type, term_alpha, type_term0;
     int n = 0, pot_dim = 0;
+    // core states per type (stored after the valence states in basis / lm),
+    // the first core function of each atom, and their total (coreDim)
+    std::vector<int> ncore, cumul_core;
+    int core_dim = 0;
 };
 
 // Device buffers of the current system.  Released buffers are kept and
@@ -156,6 +161,16 @@ struct imago_ctx {
     // panels: 0 OL, 1 KE, 2 NP, 3 EP
     double *panel[4] = {nullptr, nullptr, nullptr, nullptr};
     bool assembled[4] = {false, false, false, false};
+    // core orthogonalisation (coreDim > 0): the extended view of the
+    // system, each type's valence then core states, in its own block
+    // layout; the dense extended overlap of every k-point is kept after the
+    // overlap's assembly for the other kinds (their Scv)
+    DevSys d_ext{};
+    PairRec *pairs_ext = nullptr;
+    long nbe_ext = 0;
+    const int *fn_atom = nullptr, *fn_state = nullptr;   // [valeDim + coreDim]
+    const int *atom_nstates_ext = nullptr;               // [atoms]
+    double *core_S = nullptr;
     // (term, k-point) slices of each panel written by imago_set_packed;
     // a panel loaded from the host counts as assembled once all are
     std::vector<unsigned char> filled[4];
@@ -318,6 +333,11 @@ static void ctx_free_system(imago_ctx *c)
         c->filled[k].clear();
     }
     c->chol_valid = false;
+    c->d_ext = DevSys{};
+    c->pairs_ext = nullptr;
+    c->nbe_ext = 0;
+    c->fn_atom = c->fn_state = c->atom_nstates_ext = nullptr;
+    c->core_S = nullptr;
     c->ep_tuples = nullptr;
     c->ep_tseg = nullptr;
     c->ep_iseg = nullptr;
@@ -436,6 +456,9 @@ static int build_host(HostSys &h, const imago_system_desc *s)
     h.type_el.assign(s->type_element, s->type_element + h.nt);
     h.noa.assign(s->type_num_orb_alphas, s->type_num_orb_alphas + 4 * h.nt);
     h.nstates.assign(s->type_num_states, s->type_num_states + h.nt);
+    h.ncore.assign(h.nt, 0);
+    if (s->type_num_core_states)
+        h.ncore.assign(s->type_num_core_states, s->type_num_core_states + h.nt);
     h.alphas.assign(s->type_alphas, s->type_alphas + (size_t)h.nt * h.maxa);
     h.basis.assign(s->type_basis, s->type_basis + (size_t)h.nt * h.maxs * h.maxa);
     h.lm.assign(s->type_lm_index, s->type_lm_index + (size_t)h.nt * h.maxs);
@@ -473,7 +496,10 @@ static int build_host(HostSys &h, const imago_system_desc *s)
                 return fail(IMAGO_E_ARG, "type %d: numOrbAlphas not non-increasing in l", t);
         if (h.nstates[t] <= 0 || h.nstates[t] > h.maxs)
             return fail(IMAGO_E_ARG, "type %d: num_states %d out of range", t, h.nstates[t]);
-        for (int m = 0; m < h.nstates[t]; ++m) {
+        if (h.ncore[t] < 0 || h.nstates[t] + h.ncore[t] > h.maxs)
+            return fail(IMAGO_E_ARG, "type %d: %d valence + %d core states exceed max_states %d", t,
+                        h.nstates[t], h.ncore[t], h.maxs);
+        for (int m = 0; m < h.nstates[t] + h.ncore[t]; ++m) {
             int lmv = h.lm[t * h.maxs + m];
             if (lmv < 1 || lmv > 16) return fail(IMAGO_E_ARG, "type %d: lm index %d", t, lmv);
         }
@@ -517,12 +543,16 @@ static int build_host(HostSys &h, const imago_system_desc *s)
         for (int a = 0; a < noa[0]; ++a)
             h.lma[t * h.maxa + a] = a < noa[3] ? 16 : a < noa[2] ? 9 : a < noa[1] ? 4 : 1;
     }
-    // cumulValeStates
+    // cumulValeStates, cumulCoreStates
     h.cumul.resize(h.natoms);
+    h.cumul_core.resize(h.natoms);
     h.n = 0;
+    h.core_dim = 0;
     for (int i = 0; i < h.natoms; ++i) {
         h.cumul[i] = h.n;
         h.n += h.nstates[h.atom_type[i]];
+        h.cumul_core[i] = h.core_dim;
+        h.core_dim += h.ncore[h.atom_type[i]];
     }
     // inverse of the real vectors (columns)
     {
@@ -887,6 +917,55 @@ static int build_geometry(imago_ctx *c)
     return IMAGO_OK;
 }
 
+// The extended view for the core orthogonalisation: the same pairs with
+// block offsets for each type's valence + core states (in the valence
+// layout's block order), the atom and extended state of every function
+// (valence functions first, then core functions), and a DevSys whose state
+// counts are the extended ones.
+static int build_core_tables(imago_ctx *c)
+{
+    const HostSys &h = c->h;
+    const int N = h.n + h.core_dim;
+    std::vector<int> next(h.nt);
+    for (int t = 0; t < h.nt; ++t) next[t] = h.nstates[t] + h.ncore[t];
+    std::vector<PairRec> pe = c->hpairs;
+    std::vector<int> order(pe.size());
+    for (size_t q = 0; q < pe.size(); ++q) order[q] = (int)q;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return pe[a].blk < pe[b].blk; });
+    long off = 0;
+    for (int q : order) {
+        const long sz = (long)next[h.atom_type[pe[q].i]] * next[h.atom_type[pe[q].j]];
+        if (off + sz >= (1L << 31)) return fail(IMAGO_E_ARG, "extended block layout exceeds 2^31 entries");
+        pe[q].blk = (int)off;
+        off += sz;
+    }
+    c->nbe_ext = off;
+    std::vector<int> fa(N), fs(N), ans(h.natoms);
+    for (int a = 0; a < h.natoms; ++a) {
+        const int t = h.atom_type[a];
+        ans[a] = next[t];
+        for (int m = 0; m < h.nstates[t]; ++m) {
+            fa[h.cumul[a] + m] = a;
+            fs[h.cumul[a] + m] = m;
+        }
+        for (int m = 0; m < h.ncore[t]; ++m) {
+            fa[h.n + h.cumul_core[a] + m] = a;
+            fs[h.n + h.cumul_core[a] + m] = h.nstates[t] + m;
+        }
+    }
+    RCHK(dev_alloc(c, &c->pairs_ext, pe.size()));
+    HIPCHK(hipMemcpy(c->pairs_ext, pe.data(), pe.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+    RCHK(dev_upload(c, &c->fn_atom, fa));
+    RCHK(dev_upload(c, &c->fn_state, fs));
+    RCHK(dev_upload(c, &c->atom_nstates_ext, ans));
+    c->d_ext = c->d;
+    RCHK(dev_upload(c, &c->d_ext.type_nstates, next));
+    c->d_ext.nbe = c->nbe_ext;
+    c->d_ext.vale_dim = N;
+    return IMAGO_OK;
+}
+
 static int lma_to_l_host(int lma) { return lma == 1 ? 0 : lma == 4 ? 1 : lma == 9 ? 2 : 3; }
 
 // ---------------------------------------------------------------------------
@@ -1099,6 +1178,7 @@ extern "C" int imago_set_system(imago_ctx *c, const imago_system_desc *s, int *v
     }
     const auto t1 = clk::now();
     RCHK(build_geometry(c));
+    if (h.core_dim > 0) RCHK(build_core_tables(c));
     const auto t2 = clk::now();
     c->timing[7] = std::chrono::duration<double, std::milli>(t2 - t1).count();
     int lmax = 0;
@@ -1117,10 +1197,11 @@ extern "C" int imago_set_system(imago_ctx *c, const imago_system_desc *s, int *v
 // assembly
 
 static AssembleLaunch make_launch(imago_ctx *c, double *out, long term_stride, int term0,
-                                  int nterms, size_t lds)
+                                  int nterms, size_t lds, bool ext = false)
 {
     const HostSys &h = c->h;
-    return AssembleLaunch{c->d, c->pairs, c->imgs, c->phase, c->cands, c->cand_off, term0, out,
+    return AssembleLaunch{ext ? c->d_ext : c->d, ext ? c->pairs_ext : c->pairs, c->imgs, c->phase,
+                          c->cands, c->cand_off, term0, out,
                           term_stride, c->npairs, nterms, lds, c->stream, c->lmax,
                           c->ne <= 4 ? 4 : 16, h.gamma != 0, c->type_term0, h.npt, c->pair_order,
                           c->ep_tuples, c->ep_tseg, c->ep_item_tuple, c->ep_blocks,
@@ -1259,8 +1340,9 @@ static int ep_build_runs(imago_ctx *c)
     return IMAGO_OK;
 }
 
+// ext: the extended (valence + core) view into an extended block layout
 static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride, int term0,
-                           int nterms)
+                           int nterms, bool ext = false)
 {
     const HostSys &h = c->h;
     const size_t lds =
@@ -1269,7 +1351,7 @@ static int launch_assemble(imago_ctx *c, int kind, double *out, long term_stride
     if (lds > 64 * 1024)
         return fail(IMAGO_E_ARG, "LDS requirement %zu bytes exceeds 64 KiB (max_states %d, nk %d)",
                     lds, h.maxs, h.nk);
-    AssembleLaunch L = make_launch(c, out, term_stride, term0, nterms, lds);
+    AssembleLaunch L = make_launch(c, out, term_stride, term0, nterms, lds, ext);
     if (kind == IMAGO_NP && c->lmax == 3) {
         // the f-shell nuclear blocks accumulate in device memory
         RCHK(dev_alloc(c, &L.acc_work, (size_t)kAccSlots * kAccSlotDoubles));
@@ -1297,11 +1379,16 @@ static long panel_doubles(const imago_ctx *c)
     return c->h.gamma ? c->nbe : 2 * c->nbe * c->h.nk;
 }
 
+static int core_assemble(imago_ctx *c, int kind);
+
 extern "C" int imago_assemble(imago_ctx *c, int kind)
 {
     if (!c || !c->has_system) return fail(IMAGO_E_STATE, "imago_assemble before imago_set_system");
     if (kind < IMAGO_OL || kind > IMAGO_EP_DIRECT)
         return fail(IMAGO_E_ARG, "unknown integral kind %d", kind);
+    if (kind == IMAGO_EP_DIRECT && c->h.core_dim > 0)
+        return fail(IMAGO_E_ARG, "the direct EP path has no core orthogonalisation: assemble "
+                                 "IMAGO_EP for a system with core states");
     if (kind == IMAGO_EP_DIRECT) {
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -1322,6 +1409,19 @@ extern "C" int imago_assemble(imago_ctx *c, int kind)
     const int nterms = kind == IMAGO_EP ? c->h.pot_dim : 1;
     if (!c->panel[slot]) RCHK(dev_alloc(c, &c->panel[slot], (size_t)per * nterms));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (c->h.core_dim > 0) {
+        // extended assembly and orthogonalisation against the core states
+        RCHK(core_assemble(c, kind));
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->timing[0] = ms;
+        c->assembled[slot] = true;
+        c->filled[slot].assign((size_t)nterms * c->h.nk, 1);
+        if (kind == IMAGO_OL) c->chol_valid = false;
+        return IMAGO_OK;
+    }
     if (kind == IMAGO_EP) {
         // stage 5: screening and primitive blocks (first EP assembly of a
         // system); stage 4: the k_ep_store launch alone; stage 0: both
@@ -1566,6 +1666,71 @@ static rocblas_status gemm_sb(rocblas_handle hb, bool cx, rocblas_operation ta,
     return rocblas_dgemm_strided_batched(hb, ra, rb, m, n, k, &alpha, (const double *)A, lda, sa,
                                          (const double *)B, ldb, sb, &beta, (double *)C, ldc, sc,
                                          batch);
+}
+
+// Orthogonalisation against the core states (orthoOL / ortho,
+// integrals.F90:3608-3910): for each term the extended matrix is assembled
+// into an extended block layout, spread into dense [valence | core]
+// matrices X per k-point, and
+//   X'vv = Xvv + Scv^H (Xcc Scv) - Scv^H Xcv - Xcv^H Scv
+// is formed by four batched GEMMs over the k-points (only Xvv is written;
+// Scv, Xcv and Xcc are read from disjoint blocks), then the upper valence
+// triangle goes back to the valence block layout with the 1e-10 flush.
+// For the overlap X is S itself, which gives Svv - 2 Scv^H Scv + Scv^H Scc
+// Scv; its dense matrices are kept (core_S) for the other kinds' Scv.
+static int core_assemble(imago_ctx *c, int kind)
+{
+    const HostSys &h = c->h;
+    const int n = h.n, cd = h.core_dim, N = n + cd, nk = h.nk, na = h.natoms;
+    const bool cx = !h.gamma;
+    const int dsz = cx ? 2 : 1;
+    if (kind != IMAGO_OL && !c->core_S)
+        return fail(IMAGO_E_STATE, "a system with core states needs IMAGO_OL assembled before the "
+                                   "other kinds (their core-valence overlap)");
+    const long per_ext = (long)dsz * c->nbe_ext * nk;
+    const long per = panel_doubles(c);
+    const size_t dn = (size_t)dsz * N * N * nk;
+    const int nterms = kind == IMAGO_EP ? h.pot_dim : 1;
+    if (kind == IMAGO_EP) RCHK(ep_prepare(c));
+    double *ext = nullptr, *T = nullptr, *X = nullptr;
+    RCHK(dev_alloc(c, &ext, (size_t)per_ext));
+    RCHK(dev_alloc(c, &T, (size_t)dsz * cd * n * nk));
+    if (kind == IMAGO_OL) {
+        if (!c->core_S) RCHK(dev_alloc(c, &c->core_S, dn));
+        X = c->core_S;
+    } else {
+        RCHK(dev_alloc(c, &X, dn));
+    }
+    const double *S = c->core_S;
+    const rocblas_stride sN = (rocblas_stride)N * N, sT = (rocblas_stride)cd * n;
+    const rocblas_operation opN = rocblas_operation_none,
+                            opC = rocblas_operation_conjugate_transpose;
+    // offsets (elements) of the blocks in a dense N x N column-major matrix
+    const size_t cv = (size_t)n, cc = (size_t)n + (size_t)n * N;
+    auto at = [&](const double *base, size_t el) { return base + (size_t)dsz * el; };
+    for (int term = 0; term < nterms; ++term) {
+        // term_stride 0: the term's matrix lands at ext itself
+        RCHK(launch_assemble(c, kind, ext, 0, term, 1, true));
+        HIPCHK(launch_core_unpack(ext, c->nbe_ext, N, nk, cx, c->pairs_ext, c->fn_atom, c->fn_state,
+                                  c->pair_index, c->atom_nstates_ext, na, X, c->stream));
+        // T = Xcc Scv; Xvv += Scv^H T; Xvv -= Scv^H Xcv; Xvv -= Xcv^H Scv
+        BLASCHK(gemm_sb(c->blas, cx, opN, opN, cd, n, cd, 1.0, at(X, cc), N, sN, at(S, cv), N, sN,
+                        0.0, T, cd, sT, nk));
+        BLASCHK(gemm_sb(c->blas, cx, opC, opN, n, n, cd, 1.0, at(S, cv), N, sN, T, cd, sT, 1.0, X,
+                        N, sN, nk));
+        BLASCHK(gemm_sb(c->blas, cx, opC, opN, n, n, cd, -1.0, at(S, cv), N, sN, at(X, cv), N, sN,
+                        1.0, X, N, sN, nk));
+        BLASCHK(gemm_sb(c->blas, cx, opC, opN, n, n, cd, -1.0, at(X, cv), N, sN, at(S, cv), N, sN,
+                        1.0, X, N, sN, nk));
+        HIPCHK(launch_core_pack(X, N, n, nk, cx, c->pairs, c->nbe, c->atom_of_state, c->pair_index,
+                                c->d.cumul, na, c->panel[kind - 1] + (size_t)term * per,
+                                c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, ext);
+    dev_release(c, T);
+    if (kind != IMAGO_OL) dev_release(c, X);
+    return IMAGO_OK;
 }
 
 // The Cholesky factor S = U^H U of a batch, blocked right-looking as
@@ -3653,6 +3818,9 @@ extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
 {
     if (!c || !c->has_system) return fail(IMAGO_E_STATE, "imago_scf_setup before imago_set_system");
     if (!params || !mesh || !covalent_radius) return fail(IMAGO_E_ARG, "null argument");
+    if (c->h.core_dim > 0)
+        return fail(IMAGO_E_ARG, "the SCF update has no core charge (makeCoreRho): systems with "
+                                 "core states run the integral and secular stages only");
     if (params->xc_code != 100)
         return fail(IMAGO_E_ARG, "xc_code %d: only 100 (Wigner, spin 1, LDA) is built",
                     params->xc_code);

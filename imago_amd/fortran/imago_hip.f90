@@ -37,6 +37,7 @@ module O_ImagoHIP
       type(c_ptr) :: kpoints, kpoint_weights
       integer(c_int) :: gamma
       real(c_double) :: log_basis_fn_thresh
+      type(c_ptr) :: type_num_core_states
    end type imago_system_desc
 
    ! readExchCorrMeshParameters input block (exchCorr.f90:876-913)

@@ -97,6 +97,7 @@ program scf_driver
    sys%kpoint_weights = c_loc(kPointWeight)
    sys%gamma = gam
    sys%log_basis_fn_thresh = dhdr(1)
+   sys%type_num_core_states = c_null_ptr
 
    ! setupSCF: the integrals
    call initImagoHIP(0)

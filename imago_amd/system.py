@@ -54,18 +54,22 @@ def onsite_radial_norm(alphas: np.ndarray, coeffs: np.ndarray, l: int) -> float:
 class AtomType:
     """One atom type: an ascending exponent list shared by all shells and a
     list of contracted shells (l, radial coefficients over the first
-    numOrbAlphas(l+1) exponents)."""
+    numOrbAlphas(l+1) exponents).  core_shells are the type's core
+    functions, which the integral stage orthogonalises the valence
+    functions against (intgOrtho.F90); the reference stores each type's
+    valence states first, then its core states (basis.f90:160-243)."""
     element: int
     alphas: np.ndarray
     shells: list            # [(l, coeffs)] in valence order
     num_orb_alphas: tuple = (0, 0, 0, 0)
+    core_shells: list = field(default_factory=list)
 
     def __post_init__(self):
         self.alphas = np.asarray(self.alphas, dtype=np.float64)
         if np.any(np.diff(self.alphas) <= 0):
             raise ValueError("atom-type exponents must be ascending")
         if self.num_orb_alphas == (0, 0, 0, 0):
-            lmax = max(l for l, _ in self.shells)
+            lmax = max(l for l, _ in list(self.shells) + list(self.core_shells))
             n = len(self.alphas)
             self.num_orb_alphas = tuple(n if l <= lmax else 0 for l in range(4))
         noa = self.num_orb_alphas
@@ -76,11 +80,23 @@ class AtomType:
     def num_states(self) -> int:
         return sum(2 * l + 1 for l, _ in self.shells)
 
+    @property
+    def num_core_states(self) -> int:
+        return sum(2 * l + 1 for l, _ in self.core_shells)
+
     def states(self):
-        """(lmIndex 1..16, basis column over alphas) per state, basis.f90:160-243."""
+        """(lmIndex 1..16, basis column over alphas) per valence state,
+        basis.f90:160-243."""
+        return self._states(self.shells)
+
+    def core_states(self):
+        """The same for the core states."""
+        return self._states(self.core_shells)
+
+    def _states(self, shells):
         out = []
         na = len(self.alphas)
-        for l, c in self.shells:
+        for l, c in shells:
             nal = self.num_orb_alphas[l]
             for k in range(1, 2 * l + 2):
                 lm = l * l + k
@@ -326,6 +342,20 @@ class System:
         return int(sum(len(p.alphas) for p in self.pot_types))
 
     @property
+    def core_dim(self) -> int:
+        return int(sum(self.atom_types[t].num_core_states for t in self.atom_type_of))
+
+    def all_valence(self) -> "System":
+        """The system with every type's core shells appended to its valence
+        shells: its matrices hold, per atom, the valence then the core
+        functions (the extended blocks the orthogonalisation starts from)."""
+        from dataclasses import replace
+        types = [AtomType(element=t.element, alphas=t.alphas,
+                          shells=list(t.shells) + list(t.core_shells),
+                          num_orb_alphas=t.num_orb_alphas) for t in self.atom_types]
+        return replace(self, atom_types=types, name=self.name + "_all_valence")
+
+    @property
     def num_kpoints(self) -> int:
         return 1 if self.gamma else len(self.kpoints)
 
@@ -354,19 +384,22 @@ class System:
     def desc_arrays(self) -> dict:
         nt = len(self.atom_types)
         maxa = max(len(t.alphas) for t in self.atom_types)
-        maxs = max(t.num_states for t in self.atom_types)
+        maxs = max(t.num_states + t.num_core_states for t in self.atom_types)
         type_alphas = np.zeros((nt, maxa))
         type_basis = np.zeros((nt, maxs, maxa))
         type_lm = np.zeros((nt, maxs), dtype=np.int32)
         noa = np.zeros((nt, 4), dtype=np.int32)
         nst = np.zeros(nt, dtype=np.int32)
+        ncore = np.zeros(nt, dtype=np.int32)
         el = np.zeros(nt, dtype=np.int32)
         for ti, t in enumerate(self.atom_types):
             type_alphas[ti, :len(t.alphas)] = t.alphas
             noa[ti] = t.num_orb_alphas
             nst[ti] = t.num_states
+            ncore[ti] = t.num_core_states
             el[ti] = t.element
-            for si, (lm, col) in enumerate(t.states()):
+            # valence states, then core states
+            for si, (lm, col) in enumerate(t.states() + t.core_states()):
                 type_lm[ti, si] = lm
                 type_basis[ti, si, :len(col)] = col
         npt = len(self.pot_types)
@@ -381,6 +414,7 @@ class System:
         return dict(
             type_element=el, type_num_orb_alphas=noa, type_alphas=type_alphas,
             type_num_states=nst, type_basis=type_basis, type_lm_index=type_lm,
+            type_num_core_states=ncore,
             max_alphas=maxa, max_states=maxs,
             atom_type=self.atom_type_of, atom_pos=self.atom_pos,
             pot_type_element=np.array([p.element for p in self.pot_types], np.int32),

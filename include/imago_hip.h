@@ -73,7 +73,9 @@ typedef struct imago_system_desc {
     const double *type_alphas;    /* [num_atom_types][max_alphas] ascending */
     const int *type_num_states;   /* [num_atom_types] valence states        */
     const double *type_basis;     /* [num_atom_types][max_states][max_alphas]
-                                     radial coefficient * angNorm(lm)      */
+                                     radial coefficient * angNorm(lm); each
+                                     type's valence states, then its core
+                                     states (type_num_core_states)         */
     const int *type_lm_index;     /* [num_atom_types][max_states] 1..16     */
 
     /* atom sites (atomicSites.f90) */
@@ -108,6 +110,13 @@ typedef struct imago_system_desc {
     int gamma;                    /* 1: real Gamma-only build (imagoG)      */
 
     double log_basis_fn_thresh;   /* -ln(basisFnConvg) (lattice.f90:239)    */
+
+    /* core states (numCoreStates, basis.f90:160-243), NULL when there are
+     * none (coreDim = 0).  With core states the assembled matrices are
+     * orthogonalised against them as orthoOL / ortho do (integrals.F90:
+     * 3608-3910, intgOrtho.F90:23-213): IMAGO_OL must be assembled before
+     * the other kinds, IMAGO_EP_DIRECT is refused. */
+    const int *type_num_core_states; /* [num_atom_types]                    */
 } imago_system_desc;
 
 /*
@@ -161,7 +170,10 @@ int imago_boys_batch(imago_ctx *ctx, int n, const double *T, double *F7);
  * fused accumulation of the reference's post-SCF path, integrals.F90:
  * 2860-3605), any number of k-points.  When both are assembled the
  * stored panels are used.
- * coreDim must be 0, so orthoOL/ortho reduce to packing (intgOrtho.F90). */
+ * With core states (type_num_core_states) every kind is orthogonalised
+ * against them as orthoOL / ortho do (integrals.F90:3608-3910,
+ * intgOrtho.F90): IMAGO_OL first, then the others; IMAGO_EP_DIRECT is
+ * refused.  Without them orthoOL / ortho reduce to packing. */
 int imago_assemble(imago_ctx *ctx, int kind);
 
 /* Copy one assembled packed matrix to the host: kind, term (EP only, else
