@@ -138,40 +138,27 @@ class PotType:
 
 def _merge_sort_order(data: np.ndarray) -> np.ndarray:
     """Order produced by mergeSort (sortSubs.f90:18-177) with one element per
-    initial segment: bottom-up merges of adjacent runs; on ties the element
-    of the second run is taken first (the comparison is strict '<')."""
-    n = len(data)
-    order = list(range(n))
-    runs = [[i] for i in range(n)]
-    while len(runs) > 1:
-        merged = []
-        for r in range(0, len(runs) - 1, 2):
-            a, b = runs[r], runs[r + 1]
-            ia = ib = 0
-            out = []
-            while ia < len(a) and ib < len(b):
-                if data[a[ia]] < data[b[ib]]:
-                    out.append(a[ia]); ia += 1
-                else:
-                    out.append(b[ib]); ib += 1
-            out.extend(a[ia:]); out.extend(b[ib:])
-            merged.append(out)
-        if len(runs) % 2:
-            merged.append(runs[-1])
-        runs = merged
-    return np.array(runs[0] if runs else order, dtype=np.int64)
+    initial segment: bottom-up merges of adjacent runs in which, on equal
+    keys, the element of the second run is taken first (the comparison is
+    strict '<').  Two equal keys first meet when their runs merge, and the
+    later one then goes first and stays first, so among equal keys the
+    order is by descending position: a stable sort on (key, -position)."""
+    data = np.asarray(data)
+    return np.lexsort((-np.arange(len(data)), data)).astype(np.int64)
 
 
-def _cell_border_check(vectors, dims, negl) -> bool:
-    """cellBorderCheck (lattice.f90:700-770)."""
+def _cell_border_hits(vectors, dims, negl) -> np.ndarray:
+    """cellBorderCheck (lattice.f90:700-770) for every row of dims: the
+    point is kept if one of the seven corner moves of the cell (by the sum
+    of the lattice vectors, one axis at a time) comes within sqrt(negl) of
+    the origin."""
     shift = vectors.sum(axis=1)
-    t = dims.copy()
-    seq = [(0, +1), (1, +1), (0, -1), (2, +1), (0, +1), (1, -1), (0, -1)]
-    for ax, sg in seq:
-        t[ax] += sg * shift[ax]
-        if np.sum(t * t) < negl:
-            return True
-    return False
+    t = np.array(dims, dtype=np.float64, copy=True)
+    hit = np.zeros(len(t), dtype=bool)
+    for ax, sg in [(0, +1), (1, +1), (0, -1), (2, +1), (0, +1), (1, -1), (0, -1)]:
+        t[:, ax] += sg * shift[ax]
+        hit |= (t[:, 0] * t[:, 0] + t[:, 1] * t[:, 1] + t[:, 2] * t[:, 2]) < negl
+    return hit
 
 
 def negligible_limit_real(log_thresh: float, min_atomic_alpha: float,
@@ -205,30 +192,27 @@ def reciprocal_vectors(real_vectors: np.ndarray):
 
 def make_lattice(real_vectors: np.ndarray, negl: float):
     """Sorted real-space cell list (makeLattice, lattice.f90:571-696).
-    real_vectors: 3x3 with a,b,c as columns.  Returns (dims[n,3], sizes[n])."""
-    reps = []
-    for i in range(3):
-        reps.append(2 * math.ceil(math.sqrt(negl) /
-                                  math.sqrt(np.sum(real_vectors[:, i] ** 2))) + 1)
-    dims, sizes = [], []
-    for i in range(1, reps[0] + 1):
-        r0 = i - (1 + reps[0]) // 2
-        for j in range(1, reps[1] + 1):
-            r1 = j - (1 + reps[1]) // 2
-            for k in range(1, reps[2] + 1):
-                r2 = k - (1 + reps[2]) // 2
-                d = np.zeros(3)
-                for l, r in enumerate((r0, r1, r2)):
-                    for m in range(3):
-                        d[m] = d[m] + real_vectors[m, l] * r
-                s = float(np.sum(d ** 2))
-                if s > negl and not _cell_border_check(real_vectors, d, negl):
-                    continue
-                dims.append(d)
-                sizes.append(s)
-    sizes = np.array(sizes)
+    real_vectors: 3x3 with a,b,c as columns.  Returns (dims[n,3], sizes[n]).
+    The candidate points are the reference's repetition box in its loop
+    order (i, then j, then k), each position accumulated over the three
+    vectors in the reference's order; a point is kept when |R|^2 <= negl
+    or cellBorderCheck keeps it, then sorted as mergeSort sorts."""
+    rv = np.asarray(real_vectors, dtype=np.float64)
+    reps = [2 * math.ceil(math.sqrt(negl) / math.sqrt(np.sum(rv[:, i] ** 2))) + 1
+            for i in range(3)]
+    axes = [np.arange(1, reps[a] + 1) - (1 + reps[a]) // 2 for a in range(3)]
+    r = [g.ravel() for g in np.meshgrid(*axes, indexing="ij")]
+    d = np.zeros((len(r[0]), 3))
+    for l in range(3):
+        for m in range(3):
+            d[:, m] = d[:, m] + rv[m, l] * r[l]
+    sizes = d[:, 0] ** 2 + d[:, 1] ** 2 + d[:, 2] ** 2
+    far = sizes > negl
+    keep = ~far
+    keep[far] = _cell_border_hits(rv, d[far], negl)
+    dims, sizes = d[keep], sizes[keep]
     order = _merge_sort_order(sizes)
-    return np.array(dims)[order], sizes[order]
+    return dims[order], sizes[order]
 
 
 # ---------------------------------------------------------------------------

@@ -27,6 +27,72 @@ def test_lattice_sorted_and_complete():
                     assert tuple(np.round(R, 9)) in have
 
 
+def _lattice_by_loops(real_vectors, negl):
+    """makeLattice and cellBorderCheck (lattice.f90:571-770) loop by loop,
+    with mergeSort's bottom-up merges (sortSubs.f90:18-177) written out."""
+    reps = [2 * math.ceil(math.sqrt(negl) / math.sqrt(np.sum(real_vectors[:, i] ** 2))) + 1
+            for i in range(3)]
+    shift = real_vectors.sum(axis=1)
+    dims, sizes = [], []
+    for i in range(1, reps[0] + 1):
+        for j in range(1, reps[1] + 1):
+            for k in range(1, reps[2] + 1):
+                rr = (i - (1 + reps[0]) // 2, j - (1 + reps[1]) // 2, k - (1 + reps[2]) // 2)
+                d = np.zeros(3)
+                for l, r in enumerate(rr):
+                    for m in range(3):
+                        d[m] = d[m] + real_vectors[m, l] * r
+                sz = float(np.sum(d ** 2))
+                if sz > negl:
+                    t = d.copy()
+                    border = False
+                    for ax, sg in [(0, 1), (1, 1), (0, -1), (2, 1), (0, 1), (1, -1), (0, -1)]:
+                        t[ax] += sg * shift[ax]
+                        if np.sum(t * t) < negl:
+                            border = True
+                            break
+                    if not border:
+                        continue
+                dims.append(d)
+                sizes.append(sz)
+    runs = [[q] for q in range(len(sizes))]
+    while len(runs) > 1:
+        merged = []
+        for r in range(0, len(runs) - 1, 2):
+            a, b = runs[r], runs[r + 1]
+            ia = ib = 0
+            out = []
+            while ia < len(a) and ib < len(b):
+                if sizes[a[ia]] < sizes[b[ib]]:
+                    out.append(a[ia])
+                    ia += 1
+                else:
+                    out.append(b[ib])
+                    ib += 1
+            merged.append(out + a[ia:] + b[ib:])
+        if len(runs) % 2:
+            merged.append(runs[-1])
+        runs = merged
+    order = runs[0]
+    return np.array(dims)[order], np.array(sizes)[order]
+
+
+@pytest.mark.parametrize("case", ["skewed", "si2_real", "si64_recip"])
+def test_lattice_equals_the_loop_restatement_bit_for_bit(case):
+    from imago_amd.system import reciprocal_vectors
+    if case == "skewed":
+        rv, negl = np.array([[6.0, 0.4, 0.0], [0.0, 5.5, 0.3], [0.2, 0.0, 7.0]]).T, 150.0
+    elif case == "si2_real":
+        s = configs.si2_primitive(mesh=(1, 1, 1))
+        rv, negl = s.real_vectors, 400.0
+    else:
+        s = configs.si64_supercell(mesh=(1, 1, 1))
+        rv, negl = reciprocal_vectors(s.real_vectors)[0], 4.0 * s.log_elec_thresh * s.min_pot_alpha
+    d1, s1 = make_lattice(rv, negl)
+    d2, s2 = _lattice_by_loops(rv, negl)
+    assert np.array_equal(d1, d2) and np.array_equal(s1, s2)
+
+
 def test_merge_sort_ties_take_second_run_first():
     # mergeLists compares with a strict '<': on equal keys the element of
     # the second run is emitted first (sortSubs.f90:120-177)
