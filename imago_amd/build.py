@@ -30,7 +30,8 @@ HIP_FLAGS = [
 
 SOURCES = ["imago_hip.hip", "prim_batch.hip", "asm_ol.hip", "asm_ke.hip", "asm_np.hip",
            "asm_ep.hip", "asm_epd.hip", "hetrd.hip", "elecstat.hip",
-           "xcmesh.hip", "learned.hip", "tridiag_dc.hip", "dos.hip", "core_ortho.hip"]
+           "xcmesh.hip", "learned.hip", "tridiag_dc.hip", "dos.hip", "core_ortho.hip",
+           "gemm_mfma.hip"]
 
 
 def _compile(src: str, verbose: bool) -> str:

@@ -31,6 +31,7 @@
 #include "assemble_launch.hpp"
 #include "geom_kernel.hpp"
 #include "hetrd_launch.hpp"
+#include "gemm_mfma_launch.hpp"
 #include "tridiag_dc_launch.hpp"
 #include "dos_launch.hpp"
 #include "core_ortho_launch.hpp"
@@ -178,6 +179,10 @@ struct imago_ctx {
     // secular buffers
     double2 *Hfull = nullptr, *Sfull = nullptr;
     double *Hreal = nullptr, *Sreal = nullptr;
+    // U^-1 of every k-point's Cholesky factor (upper triangle; made with the
+    // factor) and an n x n scratch per k-point for the products around the
+    // eigensolve, typed like Sfull / Sreal
+    void *Sinv = nullptr, *Gtmp = nullptr;
     double *evals = nullptr, *ework = nullptr;
     int *info = nullptr;
     double *coef = nullptr;
@@ -306,6 +311,7 @@ struct imago_ctx {
         char *V = nullptr, *Tf = nullptr;
         void *A = nullptr;
         const void *B = nullptr;
+        const void *Binv = nullptr;   // U^-1 when known (the solve is then a GEMM)
     } pend;
     // makeSCFPot state (imago_scf_setup / imago_scf_update): the
     // electrostatic operators on the device, the per-iteration vectors on
@@ -322,6 +328,10 @@ struct imago_ctx {
         bool have_hist = false;
         std::vector<double> xl1, xl2, yl1, yl2;
     } scf;
+    // partial sums of the row-form matrix-vector product (dev_gemv),
+    // grown on demand, held with the system's buffers
+    double *gv_part = nullptr;
+    size_t gv_cap = 0;
 };
 
 static void ctx_free_system(imago_ctx *c)
@@ -366,6 +376,7 @@ static void ctx_free_system(imago_ctx *c)
     c->cand_off = nullptr;
     c->Hfull = c->Sfull = nullptr;
     c->Hreal = c->Sreal = nullptr;
+    c->Sinv = c->Gtmp = nullptr;
     c->evals = c->ework = nullptr;
     c->info = nullptr;
     c->coef = nullptr;
@@ -384,6 +395,8 @@ static void ctx_free_system(imago_ctx *c)
     for (auto &m : c->spd) m = imago_ctx::SpdMat{};
     c->scf = imago_ctx::ScfState{};
     c->pend = imago_ctx::PendingVectors{};
+    c->gv_part = nullptr;
+    c->gv_cap = 0;
     c->has_system = false;
 }
 
@@ -410,12 +423,23 @@ static int dev_alloc(imago_ctx *c, T **p, size_t count)
 
 static void dev_release(imago_ctx *c, void *p) { c->sysbuf.release_one(p); }
 
+// A blocking copy on the context's own stream.  The library never uses the
+// legacy default stream: a copy there would wait on (and, under a graph
+// capture, invalidate) the streams of other contexts in the process.
+static hipError_t ctx_memcpy(imago_ctx *c, void *dst, const void *src, size_t bytes,
+                             hipMemcpyKind kind)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
+
 template <class T>
 static int dev_upload(imago_ctx *c, const T **dst, const std::vector<T> &v)
 {
     T *p = nullptr;
     RCHK(dev_alloc(c, &p, v.size()));
-    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) HIPCHK(ctx_memcpy(c, p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     *dst = p;
     return IMAGO_OK;
 }
@@ -908,10 +932,10 @@ static int build_geometry(imago_ctx *c)
     RCHK(dev_alloc(c, &c->pairs, pairs.size()));
     RCHK(dev_alloc(c, &c->imgs, imgs.size()));
     RCHK(dev_alloc(c, &c->phase, phase.size()));
-    HIPCHK(hipMemcpy(c->pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+    HIPCHK(ctx_memcpy(c, c->pairs, pairs.data(), pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
     if (!imgs.empty()) {
-        HIPCHK(hipMemcpy(c->imgs, imgs.data(), imgs.size() * sizeof(ImgRec), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->phase, phase.data(), phase.size() * sizeof(double2), hipMemcpyHostToDevice));
+        HIPCHK(ctx_memcpy(c, c->imgs, imgs.data(), imgs.size() * sizeof(ImgRec), hipMemcpyHostToDevice));
+        HIPCHK(ctx_memcpy(c, c->phase, phase.data(), phase.size() * sizeof(double2), hipMemcpyHostToDevice));
     }
     RCHK(build_candidates(c, img_pair));
     return IMAGO_OK;
@@ -955,7 +979,7 @@ static int build_core_tables(imago_ctx *c)
         }
     }
     RCHK(dev_alloc(c, &c->pairs_ext, pe.size()));
-    HIPCHK(hipMemcpy(c->pairs_ext, pe.data(), pe.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+    HIPCHK(ctx_memcpy(c, c->pairs_ext, pe.data(), pe.size() * sizeof(PairRec), hipMemcpyHostToDevice));
     RCHK(dev_upload(c, &c->fn_atom, fa));
     RCHK(dev_upload(c, &c->fn_state, fs));
     RCHK(dev_upload(c, &c->atom_nstates_ext, ans));
@@ -1465,7 +1489,7 @@ extern "C" int imago_get_packed(imago_ctx *c, int kind, int term, int kpoint, do
     const long len = dsz * c->nbe;
     const double *src = c->panel[kind - 1] + (size_t)term * per + (size_t)kpoint * len;
     std::vector<double> blk(len);
-    HIPCHK(hipMemcpy(blk.data(), src, len * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_memcpy(c, blk.data(), src, len * sizeof(double), hipMemcpyDeviceToHost));
     // block layout -> packed upper triangle, column by column (matrixSubs.F90:506)
     for (const PairRec &pr : c->hpairs) {
         const int ns1 = h.nstates[h.atom_type[pr.i]], ns2 = h.nstates[h.atom_type[pr.j]];
@@ -1632,6 +1656,9 @@ static int ensure_secular_buffers(imago_ctx *c)
         if (!c->Hfull) RCHK(dev_alloc(c, &c->Hfull, (size_t)nn * h.nk));
         if (!c->Sfull) RCHK(dev_alloc(c, &c->Sfull, (size_t)nn * h.nk));
     }
+    const size_t mat_bytes = (size_t)nn * (h.gamma ? sizeof(double) : sizeof(double2) * h.nk);
+    if (!c->Sinv) RCHK(dev_alloc(c, (char **)&c->Sinv, mat_bytes));
+    if (!c->Gtmp) RCHK(dev_alloc(c, (char **)&c->Gtmp, mat_bytes));
     return IMAGO_OK;
 }
 
@@ -1870,7 +1897,7 @@ static int dc_ensure_plan(imago_ctx *c, int n)
     const size_t bytes = std::max<size_t>(p.host.size(), 1) * sizeof(int);
     HIPCHK(hipMalloc(&p.dev, bytes));
     if (!p.host.empty())
-        HIPCHK(hipMemcpy(p.dev, p.host.data(), p.host.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(ctx_memcpy(c, p.dev, p.host.data(), p.host.size() * sizeof(int), hipMemcpyHostToDevice));
     c->dcplan = std::move(p);
     return IMAGO_OK;
 }
@@ -1888,6 +1915,10 @@ static int dc_graph_launch(imago_ctx *c, int n, int nb, double *evals, double *e
                       g.e == ework && g.X == A && g.plan == c->dcplan.dev &&
                       g.work_bytes >= bytes;
     if (!same) {
+        // one capture at a time in the process, with no allocation or free
+        // of another context's graph work area during it
+        static std::mutex capture_mu;
+        std::lock_guard<std::mutex> capture_lock(capture_mu);
         if (g.exec || g.work) HIPCHK(hipStreamSynchronize(c->stream));
         if (g.exec) {
             HIPCHK(hipGraphExecDestroy(g.exec));
@@ -2074,7 +2105,7 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         std::vector<int> kinfo(nb, 0);
         map_stedc_info(n, h0, hs[0], kinfo.data());
         if (h1 > 0) map_stedc_info(n, h1, hs[1], kinfo.data() + h0);
-        HIPCHK(hipMemcpy(dinfo, kinfo.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+        HIPCHK(ctx_memcpy(c, dinfo, kinfo.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
     }
     return IMAGO_OK;
 }
@@ -2126,7 +2157,26 @@ static int finish_columns(imago_ctx *c, int upto)
     }
     HIPCHK(hipEventRecord(c->evs[5], c->stream));
     const rocblas_stride nns = (rocblas_stride)nn;
-    if (cx) {
+    if (p.Binv) {
+        // X = U^-1 Y on the matrix cores (U^-1 upper: the K range of row
+        // tile i starts at its first row) into the scratch, then back
+        GemmArgs g;
+        g.M = n;
+        g.N = N;
+        g.K = n;
+        g.lda = g.ldb = g.ldc = n;
+        g.batch = nb;
+        g.A = p.Binv;
+        g.sa1 = (long)nn;
+        g.B = Ac;
+        g.sb1 = (long)nn;
+        g.C = c->Gtmp;
+        g.sc1 = (long)n * N;
+        g.amask = kTriUpper;
+        HIPCHK(launch_gemm_mfma(g, cx, false, false, c->stream));
+        HIPCHK(hipMemcpy2DAsync(Ac, nn * es, c->Gtmp, (size_t)n * N * es, (size_t)n * N * es, nb,
+                                hipMemcpyDeviceToDevice, c->stream));
+    } else if (cx) {
         const rocblas_double_complex one(1.0, 0.0);
         BLASCHK(rocblas_ztrsm_strided_batched(
             c->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
@@ -2149,12 +2199,43 @@ static int finish_columns(imago_ctx *c, int upto)
     return IMAGO_OK;
 }
 
+// C = U^-H H U^-1 from the explicit inverse on the matrix cores: G = H U^-1
+// (U^-1 upper: the K range of column tile j ends at its last column), then
+// the lower tiles of C = U^-H G (U^-H lower: the K range of row tile i ends
+// at its last row), each written to both triangles with a real diagonal; H
+// is overwritten with C.
+static int gemm_transform(imago_ctx *c, bool cx, int n, int nb, void *H, const void *Uinv)
+{
+    const long nn = (long)n * n;
+    GemmArgs g;
+    g.M = g.N = g.K = n;
+    g.lda = g.ldb = g.ldc = n;
+    g.batch = nb;
+    g.A = H;
+    g.sa1 = nn;
+    g.B = Uinv;
+    g.sb1 = nn;
+    g.C = c->Gtmp;
+    g.sc1 = nn;
+    g.bmask = kTriUpper;
+    HIPCHK(launch_gemm_mfma(g, cx, false, false, c->stream));
+    GemmArgs g2 = g;
+    g2.A = Uinv;
+    g2.B = c->Gtmp;
+    g2.C = H;
+    g2.amask = kTriLower;
+    g2.bmask = kTriNone;
+    g2.herm = true;
+    HIPCHK(launch_gemm_mfma(g2, cx, true, false, c->stream));
+    return IMAGO_OK;
+}
+
 // C = U^-H A U^-1, its eigenpairs, and the eigenvectors X = U^-1 Y of the
 // generalized problem: all columns when defer is false; with defer the
 // columns are left to finish_columns.
 static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, const void *B,
                              double *evals, double *ework, int *dinfo, int *hinfo,
-                             bool defer = false)
+                             bool defer = false, const void *Binv = nullptr)
 {
     if (c->pend.active || c->pend.V || c->pend.Tf) {
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -2168,13 +2249,17 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         const rocblas_double_complex one(1.0, 0.0);
         auto *a = (rocblas_double_complex *)A;
         auto *b = (const rocblas_double_complex *)B;
-        BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
-                                              rocblas_operation_conjugate_transpose,
-                                              rocblas_diagonal_non_unit, n, n, &one, b, n, nn, a, n,
-                                              nn, nb));
-        BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
-                                              rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
-                                              &one, b, n, nn, a, n, nn, nb));
+        if (Binv) {
+            RCHK(gemm_transform(c, true, n, nb, A, Binv));
+        } else {
+            BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
+                                                  rocblas_operation_conjugate_transpose,
+                                                  rocblas_diagonal_non_unit, n, n, &one, b, n, nn,
+                                                  a, n, nn, nb));
+            BLASCHK(rocblas_ztrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
+                                                  rocblas_operation_none, rocblas_diagonal_non_unit,
+                                                  n, n, &one, b, n, nn, a, n, nn, nb));
+        }
         HIPCHK(hipEventRecord(c->evs[1], c->stream));
         const int rc = use_coop_heevd(c, n, true) ? heevd_coop(c, true, n, nb, A, evals, ework, dinfo)
                                                : kCoopRefused;
@@ -2189,12 +2274,17 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
         const double one = 1.0;
         auto *a = (double *)A;
         auto *b = (const double *)B;
-        BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
-                                              rocblas_operation_transpose, rocblas_diagonal_non_unit,
-                                              n, n, &one, b, n, nn, a, n, nn, nb));
-        BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
-                                              rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
-                                              &one, b, n, nn, a, n, nn, nb));
+        if (Binv) {
+            RCHK(gemm_transform(c, false, n, nb, A, Binv));
+        } else {
+            BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_upper,
+                                                  rocblas_operation_transpose,
+                                                  rocblas_diagonal_non_unit, n, n, &one, b, n, nn,
+                                                  a, n, nn, nb));
+            BLASCHK(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_right, rocblas_fill_upper,
+                                                  rocblas_operation_none, rocblas_diagonal_non_unit,
+                                                  n, n, &one, b, n, nn, a, n, nn, nb));
+        }
         HIPCHK(hipEventRecord(c->evs[1], c->stream));
         const int rc = use_coop_heevd(c, n, false)
                            ? heevd_coop(c, false, n, nb, A, evals, ework, dinfo)
@@ -2217,6 +2307,7 @@ static int solve_gv_factored(imago_ctx *c, bool cx, int n, int nb, void *A, cons
     p.done = 0;
     p.A = A;
     p.B = B;
+    p.Binv = Binv;
     if (!defer) RCHK(finish_columns(c, n));
     HIPCHK(hipMemcpyAsync(hinfo, dinfo, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2318,12 +2409,25 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
         HIPCHK(hipGetLastError());
         RCHK(factor_gv_batched(c, cx, h.n, nb, Sm, c->info, inf.data(), &bad));
         c->chol_valid = !bad;
+        // U^-1 once per factor: every later transform and solve is a GEMM
+        if (!bad) {
+            const rocblas_stride snn = (rocblas_stride)nn;
+            if (cx)
+                BLASCHK(rocblas_ztrtri_strided_batched(
+                    c->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, h.n,
+                    (const rocblas_double_complex *)Sm, h.n, snn,
+                    (rocblas_double_complex *)c->Sinv, h.n, snn, nb));
+            else
+                BLASCHK(rocblas_dtrtri_strided_batched(
+                    c->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, h.n,
+                    (const double *)Sm, h.n, snn, (double *)c->Sinv, h.n, snn, nb));
+        }
     }
     // the eigenvector columns are finished by the consumer (the density
     // needs the occupied ones only)
     if (!bad)
         RCHK(solve_gv_factored(c, cx, h.n, nb, Hm, Sm, c->evals, c->ework, c->info, inf.data(),
-                               true));
+                               true, c->Sinv));
     HIPCHK(hipEventRecord(e3, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms1 = 0, ms2 = 0;
@@ -2339,7 +2443,7 @@ extern "C" int imago_secular(imago_ctx *c, const double *pot_coeffs, int num_sta
     c->timing[1] = ms1;
     c->timing[2] = ms2;
     std::vector<double> ev((size_t)h.nk * h.n);
-    HIPCHK(hipMemcpy(ev.data(), c->evals, sizeof(double) * ev.size(), hipMemcpyDeviceToHost));
+    HIPCHK(ctx_memcpy(c, ev.data(), c->evals, sizeof(double) * ev.size(), hipMemcpyDeviceToHost));
     if (info_out)
         for (int k = 0; k < h.nk; ++k) info_out[k] = inf[k];
     if (eigenvalues)
@@ -2376,9 +2480,9 @@ extern "C" int imago_get_eigenvectors(imago_ctx *c, int kpoint, double *out)
     const long nn = (long)c->h.n * c->h.n;
     const long cols = (long)c->h.n * c->num_states;
     if (c->h.gamma)
-        HIPCHK(hipMemcpy(out, c->Hreal, sizeof(double) * cols, hipMemcpyDeviceToHost));
+        HIPCHK(ctx_memcpy(c, out, c->Hreal, sizeof(double) * cols, hipMemcpyDeviceToHost));
     else
-        HIPCHK(hipMemcpy(out, c->Hfull + (size_t)kpoint * nn, sizeof(double2) * cols,
+        HIPCHK(ctx_memcpy(c, out, c->Hfull + (size_t)kpoint * nn, sizeof(double2) * cols,
                          hipMemcpyDeviceToHost));
     return IMAGO_OK;
 }
@@ -2598,6 +2702,24 @@ __global__ __launch_bounds__(256) void k_trace_final_wide(const double *__restri
     if (threadIdx.x == 0) out[t] = red[0];
 }
 
+// rho = W W^H for every k-point on the matrix cores: the lower 64 x 64
+// tiles, each written to both triangles (W is n x nocc per k-point)
+static int density_gemm(imago_ctx *c, bool cx, int n, int nocc, int nk, const double *W)
+{
+    GemmArgs g;
+    g.M = g.N = n;
+    g.K = nocc;
+    g.lda = g.ldb = g.ldc = n;
+    g.batch = nk;
+    g.A = g.B = W;
+    g.sa1 = g.sb1 = (long)n * nocc;
+    g.C = c->rho;
+    g.sc1 = (long)n * n;
+    g.herm = true;
+    HIPCHK(launch_gemm_mfma(g, cx, false, true, c->stream));
+    return IMAGO_OK;
+}
+
 extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, double *traces3,
                                     double *pot_rho)
 {
@@ -2644,16 +2766,9 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
     } else if (cx) {
         // rho = W W^H as a full product: rocBLAS's zherk runs at ~1 TFLOP/s
         // here (n = 896, k = 130), its zgemm near the FP64 matrix rate
-        const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-        const rocblas_stride sw = (rocblas_stride)h.n * std::max(nocc, 1);
-        BLASCHK(rocblas_zgemm_strided_batched(
-            c->blas, rocblas_operation_none, rocblas_operation_conjugate_transpose, h.n, h.n, nocc,
-            &one, (const rocblas_double_complex *)W, h.n, sw, (const rocblas_double_complex *)W,
-            h.n, sw, &zero, (rocblas_double_complex *)c->rho, h.n, (rocblas_stride)nn, h.nk));
+        RCHK(density_gemm(c, true, h.n, nocc, h.nk, W));
     } else {
-        const double one = 1.0, zero = 0.0;
-        BLASCHK(rocblas_dsyrk(c->blas, rocblas_fill_upper, rocblas_operation_none, h.n, nocc, &one,
-                              W, h.n, &zero, c->rho, h.n));
+        RCHK(density_gemm(c, false, h.n, nocc, 1, W));
     }
     if (cx)
         hipLaunchKernelGGL(k_pack_weighted<true>, dim3(c->npairs, h.nk), dim3(256), 0, c->stream,
@@ -2718,7 +2833,7 @@ extern "C" int imago_density_traces(imago_ctx *c, const double *occupations, dou
         c->timing[9] = ms9;
     }
     std::vector<double> res(nterms);
-    HIPCHK(hipMemcpy(res.data(), outd, sizeof(double) * nterms, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_memcpy(c, res.data(), outd, sizeof(double) * nterms, hipMemcpyDeviceToHost));
     if (traces3)
         for (int q = 0; q < 3; ++q) traces3[q] = res[q];
     if (pot_rho)
@@ -2768,8 +2883,8 @@ static int single_gv(imago_ctx *c, bool cx, int n, double *A, double *B, double 
     HIPCHK(hipMalloc(&dw, n * sizeof(double)));
     HIPCHK(hipMalloc(&de, n * sizeof(double)));
     HIPCHK(hipMalloc(&di, sizeof(int)));
-    HIPCHK(hipMemcpy(dA, A, nn * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dB, B, nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_memcpy(c, dA, A, nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx_memcpy(c, dB, B, nn * sizeof(double), hipMemcpyHostToDevice));
     {
         const unsigned g = (unsigned)(((long)n * n + 255) / 256);
         if (cx)
@@ -2783,9 +2898,9 @@ static int single_gv(imago_ctx *c, bool cx, int n, double *A, double *B, double 
     // B returns the factor in its upper triangle and, as in LAPACK, its
     // strictly lower triangle as it came in (the device copy's is overwritten)
     std::vector<double> hB(nn);
-    if (rc == IMAGO_OK && hipMemcpy(A, dA, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
-        hipMemcpy(hB.data(), dB, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
-        hipMemcpy(w, dw, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
+    if (rc == IMAGO_OK && ctx_memcpy(c, A, dA, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+        ctx_memcpy(c, hB.data(), dB, nn * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+        ctx_memcpy(c, w, dw, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
         const int es = cx ? 2 : 1;
         for (int col = 0; col < n; ++col)
             for (int row = 0; row <= col; ++row)
@@ -2831,7 +2946,7 @@ extern "C" int imago_ep_counts(imago_ctx *c, long *tuples, long *items, int *blo
     if (partial_doubles) {
         long w = 0;
         if (c->ep_runs)
-            HIPCHK(hipMemcpy(&w, c->hrun_woff + c->npairs, sizeof(long), hipMemcpyDeviceToHost));
+            HIPCHK(ctx_memcpy(c, &w, c->hrun_woff + c->npairs, sizeof(long), hipMemcpyDeviceToHost));
         *partial_doubles = w;
     }
     return IMAGO_OK;
@@ -3095,7 +3210,7 @@ extern "C" int imago_residual_q(imago_ctx *c, double log_elec_thresh, int num_ce
     RCHK(residual_q_run(c, log_elec_thresh, num_cells_recip, cell_dims_recip, cell_sizes_recip,
                         &out));
     const size_t nout = (size_t)c->h.npt * c->h.pot_dim;
-    HIPCHK(hipMemcpy(non_local_residual_q, out, nout * 8, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_memcpy(c, non_local_residual_q, out, nout * 8, hipMemcpyDeviceToHost));
     dev_release(c, out);
     return IMAGO_OK;
 }
@@ -3620,6 +3735,152 @@ __global__ __launch_bounds__(256) void k_spd_rows(double *__restrict__ B,
     B[g] *= s[g % n];
 }
 
+// Matrix-vector products of the potential-side operators (potDim x potDim,
+// a few right-hand sides), spread over the whole device: rocBLAS runs these
+// shapes as GEMMs or row-block GEMVs on 16 work-groups (35-60 us for 8 MB).
+// Every sum has a fixed order, so results do not depend on timing.
+__device__ __forceinline__ double gv_wave_sum(double v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// y(j, r) = alpha * sum_i M(i, j) x(i, r) + beta * y(j, r) for an m x n
+// column-major M (leading dimension ld) and R right-hand sides: one wave per
+// column, lanes over rows
+template <int R>
+__global__ __launch_bounds__(256) void k_gemv_cols(const double *__restrict__ M, int m, int n,
+                                                   int ld, const double *__restrict__ x, int ldx,
+                                                   double alpha, double beta,
+                                                   double *__restrict__ y, int ldy)
+{
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (j >= n) return;
+    const double *col = M + (size_t)j * ld;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    int i = lane;
+    for (; i + 192 < m; i += 256) {
+        double a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = col[i + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = fma(a[u], x[i + 64 * u + (size_t)r * ldx], acc[r]);
+    }
+    for (; i < m; i += 64) {
+        const double a = col[i];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = fma(a, x[i + (size_t)r * ldx], acc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = gv_wave_sum(acc[r]);
+    if (lane == 0)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double *yr = y + j + (size_t)r * ldy;
+            *yr = beta == 0.0 ? alpha * acc[r] : alpha * acc[r] + beta * *yr;
+        }
+}
+
+// part(i, r, chunk) = sum over the chunk's columns j of M(i, j) x(j, r):
+// one thread per row, column chunks of cw columns along blockIdx.y
+template <int R>
+__global__ __launch_bounds__(256) void k_gemv_rows_part(const double *__restrict__ M, int m,
+                                                        int n, int ld,
+                                                        const double *__restrict__ x, int ldx,
+                                                        int cw, double *__restrict__ part)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x, ch = blockIdx.y;
+    if (i >= m) return;
+    const int j0 = ch * cw, j1 = min(n, j0 + cw);
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double a = M[i + (size_t)j * ld];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = fma(a, x[j + (size_t)r * ldx], acc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) part[((size_t)ch * R + r) * m + i] = acc[r];
+}
+
+// y(i, r) = alpha * sum over chunks (in order) of part(i, r, chunk) + beta * y(i, r)
+template <int R>
+__global__ __launch_bounds__(256) void k_gemv_rows_sum(const double *__restrict__ part, int m,
+                                                       int nch, double alpha, double beta,
+                                                       double *__restrict__ y, int ldy)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double s = 0.0;
+        for (int ch = 0; ch < nch; ++ch) s += part[((size_t)ch * R + r) * m + i];
+        double *yr = y + i + (size_t)r * ldy;
+        *yr = beta == 0.0 ? alpha * s : alpha * s + beta * *yr;
+    }
+}
+
+template <int R>
+static hipError_t gemv_launch(bool trans, int m, int n, const double *M, int ld, const double *x,
+                              int ldx, double alpha, double beta, double *y, int ldy,
+                              double *part, int cw, hipStream_t st)
+{
+    if (trans) {
+        hipLaunchKernelGGL(k_gemv_cols<R>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, M, m,
+                           n, ld, x, ldx, alpha, beta, y, ldy);
+    } else {
+        const int nch = (n + cw - 1) / cw;
+        hipLaunchKernelGGL(k_gemv_rows_part<R>, dim3((unsigned)((m + 255) / 256), (unsigned)nch),
+                           dim3(256), 0, st, M, m, n, ld, x, ldx, cw, part);
+        hipLaunchKernelGGL(k_gemv_rows_sum<R>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                           part, m, nch, alpha, beta, y, ldy);
+    }
+    return hipGetLastError();
+}
+
+// y = alpha op(M) x + beta y on the device (op(M) = M^T when trans) for
+// nrhs <= 4 right-hand sides; x, y device arrays (leading dimensions ldx,
+// ldy), M m x n column-major with leading dimension ld
+static int dev_gemv(imago_ctx *c, bool trans, int m, int n, const double *M, int ld,
+                    const double *x, int ldx, int nrhs, double alpha, double beta, double *y,
+                    int ldy)
+{
+    if (nrhs < 1 || nrhs > 4) return fail(IMAGO_E_ARG, "dev_gemv: %d right-hand sides", nrhs);
+    if (m <= 0 || n <= 0) return IMAGO_OK;
+    constexpr int cw = 32;
+    const int nch = (n + cw - 1) / cw;
+    double *part = nullptr;
+    if (!trans) {
+        const size_t need = (size_t)nch * nrhs * m;
+        if (need > c->gv_cap) {
+            if (c->gv_part) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                dev_release(c, c->gv_part);
+                c->gv_part = nullptr;
+                c->gv_cap = 0;
+            }
+            RCHK(dev_alloc(c, &c->gv_part, need));
+            c->gv_cap = need;
+        }
+        part = c->gv_part;
+    }
+    hipError_t e = hipSuccess;
+    switch (nrhs) {
+    case 1: e = gemv_launch<1>(trans, m, n, M, ld, x, ldx, alpha, beta, y, ldy, part, cw, c->stream); break;
+    case 2: e = gemv_launch<2>(trans, m, n, M, ld, x, ldx, alpha, beta, y, ldy, part, cw, c->stream); break;
+    case 3: e = gemv_launch<3>(trans, m, n, M, ld, x, ldx, alpha, beta, y, ldy, part, cw, c->stream); break;
+    default: e = gemv_launch<4>(trans, m, n, M, ld, x, ldx, alpha, beta, y, ldy, part, cw, c->stream); break;
+    }
+    HIPCHK(e);
+    return IMAGO_OK;
+}
+
 // The equilibration of dposvx('E') (dpoequ / dlaqsy): s_i = 1/sqrt(a_ii),
 // applied when min(s)/max(s) < 0.1 or the largest diagonal is outside
 // [tiny/eps, eps/tiny]; then the Cholesky factor of the (scaled) upper
@@ -3717,15 +3978,26 @@ extern "C" int imago_pot_solve(imago_ctx *c, int matrix, int nrhs, double *b, in
     HIPCHK(hipMemcpyAsync(bs, b, nb * 8, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, bs, m.s, n, nrhs);
     HIPCHK(hipGetLastError());
+    // Ainv and Af hold both triangles of symmetric matrices, so up to four
+    // right-hand sides go through the column-dot product M^T x = M x
+    const bool gv = nrhs <= 4;
     const double one = 1.0, mone = -1.0, zero = 0.0;
-    BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, nrhs, n,
-                          &one, m.Ainv, n, bs, n, &zero, x, n));
+    if (gv)
+        RCHK(dev_gemv(c, true, n, n, m.Ainv, n, bs, n, nrhs, 1.0, 0.0, x, n));
+    else
+        BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, nrhs, n,
+                              &one, m.Ainv, n, bs, n, &zero, x, n));
     for (int it = 0; it < 2; ++it) {
         HIPCHK(hipMemcpyAsync(r, bs, nb * 8, hipMemcpyDeviceToDevice, c->stream));
-        BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, nrhs,
-                              n, &mone, m.Af, n, x, n, &one, r, n));
-        BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, nrhs,
-                              n, &one, m.Ainv, n, r, n, &one, x, n));
+        if (gv) {
+            RCHK(dev_gemv(c, true, n, n, m.Af, n, x, n, nrhs, -1.0, 1.0, r, n));
+            RCHK(dev_gemv(c, true, n, n, m.Ainv, n, r, n, nrhs, 1.0, 1.0, x, n));
+        } else {
+            BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n,
+                                  nrhs, n, &mone, m.Af, n, x, n, &one, r, n));
+            BLASCHK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n,
+                                  nrhs, n, &one, m.Ainv, n, r, n, &one, x, n));
+        }
     }
     hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, x, m.s, n, nrhs);
     HIPCHK(hipGetLastError());
@@ -3852,8 +4124,8 @@ extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
     S.neut_l = es + 2 * PP;
     S.nuc_nl.resize(P);
     S.nuc_l.resize(P);
-    HIPCHK(hipMemcpy(S.nuc_nl.data(), es + 3 * PP, (size_t)P * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(S.nuc_l.data(), es + 3 * PP + P, (size_t)P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_memcpy(c, S.nuc_nl.data(), es + 3 * PP, (size_t)P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(ctx_memcpy(c, S.nuc_l.data(), es + 3 * PP + P, (size_t)P * 8, hipMemcpyDeviceToHost));
     RCHK(residual_q_run(c, params->log_elec_thresh, num_cells_recip, cell_dims_recip,
                         cell_sizes_recip, &S.resq));
     // exchange-correlation mesh, overlap and operator blocks
@@ -3870,18 +4142,19 @@ extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
     return IMAGO_OK;
 }
 
-// y = op(M) x on the device for an m x n column-major M (trans: M^T x);
-// x and y are host vectors
+// y_r = op(M) x_r on the device for an m x n column-major M (trans:
+// M^T x) and nrhs <= 4 vectors; x (nrhs vectors of length lx back to back)
+// and y (nrhs of length ly) are host arrays
 static int scf_gemv(imago_ctx *c, bool trans, int m, int n, const double *M, const double *x,
-                    double *y)
+                    double *y, int nrhs = 1)
 {
-    double *dx = c->scf.vec, *dy = c->scf.vec + 4 * (size_t)c->h.pot_dim;
     const int lx = trans ? m : n, ly = trans ? n : m;
-    const double one = 1.0, zero = 0.0;
-    HIPCHK(hipMemcpyAsync(dx, x, (size_t)lx * 8, hipMemcpyHostToDevice, c->stream));
-    BLASCHK(rocblas_dgemv(c->blas, trans ? rocblas_operation_transpose : rocblas_operation_none,
-                          m, n, &one, M, m, dx, 1, &zero, dy, 1));
-    HIPCHK(hipMemcpyAsync(y, dy, (size_t)ly * 8, hipMemcpyDeviceToHost, c->stream));
+    if (nrhs < 1 || nrhs > 4 || (size_t)nrhs * std::max(lx, ly) > 4 * (size_t)c->h.pot_dim)
+        return fail(IMAGO_E_ARG, "scf_gemv: %d vectors of %d", nrhs, std::max(lx, ly));
+    double *dx = c->scf.vec, *dy = c->scf.vec + 4 * (size_t)c->h.pot_dim;
+    HIPCHK(hipMemcpyAsync(dx, x, (size_t)lx * nrhs * 8, hipMemcpyHostToDevice, c->stream));
+    RCHK(dev_gemv(c, trans, m, n, M, m, dx, lx, nrhs, 1.0, 0.0, dy, ly));
+    HIPCHK(hipMemcpyAsync(y, dy, (size_t)ly * nrhs * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return IMAGO_OK;
 }
@@ -4042,8 +4315,12 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
             d1[i] = rl0[i] - (S.yl1[i] - S.xl1[i]);
             d2[i] = rl0[i] - (S.yl2[i] - S.xl2[i]);
         }
-        RCHK(scf_gemv(c, false, P, P, c->spd[0].A, d1.data(), u1.data()));
-        RCHK(scf_gemv(c, false, P, P, c->spd[0].A, d2.data(), u2.data()));
+        std::vector<double> dd(2 * (size_t)P), uu(2 * (size_t)P);
+        std::copy(d1.begin(), d1.end(), dd.begin());
+        std::copy(d2.begin(), d2.end(), dd.begin() + P);
+        RCHK(scf_gemv(c, false, P, P, c->spd[0].A, dd.data(), uu.data(), 2));
+        std::copy(uu.begin(), uu.begin() + P, u1.begin());
+        std::copy(uu.begin() + P, uu.end(), u2.begin());
         const double s11 = dot(u1.data(), d1.data()), t1 = dot(u1.data(), rl0.data());
         const double s12 = dot(u2.data(), d1.data()), s22 = dot(u2.data(), d2.data());
         const double t2 = dot(u2.data(), rl0.data());

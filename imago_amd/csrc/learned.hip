@@ -400,10 +400,15 @@ extern "C" int imago_mlp_create(imago_ctx *c, int num_layers, const int *widths,
     hipError_t e = hipSuccess;
     for (float **q : {&m->p, &m->g, &m->m, &m->v})
         if (e == hipSuccess) e = hipMalloc(q, sizeof(float) * m->np);
-    if (e == hipSuccess) e = hipMemcpy(m->p, p.data(), sizeof(float) * m->np, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(m->m, 0, sizeof(float) * m->np);
-    if (e == hipSuccess) e = hipMemset(m->v, 0, sizeof(float) * m->np);
-    if (e == hipSuccess) e = hipMemset(m->g, 0, sizeof(float) * m->np);
+    // on the context's stream: the legacy default stream would wait on the
+    // streams of other contexts in the process
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(m->p, p.data(), sizeof(float) * m->np, hipMemcpyHostToDevice,
+                           m->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(m->m, 0, sizeof(float) * m->np, m->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(m->v, 0, sizeof(float) * m->np, m->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(m->g, 0, sizeof(float) * m->np, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     if (e != hipSuccess) {
         for (float *q : {m->p, m->g, m->m, m->v})
             if (q) (void)hipFree(q);

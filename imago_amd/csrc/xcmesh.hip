@@ -282,9 +282,15 @@ __global__ __launch_bounds__(256) void k_xc_point_pot(const double *__restrict__
     const int b = pt_slot[p], m = count[b], q = p - slot_first[b];
     const double *Bb = B + (size_t)b * lda * ldn + q;
     const int *rb = rows + (size_t)b * pot_dim;
+    // each operator value is read once for both sums (each sum keeps its
+    // own order, term by term)
     double core = 0.0, total = 0.0;
-    for (int c = 0; c < m; ++c) core = core + Bb[(size_t)c * ldn] * rho_core[rb[c]];
-    for (int c = 0; c < m; ++c) total = total + Bb[(size_t)c * ldn] * rho_total[rb[c]];
+    for (int c = 0; c < m; ++c) {
+        const double op = Bb[(size_t)c * ldn];
+        const int row = rb[c];
+        core = core + op * rho_core[row];
+        total = total + op * rho_total[row];
+    }
     if (core < 0.0) core = 0.0;
     if (total < 0.0) total = 0.0;
     double pot, en;

@@ -37,6 +37,9 @@ def test_stream_matches_one_context(gpu_ctx):
     results, report = stream.run(cells, workers=2, model=None, tuner=tune.Autotuner(),
                                  max_iter=80)
     assert report["cells"] == 6
+    for rec in results:
+        if rec and rec.get("error"):
+            print("[gpu] %s failed: %s" % (rec["name"], rec["error"]), flush=True)
     db = guess.AtomicPotentials()
     for cell, rec in zip(cells, results):
         if not rec.get("converged"):
