@@ -53,7 +53,7 @@ struct TileRange {
     bool ok;
 };
 
-__device__ __forceinline__ TileRange tile_of(const GemmArgs &g, int idx)
+__device__ __forceinline__ TileRange tile_of(const GemmArgs &g, int idx, int inner)
 {
     const int tm = (g.M + kGT - 1) / kGT, tn = (g.N + kGT - 1) / kGT;
     int ti, tj;
@@ -70,7 +70,7 @@ __device__ __forceinline__ TileRange tile_of(const GemmArgs &g, int idx)
     t.i0 = ti * kGT;
     t.j0 = tj * kGT;
     t.ok = ti < tm && tj < tn;
-    int klo = 0, khi = g.K;
+    int klo = inner * g.koff2, khi = g.K;
     if (g.amask == kTriUpper) klo = max(klo, t.i0);
     if (g.amask == kTriLower) khi = min(khi, t.i0 + kGT);
     if (g.bmask == kTriUpper) khi = min(khi, t.j0 + kGT);
@@ -87,9 +87,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
     using T = typename E::T;
     __shared__ double As[CX ? 2 : 1][kGK][kGLd];
     __shared__ double Bs[CX ? 2 : 1][kGK][kGLd];
-    const TileRange tr = tile_of(g, blockIdx.x);
-    if (!tr.ok) return;
     const int bi = blockIdx.y, outer = bi / g.nb2, inner = bi % g.nb2;
+    const TileRange tr = tile_of(g, blockIdx.x, inner);
+    if (!tr.ok) return;
     const T *A = static_cast<const T *>(g.A) + outer * g.sa1 + inner * g.sa2;
     const T *B = static_cast<const T *>(g.B) + outer * g.sb1 + inner * g.sb2;
     T *C = static_cast<T *>(g.C) + outer * g.sc1 + inner * g.sc2;

@@ -32,6 +32,9 @@ struct GemmArgs {
     double alpha_re = 1.0, alpha_im = 0.0, beta_re = 0.0, beta_im = 0.0;
     int amask = kTriNone, bmask = kTriNone;
     bool herm = false;
+    // the K range of entry (outer, inner) starts at inner * koff2 (the
+    // operands are zero before it)
+    int koff2 = 0;
 };
 
 // complex: double2 elements (interleaved re, im), else double; conj_a /

@@ -203,6 +203,71 @@ __global__ __launch_bounds__(256) void k_trd_larft(const T *__restrict__ tau, in
     for (int e = threadIdx.x; e < K * K; e += 256) Tg[e] = Ts[e % K][e / K];
 }
 
+// The UT form of Q = H(0) ... H(n-2) = I - V T V^H for all n - 1
+// reflectors at once: T^-1 = diag(1 / tau) + the strict upper triangle of
+// G = V^H V (the zlarft recurrence T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i),
+// T(i, i) = tau_i, inverted column by column).  A reflector with tau = 0 is
+// the identity (zhetd2 leaves v = e_{j+1} then): its column of V is zeroed
+// first and its diagonal entry of T^-1 set to 1, so it adds nothing.
+template <class T>
+__global__ __launch_bounds__(256) void k_ut_null(const T *__restrict__ tau, int n,
+                                                 T *__restrict__ V)
+{
+    const int m = blockIdx.y, j = blockIdx.x;
+    const T t = tau[(size_t)m * n + j];
+    if (sc_re(t) != 0.0 || sc_im(t) != 0.0) return;
+    T *col = V + (size_t)m * n * n + (size_t)j * n;
+    for (int r = threadIdx.x; r < n; r += 256) col[r] = sc_zero<T>();
+}
+
+// in place of G (order n - 1, leading dimension n): the strict upper
+// triangle kept, 1 / tau on the diagonal, zero below
+template <class T>
+__global__ __launch_bounds__(256) void k_ut_tinv(const T *__restrict__ tau, int n,
+                                                 T *__restrict__ G)
+{
+    const int m = blockIdx.y, n1 = n - 1;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (size_t)n1 * n1) return;
+    const int i = (int)(e % n1), j = (int)(e / n1);
+    if (j > i) return;
+    T *g = G + (size_t)m * n * n + (size_t)j * n + i;
+    if (j < i) {
+        *g = sc_zero<T>();
+        return;
+    }
+    const T t = tau[(size_t)m * n + i];
+    const double tr = sc_re(t), ti = sc_im(t), d = tr * tr + ti * ti;
+    *g = d == 0.0 ? sc_one<T>() : sc_make<T>(tr / d, -ti / d);
+}
+
+hipError_t launch_ut_null(const void *tau, int n, int nb, bool real, void *V, hipStream_t stream)
+{
+    if (n < 2 || nb <= 0) return hipSuccess;
+    const dim3 grid((unsigned)(n - 1), (unsigned)nb);
+    if (real)
+        hipLaunchKernelGGL(k_ut_null<double>, grid, dim3(256), 0, stream,
+                           static_cast<const double *>(tau), n, static_cast<double *>(V));
+    else
+        hipLaunchKernelGGL(k_ut_null<double2>, grid, dim3(256), 0, stream,
+                           static_cast<const double2 *>(tau), n, static_cast<double2 *>(V));
+    return hipGetLastError();
+}
+
+hipError_t launch_ut_tinv(const void *tau, int n, int nb, bool real, void *G, hipStream_t stream)
+{
+    if (n < 2 || nb <= 0) return hipSuccess;
+    const size_t n1 = (size_t)(n - 1);
+    const dim3 grid((unsigned)((n1 * n1 + 255) / 256), (unsigned)nb);
+    if (real)
+        hipLaunchKernelGGL(k_ut_tinv<double>, grid, dim3(256), 0, stream,
+                           static_cast<const double *>(tau), n, static_cast<double *>(G));
+    else
+        hipLaunchKernelGGL(k_ut_tinv<double2>, grid, dim3(256), 0, stream,
+                           static_cast<const double2 *>(tau), n, static_cast<double2 *>(G));
+    return hipGetLastError();
+}
+
 hipError_t launch_trd_larft(const void *tau, int n, int nb, bool real, void *T, hipStream_t stream)
 {
     const int nblocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;

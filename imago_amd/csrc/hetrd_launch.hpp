@@ -56,6 +56,15 @@ hipError_t launch_trd_unmix(const double *Z, const double *wall, int n, int nb, 
 // read) and T on exit.
 hipError_t launch_trd_larft(const void *tau, int n, int nb, bool real, void *T, hipStream_t stream);
 
+// The UT form of all n - 1 reflectors of every matrix, Q = I - V T V^H with
+// T^-1 = diag(1 / tau) + strict upper triangle of V^H V: launch_ut_null
+// zeroes the columns of V whose tau is 0 (identity reflectors), and after
+// the caller has formed G = V^H V (order n - 1, leading dimension n, stride
+// n * n), launch_ut_tinv turns G into T^-1 in place (zero below the
+// diagonal, 1 / tau on it, 1 where tau = 0).
+hipError_t launch_ut_null(const void *tau, int n, int nb, bool real, void *V, hipStream_t stream);
+hipError_t launch_ut_tinv(const void *tau, int n, int nb, bool real, void *G, hipStream_t stream);
+
 // Per-matrix eigensolver codes from the divide and conquer of one half of
 // a batch.  The half's nh matrices of order n were solved as one
 // block-diagonal problem of order N = nh * n; a nonzero dstedc info names

@@ -288,6 +288,13 @@ struct imago_ctx {
     // for the graph's lifetime, which is keyed by the order, the batch and
     // the buffers it reads and writes
     int tune_graph = 1;
+    // the back-transform of the reduction's reflectors: 1 (the default) in
+    // compact-WY blocks of kTrdBlock, the Gram blocks on the matrix cores and
+    // the block products with rocBLAS; 0 all n - 1 reflectors at once in the
+    // UT form (one Gram GEMM and T = (T^-1)^-1 per eigensolve, three GEMMs
+    // per finished column range on the matrix cores; IMAGO_BACKTRANSFORM =
+    // ut), slower at the workloads' orders because of the triangular inverse
+    int tune_wy = 1;
     struct DcGraph {
         hipGraphExec_t exec = nullptr;
         int n = 0, nb = 0;
@@ -306,6 +313,7 @@ struct imago_ctx {
     struct PendingVectors {
         bool active = false;
         bool reflectors = false;   // false: the back-transform is done, only U^-1 left
+        bool ut = false;           // Tf holds the UT factor of all reflectors (n x n each)
         bool cx = true;
         int n = 0, nb = 0, done = 0;
         char *V = nullptr, *Tf = nullptr;
@@ -1029,6 +1037,8 @@ extern "C" int imago_ctx_create(int device, imago_ctx **out)
         c->tune_tridiag = (e3 && strcmp(e3, "rocsolver") == 0) ? 1 : 0;
         const char *e4 = getenv("IMAGO_GRAPH");
         c->tune_graph = (e4 && strcmp(e4, "0") == 0) ? 0 : 1;
+        const char *e5 = getenv("IMAGO_BACKTRANSFORM");
+        c->tune_wy = (e5 && strcmp(e5, "ut") == 0) ? 0 : 1;
     }
     *out = c;
     return IMAGO_OK;
@@ -1988,7 +1998,8 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         RCHK(dev_alloc(c, &rank, (size_t)N));
     }
     RCHK(dev_alloc(c, &tau, (size_t)n * nb * es));
-    RCHK(dev_alloc(c, &Tf, (size_t)nb * std::max(nblk, 1) * K * K * es));
+    const bool ut = c->tune_wy == 0;
+    RCHK(dev_alloc(c, &Tf, ut ? nn * nb * es : (size_t)nb * std::max(nblk, 1) * K * K * es));
     RCHK(dev_alloc(c, &vbuf, nn * nb * es));
     RCHK(dev_alloc(c, &pbuf, nn * nb * es));
     RCHK(dev_alloc(c, &dbuf, (size_t)n * 128 * nb * es));
@@ -2061,24 +2072,71 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
                                 err, c->stream));
     }
     HIPCHK(hipEventRecord(c->evs[2], c->stream));
-    const rocblas_stride sw = (rocblas_stride)K * n,
-                         st = (rocblas_stride)std::max(nblk, 1) * K * K;
-    // G_b = V_b^H V_b for every block of each matrix (one strided-batched
-    // GEMM per matrix over all n rows), then the factors T_b in place of G_b;
-    // the columns a last block reads beyond n only reach G entries the
-    // factor ignores
-    for (int m = 0; m < nb && nblk > 0; ++m) {
-        const char *Vm = V + (size_t)m * nn * es;
-        BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_conjugate_transpose, rocblas_operation_none,
-                        K, K, n, 1.0, Vm, n, sw, Vm, n, sw, 0.0, Tf + (size_t)m * st * es, K,
-                        (rocblas_stride)K * K, nblk));
+    if (ut && n > 1) {
+        // G = V^H V over the n - 1 reflectors (V^H upper, V lower: the K range
+        // of tile (i, j) starts at max(i0, j0); only the lower tiles, mirrored),
+        // T^-1 from it in place, then T = (T^-1)^-1
+        char *G = nullptr;
+        RCHK(dev_alloc(c, &G, nn * nb * es));
+        HIPCHK(launch_ut_null(tau, n, nb, !cx, V, c->stream));
+        GemmArgs g;
+        g.M = g.N = n - 1;
+        g.K = n;
+        g.lda = g.ldb = g.ldc = n;
+        g.batch = nb;
+        g.A = g.B = V;
+        g.sa1 = g.sb1 = g.sc1 = (long)nn;
+        g.C = G;
+        g.amask = kTriUpper;
+        g.bmask = kTriLower;
+        g.herm = true;
+        HIPCHK(launch_gemm_mfma(g, cx, true, false, c->stream));
+        HIPCHK(launch_ut_tinv(tau, n, nb, !cx, G, c->stream));
+        const rocblas_stride snn = (rocblas_stride)nn;
+        if (cx)
+            BLASCHK(rocblas_ztrtri_strided_batched(
+                c->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, n - 1,
+                (const rocblas_double_complex *)G, n, snn, (rocblas_double_complex *)Tf, n, snn,
+                nb));
+        else
+            BLASCHK(rocblas_dtrtri_strided_batched(c->blas, rocblas_fill_upper,
+                                                   rocblas_diagonal_non_unit, n - 1,
+                                                   (const double *)G, n, snn, (double *)Tf, n, snn,
+                                                   nb));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dev_release(c, G);
+    } else if (!ut) {
+        const rocblas_stride sw = (rocblas_stride)K * n,
+                             st = (rocblas_stride)std::max(nblk, 1) * K * K;
+        // G_b = V_b^H V_b for every block of every matrix in one launch on the
+        // matrix cores (V_b is zero above row K b, so its K range starts
+        // there), then the factors T_b in place of G_b; the columns a last
+        // block reads beyond n only reach G entries the factor ignores
+        if (nblk > 0) {
+            GemmArgs g;
+            g.M = g.N = K;
+            g.K = n;
+            g.lda = g.ldb = n;
+            g.ldc = K;
+            g.batch = nb * nblk;
+            g.nb2 = nblk;
+            g.A = g.B = V;
+            g.sa1 = g.sb1 = (long)nn;
+            g.sa2 = g.sb2 = (long)sw;
+            g.C = Tf;
+            g.sc1 = (long)st;
+            g.sc2 = (long)K * K;
+            g.koff2 = K;
+            HIPCHK(launch_gemm_mfma(g, cx, true, false, c->stream));
+        }
+        HIPCHK(launch_trd_larft(tau, n, nb, !cx, Tf, c->stream));
     }
-    HIPCHK(launch_trd_larft(tau, n, nb, !cx, Tf, c->stream));
     HIPCHK(hipEventRecord(c->evs[3], c->stream));
     // the reflectors and their factors stay for the back-transform
     c->pend.V = V;
     c->pend.Tf = Tf;
     c->pend.reflectors = true;
+    c->pend.ut = ut;
     V = nullptr;
     Tf = nullptr;
     int herr = 0, hs[2] = {0, 0};
@@ -2132,7 +2190,67 @@ static int finish_columns(imago_ctx *c, int upto)
     char *Ac = static_cast<char *>(p.A) + (size_t)c0 * n * es;
     RCHK(ensure_stage_events(c));
     HIPCHK(hipEventRecord(c->evs[4], c->stream));
-    if (p.reflectors) {
+    if (p.reflectors && p.ut) {
+        // Y = (I - V T V^H) Y: W = V^H Y (V^H upper: row tile i from row i0),
+        // W2 = T W (T upper), Y -= V W2 (V lower: row tile i up to its last row)
+        if (n > 1) {
+            const int n1 = n - 1;
+            char *W = nullptr, *W2 = nullptr;
+            RCHK(dev_alloc(c, &W, (size_t)nb * n1 * N * es));
+            RCHK(dev_alloc(c, &W2, (size_t)nb * n1 * N * es));
+            GemmArgs g;
+            g.M = n1;
+            g.N = N;
+            g.K = n;
+            g.lda = n;
+            g.ldb = n;
+            g.ldc = n1;
+            g.batch = nb;
+            g.A = p.V;
+            g.sa1 = (long)nn;
+            g.B = Ac;
+            g.sb1 = (long)nn;
+            g.C = W;
+            g.sc1 = (long)n1 * N;
+            g.amask = kTriUpper;
+            HIPCHK(launch_gemm_mfma(g, cx, true, false, c->stream));
+            GemmArgs g2;
+            g2.M = n1;
+            g2.N = N;
+            g2.K = n1;
+            g2.lda = n;
+            g2.ldb = g2.ldc = n1;
+            g2.batch = nb;
+            g2.A = p.Tf;
+            g2.sa1 = (long)nn;
+            g2.B = W;
+            g2.sb1 = (long)n1 * N;
+            g2.C = W2;
+            g2.sc1 = (long)n1 * N;
+            g2.amask = kTriUpper;
+            HIPCHK(launch_gemm_mfma(g2, cx, false, false, c->stream));
+            GemmArgs g3;
+            g3.M = n;
+            g3.N = N;
+            g3.K = n1;
+            g3.lda = n;
+            g3.ldb = n1;
+            g3.ldc = n;
+            g3.batch = nb;
+            g3.A = p.V;
+            g3.sa1 = (long)nn;
+            g3.B = W2;
+            g3.sb1 = (long)n1 * N;
+            g3.C = Ac;
+            g3.sc1 = (long)nn;
+            g3.alpha_re = -1.0;
+            g3.beta_re = 1.0;
+            g3.amask = kTriLower;
+            HIPCHK(launch_gemm_mfma(g3, cx, false, false, c->stream));
+            dev_release(c, W);
+            dev_release(c, W2);
+        }
+    } else if (p.reflectors) {
         const int K = kTrdBlock, nblk = (n - 1 + K - 1) / K;
         char *W = nullptr, *W2 = nullptr;
         RCHK(dev_alloc(c, &W, (size_t)nb * K * N * es));
