@@ -112,6 +112,44 @@ def electrons(system):
                          for t in system.atom_type_of)))
 
 
+def eigensolve_kernels(system, st, ncols):
+    """Rooflines of the eigensolve's stages from their HIP-event times (ms in
+    st) at order n, nk matrices, ncols finished eigenvector columns.
+
+    k_hetrd_coop, the Householder tridiagonalisation, is half BLAS2: LAPACK's
+    zhetrd reads the trailing matrix once per column for the Hermitian
+    matrix-vector product (zlatrd), so its algorithmic traffic is
+    sum_{m<n} m^2 elements per matrix (16 B complex, 8 B real), and its
+    roofline is HBM bandwidth; its LAWN-41 flops (16/3 n^3 complex, 4/3 n^3
+    real) are reported beside it against the FP64 peak.  The dense stages on
+    the matrix cores are priced with LAWN-41 flops against the FP64 matrix
+    peak: the Cholesky transform as zhegst / dsygst (4 n^3 / n^3), the
+    solve X = U^-1 Y as ztrsm (4 n^2 ncols / n^2 ncols), the back-transform
+    as zunmtr (8 n^2 ncols / 2 n^2 ncols)."""
+    n, nk = system.vale_dim, (1 if system.gamma else system.num_kpoints)
+    cx = 1.0 if system.gamma else 4.0
+    es = 8.0 if system.gamma else 16.0
+    out = {}
+    if st.get("k_hetrd_coop", 0.0) > 0.0:
+        ms = st["k_hetrd_coop"]
+        nbytes = es * (n - 1) * n * (2 * n - 1) / 6.0 * nk
+        flop = (4.0 / 3.0 if system.gamma else 16.0 / 3.0) * n ** 3 * nk
+        out["k_hetrd_coop"] = {
+            "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "ms": ms, "bytes": nbytes,
+            "achieved": nbytes / (ms * 1e-3) / 1e9,
+            "fp64_flop": flop, "fp64_tflops": flop / (ms * 1e-3) / 1e12,
+            "fp64_frac": flop / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+    for name, key, flop in (
+            ("cholesky_transform", "cholesky_transform", cx * n ** 3 * nk),
+            ("vector_solve", "vector_solve", cx * n ** 2 * ncols * nk),
+            ("back_transform", "back_transform", 2.0 * cx * n ** 2 * ncols * nk)):
+        ms = st.get(key, 0.0)
+        if ms > 0.0:
+            out[name] = {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "ms": ms,
+                         "flop": flop, "achieved": flop / (ms * 1e-3) / 1e12}
+    return out
+
+
 def gpu_step(ctx, system, coeffs, num_states, oracle_populate, ep_kind):
     t0 = time.perf_counter()
     ctx.set_system(system)
@@ -128,12 +166,15 @@ def gpu_step(ctx, system, coeffs, num_states, oracle_populate, ep_kind):
     stage["hbuild"] = ctx.timing(1)
     stage["eigensolve"] = ctx.timing(2)
     stage["k_hetrd_coop"] = ctx.timing(10)
+    stage["cholesky_transform"] = ctx.timing(18)
     if ep_kind == "ep_direct":
         stage["k_epd_hsum"] = ctx.timing(8)
     t3 = time.perf_counter()
     occ = oracle_populate(ev, system.kpoint_weights, electrons(system))
     tr, prho = ctx.density_traces(occ)
     stage["traces"] = ctx.timing(3)
+    stage["back_transform"] = ctx.timing(20)
+    stage["vector_solve"] = ctx.timing(21)
     if ep_kind == "ep_direct":
         stage["k_epd_trace"] = ctx.timing(9)
     t4 = time.perf_counter()
@@ -599,13 +640,8 @@ def bench_scf(args, system, coeffs, ctx, log):
           "tridiagonal_dc": ctx.timing(19), "wy_factors": ctx.timing(22),
           "back_transform": ctx.timing(20), "vector_solve": ctx.timing(21),
           "traces": ctx.timing(3), "xc_potential": ctx.timing(16)}
-    kernels = {}
-    if st["k_hetrd_coop"] > 0.0:
-        kernels["k_hetrd_coop"] = {
-            "bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "ms": st["k_hetrd_coop"],
-            "flop": (4.0 / 3.0 if system.gamma else 16.0 / 3.0) * n ** 3 * nk}
-    for v in kernels.values():
-        v["achieved"] = v["flop"] / (v["ms"] * 1e-3) / 1e12
+    # occupied columns the traces finish each iteration (spin 1, no smearing)
+    kernels = eigensolve_kernels(system, st, (electrons(system) + 1) // 2)
     roofline = None
     if "k_hetrd_coop" in kernels:   # absent when rocSOLVER's eigensolver ran
         dom = "k_hetrd_coop"
@@ -613,7 +649,10 @@ def bench_scf(args, system, coeffs, ctx, log):
         roofline = {"kernel": dom, "bound": d["bound"], "achieved": round(d["achieved"], 3),
                     "peak": d["peak"], "unit": d["unit"],
                     "frac": round(d["achieved"] / d["peak"], 5),
-                    "traffic": profiled_traffic(dom, args.workload)}
+                    "traffic": profiled_traffic(dom, args.workload),
+                    # the same launch priced by its LAWN-41 flops
+                    "fp64_tflops": round(d["fp64_tflops"], 3),
+                    "fp64_frac": round(d["fp64_frac"], 5)}
     line = {
         "metric": METRIC,
         "value": round(wall, 6),
@@ -768,15 +807,7 @@ def run(args, system, coeffs, num_states, ep_kind, ctx):
             "bytes": (system.pot_dim + 3) * per * 8.0}
     for v in kernels.values():
         v["achieved"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
-    if st.get("k_hetrd_coop", 0.0) > 0.0:
-        # Householder tridiagonalisation, LAWN 41: 16/3 n^3 real flops per
-        # complex Hermitian matrix, 4/3 n^3 per real symmetric one; bound by
-        # the exchange latency between its work-groups, priced against FP64
-        kernels["k_hetrd_coop"] = {
-            "bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "ms": st["k_hetrd_coop"],
-            "flop": (4.0 / 3.0 if system.gamma else 16.0 / 3.0) * n ** 3 * nk}
-        kernels["k_hetrd_coop"]["achieved"] = (kernels["k_hetrd_coop"]["flop"] /
-                                               (st["k_hetrd_coop"] * 1e-3) / 1e12)
+    kernels.update(eigensolve_kernels(system, st, (electrons(system) + 1) // 2))
     # the roofline line is the kernel of this library that takes the most
     # time per step; the eigensolve stage (the Cholesky transform, the
     # reduction, the divide and conquer; its back-transform of the occupied
