@@ -34,10 +34,19 @@ SOURCES = ["imago_hip.hip", "prim_batch.hip", "asm_ol.hip", "asm_ke.hip", "asm_n
            "gemm_mfma.hip"]
 
 
+# The MFMA kernels keep their accumulators in ordinary VGPRs: with the
+# AGPR form the compiler copied every accumulator out of and back into the
+# AGPRs on each K step, draining the matrix pipeline each time.
+EXTRA_FLAGS = {
+    "gemm_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+    "tridiag_dc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
+
 def _compile(src: str, verbose: bool) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c",
-                                 os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + HIP_FLAGS + EXTRA_FLAGS.get(src, []) + [
+        "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -4,8 +4,8 @@
 // One 64 x 64 tile of C per work-group of four waves; each wave owns a
 // 32 x 32 quarter as 2 x 2 blocks of 16 x 16 accumulators.  op(A) and
 // op(B) are staged through LDS in K steps of 16, split into real and
-// imaginary planes, with the next step's global loads in flight during the
-// current step's MFMAs.  A complex product takes four real MFMAs per block
+// imaginary planes, in two stages: the next step's global loads are in
+// flight during the current step's MFMAs and land in the other stage.  A complex product takes four real MFMAs per block
 // (Re += ar br - ai bi, Im += ar bi + ai br); the eight accumulators of a
 // wave are independent, so consecutive MFMAs never wait on each other.
 //
@@ -56,15 +56,31 @@ struct TileRange {
 __device__ __forceinline__ TileRange tile_of(const GemmArgs &g, int idx, int inner)
 {
     const int tm = (g.M + kGT - 1) / kGT, tn = (g.N + kGT - 1) / kGT;
+    // Tiles whose K range is longest are handed out first (the work-groups
+    // are dispatched in index order), so the short ones fill the tail.
     int ti, tj;
     if (g.herm) {
-        // lower tiles in row order: row ti holds tiles (ti, 0 .. ti)
-        ti = 0;
-        while ((ti + 1) * (ti + 2) / 2 <= idx) ++ti;
-        tj = idx - ti * (ti + 1) / 2;
+        if (g.amask == kTriLower) {
+            // K grows with the row: lower tiles from the bottom row up, row
+            // tm - 1 - r holding tiles (., 0 .. tm - 1 - r)
+            int r = 0, s0 = 0;
+            while (s0 + (tm - r) <= idx) {
+                s0 += tm - r;
+                ++r;
+            }
+            ti = tm - 1 - r;
+            tj = idx - s0;
+        } else {
+            // lower tiles in row order: row ti holds tiles (ti, 0 .. ti)
+            ti = 0;
+            while ((ti + 1) * (ti + 2) / 2 <= idx) ++ti;
+            tj = idx - ti * (ti + 1) / 2;
+        }
     } else {
         ti = idx % tm;
         tj = idx / tm;
+        if (g.bmask == kTriUpper) tj = tn - 1 - tj;   // K grows with the column
+        if (g.amask == kTriLower) ti = tm - 1 - ti;   // K grows with the row
     }
     TileRange t;
     t.i0 = ti * kGT;
@@ -85,8 +101,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
 {
     using E = Elem<CX>;
     using T = typename E::T;
-    __shared__ double As[CX ? 2 : 1][kGK][kGLd];
-    __shared__ double Bs[CX ? 2 : 1][kGK][kGLd];
+    // two stages: the MFMAs read one while the next K step is written to the other
+    __shared__ double As[2][CX ? 2 : 1][kGK][kGLd];
+    __shared__ double Bs[2][CX ? 2 : 1][kGK][kGLd];
     const int bi = blockIdx.y, outer = bi / g.nb2, inner = bi % g.nb2;
     const TileRange tr = tile_of(g, blockIdx.x, inner);
     if (!tr.ok) return;
@@ -139,7 +156,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
             rb[q] = v;
         }
     };
-    auto stash = [&]() {
+    auto stash = [&](int st) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             int i, k;
@@ -150,8 +167,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
                 k = tid & 15;
                 i = (tid >> 4) + 16 * q;
             }
-            As[0][k][i] = E::re(ra[q]);
-            if (CX) As[CX ? 1 : 0][k][i] = E::im(ra[q]);
+            As[st][0][k][i] = E::re(ra[q]);
+            if (CX) As[st][CX ? 1 : 0][k][i] = E::im(ra[q]);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -163,8 +180,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
                 j = tid & 63;
                 k = (tid >> 6) + 4 * q;
             }
-            Bs[0][k][j] = E::re(rb[q]);
-            if (CX) Bs[CX ? 1 : 0][k][j] = E::im(rb[q]);
+            Bs[st][0][k][j] = E::re(rb[q]);
+            if (CX) Bs[st][CX ? 1 : 0][k][j] = E::im(rb[q]);
         }
     };
 
@@ -177,20 +194,23 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
             accI[p][q] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
     const int l16 = lane & 15;
-    if (tr.klo < tr.khi) load(tr.klo);
+    if (tr.klo < tr.khi) {
+        load(tr.klo);
+        stash(0);
+    }
+    __syncthreads();
+    int cur = 0;
     for (int k0 = tr.klo; k0 < tr.khi; k0 += kGK) {
-        __syncthreads();
-        stash();
-        __syncthreads();
-        if (k0 + kGK < tr.khi) load(k0 + kGK);
+        const bool more = k0 + kGK < tr.khi;
+        if (more) load(k0 + kGK);
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
             const int kk = 4 * s4 + (lane >> 4);
             double ar[2], br[2];
 #pragma unroll
-            for (int p = 0; p < 2; ++p) ar[p] = As[0][kk][wi * 32 + 16 * p + l16];
+            for (int p = 0; p < 2; ++p) ar[p] = As[cur][0][kk][wi * 32 + 16 * p + l16];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) br[q] = Bs[0][kk][wj * 32 + 16 * q + l16];
+            for (int q = 0; q < 2; ++q) br[q] = Bs[cur][0][kk][wj * 32 + 16 * q + l16];
             if (!CX) {
 #pragma unroll
                 for (int p = 0; p < 2; ++p)
@@ -201,10 +221,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
             } else {
                 double ai[2], bi_[2], nbi[2];
 #pragma unroll
-                for (int p = 0; p < 2; ++p) ai[p] = As[CX ? 1 : 0][kk][wi * 32 + 16 * p + l16];
+                for (int p = 0; p < 2; ++p) ai[p] = As[cur][CX ? 1 : 0][kk][wi * 32 + 16 * p + l16];
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
-                    bi_[q] = Bs[CX ? 1 : 0][kk][wj * 32 + 16 * q + l16];
+                    bi_[q] = Bs[cur][CX ? 1 : 0][kk][wj * 32 + 16 * q + l16];
                     nbi[q] = -bi_[q];
                 }
 #pragma unroll
@@ -227,6 +247,12 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
                     }
             }
         }
+        // the next step into the other stage; one barrier orders both the
+        // reads of this stage before its rewrite two steps on and the writes
+        // of the next before its reads
+        if (more) stash(cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
     }
 
     // epilogue: lane l, register r holds C(i0 + .. + l % 16, j0 + .. + l / 16 + 4 r)

@@ -88,8 +88,95 @@ __global__ __launch_bounds__(256) void k_dc_init(const double *__restrict__ d,
     }
 }
 
+// deflate_scan on one wave with the same decisions and the same output
+// order, 64 entries a round.  Each lane takes one entry: the |z| test, its
+// predecessor (the last earlier entry that survived it, or the pending one
+// carried from the last round) and the neighbour test against it.  All
+// entries up to the first lane whose neighbour test deflates are final; that
+// rotation changes the entry the next test reads, so the round ends there
+// and the next one starts behind it.  Output slots come from prefix counts of
+// each kind of event in lane order, as the sequential scan appends them.
+__device__ int deflate_scan_wave(int m, double rho, double tol, double *Ds, double *zs,
+                                 double *dl, double *zz, int *kpos, int *dpos, double *dval,
+                                 int *rp, int *rq, double *rc, double *rs, int *nrot_out)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int K = 0, nd = 0, nrot = 0, pj = -1;
+    for (int t0 = 0; t0 < m;) {
+        const int t = t0 + lane;
+        const bool in = t < m;
+        const double z = in ? zs[t] : 0.0, D = in ? Ds[t] : 0.0;
+        const bool f1 = in && rho * fabs(z) <= tol;
+        const bool cand = in && !f1;
+        const unsigned long long cm = __ballot(cand);
+        const unsigned long long prev = cm & below;
+        const int pred = prev ? t0 + 63 - __clzll((long long)prev) : pj;
+        bool rot = false;
+        double c = 0.0, sn = 0.0, tau = 0.0, Dp = 0.0, zp = 0.0;
+        if (cand && pred >= 0) {
+            zp = zs[pred];
+            Dp = Ds[pred];
+            rot = pair_deflates(zp, Dp, z, D, tol, &c, &sn, &tau);
+        }
+        const unsigned long long rm = __ballot(rot);
+        const int f = rm ? __ffsll((long long)rm) - 1 : 64;
+        const bool act = in && lane <= f;
+        const bool ev_def = act && (f1 || lane == f);
+        const bool ev_keep = act && cand && pred >= 0 && lane != f;
+        const unsigned long long dm = __ballot(ev_def), km = __ballot(ev_keep);
+        if (ev_keep) {
+            const int k = K + __popcll(km & below);
+            dl[k] = Dp;
+            zz[k] = zp;
+            kpos[k] = pred;
+        }
+        if (ev_def) {
+            const int d = nd + __popcll(dm & below);
+            if (f1) {
+                dpos[d] = t;
+                dval[d] = D;
+            } else {
+                const double dp = Dp * c * c + D * sn * sn;
+                zs[t] = tau;
+                zs[pred] = 0.0;
+                Ds[t] = Dp * sn * sn + D * c * c;
+                Ds[pred] = dp;
+                rp[nrot] = pred;
+                rq[nrot] = t;
+                rc[nrot] = c;
+                rs[nrot] = sn;
+                dpos[d] = pred;
+                dval[d] = dp;
+            }
+        }
+        K += __popcll(km);
+        nd += __popcll(dm);
+        if (rm) {
+            ++nrot;
+            pj = t0 + f;
+            t0 += f + 1;
+        } else {
+            if (cm) pj = t0 + 63 - __clzll((long long)cm);
+            t0 += 64;
+        }
+        // the rotated entries are read by other lanes in the next round
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (pj >= 0 && lane == 0) {
+        dl[K] = Ds[pj];
+        zz[K] = zs[pj];
+        kpos[K] = pj;
+    }
+    if (pj >= 0) ++K;
+    *nrot_out = nrot;
+    return K;
+}
+
 // one work-group per merge: z, the sort of the children's eigenvalues, the
-// deflation scan (thread 0)
+// deflation scan (wave 0)
 __global__ __launch_bounds__(256) void k_dc_deflate(const int *__restrict__ merges, int nmerge,
                                                     int n, const double *__restrict__ e,
                                                     DcWork w)
@@ -146,16 +233,18 @@ __global__ __launch_bounds__(256) void k_dc_deflate(const int *__restrict__ merg
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
         const double tol = 8.0 * kEps * fmax(red[0][0], red[1][0]);
         const double rho = 2.0 * fabs(beta);
         int nrot = 0;
-        const int K = deflate_scan(m, rho, tol, Ds, zs, w.dl + base, w.zz + base, w.kpos + base,
-                                   w.dpos + base, w.dv + base, w.rp + base, w.rq + base,
-                                   w.rc + base, w.rs + base, &nrot);
-        w.K[base] = K;
-        w.nrot[base] = nrot;
-        w.rho[base] = rho;
+        const int K = deflate_scan_wave(m, rho, tol, Ds, zs, w.dl + base, w.zz + base,
+                                        w.kpos + base, w.dpos + base, w.dv + base, w.rp + base,
+                                        w.rq + base, w.rc + base, w.rs + base, &nrot);
+        if (threadIdx.x == 0) {
+            w.K[base] = K;
+            w.nrot[base] = nrot;
+            w.rho[base] = rho;
+        }
     }
 }
 

@@ -78,6 +78,23 @@ struct Seq {
 // (dl ascending, zz, their sorted positions kpos), the deflated ones
 // (dpos, dval) and the rotations (rp, rq, rc, rs: columns p, q of the
 // sorted basis, x_p' = c x_p + s x_q, x_q' = c x_q - s x_p).  Returns K.
+// The neighbour test of dlaed2 between the pending entry (zp, Dp) and the
+// next one (zt, Dt): the Givens rotation that zeroes zp (c, s, and the
+// combined z, tau) and whether it deflates, |(Dt - Dp) c s| <= tol.
+DC_HD bool pair_deflates(double zp, double Dp, double zt, double Dt, double tol, double *c_out,
+                         double *s_out, double *tau_out)
+{
+    double s = zp, c = zt;
+    const double tau = sqrt(c * c + s * s);
+    const double gap = Dt - Dp;
+    c /= tau;
+    s = -s / tau;
+    *c_out = c;
+    *s_out = s;
+    *tau_out = tau;
+    return fabs(gap * c * s) <= tol;
+}
+
 DC_HD int deflate_scan(int m, double rho, double tol, double *Ds, double *zs, double *dl,
                        double *zz, int *kpos, int *dpos, double *dval, int *rp, int *rq,
                        double *rc, double *rs, int *nrot_out)
@@ -94,12 +111,8 @@ DC_HD int deflate_scan(int m, double rho, double tol, double *Ds, double *zs, do
             pj = t;
             continue;
         }
-        double s = zs[pj], c = zs[t];
-        const double tau = sqrt(c * c + s * s);
-        const double gap = Ds[t] - Ds[pj];
-        c /= tau;
-        s = -s / tau;
-        if (fabs(gap * c * s) <= tol) {
+        double c, s, tau;
+        if (pair_deflates(zs[pj], Ds[pj], zs[t], Ds[t], tol, &c, &s, &tau)) {
             zs[t] = tau;
             zs[pj] = 0.0;
             rp[nrot] = pj;
