@@ -331,7 +331,11 @@ struct imago_ctx {
         std::vector<double> intg, nuc_nl, nuc_l;
         double *es_buf = nullptr;   // device: the elecstat arrays (one allocation)
         double *neut_nl = nullptr, *neut_l = nullptr, *resq = nullptr;   // device
-        double *vec = nullptr;      // device scratch, 8 potDim
+        // device scratch of imago_scf_update (its vectors, the XC potential's
+        // point values and partial sums, the convergence measure's rows and
+        // point values), sized by imago_scf_setup
+        double *work = nullptr, *xc_v = nullptr, *xc_part = nullptr, *td_u = nullptr,
+               *td_d = nullptr;
         int iteration = 1;
         bool have_hist = false;
         std::vector<double> xl1, xl2, yl1, yl2;
@@ -3772,30 +3776,24 @@ static int xc_build_cache(imago_ctx *c)
     return IMAGO_OK;
 }
 
-extern "C" int imago_xc_potential(imago_ctx *c, const double *rho_total, const double *rho_core,
-                                  double *exch_corr_pot)
+// Point-value and partial-sum scratch the cached operator blocks need.
+static void xc_scratch_sizes(const imago_ctx *c, size_t *v_need, size_t *p_need)
 {
-    if (!c) return fail(IMAGO_E_ARG, "null context");
-    if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
-    if (!rho_total || !exch_corr_pot) return fail(IMAGO_E_ARG, "null array");
-    HIPCHK(hipSetDevice(c->device));
-    const int P = c->h.pot_dim;
-    if (!c->xc_cached) RCHK(xc_build_cache(c));
-    std::vector<double> hr(2 * (size_t)P, 0.0);
-    std::copy(rho_total, rho_total + P, hr.begin());
-    if (rho_core) std::copy(rho_core, rho_core + P, hr.begin() + P);
-    size_t v_need = 1, p_need = 1;
+    *v_need = 1;
+    *p_need = 1;
     for (const auto &q : c->xc_cache) {
-        v_need = std::max(v_need, (size_t)q.nb * 3 * q.nn);
-        p_need = std::max(p_need, (size_t)q.nb * 3 * q.mm);
+        *v_need = std::max(*v_need, (size_t)q.nb * 3 * q.nn);
+        *p_need = std::max(*p_need, (size_t)q.nb * 3 * q.mm);
     }
-    double *rho = nullptr, *pot = nullptr, *v = nullptr, *part = nullptr;
-    RCHK(dev_alloc(c, &rho, 2 * (size_t)P));
-    RCHK(dev_alloc(c, &pot, 3 * (size_t)P));
-    RCHK(dev_alloc(c, &v, v_need));
-    RCHK(dev_alloc(c, &part, p_need));
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(hipMemcpyAsync(rho, hr.data(), 2 * (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+}
+
+// The XC potential terms on the device: rho holds rho_total then rho_core
+// (2 potDim), pot receives the 3 potDim terms; v and part scratch of the
+// sizes above.  Enqueued on the context's stream.
+static int xc_potential_dev(imago_ctx *c, const double *rho, double *pot, double *v,
+                            double *part)
+{
+    const int P = c->h.pot_dim;
     HIPCHK(hipMemsetAsync(pot, 0, 3 * (size_t)P * 8, c->stream));
     for (const auto &q : c->xc_cache) {
         if (!q.B) continue;
@@ -3810,6 +3808,31 @@ extern "C" int imago_xc_potential(imago_ctx *c, const double *rho_total, const d
             (rocblas_stride)3 * q.mm, q.nb));
         HIPCHK(imago::launch_xc_pot_add(part, q.mm, q.rank, q.nb, P, pot, c->stream));
     }
+    return IMAGO_OK;
+}
+
+extern "C" int imago_xc_potential(imago_ctx *c, const double *rho_total, const double *rho_core,
+                                  double *exch_corr_pot)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (!c->xc_ready) return fail(IMAGO_E_STATE, "imago_xc_mesh must come first");
+    if (!rho_total || !exch_corr_pot) return fail(IMAGO_E_ARG, "null array");
+    HIPCHK(hipSetDevice(c->device));
+    const int P = c->h.pot_dim;
+    if (!c->xc_cached) RCHK(xc_build_cache(c));
+    std::vector<double> hr(2 * (size_t)P, 0.0);
+    std::copy(rho_total, rho_total + P, hr.begin());
+    if (rho_core) std::copy(rho_core, rho_core + P, hr.begin() + P);
+    size_t v_need = 1, p_need = 1;
+    xc_scratch_sizes(c, &v_need, &p_need);
+    double *rho = nullptr, *pot = nullptr, *v = nullptr, *part = nullptr;
+    RCHK(dev_alloc(c, &rho, 2 * (size_t)P));
+    RCHK(dev_alloc(c, &pot, 3 * (size_t)P));
+    RCHK(dev_alloc(c, &v, v_need));
+    RCHK(dev_alloc(c, &part, p_need));
+    HIPCHK(hipMemcpyAsync(rho, hr.data(), 2 * (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    RCHK(xc_potential_dev(c, rho, pot, v, part));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipMemcpyAsync(exch_corr_pot, pot, 3 * (size_t)P * 8, hipMemcpyDeviceToHost,
                           c->stream));
@@ -4068,32 +4091,17 @@ static int spd_factor(imago_ctx *c, imago_ctx::SpdMat &m, int n)
     return IMAGO_OK;
 }
 
-extern "C" int imago_pot_solve(imago_ctx *c, int matrix, int nrhs, double *b, int *info)
+// x = S (As^-1 (S b)) for a factored overlap with the inverse from its
+// factor, then two steps of refinement on the scaled system
+// (r = S b - As x, x += As^-1 r); b and x device arrays of n x nrhs, scr
+// 2 n nrhs doubles of device scratch.  Enqueued on the context's stream.
+static int pot_solve_dev(imago_ctx *c, imago_ctx::SpdMat &m, int n, int nrhs, const double *b,
+                         double *x, double *scr)
 {
-    if (!c) return fail(IMAGO_E_ARG, "null context");
-    if (matrix != IMAGO_SOLVE_POT_OVERLAP && matrix != IMAGO_SOLVE_XC_OVERLAP)
-        return fail(IMAGO_E_ARG, "unknown matrix %d", matrix);
-    if (nrhs < 1 || !b) return fail(IMAGO_E_ARG, "no right-hand sides");
-    imago_ctx::SpdMat &m = c->spd[matrix == IMAGO_SOLVE_POT_OVERLAP ? 0 : 1];
-    if (!m.A)
-        return fail(IMAGO_E_STATE, "%s first", matrix == IMAGO_SOLVE_POT_OVERLAP
-                                                  ? "imago_elecstat" : "imago_xc_overlap");
-    HIPCHK(hipSetDevice(c->device));
-    const int n = c->h.pot_dim;
-    if (!m.factored) RCHK(spd_factor(c, m, n));
-    if (info) *info = m.info;
-    if (m.info != 0)
-        return fail(IMAGO_E_NOTPD, "the overlap is not positive definite (minor %d)", m.info);
     const size_t nb = (size_t)n * nrhs;
-    double *x = nullptr, *r = nullptr;
-    RCHK(dev_alloc(c, &x, nb));
-    RCHK(dev_alloc(c, &r, nb));
+    double *bs = scr, *r = scr + nb;
     const unsigned g = (unsigned)((nb + 255) / 256);
-    // x = S (As^-1 (S b)) with the inverse from the factor, then two steps
-    // of refinement on the scaled system: r = S b - As x, x += As^-1 r
-    double *bs = nullptr;
-    RCHK(dev_alloc(c, &bs, nb));
-    HIPCHK(hipMemcpyAsync(bs, b, nb * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(bs, b, nb * 8, hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, bs, m.s, n, nrhs);
     HIPCHK(hipGetLastError());
     // Ainv and Af hold both triangles of symmetric matrices, so up to four
@@ -4119,11 +4127,37 @@ extern "C" int imago_pot_solve(imago_ctx *c, int matrix, int nrhs, double *b, in
     }
     hipLaunchKernelGGL(k_spd_rows, dim3(g), dim3(256), 0, c->stream, x, m.s, n, nrhs);
     HIPCHK(hipGetLastError());
+    return IMAGO_OK;
+}
+
+extern "C" int imago_pot_solve(imago_ctx *c, int matrix, int nrhs, double *b, int *info)
+{
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    if (matrix != IMAGO_SOLVE_POT_OVERLAP && matrix != IMAGO_SOLVE_XC_OVERLAP)
+        return fail(IMAGO_E_ARG, "unknown matrix %d", matrix);
+    if (nrhs < 1 || !b) return fail(IMAGO_E_ARG, "no right-hand sides");
+    imago_ctx::SpdMat &m = c->spd[matrix == IMAGO_SOLVE_POT_OVERLAP ? 0 : 1];
+    if (!m.A)
+        return fail(IMAGO_E_STATE, "%s first", matrix == IMAGO_SOLVE_POT_OVERLAP
+                                                  ? "imago_elecstat" : "imago_xc_overlap");
+    HIPCHK(hipSetDevice(c->device));
+    const int n = c->h.pot_dim;
+    if (!m.factored) RCHK(spd_factor(c, m, n));
+    if (info) *info = m.info;
+    if (m.info != 0)
+        return fail(IMAGO_E_NOTPD, "the overlap is not positive definite (minor %d)", m.info);
+    const size_t nb = (size_t)n * nrhs;
+    double *db = nullptr, *x = nullptr, *scr = nullptr;
+    RCHK(dev_alloc(c, &db, nb));
+    RCHK(dev_alloc(c, &x, nb));
+    RCHK(dev_alloc(c, &scr, 2 * nb));
+    HIPCHK(hipMemcpyAsync(db, b, nb * 8, hipMemcpyHostToDevice, c->stream));
+    RCHK(pot_solve_dev(c, m, n, nrhs, db, x, scr));
     HIPCHK(hipMemcpyAsync(b, x, nb * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    dev_release(c, db);
     dev_release(c, x);
-    dev_release(c, r);
-    dev_release(c, bs);
+    dev_release(c, scr);
     return IMAGO_OK;
 }
 
@@ -4201,6 +4235,63 @@ __global__ __launch_bounds__(256) void k_scf_site_rms(const double *__restrict__
     }
 }
 
+static void td_scratch_sizes(const imago_ctx *c, size_t *u_need, size_t *d_need);
+
+// imago_scf_update's device vectors, carved out of ScfState::work in this
+// order (P = potDim, NPT = max(P, potential types)); the groups marked
+// "one copy" are contiguous so each phase of the update moves them in one
+// transfer.
+struct ScfWork {
+    double *fit_b, *fit_x;        // 2P each: [potRho intgConsts], the fit
+    double *scr;                  // 6P: solve scratch
+    double *g4, *gv, *g6, *rho;   // P, P, NPT, 2P (g1 then zeros): one copy in
+    double *e1, *e2, *e6, *xc;    // P, P, P, 3P: one copy out
+    double *pes_b;                // 2P
+    double *pes_x, *xc_x;         // 2P, 3P: one copy out
+    double *outpot, *dd;          // P, 2P: one copy in
+    double *sx, *uu, *sums;       // P, 2P, 2 nsites: one copy out
+};
+
+static ScfWork scf_work(const imago_ctx *c, double *base, size_t *total)
+{
+    const size_t P = (size_t)c->h.pot_dim, NPT = std::max<size_t>(P, (size_t)c->h.npt);
+    ScfWork w;
+    double *p = base;
+    auto take = [&p](size_t n) {
+        double *q = p;
+        p += n;
+        return q;
+    };
+    w.fit_b = take(2 * P);
+    w.fit_x = take(2 * P);
+    w.scr = take(6 * P);
+    w.g4 = take(P);
+    w.gv = take(P);
+    w.g6 = take(NPT);
+    w.rho = take(2 * P);
+    w.e1 = take(P);
+    w.e2 = take(P);
+    w.e6 = take(P);
+    w.xc = take(3 * P);
+    w.pes_b = take(2 * P);
+    w.pes_x = take(2 * P);
+    w.xc_x = take(3 * P);
+    w.outpot = take(P);
+    w.dd = take(2 * P);
+    w.sx = take(P);
+    w.uu = take(2 * P);
+    w.sums = take(2 * (size_t)c->h.nsites);
+    if (total) *total = (size_t)(p - base);
+    return w;
+}
+
+static size_t scf_work_doubles(const imago_ctx *c)
+{
+    size_t total = 0;
+    (void)scf_work(c, nullptr, &total);
+    return total;
+}
+
 extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
                                const imago_xc_mesh_params *mesh, const double *covalent_radius,
                                int num_cells_recip, const double *cell_dims_recip,
@@ -4220,7 +4311,8 @@ extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
     const HostSys &h = c->h;
     const int P = h.pot_dim;
     imago_ctx::ScfState &S = c->scf;
-    for (void *q : {(void *)S.es_buf, (void *)S.resq, (void *)S.vec})
+    for (void *q : {(void *)S.es_buf, (void *)S.resq, (void *)S.work,
+                    (void *)S.xc_v, (void *)S.xc_part, (void *)S.td_u, (void *)S.td_d})
         if (q) dev_release(c, q);
     S = imago_ctx::ScfState{};
     S.p = *params;
@@ -4251,51 +4343,46 @@ extern "C" int imago_scf_setup(imago_ctx *c, const imago_scf_params *params,
     RCHK(imago_xc_mesh(c, mesh, params->log_elec_thresh, covalent_radius, &maxpts));
     RCHK(xc_overlap_run(c, nullptr));
     if (!c->xc_cached) RCHK(xc_build_cache(c));
-    RCHK(dev_alloc(c, &S.vec, 8 * (size_t)P));
     // factor both overlaps now (the first solve would)
     for (int m = 0; m < 2; ++m)
         if (!c->spd[m].factored) RCHK(spd_factor(c, c->spd[m], P));
+    // imago_scf_update's device scratch
+    {
+        size_t v_need, p_need, u_need, d_need;
+        xc_scratch_sizes(c, &v_need, &p_need);
+        td_scratch_sizes(c, &u_need, &d_need);
+        RCHK(dev_alloc(c, &S.work, scf_work_doubles(c)));
+        RCHK(dev_alloc(c, &S.xc_v, v_need));
+        RCHK(dev_alloc(c, &S.xc_part, p_need));
+        RCHK(dev_alloc(c, &S.td_u, u_need));
+        RCHK(dev_alloc(c, &S.td_d, d_need));
+    }
     S.iteration = 1;
     S.ready = true;
     return IMAGO_OK;
 }
 
-// y_r = op(M) x_r on the device for an m x n column-major M (trans:
-// M^T x) and nrhs <= 4 vectors; x (nrhs vectors of length lx back to back)
-// and y (nrhs of length ly) are host arrays
-static int scf_gemv(imago_ctx *c, bool trans, int m, int n, const double *M, const double *x,
-                    double *y, int nrhs = 1)
+// The convergence measure testableDelta: the largest site RMS of the
+// real-space potential change over the ray points
+// (potentialUpdate.F90:1350-1431), per site sqrt(sum w d^2 / sum w) with
+// d = exchRhoOp(:, k) . outputPot at each of its points.
+// Row and point scratch of the convergence measure.
+static void td_scratch_sizes(const imago_ctx *c, size_t *u_need, size_t *d_need)
 {
-    const int lx = trans ? m : n, ly = trans ? n : m;
-    if (nrhs < 1 || nrhs > 4 || (size_t)nrhs * std::max(lx, ly) > 4 * (size_t)c->h.pot_dim)
-        return fail(IMAGO_E_ARG, "scf_gemv: %d vectors of %d", nrhs, std::max(lx, ly));
-    double *dx = c->scf.vec, *dy = c->scf.vec + 4 * (size_t)c->h.pot_dim;
-    HIPCHK(hipMemcpyAsync(dx, x, (size_t)lx * nrhs * 8, hipMemcpyHostToDevice, c->stream));
-    RCHK(dev_gemv(c, trans, m, n, M, m, dx, lx, nrhs, 1.0, 0.0, dy, ly));
-    HIPCHK(hipMemcpyAsync(y, dy, (size_t)ly * nrhs * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return IMAGO_OK;
+    *u_need = 1;
+    *d_need = 1;
+    for (const auto &q : c->xc_cache) {
+        *u_need = std::max(*u_need, (size_t)q.nb * q.mm);
+        *d_need = std::max(*d_need, (size_t)q.nb * q.nn);
+    }
 }
 
-// the largest site RMS of the real-space potential change over the ray
-// points (potentialUpdate.F90:1350-1431): per site sqrt(sum w d^2 / sum w)
-// with d = exchRhoOp(:, k) . outputPot at each of its points
-static int scf_testable_delta(imago_ctx *c, const std::vector<double> &outpot, double *delta)
+// The per-site sums of the measure for the device vector v (potDim): sums
+// receives (sum w d^2, sum w) per site; u, d scratch of the sizes above.
+// Enqueued on the context's stream.
+static int testable_delta_dev(imago_ctx *c, const double *v, double *sums, double *u, double *d)
 {
     const int P = c->h.pot_dim;
-    size_t u_need = 1, d_need = 1;
-    int nbmax = 1;
-    for (const auto &q : c->xc_cache) {
-        u_need = std::max(u_need, (size_t)q.nb * q.mm);
-        d_need = std::max(d_need, (size_t)q.nb * q.nn);
-        nbmax = std::max(nbmax, q.nb);
-    }
-    double *v = nullptr, *u = nullptr, *d = nullptr, *sums = nullptr;
-    RCHK(dev_alloc(c, &v, (size_t)P));
-    RCHK(dev_alloc(c, &u, u_need));
-    RCHK(dev_alloc(c, &d, d_need));
-    RCHK(dev_alloc(c, &sums, 2 * (size_t)c->h.nsites));
-    HIPCHK(hipMemcpyAsync(v, outpot.data(), (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemsetAsync(sums, 0, 2 * (size_t)c->h.nsites * 8, c->stream));
     for (const auto &q : c->xc_cache) {
         if (!q.B) continue;
@@ -4312,15 +4399,16 @@ static int scf_testable_delta(imago_ctx *c, const std::vector<double> &outpot, d
                            c->xc_w + q.p0, q.first, q.nb, q.npb, sums + 2 * (size_t)q.s0);
         HIPCHK(hipGetLastError());
     }
-    std::vector<double> hs(2 * (size_t)c->h.nsites);
-    HIPCHK(hipMemcpyAsync(hs.data(), sums, hs.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (void *x : {(void *)v, (void *)u, (void *)d, (void *)sums}) dev_release(c, x);
+    return IMAGO_OK;
+}
+
+// the largest site RMS from the sums (sites with ray points only)
+static double testable_delta_of(const imago_ctx *c, const double *hs)
+{
     double best = 0.0;
     for (int i = 0; i < c->h.nsites; ++i)
         if (c->xc_npts[i] > 0) best = std::max(std::sqrt(hs[2 * i] / hs[2 * i + 1]), best);
-    *delta = best;
-    return IMAGO_OK;
+    return best;
 }
 
 extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const double *pot_rho,
@@ -4332,20 +4420,28 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
     HIPCHK(hipSetDevice(c->device));
     const HostSys &h = c->h;
     imago_ctx::ScfState &S = c->scf;
-    const int P = h.pot_dim;
+    const int P = h.pot_dim, NPT = std::max(P, h.npt);
     const std::vector<double> &ic = S.intg;
+    const ScfWork W = scf_work(c, S.work, nullptr);
+    imago_ctx::SpdMat &mp = c->spd[0], &mx = c->spd[1];
     auto dot = [P](const double *a, const double *b) {
         double t = 0.0;
         for (int i = 0; i < P; ++i) t += a[i] * b[i];
         return t;
     };
-    // charge fit: potAlphaOverlap [g1 g2] = [potRho intgConsts] (:191-239)
+    // The update runs in four phases, each one transfer in, its kernels and
+    // one transfer out with a single wait; the host does the vector
+    // arithmetic between them in the reference's order.
+    // ---- 1: charge fit, potAlphaOverlap [g1 g2] = [potRho intgConsts]
+    // (:191-239)
+    if (mp.info != 0) return fail(IMAGO_E_SOLVE, "Fitting dposvx failed. INFO= %d", mp.info);
     std::vector<double> g(2 * (size_t)P);
     std::copy(pot_rho, pot_rho + P, g.begin());
     std::copy(ic.begin(), ic.end(), g.begin() + P);
-    int info = 0;
-    if (imago_pot_solve(c, IMAGO_SOLVE_POT_OVERLAP, 2, g.data(), &info) != IMAGO_OK)
-        return fail(IMAGO_E_SOLVE, "Fitting dposvx failed. INFO= %d", info);
+    HIPCHK(hipMemcpyAsync(W.fit_b, g.data(), 2 * (size_t)P * 8, hipMemcpyHostToDevice, c->stream));
+    RCHK(pot_solve_dev(c, mp, P, 2, W.fit_b, W.fit_x, W.scr));
+    HIPCHK(hipMemcpyAsync(g.data(), W.fit_x, 2 * (size_t)P * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     std::vector<double> g1(g.begin(), g.begin() + P), g2(g.begin() + P, g.end());
     const double fitted = dot(ic.data(), g1.data());
     const double electron_diff = S.p.num_electrons - fitted;
@@ -4355,7 +4451,7 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
     }
     // no core charge: total = valence (:329-342)
     g1 = g2;
-    std::vector<double> g4 = g1, g5(P, 0.0), g6(std::max(P, h.npt), 0.0), g7(P, 0.0);
+    std::vector<double> g4 = g1, g5(P, 0.0), g6(NPT, 0.0), g7(P, 0.0);
     for (int t = 0; t < h.npt; ++t) {   // neutral / residual / nuclear split (:416-484)
         const int i0 = h.type_term0[t], i1 = i0 + h.pot_nalpha[t] - 1;
         const double nuc = h.pot_z[t] * (double)h.pot_mult[t];
@@ -4366,12 +4462,30 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
         g6[t] = g5[i0];
         g7[i1] = -nuc / ic[i1];
     }
-    // electrostatic potential components (:536-606); the core terms vanish
-    std::vector<double> e1(P), e2(P), e6(P), gv(P);
-    RCHK(scf_gemv(c, true, P, P, S.neut_nl, g4.data(), e1.data()));
-    for (int i = 0; i < P; ++i) gv[i] = g2[i] - g5[i];
-    RCHK(scf_gemv(c, true, P, P, S.neut_l, gv.data(), e2.data()));
-    RCHK(scf_gemv(c, true, h.npt, P, S.resq, g6.data(), e6.data()));
+    // ---- 2: the electrostatic potential components (:536-606; the core
+    // terms vanish) and the exchange-correlation potential on the mesh
+    // (:720-1015) of rho_total = g1
+    std::vector<double> in2(4 * (size_t)P + NPT, 0.0);
+    std::copy(g4.begin(), g4.end(), in2.begin());
+    for (int i = 0; i < P; ++i) in2[P + i] = g2[i] - g5[i];   // gv
+    std::copy(g6.begin(), g6.end(), in2.begin() + 2 * P);
+    std::copy(g1.begin(), g1.end(), in2.begin() + 2 * P + NPT);   // rho_total, then rho_core = 0
+    HIPCHK(hipMemcpyAsync(W.g4, in2.data(), in2.size() * 8, hipMemcpyHostToDevice, c->stream));
+    RCHK(dev_gemv(c, true, P, P, S.neut_nl, P, W.g4, P, 1, 1.0, 0.0, W.e1, P));
+    RCHK(dev_gemv(c, true, P, P, S.neut_l, P, W.gv, P, 1, 1.0, 0.0, W.e2, P));
+    RCHK(dev_gemv(c, true, h.npt, P, S.resq, h.npt, W.g6, h.npt, 1, 1.0, 0.0, W.e6, P));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    RCHK(xc_potential_dev(c, W.rho, W.xc, S.xc_v, S.xc_part));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    std::vector<double> out2(6 * (size_t)P);
+    HIPCHK(hipMemcpyAsync(out2.data(), W.e1, out2.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->timing[16] = ms;
+    }
+    const double *e1 = out2.data(), *e2 = e1 + P, *e6 = e2 + P;
     const std::vector<double> &e4 = S.nuc_nl, &e5 = S.nuc_l;
     double es_energy = 0.0;
     for (int i = 0; i < P; ++i)
@@ -4383,18 +4497,47 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
         pes[i] = e1[i] + e2[i] - e4[i] + e5[i];
         pes[P + i] = e6[i];
     }
-    if (imago_pot_solve(c, IMAGO_SOLVE_POT_OVERLAP, 2, pes.data(), &info) != IMAGO_OK)
-        return fail(IMAGO_E_SOLVE, "Electrostatics dposvx failed. INFO= %d", info);
-    // exchange-correlation potential on the mesh (:720-1015)
-    std::vector<double> xc(3 * (size_t)P);
-    RCHK(imago_xc_potential(c, g1.data(), nullptr, xc.data()));
-    if (imago_pot_solve(c, IMAGO_SOLVE_XC_OVERLAP, 3, xc.data(), &info) != IMAGO_OK)
-        return fail(IMAGO_E_SOLVE, "Exchange-correlation dposvx failed. INFO= %d", info);
-    // exchCorrEnergy = sum_i (potAlphaOverlap(:, i) . xc_energy) rho_total(i)
-    // (:1033-1039; the core term is zero)
-    std::vector<double> sx(P);
-    RCHK(scf_gemv(c, true, P, P, c->spd[0].A, xc.data() + P, sx.data()));
-    const double xc_energy = dot(sx.data(), g1.data());
+    // ---- 3: the electrostatic and exchange-correlation solves
+    if (mx.info != 0)
+        return fail(IMAGO_E_SOLVE, "Exchange-correlation dposvx failed. INFO= %d", mx.info);
+    HIPCHK(hipMemcpyAsync(W.pes_b, pes.data(), pes.size() * 8, hipMemcpyHostToDevice, c->stream));
+    RCHK(pot_solve_dev(c, mp, P, 2, W.pes_b, W.pes_x, W.scr));
+    RCHK(pot_solve_dev(c, mx, P, 3, W.xc, W.xc_x, W.scr));
+    std::vector<double> out3(5 * (size_t)P);
+    HIPCHK(hipMemcpyAsync(out3.data(), W.pes_x, out3.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::copy(out3.begin(), out3.begin() + 2 * P, pes.begin());
+    const double *xc = out3.data() + 2 * P;
+    // ---- 4: exchCorrEnergy's product (:1033-1039; the core term is zero),
+    // the convergence measure of outputPot = yl0 - xl0 and the feedback's
+    // products with potAlphaOverlap
+    std::vector<double> yl0(P), xl0(pot_coeffs, pot_coeffs + P), outpot(P);
+    for (int i = 0; i < P; ++i) yl0[i] = pes[i] + pes[P + i] + xc[i];
+    for (int i = 0; i < P; ++i) outpot[i] = yl0[i] - xl0[i];
+    const bool mixing = S.p.feedback_level != 0 && S.iteration > 1;
+    std::vector<double> in4(3 * (size_t)P, 0.0), rl0, d1, d2;
+    std::copy(outpot.begin(), outpot.end(), in4.begin());
+    if (mixing) {
+        rl0.resize(P);
+        d1.resize(P);
+        d2.resize(P);
+        for (int i = 0; i < P; ++i) {
+            rl0[i] = yl0[i] - xl0[i];
+            d1[i] = rl0[i] - (S.yl1[i] - S.xl1[i]);
+            d2[i] = rl0[i] - (S.yl2[i] - S.xl2[i]);
+        }
+        std::copy(d1.begin(), d1.end(), in4.begin() + P);
+        std::copy(d2.begin(), d2.end(), in4.begin() + 2 * P);
+    }
+    HIPCHK(hipMemcpyAsync(W.outpot, in4.data(), in4.size() * 8, hipMemcpyHostToDevice, c->stream));
+    RCHK(dev_gemv(c, true, P, P, mp.A, P, W.xc_x + P, P, 1, 1.0, 0.0, W.sx, P));
+    RCHK(testable_delta_dev(c, W.outpot, W.sums, S.td_u, S.td_d));
+    if (mixing) RCHK(dev_gemv(c, false, P, P, mp.A, P, W.dd, P, 2, 1.0, 0.0, W.uu, P));
+    std::vector<double> out4(3 * (size_t)P + 2 * (size_t)h.nsites);
+    HIPCHK(hipMemcpyAsync(out4.data(), W.sx, out4.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double *sx = out4.data(), *u1 = sx + P, *u2 = u1 + P, *sums = u2 + P;
+    const double xc_energy = dot(sx, g1.data());
     const double kinetic = traces3[2];
     const double total = kinetic + es_energy + xc_energy;
     record->iteration = S.iteration;
@@ -4413,35 +4556,19 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
         return fail(IMAGO_E_SOLVE, "Total energy is positive (%.8f Ha): the structure might have a "
                                    "serious problem", total);
     // mixing history (:1301-1340)
-    std::vector<double> yl0(P), xl0(pot_coeffs, pot_coeffs + P), outpot(P);
-    for (int i = 0; i < P; ++i) yl0[i] = pes[i] + pes[P + i] + xc[i];
     if (!S.have_hist) {
         S.xl1 = S.xl2 = xl0;
         S.yl1 = S.yl2 = yl0;
         S.have_hist = true;
     }
-    for (int i = 0; i < P; ++i) outpot[i] = yl0[i] - xl0[i];
-    double delta = 0.0;
-    RCHK(scf_testable_delta(c, outpot, &delta));
+    const double delta = testable_delta_of(c, sums);
     record->testable_delta = delta;
     // the two-step feedback (:1604-1666)
     double th1 = 0.0, th2 = 0.0;
-    if (S.p.feedback_level != 0 && S.iteration > 1) {
-        std::vector<double> rl0(P), d1(P), d2(P), u1(P), u2(P);
-        for (int i = 0; i < P; ++i) {
-            rl0[i] = yl0[i] - xl0[i];
-            d1[i] = rl0[i] - (S.yl1[i] - S.xl1[i]);
-            d2[i] = rl0[i] - (S.yl2[i] - S.xl2[i]);
-        }
-        std::vector<double> dd(2 * (size_t)P), uu(2 * (size_t)P);
-        std::copy(d1.begin(), d1.end(), dd.begin());
-        std::copy(d2.begin(), d2.end(), dd.begin() + P);
-        RCHK(scf_gemv(c, false, P, P, c->spd[0].A, dd.data(), uu.data(), 2));
-        std::copy(uu.begin(), uu.begin() + P, u1.begin());
-        std::copy(uu.begin() + P, uu.end(), u2.begin());
-        const double s11 = dot(u1.data(), d1.data()), t1 = dot(u1.data(), rl0.data());
-        const double s12 = dot(u2.data(), d1.data()), s22 = dot(u2.data(), d2.data());
-        const double t2 = dot(u2.data(), rl0.data());
+    if (mixing) {
+        const double s11 = dot(u1, d1.data()), t1 = dot(u1, rl0.data());
+        const double s12 = dot(u2, d1.data()), s22 = dot(u2, d2.data());
+        const double t2 = dot(u2, rl0.data());
         th1 = t1 / s11;
         if (S.p.feedback_level != 1 && S.iteration > 2) {
             const double det = s11 * s22 - s12 * s12;
