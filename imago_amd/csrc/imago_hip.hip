@@ -133,6 +133,14 @@ struct DevBuf {
 
 }  // namespace
 
+// A device buffer with guard bands (IMAGO_DEBUG_GUARD)
+struct GuardedBuf {
+    void *user, *raw;
+    size_t bytes;
+    int line;
+    long seq;
+};
+
 struct imago_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -141,6 +149,8 @@ struct imago_ctx {
     HostSys h;
     DevSys d{};
     DevBuf sysbuf;
+    std::vector<GuardedBuf> guarded;
+    long guard_seq = 0;
     // geometry tables
     PairRec *pairs = nullptr;
     ImgRec *imgs = nullptr;
@@ -346,9 +356,12 @@ struct imago_ctx {
     size_t gv_cap = 0;
 };
 
+static void guard_release_all(imago_ctx *c);
+
 static void ctx_free_system(imago_ctx *c)
 {
     c->sysbuf.release();
+    guard_release_all(c);
     for (int k = 0; k < 4; ++k) {
         c->panel[k] = nullptr;
         c->assembled[k] = false;
@@ -412,10 +425,91 @@ static void ctx_free_system(imago_ctx *c)
     c->has_system = false;
 }
 
+// IMAGO_DEBUG_GUARD (a check for writes outside a buffer's bounds; slow):
+// every device buffer is a fresh allocation between two guard bands of a
+// fixed byte pattern, and the bands are compared with the pattern when the
+// buffer is returned or the system's buffers are; a changed band is
+// reported on stderr with the buffer's size and the source line that
+// allocated it.
+constexpr size_t kGuardBytes = 256 << 10;
+constexpr unsigned char kGuardByte = 0xcb;
+
+static bool guard_mode()
+{
+    static const bool on = [] {
+        const bool v = getenv("IMAGO_DEBUG_GUARD") != nullptr;
+        if (v) fprintf(stderr, "[imago guard] guard bands of %zu bytes on\n", kGuardBytes);
+        return v;
+    }();
+    return on;
+}
+
+static void guard_check(imago_ctx *c, const GuardedBuf &g)
+{
+    std::vector<unsigned char> band(kGuardBytes);
+    for (int side = 0; side < 2; ++side) {
+        const unsigned char *src = static_cast<const unsigned char *>(g.raw) +
+                                   (side ? kGuardBytes + g.bytes : 0);
+        if (hipMemcpyAsync(band.data(), src, kGuardBytes, hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            fprintf(stderr, "[imago guard] could not read the guard of a buffer from line %d\n",
+                    g.line);
+            return;
+        }
+        size_t changed = 0, first = 0, last = 0;
+        for (size_t b = 0; b < kGuardBytes; ++b)
+            if (band[b] != kGuardByte) {
+                if (!changed) first = b;
+                last = b;
+                ++changed;
+            }
+        if (changed)
+            fprintf(stderr,
+                    "[imago guard] %zu bytes %s the buffer of %zu bytes allocated at line %d "
+                    "(#%ld) changed, offsets %ld .. %ld from its %s\n",
+                    changed, side ? "after" : "before", g.bytes, g.line, g.seq,
+                    side ? (long)first : (long)first - (long)kGuardBytes,
+                    side ? (long)last : (long)last - (long)kGuardBytes, side ? "end" : "start");
+    }
+}
+
+static void guard_release(imago_ctx *c, void *p)
+{
+    for (size_t i = 0; i < c->guarded.size(); ++i)
+        if (c->guarded[i].user == p) {
+            guard_check(c, c->guarded[i]);
+            (void)hipFree(c->guarded[i].raw);
+            c->guarded.erase(c->guarded.begin() + i);
+            return;
+        }
+}
+
+static void guard_release_all(imago_ctx *c)
+{
+    if (c->guarded.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (const GuardedBuf &g : c->guarded) {
+        guard_check(c, g);
+        (void)hipFree(g.raw);
+    }
+    c->guarded.clear();
+}
+
 template <class T>
-static int dev_alloc(imago_ctx *c, T **p, size_t count)
+static int dev_alloc_at(imago_ctx *c, T **p, size_t count, int line)
 {
     const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    if (guard_mode()) {
+        void *raw = nullptr;
+        HIPCHK(hipMalloc(&raw, bytes + 2 * kGuardBytes));
+        unsigned char *r = static_cast<unsigned char *>(raw);
+        HIPCHK(hipMemsetAsync(r, kGuardByte, kGuardBytes, c->stream));
+        HIPCHK(hipMemsetAsync(r + kGuardBytes + bytes, kGuardByte, kGuardBytes, c->stream));
+        c->guarded.push_back({r + kGuardBytes, raw, bytes, line, c->guard_seq++});
+        *p = (T *)(r + kGuardBytes);
+        return IMAGO_OK;
+    }
     void *q = c->sysbuf.take(bytes);
     if (!q) {
         hipError_t e = hipMalloc(&q, bytes);
@@ -433,7 +527,27 @@ static int dev_alloc(imago_ctx *c, T **p, size_t count)
     return IMAGO_OK;
 }
 
-static void dev_release(imago_ctx *c, void *p) { c->sysbuf.release_one(p); }
+#define dev_alloc(c, p, count) dev_alloc_at((c), (p), (count), __LINE__)
+
+static void dev_release(imago_ctx *c, void *p)
+{
+    if (guard_mode()) {
+        (void)hipStreamSynchronize(c->stream);
+        guard_release(c, p);
+        return;
+    }
+    c->sysbuf.release_one(p);
+}
+
+// Returns an SPD matrix's buffers (the matrix, its factor and inverse) and
+// leaves it empty.
+static void spd_release(imago_ctx *c, imago_ctx::SpdMat &m)
+{
+    for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.Af, (void *)m.Ainv,
+                    (void *)m.s})
+        if (q) dev_release(c, q);
+    m = imago_ctx::SpdMat{};
+}
 
 // A blocking copy on the context's own stream.  The library never uses the
 // legacy default stream: a copy there would wait on (and, under a graph
@@ -1027,8 +1141,13 @@ extern "C" int imago_ctx_create(int device, imago_ctx **out)
         delete c;
         return fail(IMAGO_E_HIP, "context set-up failed: %s", hipGetErrorString(e));
     }
+    // rocBLAS and rocSOLVER without atomic accumulation (split-K GEMMs
+    // summing partial tiles in arrival order): results then do not depend
+    // on how work groups are scheduled, which other contexts on the device
+    // change from run to run
     if (rocblas_create_handle(&c->blas) != rocblas_status_success ||
-        rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) {
+        rocblas_set_stream(c->blas, c->stream) != rocblas_status_success ||
+        rocblas_set_atomics_mode(c->blas, rocblas_atomics_not_allowed) != rocblas_status_success) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(IMAGO_E_HIP, "rocblas handle creation failed");
@@ -1782,11 +1901,12 @@ static int core_assemble(imago_ctx *c, int kind)
 // (rocBLAS's herk runs at a fraction of its speed at these shapes), so the
 // strictly lower triangle is overwritten as well; every reader of the
 // factor takes the upper triangle only.  IMAGO_POTRF=rocsolver selects
-// rocSOLVER's batched potrf instead.
-static int potrf_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int *dinfo)
+// rocSOLVER's batched potrf instead where allow_rocsolver.
+static int potrf_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int *dinfo,
+                         bool allow_rocsolver = true)
 {
     const rocblas_stride nn = (rocblas_stride)n * n;
-    if (c->tune_potrf == 1) {
+    if (c->tune_potrf == 1 && allow_rocsolver) {
         if (cx)
             BLASCHK(rocsolver_zpotrf_strided_batched(c->blas, rocblas_fill_upper, n,
                                                      (rocblas_double_complex *)B, n, nn, dinfo, nb));
@@ -2057,7 +2177,9 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         if (!c->stream2) {
             HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
             if (rocblas_create_handle(&c->blas2) != rocblas_status_success ||
-                rocblas_set_stream(c->blas2, c->stream2) != rocblas_status_success)
+                rocblas_set_stream(c->blas2, c->stream2) != rocblas_status_success ||
+                rocblas_set_atomics_mode(c->blas2, rocblas_atomics_not_allowed) !=
+                    rocblas_status_success)
                 return fail(IMAGO_E_HIP, "rocblas handle for the second stream");
             HIPCHK(hipEventCreateWithFlags(&c->evf, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&c->evj, hipEventDisableTiming));
@@ -3130,9 +3252,7 @@ static int elecstat_run(imago_ctx *c, double log_elec_thresh, double **out)
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     // potAlphaOverlap stays on the device for imago_pot_solve
     imago_ctx::SpdMat &m = c->spd[0];
-    for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.s})
-        if (q) dev_release(c, q);
-    m = imago_ctx::SpdMat{};
+    spd_release(c, m);
     RCHK(dev_alloc(c, &m.A, PP));
     HIPCHK(hipMemcpyAsync(m.A, buf, PP * 8, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -3152,6 +3272,10 @@ extern "C" int imago_elecstat(imago_ctx *c, double log_elec_thresh, double *pot_
     if (!pot_alpha_overlap || !non_local_neut_q_pot || !local_neut_q_pot ||
         !non_local_nuc_q_pot || !local_nuc_q_pot)
         return fail(IMAGO_E_ARG, "null output array");
+    if (!c || !c->has_system) return fail(IMAGO_E_STATE, "imago_set_system must come first");
+    // potAlphaOverlap is formed again: imago_scf_setup must follow before
+    // the next imago_scf_update
+    c->scf.ready = false;
     double *buf = nullptr;
     RCHK(elecstat_run(c, log_elec_thresh, &buf));
     const size_t P = (size_t)c->h.pot_dim, PP = P * P;
@@ -3403,6 +3527,8 @@ extern "C" int imago_xc_mesh(imago_ctx *c, const imago_xc_mesh_params *params,
     if (!c) return fail(IMAGO_E_ARG, "null context");
     if (!c->has_system) return fail(IMAGO_E_STATE, "imago_set_system must come first");
     if (!params || !covalent_radius) return fail(IMAGO_E_ARG, "null argument");
+    // a new mesh replaces the operators imago_scf_setup built
+    c->scf.ready = false;
     const imago_xc_mesh_params &p = *params;
     if (p.num_sample_vectors < 1 || p.num_sample_vectors > 100000)
         return fail(IMAGO_E_ARG, "num_sample_vectors %d out of range", p.num_sample_vectors);
@@ -3429,10 +3555,7 @@ extern "C" int imago_xc_mesh(imago_ctx *c, const imago_xc_mesh_params *params,
     c->xc_cached = false;
     {
         // an overlap of the previous mesh no longer applies
-        imago_ctx::SpdMat &m = c->spd[1];
-        for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.s})
-            if (q) dev_release(c, q);
-        m = imago_ctx::SpdMat{};
+        spd_release(c, c->spd[1]);
     }
     c->xc_ready = false;
     c->xcp = p;
@@ -3686,9 +3809,7 @@ static int xc_overlap_run(imago_ctx *c, double *exch_corr_overlap)
     {
         // exchCorrOverlap stays on the device for imago_pot_solve
         imago_ctx::SpdMat &m = c->spd[1];
-        for (void *q : {(void *)m.A, (void *)m.As, (void *)m.U, (void *)m.s})
-            if (q) dev_release(c, q);
-        m = imago_ctx::SpdMat{};
+        spd_release(c, m);
         m.A = C;
     }
     for (void *q : {(void *)A, (void *)Cs, (void *)nz, (void *)rows, (void *)rank, (void *)cnt,
@@ -3703,6 +3824,10 @@ static int xc_overlap_run(imago_ctx *c, double *exch_corr_overlap)
 extern "C" int imago_xc_overlap(imago_ctx *c, double *exch_corr_overlap)
 {
     if (!exch_corr_overlap) return fail(IMAGO_E_ARG, "null output array");
+    if (!c) return fail(IMAGO_E_ARG, "null context");
+    // exchCorrOverlap is formed again: imago_scf_setup must follow before
+    // the next imago_scf_update
+    c->scf.ready = false;
     return xc_overlap_run(c, exch_corr_overlap);
 }
 
@@ -3845,6 +3970,14 @@ extern "C" int imago_xc_potential(imago_ctx *c, const double *rho_total, const d
 }
 
 // ---- potential-side SPD solves (solveDPOSVX, interfaces.F90:236-293) ----
+
+// d(i) = A(i, i) of an n x n column-major matrix
+__global__ __launch_bounds__(256) void k_spd_diag(const double *__restrict__ A, int n,
+                                                  double *__restrict__ d)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = A[(size_t)i * (n + 1)];
+}
 
 // As(i, j) = s_i A(i, j) s_j on the upper triangle (j >= i), 0 below
 __global__ __launch_bounds__(256) void k_spd_scale(const double *__restrict__ A,
@@ -4029,10 +4162,17 @@ static int dev_gemv(imago_ctx *c, bool trans, int m, int n, const double *M, int
 // i for a_ii <= 0 as dpoequ.
 static int spd_factor(imago_ctx *c, imago_ctx::SpdMat &m, int n)
 {
+    // the diagonal, gathered on the device
     std::vector<double> diag(n), sc(n, 1.0);
-    HIPCHK(hipMemcpy2DAsync(diag.data(), sizeof(double), m.A, (size_t)(n + 1) * sizeof(double),
-                            sizeof(double), n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    {
+        double *dd = nullptr;
+        RCHK(dev_alloc(c, &dd, (size_t)n));
+        hipLaunchKernelGGL(k_spd_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           m.A, n, dd);
+        HIPCHK(hipGetLastError());
+        HIPCHK(ctx_memcpy(c, diag.data(), dd, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        dev_release(c, dd);
+    }
     double smin = diag[0], amax = diag[0];
     for (int i = 0; i < n; ++i) {
         smin = std::min(smin, diag[i]);
@@ -4063,29 +4203,36 @@ static int spd_factor(imago_ctx *c, imago_ctx::SpdMat &m, int n)
     HIPCHK(hipMemcpyAsync(m.U, m.As, nn * 8, hipMemcpyDeviceToDevice, c->stream));
     int *dinfo = nullptr;
     RCHK(dev_alloc(c, &dinfo, 1));
-    BLASCHK(rocsolver_dpotrf(c->blas, rocblas_fill_upper, n, m.U, n, dinfo));
+    RCHK(potrf_batched(c, false, n, 1, m.U, dinfo, false));
     int hinfo = 0;
-    HIPCHK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(ctx_memcpy(c, &hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost));
     m.info = hinfo;
     m.factored = true;
     if (hinfo == 0) {
-        // the inverse from the factor (dpotri), kept with the equilibrated
-        // matrix in full storage: every later solve is GEMMs against them
+        // the inverse from the factor (dpotri: U^-1 U^-T), kept with the
+        // equilibrated matrix in full storage: every later solve is
+        // products with them
         const unsigned g = (unsigned)((nn + 255) / 256);
         double *Ui = nullptr;
         RCHK(dev_alloc(c, &m.Af, nn));
         RCHK(dev_alloc(c, &m.Ainv, nn));
         RCHK(dev_alloc(c, &Ui, nn));
-        HIPCHK(hipMemcpyAsync(Ui, m.U, nn * 8, hipMemcpyDeviceToDevice, c->stream));
-        BLASCHK(rocsolver_dpotri(c->blas, rocblas_fill_upper, n, Ui, n, dinfo));
-        hipLaunchKernelGGL(k_sym_full, dim3(g), dim3(256), 0, c->stream, Ui, n, m.Ainv);
+        BLASCHK(rocblas_dtrtri(c->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, n, m.U, n,
+                               Ui, n));
+        imago::GemmArgs ga;
+        ga.M = ga.N = ga.K = n;
+        ga.lda = ga.ldb = ga.ldc = n;
+        ga.A = Ui;
+        ga.B = Ui;
+        ga.C = m.Ainv;
+        ga.amask = imago::kTriUpper;
+        ga.bmask = imago::kTriLower;
+        ga.herm = true;
+        HIPCHK(imago::launch_gemm_mfma(ga, false, false, true, c->stream));
         hipLaunchKernelGGL(k_sym_full, dim3(g), dim3(256), 0, c->stream, m.As, n, m.Af);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         dev_release(c, Ui);
-        if (hinfo != 0) return fail(IMAGO_E_SOLVE, "dpotri of the factored overlap: info %d", hinfo);
     }
     dev_release(c, dinfo);
     return IMAGO_OK;
@@ -4098,6 +4245,8 @@ static int spd_factor(imago_ctx *c, imago_ctx::SpdMat &m, int n)
 static int pot_solve_dev(imago_ctx *c, imago_ctx::SpdMat &m, int n, int nrhs, const double *b,
                          double *x, double *scr)
 {
+    if (!m.factored || m.info != 0 || !m.Ainv || !m.Af || !m.s)
+        return fail(IMAGO_E_STATE, "solve with an overlap that has no factor");
     const size_t nb = (size_t)n * nrhs;
     double *bs = scr, *r = scr + nb;
     const unsigned g = (unsigned)((nb + 255) / 256);
@@ -4415,7 +4564,9 @@ extern "C" int imago_scf_update(imago_ctx *c, const double *traces3, const doubl
                                 double *pot_coeffs, double *rho_coeffs,
                                 imago_scf_record *record)
 {
-    if (!c || !c->scf.ready) return fail(IMAGO_E_STATE, "imago_scf_update before imago_scf_setup");
+    if (!c || !c->scf.ready)
+        return fail(IMAGO_E_STATE, "imago_scf_update before imago_scf_setup (or after a call that "
+                                   "formed its operators again)");
     if (!traces3 || !pot_rho || !pot_coeffs || !record) return fail(IMAGO_E_ARG, "null argument");
     HIPCHK(hipSetDevice(c->device));
     const HostSys &h = c->h;

@@ -4,7 +4,8 @@ imago_tune exposes the run-time choices of the per-iteration stage: the
 columns per diagonal block of the blocked Cholesky (8, 16, 24, 32), and the
 eigensolver (this library's cooperative tridiagonal reduction with divide
 and conquer and the compact-WY back-transform, or rocSOLVER's heevd /
-syevd).  The best choice depends on the matrix order, the batch (k-points)
+syevd, which the tuner offers only beyond the orders the library's
+reduction takes).  The best choice depends on the matrix order, the batch (k-points)
 and the scalar type; measured optima are 32 columns for 4 x 896 complex
 and 16 for 1 x 1944 real.  ``Autotuner`` times each
 candidate on the system the context holds, two passes of imago_secular,
@@ -50,7 +51,12 @@ class Autotuner:
                 self.cache = json.load(f)
 
     def candidates(self, system):
-        solvers = (0, 1) if coop_supported(system) else (1,)
+        """Cholesky block sizes with this library's eigensolver, or with
+        rocSOLVER's heevd / syevd only for orders the library's reduction
+        does not take: rocSOLVER's factorizations have returned wrong
+        results now and then while other contexts drove the device from
+        other host threads, as the batched stream does."""
+        solvers = (0,) if coop_supported(system) else (1,)
         return [dict(potrf_block=b, eigensolver=e) for e in solvers for b in BLOCKS]
 
     def choose(self, ctx, system, coeffs):

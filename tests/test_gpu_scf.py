@@ -51,3 +51,28 @@ def test_scf_si2_primitive_matches_oracle(gpu_ctx):
     res = _compare(gpu_ctx, s, configs.pot_coefficients(s), "si2")
     # the fort.7 row and the driver's read-out of it
     assert res.records[-1]["testable_delta"] < 1e-4
+
+
+def test_scf_update_after_operators_formed_again(gpu_ctx):
+    """imago_elecstat, imago_xc_mesh and imago_xc_overlap form the
+    operators imago_scf_setup holds again: the next imago_scf_update is
+    refused until the set-up runs again, and then gives the same result."""
+    from imago_amd import lib
+    s = configs.small_test_system(seed=5)
+    c0 = configs.pot_coefficients(s)
+    scf.setup(gpu_ctx, s)
+    ev = gpu_ctx.secular(c0, s.vale_dim)
+    occ, _ = lib.populate(ev, s.kpoint_weights, num(s))
+    tr, prho = gpu_ctx.density_traces(occ)
+    c_ref, rec_ref, _ = gpu_ctx.scf_update(tr, prho, c0)
+    for again in (lambda: gpu_ctx.xc_overlap(), lambda: gpu_ctx.elecstat(),
+                  lambda: gpu_ctx.xc_mesh()):
+        print("[gpu] operators formed again, then scf_update", flush=True)
+        gpu_ctx.scf_setup(num(s))
+        again()
+        with pytest.raises(lib.ImagoError):
+            gpu_ctx.scf_update(tr, prho, c0)
+        gpu_ctx.scf_setup(num(s))
+        c1, rec1, _ = gpu_ctx.scf_update(tr, prho, c0)
+        assert rec1["total_energy"] == rec_ref["total_energy"]
+        assert np.array_equal(c1, c_ref)
