@@ -17,8 +17,9 @@
 //   w = p - tau/2 (p^H v) v, applies A -= v w^H + w v^H to its rows and
 //   the same update to row j+1, the next pivot row.
 // Only p, the dot partials and one row per step go through memory.  The
-// tridiagonal (d, e) and tau are written by work-group 0, the reflectors
-// (LAPACK's lower layout) by the work-group owning their rows.
+// tridiagonal (d, e) and tau are written by the last work-group, the
+// reflectors (LAPACK's lower layout) by the work-group owning their rows; a
+// work-group leaves once all its rows are reduced.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -231,7 +232,9 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     T *V = A_.V + m * nn;
     T *rbuf = A_.rbuf + m * nn, *pbuf = A_.pbuf + m * nn;
     T *dbuf = A_.dbuf + (size_t)m * n * kTrdMaxGroups;
-    const bool out = w == 0;   // work-group 0 writes d, e and tau
+    // the last work-group, which takes part in every step, writes d, e
+    // and tau
+    const bool out = w == P - 1;
 
     T a[S2][RPL];
 #pragma unroll
@@ -264,6 +267,10 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int j = 0; j + 1 < n; ++j) {
+        // A work-group whose rows all lie at or before j has nothing left
+        // to do: it leaves, and the others no longer wait for its (zero)
+        // dot partial.
+        if (kTrdRows * w + kTrdRows - 1 <= j) break;
         T *ps = pbuf + (size_t)j * n, *rs = rbuf + (size_t)j * n;
         T *ds = dbuf + (size_t)j * kTrdMaxGroups;
         if (w == ((j + 1) >> 4))
@@ -367,8 +374,9 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
             for (int k = 0; k < MC; ++k)
                 if (j + 1 + t + kTrdThreads * k < n) pend |= 3u << (2 * k);
             if (wave == 0) {
-                if (lane < P) pend |= 1u << 30;
-                if (lane + 64 < P) pend |= 1u << 31;
+                const int first = (j + 1) / kTrdRows;   // the first work-group still active
+                if (lane < P && lane >= first) pend |= 1u << 30;
+                if (lane + 64 < P && lane + 64 >= first) pend |= 1u << 31;
             }
             T d0 = sc_zero<T>(), d1 = sc_zero<T>();
             int polls = 0;
