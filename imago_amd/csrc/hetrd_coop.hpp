@@ -112,10 +112,67 @@ __device__ __forceinline__ double2 sc_sel(bool c, double2 x, double2 y)
 // issued where it stands and not moved into a later conditional block.
 __device__ __forceinline__ void sc_pin(double &x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void sc_pin(double2 &x) { asm volatile("" : "+v"(x.x), "+v"(x.y)); }
-__device__ __forceinline__ double sc_shfl_xor(double v, int m) { return __shfl_xor(v, m); }
-__device__ __forceinline__ double2 sc_shfl_xor(double2 v, int m)
+// v of lane (this lane xor O) as VALU lane exchanges, not LDS permutes:
+// xor 1 and 2 by quad permutes, xor 4 as xor 3 (quad permute) then xor 7
+// (half-row mirror), xor 8 by a row rotation of 8, xor 16 and 32 by the
+// gfx950 row / half-wave swaps.  Every lane of the exchange must be active.
+template <int O>
+__device__ __forceinline__ unsigned xor_lane_u32(unsigned x)
 {
-    return make_double2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
+    if constexpr (O == 1) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (O == 2) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (O == 4) {
+        const int y = __builtin_amdgcn_mov_dpp((int)x, 0x1B, 0xF, 0xF, false);
+        return (unsigned)__builtin_amdgcn_mov_dpp(y, 0x141, 0xF, 0xF, false);
+    } else if constexpr (O == 8) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, false);
+    } else if constexpr (O == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        return (threadIdx.x & 16) ? r[0] : r[1];
+    } else {
+        static_assert(O == 32, "lane exchange");
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return (threadIdx.x & 32) ? r[0] : r[1];
+    }
+}
+template <int O>
+__device__ __forceinline__ double sc_xor_lane(double v)
+{
+    const unsigned lo = xor_lane_u32<O>((unsigned)__double2loint(v));
+    const unsigned hi = xor_lane_u32<O>((unsigned)__double2hiint(v));
+    return __hiloint2double((int)hi, (int)lo);
+}
+template <int O>
+__device__ __forceinline__ double2 sc_xor_lane(double2 v)
+{
+    return make_double2(sc_xor_lane<O>(v.x), sc_xor_lane<O>(v.y));
+}
+// butterfly sums (every lane of the group ends with the same total): over
+// the lanes xor 8, 16, 32 (the column groups of one row phase), over 16
+// lanes, over the wave
+template <class T>
+__device__ __forceinline__ T sc_sum_xor_8_16_32(T v)
+{
+    v = sc_add(v, sc_xor_lane<8>(v));
+    v = sc_add(v, sc_xor_lane<16>(v));
+    return sc_add(v, sc_xor_lane<32>(v));
+}
+template <class T>
+__device__ __forceinline__ T sc_sum16(T v)
+{
+    v = sc_add(v, sc_xor_lane<8>(v));
+    v = sc_add(v, sc_xor_lane<4>(v));
+    v = sc_add(v, sc_xor_lane<2>(v));
+    return sc_add(v, sc_xor_lane<1>(v));
+}
+template <class T>
+__device__ __forceinline__ T sc_sum64(T v)
+{
+    v = sc_add(v, sc_xor_lane<32>(v));
+    v = sc_add(v, sc_xor_lane<16>(v));
+    return sc_sum16(v);
 }
 
 // The vectors exchanged between work-groups (p, pivot rows, dot partials)
@@ -181,12 +238,7 @@ struct TrdArgs {
 };
 
 // wave sum of a double; every lane gets the total
-__device__ __forceinline__ double trd_wave_sum(double v)
-{
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
+__device__ __forceinline__ double trd_wave_sum(double v) { return sc_sum64(v); }
 
 // grid = nb * P work-groups of kTrdThreads, all co-resident (cooperative
 // launch); n <= 32 * S and P <= kTrdMaxGroups.  One column step j:
@@ -343,12 +395,13 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        // sums over the column groups of the wave, each of the lane's rows
+        // sums over the column groups of the wave (lanes xor 8, 16, 32),
+        // each of the lane's rows
+        static_assert(LR == 8, "column groups of a row phase");
 #pragma unroll
         for (int h = 0; h < RPL; ++h) {
             sp[h] = on[h] ? sp[h] : sc_zero<T>();
-#pragma unroll
-            for (int o = LR; o < 64; o <<= 1) sp[h] = sc_add(sp[h], sc_shfl_xor(sp[h], o));
+            sp[h] = sc_sum_xor_8_16_32(sp[h]);
         }
         if (lane < LR) {
 #pragma unroll
@@ -362,7 +415,7 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
             if (row_on) pub_store(ps + it, p);
             // conj(p_i) v_i over the 16 rows, the same tree in every work-group
             T dp = row_on ? sc_mulc(vl[it], p) : sc_zero<T>();
-            for (int o = 8; o >= 1; o >>= 1) dp = sc_add(dp, sc_shfl_xor(dp, o));
+            dp = sc_sum16(dp);
             if (t == 0) pub_store(ds + w, dp);
         }
 
@@ -429,7 +482,7 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
             if (wave == 0) {
                 // partials of work-groups lane and lane + 64, then a fixed tree
                 T x = sc_add(d0, d1);
-                for (int o = 32; o >= 1; o >>= 1) x = sc_add(x, sc_shfl_xor(x, o));
+                x = sc_sum64(x);
                 if (lane == 0) dot = x;
             }
         }
