@@ -30,6 +30,7 @@
 // launch per group of four.
 #pragma once
 #include "ep_kernel.hpp"
+#include "lane_xor.hpp"
 
 namespace imago {
 
@@ -100,12 +101,7 @@ __device__ __forceinline__ double epd_readlane(double v, int src)
 }
 
 // wave sum of a double (every lane gets the total)
-__device__ __forceinline__ double epd_wave_sum(double v)
-{
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
+__device__ __forceinline__ double epd_wave_sum(double v) { return wave_sum_xor(v); }
 
 // ---------------------------------------------------------------------------
 // set-up: image keys, pair ranges, runs
@@ -454,14 +450,12 @@ __global__ __launch_bounds__(64) void k_epd_trace(DevSys S, const PairRec *__res
                 // xor-32 and xor-16 steps, then sum within 16 lanes; lane
                 // 16 g ends with item g's total
                 const bool hi32 = lane >= 32, hi16 = (lane & 16) != 0;
-                const double x0 = __shfl_xor(hi32 ? s[0] : s[2 % G], 32, 64);
-                const double x1 = __shfl_xor(hi32 ? s[1 % G] : s[3 % G], 32, 64);
+                const double x0 = xor_lane<32>(hi32 ? s[0] : s[2 % G]);
+                const double x1 = xor_lane<32>(hi32 ? s[1 % G] : s[3 % G]);
                 const double a0 = (hi32 ? s[2 % G] : s[0]) + x0;
                 const double a1v = (hi32 ? s[3 % G] : s[1 % G]) + x1;
-                const double y = __shfl_xor(hi16 ? a0 : a1v, 16, 64);
-                tot = (hi16 ? a1v : a0) + y;
-#pragma unroll
-                for (int dd = 8; dd > 0; dd >>= 1) tot += __shfl_xor(tot, dd, 64);
+                const double y = xor_lane<16>(hi16 ? a0 : a1v);
+                tot = sum16_xor((hi16 ? a1v : a0) + y);
                 slot = (lane >> 4) & 3;
             } else {
                 tot = epd_wave_sum(s[0]);

@@ -23,6 +23,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "lane_xor.hpp"
+
 namespace imago {
 
 constexpr int kTrdThreads = 512;
@@ -112,37 +114,11 @@ __device__ __forceinline__ double2 sc_sel(bool c, double2 x, double2 y)
 // issued where it stands and not moved into a later conditional block.
 __device__ __forceinline__ void sc_pin(double &x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void sc_pin(double2 &x) { asm volatile("" : "+v"(x.x), "+v"(x.y)); }
-// v of lane (this lane xor O) as VALU lane exchanges, not LDS permutes:
-// xor 1 and 2 by quad permutes, xor 4 as xor 3 (quad permute) then xor 7
-// (half-row mirror), xor 8 by a row rotation of 8, xor 16 and 32 by the
-// gfx950 row / half-wave swaps.  Every lane of the exchange must be active.
-template <int O>
-__device__ __forceinline__ unsigned xor_lane_u32(unsigned x)
-{
-    if constexpr (O == 1) {
-        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
-    } else if constexpr (O == 2) {
-        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
-    } else if constexpr (O == 4) {
-        const int y = __builtin_amdgcn_mov_dpp((int)x, 0x1B, 0xF, 0xF, false);
-        return (unsigned)__builtin_amdgcn_mov_dpp(y, 0x141, 0xF, 0xF, false);
-    } else if constexpr (O == 8) {
-        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, false);
-    } else if constexpr (O == 16) {
-        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
-        return (threadIdx.x & 16) ? r[0] : r[1];
-    } else {
-        static_assert(O == 32, "lane exchange");
-        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
-        return (threadIdx.x & 32) ? r[0] : r[1];
-    }
-}
+// v of lane (this lane xor O), by the VALU exchanges of lane_xor.hpp
 template <int O>
 __device__ __forceinline__ double sc_xor_lane(double v)
 {
-    const unsigned lo = xor_lane_u32<O>((unsigned)__double2loint(v));
-    const unsigned hi = xor_lane_u32<O>((unsigned)__double2hiint(v));
-    return __hiloint2double((int)hi, (int)lo);
+    return xor_lane<O>(v);
 }
 template <int O>
 __device__ __forceinline__ double2 sc_xor_lane(double2 v)

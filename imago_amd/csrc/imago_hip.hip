@@ -32,6 +32,7 @@
 #include "geom_kernel.hpp"
 #include "hetrd_launch.hpp"
 #include "gemm_mfma_launch.hpp"
+#include "lane_xor.hpp"
 #include "tridiag_dc_launch.hpp"
 #include "dos_launch.hpp"
 #include "core_ortho_launch.hpp"
@@ -4013,12 +4014,7 @@ __global__ __launch_bounds__(256) void k_spd_rows(double *__restrict__ B,
 // a few right-hand sides), spread over the whole device: rocBLAS runs these
 // shapes as GEMMs or row-block GEMVs on 16 work-groups (35-60 us for 8 MB).
 // Every sum has a fixed order, so results do not depend on timing.
-__device__ __forceinline__ double gv_wave_sum(double v)
-{
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
+__device__ __forceinline__ double gv_wave_sum(double v) { return imago::wave_sum_xor(v); }
 
 // y(j, r) = alpha * sum_i M(i, j) x(i, r) + beta * y(j, r) for an m x n
 // column-major M (leading dimension ld) and R right-hand sides: one wave per

@@ -18,6 +18,7 @@
 // The children of a merge of height h have heights h-1 and h-2 at most, so
 // three eigenvector buffers indexed by height mod 3 never overlap a merge's
 // inputs with its output.
+#include "lane_xor.hpp"
 #include "tridiag_dc.hpp"
 #include "tridiag_dc_launch.hpp"
 
@@ -34,15 +35,13 @@ namespace {
 struct Wave {
     __device__ int lane() const { return threadIdx.x & 63; }
     __device__ int width() const { return 64; }
-    __device__ double sum(double x) const
+    __device__ double sum(double x) const { return bcast0(wave_sum_xor(x)); }
+    __device__ double prod(double x) const { return bcast0(wave_prod_xor(x)); }
+    static __device__ double bcast0(double x)
     {
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        return __shfl(x, 0, 64);
-    }
-    __device__ double prod(double x) const
-    {
-        for (int o = 32; o > 0; o >>= 1) x *= __shfl_xor(x, o, 64);
-        return __shfl(x, 0, 64);
+        const int lo = __builtin_amdgcn_readlane(__double2loint(x), 0);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 0);
+        return __hiloint2double(hi, lo);
     }
 };
 
