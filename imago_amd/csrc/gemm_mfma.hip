@@ -87,6 +87,7 @@ __device__ __forceinline__ TileRange tile_of(const GemmArgs &g, int idx, int inn
     t.j0 = tj * kGT;
     t.ok = ti < tm && tj < tn;
     int klo = inner * g.koff2, khi = g.K;
+    if (g.klen2 > 0) khi = min(khi, klo + g.klen2);
     if (g.amask == kTriUpper) klo = max(klo, t.i0);
     if (g.amask == kTriLower) khi = min(khi, t.i0 + kGT);
     if (g.bmask == kTriUpper) khi = min(khi, t.j0 + kGT);
@@ -306,6 +307,7 @@ hipError_t launch_gemm_mfma(const GemmArgs &g, bool complex, bool conj_a, bool c
 {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return hipSuccess;
     if (g.nb2 <= 0 || g.batch > 65535) return hipErrorInvalidValue;
+    if (g.klen2 > 0 && g.koff2 % 16 != 0) return hipErrorInvalidValue;
     if (g.herm && (g.M != g.N || g.beta_re != 0.0 || g.beta_im != 0.0))
         return hipErrorInvalidValue;
     if (complex) {

@@ -33,8 +33,11 @@ struct GemmArgs {
     int amask = kTriNone, bmask = kTriNone;
     bool herm = false;
     // the K range of entry (outer, inner) starts at inner * koff2 (the
-    // operands are zero before it)
+    // operands are zero before it) and, with klen2 > 0, holds at most klen2
+    // values (a split of K over the inner entries, each into its own C);
+    // koff2 a multiple of 16 then
     int koff2 = 0;
+    int klen2 = 0;
 };
 
 // complex: double2 elements (interleaved re, im), else double; conj_a /

@@ -2307,6 +2307,42 @@ static void pending_clear(imago_ctx *c)
 // W = T_b W, Y -= V_b W as batched GEMMs over these columns only), then
 // X = U^-1 Y.  Columns are independent, so later calls continue where this
 // one stops.
+__device__ __forceinline__ double part_add(double a, double b) { return a + b; }
+__device__ __forceinline__ double2 part_add(double2 a, double2 b)
+{
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+
+// W[m](r, j) = sum over c < nch in order of P[m][c](r, j) for r < kb
+// (each partial K x N with leading dimension K, rows kb.. unused)
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_partials(const T *__restrict__ P, int nch, int kb,
+                                                      int K, int N, T *__restrict__ W)
+{
+    const int m = blockIdx.y;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x, per = (size_t)K * N;
+    if (e >= per || (int)(e % K) >= kb) return;
+    const T *p = P + (size_t)m * nch * per + e;
+    T s = p[0];
+    for (int q = 1; q < nch; ++q) s = part_add(s, p[(size_t)q * per]);
+    W[(size_t)m * per + e] = s;
+}
+
+static hipError_t launch_sum_partials(const void *P, int nch, int kb, int K, int N, int nb,
+                                      void *W, bool cx, hipStream_t st)
+{
+    const dim3 grid((unsigned)(((size_t)K * N + 255) / 256), (unsigned)nb);
+    if (cx)
+        hipLaunchKernelGGL(k_sum_partials<double2>, grid, dim3(256), 0, st,
+                           static_cast<const double2 *>(P), nch, kb, K, N,
+                           static_cast<double2 *>(W));
+    else
+        hipLaunchKernelGGL(k_sum_partials<double>, grid, dim3(256), 0, st,
+                           static_cast<const double *>(P), nch, kb, K, N,
+                           static_cast<double *>(W));
+    return hipGetLastError();
+}
+
 static int finish_columns(imago_ctx *c, int upto)
 {
     imago_ctx::PendingVectors &p = c->pend;
@@ -2379,18 +2415,42 @@ static int finish_columns(imago_ctx *c, int upto)
         }
     } else if (p.reflectors) {
         const int K = kTrdBlock, nblk = (n - 1 + K - 1) / K;
-        char *W = nullptr, *W2 = nullptr;
+        // W = V_b^H Y_b is K x N over a long K range: it is split into
+        // chunks of kBtChunk rows, each product a separate matrix-core
+        // launch entry, and the partials summed in order
+        constexpr int kBtChunk = 128;
+        const int nchm = (n - 1 + kBtChunk - 1) / kBtChunk;
+        char *W = nullptr, *W2 = nullptr, *Wp = nullptr;
         RCHK(dev_alloc(c, &W, (size_t)nb * K * N * es));
         RCHK(dev_alloc(c, &W2, (size_t)nb * K * N * es));
+        RCHK(dev_alloc(c, &Wp, (size_t)nb * nchm * K * N * es));
         const rocblas_stride snn = (rocblas_stride)nn, sw = (rocblas_stride)K * N,
                              st = (rocblas_stride)std::max(nblk, 1) * K * K;
         for (int b = nblk - 1; b >= 0; --b) {
             const int j0 = b * K, kb = std::min(K, n - 1 - j0), r0 = j0 + 1, mb = n - r0;
             const char *Vb = p.V + ((size_t)j0 * n + r0) * es;
             char *Yb = Ac + (size_t)r0 * es;
-            BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_conjugate_transpose,
-                            rocblas_operation_none, kb, N, mb, 1.0, Vb, n, snn, Yb, n, snn, 0.0, W,
-                            K, sw, nb));
+            const int nch = (mb + kBtChunk - 1) / kBtChunk;
+            GemmArgs gw;
+            gw.M = kb;
+            gw.N = N;
+            gw.K = mb;
+            gw.lda = n;
+            gw.ldb = n;
+            gw.ldc = K;
+            gw.batch = nb * nch;
+            gw.nb2 = nch;
+            gw.A = Vb;
+            gw.sa1 = (long)nn;
+            gw.B = Yb;
+            gw.sb1 = (long)nn;
+            gw.C = Wp;
+            gw.sc1 = (long)nch * K * N;
+            gw.sc2 = (long)K * N;
+            gw.koff2 = kBtChunk;
+            gw.klen2 = kBtChunk;
+            HIPCHK(launch_gemm_mfma(gw, cx, true, false, c->stream));
+            HIPCHK(launch_sum_partials(Wp, nch, kb, K, N, nb, W, cx, c->stream));
             BLASCHK(gemm_sb(c->blas, cx, rocblas_operation_none, rocblas_operation_none, kb, N, kb,
                             1.0, p.Tf + (size_t)b * K * K * es, K, st, W, K, sw, 0.0, W2, K, sw,
                             nb));
@@ -2399,6 +2459,7 @@ static int finish_columns(imago_ctx *c, int upto)
         }
         dev_release(c, W);
         dev_release(c, W2);
+        dev_release(c, Wp);
     }
     HIPCHK(hipEventRecord(c->evs[5], c->stream));
     const rocblas_stride nns = (rocblas_stride)nn;
