@@ -2418,7 +2418,7 @@ static int finish_columns(imago_ctx *c, int upto)
         // W = V_b^H Y_b is K x N over a long K range: it is split into
         // chunks of kBtChunk rows, each product a separate matrix-core
         // launch entry, and the partials summed in order
-        constexpr int kBtChunk = 128;
+        constexpr int kBtChunk = 64;
         const int nchm = (n - 1 + kBtChunk - 1) / kBtChunk;
         char *W = nullptr, *W2 = nullptr, *Wp = nullptr;
         RCHK(dev_alloc(c, &W, (size_t)nb * K * N * es));
