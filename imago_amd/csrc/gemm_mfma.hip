@@ -53,7 +53,7 @@ struct TileRange {
     bool ok;
 };
 
-__device__ __forceinline__ TileRange tile_of(const GemmArgs &g, int idx, int inner)
+__device__ __forceinline__ TileRange tile_of(const GemmArgs &g, int idx, int inner, int chunk)
 {
     const int tm = (g.M + kGT - 1) / kGT, tn = (g.N + kGT - 1) / kGT;
     // Tiles whose K range is longest are handed out first (the work-groups
@@ -87,7 +87,10 @@ __device__ __forceinline__ TileRange tile_of(const GemmArgs &g, int idx, int inn
     t.j0 = tj * kGT;
     t.ok = ti < tm && tj < tn;
     int klo = inner * g.koff2, khi = g.K;
-    if (g.klen2 > 0) khi = min(khi, klo + g.klen2);
+    if (g.klen2 > 0) {
+        klo += chunk * g.klen2;
+        khi = min(khi, klo + g.klen2);
+    }
     if (g.amask == kTriUpper) klo = max(klo, t.i0);
     if (g.amask == kTriLower) khi = min(khi, t.i0 + kGT);
     if (g.bmask == kTriUpper) khi = min(khi, t.j0 + kGT);
@@ -105,12 +108,13 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
     // two stages: the MFMAs read one while the next K step is written to the other
     __shared__ double As[2][CX ? 2 : 1][kGK][kGLd];
     __shared__ double Bs[2][CX ? 2 : 1][kGK][kGLd];
-    const int bi = blockIdx.y, outer = bi / g.nb2, inner = bi % g.nb2;
-    const TileRange tr = tile_of(g, blockIdx.x, inner);
+    const int chunk = blockIdx.y % g.nb3, bi = blockIdx.y / g.nb3;
+    const int outer = bi / g.nb2, inner = bi % g.nb2;
+    const TileRange tr = tile_of(g, blockIdx.x, inner, chunk);
     if (!tr.ok) return;
     const T *A = static_cast<const T *>(g.A) + outer * g.sa1 + inner * g.sa2;
     const T *B = static_cast<const T *>(g.B) + outer * g.sb1 + inner * g.sb2;
-    T *C = static_cast<T *>(g.C) + outer * g.sc1 + inner * g.sc2;
+    T *C = static_cast<T *>(g.C) + outer * g.sc1 + inner * g.sc2 + chunk * g.sc3;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wi = wv & 1, wj = wv >> 1;
     const int i0 = tr.i0, j0 = tr.j0;
@@ -307,7 +311,9 @@ hipError_t launch_gemm_mfma(const GemmArgs &g, bool complex, bool conj_a, bool c
 {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return hipSuccess;
     if (g.nb2 <= 0 || g.batch > 65535) return hipErrorInvalidValue;
-    if (g.klen2 > 0 && g.koff2 % 16 != 0) return hipErrorInvalidValue;
+    if (g.nb3 <= 0 || g.batch % g.nb3 != 0) return hipErrorInvalidValue;
+    if (g.klen2 > 0 && (g.koff2 % 16 != 0 || g.klen2 % 16 != 0)) return hipErrorInvalidValue;
+    if (g.nb3 > 1 && g.klen2 <= 0) return hipErrorInvalidValue;
     if (g.herm && (g.M != g.N || g.beta_re != 0.0 || g.beta_im != 0.0))
         return hipErrorInvalidValue;
     if (complex) {

@@ -17,8 +17,9 @@ enum GemmTri : int {
 };
 
 // C_b = alpha op(A_b) op(B_b) + beta C_b for every batch entry b, column-
-// major, op(X) = X or X^H (X^T for real data).  Entry b = outer * nb2 +
-// inner takes its matrices at X + outer * sx1 + inner * sx2 (in elements).
+// major, op(X) = X or X^H (X^T for real data).  Entry b = (outer * nb2 +
+// inner) * nb3 + chunk takes its matrices at X + outer * sx1 + inner * sx2
+// (in elements), and C at C + outer * sc1 + inner * sc2 + chunk * sc3.
 // herm: C is Hermitian (symmetric): only its lower 64 x 64 tiles are formed
 // and each element is written to (i, j) and, conjugated, to (j, i), with
 // the diagonal made real; beta must be 0 then and M == N.
@@ -32,12 +33,14 @@ struct GemmArgs {
     double alpha_re = 1.0, alpha_im = 0.0, beta_re = 0.0, beta_im = 0.0;
     int amask = kTriNone, bmask = kTriNone;
     bool herm = false;
-    // the K range of entry (outer, inner) starts at inner * koff2 (the
-    // operands are zero before it) and, with klen2 > 0, holds at most klen2
-    // values (a split of K over the inner entries, each into its own C);
-    // koff2 a multiple of 16 then
+    // the K range of entry (outer, inner, chunk) starts at inner * koff2
+    // (the operands are zero before it) plus, with klen2 > 0, chunk * klen2
+    // and holds at most klen2 values (a split of K, each part into its own
+    // C, to be summed by the caller); koff2 and klen2 multiples of 16 then
     int koff2 = 0;
     int klen2 = 0;
+    int nb3 = 1;
+    long sc3 = 0;
 };
 
 // complex: double2 elements (interleaved re, im), else double; conj_a /

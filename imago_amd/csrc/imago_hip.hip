@@ -1983,6 +1983,42 @@ static int factor_gv_batched(imago_ctx *c, bool cx, int n, int nb, void *B, int 
 // divide and conquer of this library).
 // The default up to order 1024 (complex) / 2048 (real); IMAGO_HEEVD =
 // rocsolver selects rocSOLVER's batched heevd / syevd instead.
+__device__ __forceinline__ double part_add(double a, double b) { return a + b; }
+__device__ __forceinline__ double2 part_add(double2 a, double2 b)
+{
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+
+// W[m](r, j) = sum over c < nch in order of P[m][c](r, j) for r < kb
+// (each partial K x N with leading dimension K, rows kb.. unused)
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_partials(const T *__restrict__ P, int nch, int kb,
+                                                      int K, int N, T *__restrict__ W)
+{
+    const int m = blockIdx.y;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x, per = (size_t)K * N;
+    if (e >= per || (int)(e % K) >= kb) return;
+    const T *p = P + (size_t)m * nch * per + e;
+    T s = p[0];
+    for (int q = 1; q < nch; ++q) s = part_add(s, p[(size_t)q * per]);
+    W[(size_t)m * per + e] = s;
+}
+
+static hipError_t launch_sum_partials(const void *P, int nch, int kb, int K, int N, int nb,
+                                      void *W, bool cx, hipStream_t st)
+{
+    const dim3 grid((unsigned)(((size_t)K * N + 255) / 256), (unsigned)nb);
+    if (cx)
+        hipLaunchKernelGGL(k_sum_partials<double2>, grid, dim3(256), 0, st,
+                           static_cast<const double2 *>(P), nch, kb, K, N,
+                           static_cast<double2 *>(W));
+    else
+        hipLaunchKernelGGL(k_sum_partials<double>, grid, dim3(256), 0, st,
+                           static_cast<const double *>(P), nch, kb, K, N,
+                           static_cast<double *>(W));
+    return hipGetLastError();
+}
+
 static bool use_coop_heevd(const imago_ctx *c, int n, bool cx)
 {
     return hetrd_coop_supported(n, !cx) && c->tune_heevd == 0;
@@ -2240,21 +2276,32 @@ static int heevd_coop(imago_ctx *c, bool cx, int n, int nb, void *A, double *eva
         // there), then the factors T_b in place of G_b; the columns a last
         // block reads beyond n only reach G entries the factor ignores
         if (nblk > 0) {
+            // K split into chunks of kGramChunk rows (a launch entry each),
+            // the partials summed in order into T's buffer
+            constexpr int kGramChunk = 128;
+            const int nch = (n + kGramChunk - 1) / kGramChunk;
+            char *Gp = nullptr;
+            RCHK(dev_alloc(c, &Gp, (size_t)nb * nblk * nch * K * K * es));
             GemmArgs g;
             g.M = g.N = K;
             g.K = n;
             g.lda = g.ldb = n;
             g.ldc = K;
-            g.batch = nb * nblk;
+            g.batch = nb * nblk * nch;
             g.nb2 = nblk;
+            g.nb3 = nch;
             g.A = g.B = V;
             g.sa1 = g.sb1 = (long)nn;
             g.sa2 = g.sb2 = (long)sw;
-            g.C = Tf;
-            g.sc1 = (long)st;
-            g.sc2 = (long)K * K;
+            g.C = Gp;
+            g.sc1 = (long)nblk * nch * K * K;
+            g.sc2 = (long)nch * K * K;
+            g.sc3 = (long)K * K;
             g.koff2 = K;
+            g.klen2 = kGramChunk;
             HIPCHK(launch_gemm_mfma(g, cx, true, false, c->stream));
+            HIPCHK(launch_sum_partials(Gp, nch, K, K, K, nb * nblk, Tf, cx, c->stream));
+            dev_release(c, Gp);
         }
         HIPCHK(launch_trd_larft(tau, n, nb, !cx, Tf, c->stream));
     }
@@ -2307,42 +2354,6 @@ static void pending_clear(imago_ctx *c)
 // W = T_b W, Y -= V_b W as batched GEMMs over these columns only), then
 // X = U^-1 Y.  Columns are independent, so later calls continue where this
 // one stops.
-__device__ __forceinline__ double part_add(double a, double b) { return a + b; }
-__device__ __forceinline__ double2 part_add(double2 a, double2 b)
-{
-    return make_double2(a.x + b.x, a.y + b.y);
-}
-
-// W[m](r, j) = sum over c < nch in order of P[m][c](r, j) for r < kb
-// (each partial K x N with leading dimension K, rows kb.. unused)
-template <class T>
-__global__ __launch_bounds__(256) void k_sum_partials(const T *__restrict__ P, int nch, int kb,
-                                                      int K, int N, T *__restrict__ W)
-{
-    const int m = blockIdx.y;
-    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x, per = (size_t)K * N;
-    if (e >= per || (int)(e % K) >= kb) return;
-    const T *p = P + (size_t)m * nch * per + e;
-    T s = p[0];
-    for (int q = 1; q < nch; ++q) s = part_add(s, p[(size_t)q * per]);
-    W[(size_t)m * per + e] = s;
-}
-
-static hipError_t launch_sum_partials(const void *P, int nch, int kb, int K, int N, int nb,
-                                      void *W, bool cx, hipStream_t st)
-{
-    const dim3 grid((unsigned)(((size_t)K * N + 255) / 256), (unsigned)nb);
-    if (cx)
-        hipLaunchKernelGGL(k_sum_partials<double2>, grid, dim3(256), 0, st,
-                           static_cast<const double2 *>(P), nch, kb, K, N,
-                           static_cast<double2 *>(W));
-    else
-        hipLaunchKernelGGL(k_sum_partials<double>, grid, dim3(256), 0, st,
-                           static_cast<const double *>(P), nch, kb, K, N,
-                           static_cast<double *>(W));
-    return hipGetLastError();
-}
-
 static int finish_columns(imago_ctx *c, int upto)
 {
     imago_ctx::PendingVectors &p = c->pend;
