@@ -318,11 +318,12 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd_coop(TrdArgs<T> A_)
                 scale = sc_zero<T>();
             } else {
                 beta = -copysign(sqrt(ar * ar + ai * ai + xn2), ar);
-                tau = sc_make<T>((beta - ar) / beta, -ai / beta);
-                // 1 / (alpha - beta)
+                // one reciprocal each for tau and 1 / (alpha - beta)
+                const double rb = 1.0 / beta;
+                tau = sc_make<T>((beta - ar) * rb, -ai * rb);
                 const double zr = ar - beta, zi = ai;
-                const double den = zr * zr + zi * zi;
-                scale = sc_make<T>(zr / den, -zi / den);
+                const double rden = 1.0 / (zr * zr + zi * zi);
+                scale = sc_make<T>(zr * rden, -zi * rden);
             }
         }
         // v into LDS for the hemv, the dot and the update
